@@ -1,0 +1,150 @@
+"""Benchmark: hex Ax GDOF/s (N=7, FP64) on one MI355X.
+
+Workload: Poisson on the hex box [-1,1]^3 with 48^3 elements, degree N=7
+(512 nodes/element, 37,595,375 gathered dofs), affine geometry with 7 FP64
+geometric factors per node, Dirichlet boundary, sine manufactured solution.
+One step = one application of the partial Ax over all 110,592 elements, with
+every input resident in HBM. The Ax kernel is timed with HIP events on the
+stream it is launched on. Also reported: the full C0 operator (Ax + gather),
+the gather alone, a PCG+Jacobi solve to the reference's stopping rule, and the
+CPU oracle's Ax on a bounded sample of the same workload (16^3 elements, N=7).
+
+Run: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve] [--no-cpu]
+"""
+import argparse
+import ctypes
+import json
+import os
+import platform as pyplatform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+
+
+def workload_settings(nx=48, N=7, precon="JACOBI"):
+    return {"MESH FILE": "BOX", "MESH DIMENSION": 3, "ELEMENT TYPE": 12, "BOX NX": nx, "BOX NY": nx, "BOX NZ": nx,
+            "BOX DIMX": 2, "BOX DIMY": 2, "BOX DIMZ": 2, "BOX BOUNDARY FLAG": 1, "POLYNOMIAL DEGREE": N,
+            "LAMBDA": 0, "DATA FILE": "data/ellipticSine3D.h", "DISCRETIZATION": "CONTINUOUS",
+            "LINEAR SOLVER": "PCG", "PRECONDITIONER": precon}
+
+
+def cpu_baseline(reps=20):
+    """Oracle partial Ax (serial, 1 thread) on a 16^3 x N=7 sample of the workload."""
+    from tests import oracle_lib
+    s = workload_settings(nx=16)
+    t0 = time.time()
+    o = oracle_lib.OracleElliptic(s)
+    setup = time.time() - t0
+    L = oracle_lib.load()
+    t = L.orc_time_partial_ax(o.h, reps)
+    ng = o.geti("Ngather")
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": ng / t / 1e9, "unit": "GDOF/s", "cores": 1, "kind": "port",
+            "sample": f"oracle partial Ax, 16^3 elements N=7 ({ng} gathered dofs), median of {reps} applications, "
+                      f"serial (1 thread) on {cpu}; setup {setup:.1f}s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--nx", type=int, default=48)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(the path would shard by element slabs with a halo exchange, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import libparanumal_amd as lp
+
+    t0 = time.time()
+    el = lp.Elliptic(workload_settings(nx=args.nx))
+    setup_s = time.time() - t0
+    E, Np, Ng = el.geti("Nelements"), el.geti("Np"), el.geti("Ngather")
+    NL = E * Np
+    stream = torch.cuda.ExternalStream(el.stream)
+    q = torch.rand(Ng, dtype=torch.float64, device="cuda") * 2 - 1
+    AqL = torch.empty(NL, dtype=torch.float64, device="cuda")
+    Aq = torch.empty(Ng, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        stream.synchronize()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for _ in range(steps):
+            fn()
+        stop.record(stream)
+        stop.synchronize()
+        return start.elapsed_time(stop) / steps  # ms per call
+
+    torch.cuda.synchronize()
+    ms_ax = timed(lambda: el.partial_ax(q, AqL), args.steps, args.warmup)
+    ms_op = timed(lambda: el.operator(q, Aq), max(10, args.steps // 2), 2)
+    L = lp.lib()
+    rs, _, _, _ = el.array_info("rowStartsT")
+    ci, _, _, _ = el.array_info("colIdsT")
+    ms_gather = timed(lambda: lp.check(L.libp_ogs_gather_f64(Ng, rs, ci, AqL.data_ptr(), Aq.data_ptr(), el.stream)),
+                      max(10, args.steps // 2), 2)
+
+    bytes_ax = 68 * NL + 8 * Ng          # algorithmic bytes per Ax launch
+    flops_ax = NL * (12 * (7 + 1) + 18)  # algorithmic flops per Ax launch
+    bytes_gather = 12 * NL + 12 * Ng
+    gbs = bytes_ax / (ms_ax * 1e-3) / 1e9
+    result = {
+        "metric": "hex Ax GDOF/s (N=7, FP64)",
+        "value": Ng / (ms_ax * 1e-3) / 1e9,
+        "unit": "GDOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_ax,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"Poisson hex box {args.nx}^3 elements, N=7, affine, Dirichlet, FP64 partial Ax",
+                   "elements": E, "N": 7, "local_nodes": NL, "gathered_dofs": Ng},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs / HBM_PEAK_GBS, "traffic": None},
+        "ax_gflops": flops_ax / (ms_ax * 1e-3) / 1e9,
+        "operator": {"ms": ms_op, "gdofs": Ng / (ms_op * 1e-3) / 1e9},
+        "gather": {"ms": ms_gather, "GBs": bytes_gather / (ms_gather * 1e-3) / 1e9},
+        "setup_seconds": setup_s,
+    }
+    if not args.no_solve:
+        r = el.run()
+        result["solve"] = {"solver": "PCG+Jacobi, stop |r|<=max(1e-8|r0|,1e-8)", "iterations": r["iterations"],
+                           "seconds": r["seconds"], "r0": r["r0"], "r_final": r["r_final"],
+                           "error_l2_vs_exact": r["error_l2"], "solution_norm": r["norm"]}
+    el.close()
+    if not args.no_cpu:
+        try:
+            result["cpu_baseline"] = cpu_baseline()
+        except Exception as e:  # the baseline is reported, never required for the GPU number
+            result["cpu_baseline"] = {"value": None, "error": str(e)}
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

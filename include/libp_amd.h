@@ -1,0 +1,258 @@
+/*
+ * libp_amd — MI355X (gfx950) kernels for libParanumal's C0 hex elliptic path.
+ *
+ * Thin C ABI: plain pointers (device pointers unless stated), sizes and a HIP
+ * stream passed as void* (NULL = the null stream). Every entry point returns 0
+ * on success and a nonzero status otherwise (libp_status_string() names it);
+ * nothing here throws. The C++ layer (elliptic_t, MGLevel, pcg, ogs_t) wraps
+ * these calls and turns a nonzero status into libp::exception, which is how
+ * the reference reports errors (include/utils.hpp:57-76).
+ *
+ * Index types follow the reference: dlong = int32 for local/gathered indices
+ * (include/types.h:53-60). Element lists may be NULL, meaning elements
+ * 0..Nelements-1 in order (the single-rank case, where the reference's local
+ * gather list is exactly that sequence).
+ */
+#ifndef LIBP_AMD_H
+#define LIBP_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  LIBP_OK = 0,
+  LIBP_ERR_ARG = 1,       /* invalid size / degree / null pointer */
+  LIBP_ERR_HIP = 2,       /* a HIP runtime call failed */
+  LIBP_ERR_SETUP = 3,     /* setup error (settings, mesh, geometry) */
+  LIBP_ERR_UNSUPPORTED = 4
+};
+
+const char* libp_status_string(int status);
+/* Last error message recorded by a failing call on this thread. */
+const char* libp_last_error(void);
+
+/* ------------------------------------------------------------------ */
+/* Matrix-free hex Ax                                                   */
+/* ------------------------------------------------------------------ */
+
+/* Replaces ellipticPartialAxHex3D (solvers/elliptic/okl/ellipticAxHex3D.okl:156-295),
+ * launched by elliptic_t::Operator (solvers/elliptic/src/ellipticOperator.cpp:42-47,69-75,97-104).
+ * For each listed element e: q_L = q[GlobalToLocal[e*Np+n]] (0 where the map is -1),
+ * AqL[e*Np+n] = (sum_ab D_a^T G_ab D_b q_L + lambda*wJ*q_L)[n].
+ * Layouts: ggeo [e][6][Np] (G00,G01,G02,G11,G12,G22), wJ [e][Np], D [Nq][Nq]
+ * row-major with D[i*Nq+m] = l'_m(r_i). 1 <= N <= 15. */
+int libp_ax_hex_partial_f64(int N, int Nelements, const int* elementList, const int* GlobalToLocal,
+                            const double* wJ, const double* ggeo, const double* D, double lambda,
+                            const double* q, double* AqL, void* stream);
+/* FP32 variant (no reference counterpart; the reference fixes pfloat at build time,
+ * include/types.h:31-45) used by the p-multigrid smoother. */
+int libp_ax_hex_partial_f32(int N, int Nelements, const int* elementList, const int* GlobalToLocal,
+                            const float* wJ, const float* ggeo, const float* D, float lambda,
+                            const float* q, float* AqL, void* stream);
+/* E-vector in, E-vector out (ellipticAxHex3D, ellipticAxHex3D.okl:28-152). Used for
+ * the Dirichlet lift of ellipticRhsBCHex3D.okl:28-289. */
+int libp_ax_hex_local_f64(int N, int Nelements, const double* wJ, const double* ggeo, const double* D,
+                          double lambda, const double* qL, double* AqL, void* stream);
+/* N=15 variant whose 16x16 tensor contractions run on FP64 MFMA
+ * (v_mfma_f64_16x16x4f64). Same arguments and result as libp_ax_hex_partial_f64. */
+int libp_ax_hex_partial_mfma_n15(int Nelements, const int* elementList, const int* GlobalToLocal,
+                                 const double* wJ, const double* ggeo, const double* D, double lambda,
+                                 const double* q, double* AqL, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Gather-scatter (single rank, signed, unique)                         */
+/* ------------------------------------------------------------------ */
+
+/* Replaces the ogs "gather" kernel with op=Add (libs/ogs/okl/ogsKernels.okl:86-122)
+ * as called by ogsOperator_t::Gather (libs/ogs/ogsOperator.cpp:176-203):
+ * gv[row] = sum_{g in [rowStarts[row], rowStarts[row+1])} v[colIds[g]],
+ * summed from zero in ascending g (the reference order, so results are bit-exact).
+ * Pass the T (all copies) CSR for Trans and the N (owner) CSR for NoTrans. */
+int libp_ogs_gather_f64(int Nrows, const int* rowStarts, const int* colIds, const double* v, double* gv,
+                        void* stream);
+int libp_ogs_gather_f32(int Nrows, const int* rowStarts, const int* colIds, const float* v, float* gv,
+                        void* stream);
+/* Replaces "scatter" (ogsKernels.okl:127-164, ogsOperator.cpp:295-321):
+ * v[colIds[g]] = gv[row] for every g of the row. */
+int libp_ogs_scatter_f64(int Nrows, const int* rowStarts, const int* colIds, const double* gv, double* v,
+                         void* stream);
+int libp_ogs_scatter_i32(int Nrows, const int* rowStarts, const int* colIds, const int* gv, int* v,
+                         void* stream);
+
+/* ------------------------------------------------------------------ */
+/* BLAS-1 (libs/linAlg/okl, wrapped by libs/linAlg/linAlg.cpp)           */
+/* ------------------------------------------------------------------ */
+/* y = alpha*x + beta*y (y not read when beta == 0) — linAlgAXPY.okl:27-39 */
+int libp_axpy_f64(int N, double alpha, const double* x, double beta, double* y, void* stream);
+int libp_axpy_f32(int N, float alpha, const float* x, float beta, float* y, void* stream);
+/* z = alpha*x + beta*y — linAlgAXPY.okl:41-51 */
+int libp_zaxpy_f64(int N, double alpha, const double* x, double beta, const double* y, double* z, void* stream);
+/* x = alpha*a.*x — linAlgAMXPY.okl:27-35 */
+int libp_amx_f64(int N, double alpha, const double* a, double* x, void* stream);
+int libp_amx_f32(int N, float alpha, const float* a, float* x, void* stream);
+/* y = alpha*a.*x + beta*y (y not read when beta == 0) — linAlgAMXPY.okl:37-50 */
+int libp_amxpy_f64(int N, double alpha, const double* a, const double* x, double beta, double* y, void* stream);
+int libp_amxpy_f32(int N, float alpha, const float* a, const float* x, float beta, float* y, void* stream);
+/* x = alpha — linAlgSet.okl; x += alpha — linAlgAdd.okl; x *= alpha — linAlgScale.okl */
+int libp_set_f64(int N, double alpha, double* x, void* stream);
+int libp_add_f64(int N, double alpha, double* x, void* stream);
+int libp_scale_f64(int N, double alpha, double* x, void* stream);
+/* Precision conversion (no reference counterpart). */
+int libp_convert_f64_to_f32(int N, const double* x, float* y, void* stream);
+int libp_convert_f32_to_f64(int N, const float* x, double* y, void* stream);
+
+/* Reductions to a device scalar. Replace innerProd1/2, norm2_1/2, sum1/2
+ * (linAlgInnerProd.okl:28-100, linAlgNorm2.okl, linAlgSum.okl) plus the
+ * host copy of linAlg.cpp:160-224: the result stays in device memory at
+ * *result. ws is a device workspace of at least libp_reduce_ws_bytes()
+ * bytes; its trailing counter must be zero before the first call (the
+ * kernels leave it zero). Deterministic: fixed block partition, partials
+ * summed in block order. */
+long libp_reduce_ws_bytes(void);
+int libp_inner_prod_f64(int N, const double* x, const double* y, double* result, void* ws, void* stream);
+int libp_weighted_inner_prod_f64(int N, const double* w, const double* x, const double* y, double* result,
+                                 void* ws, void* stream);
+int libp_norm2sq_f64(int N, const double* x, double* result, void* ws, void* stream);
+int libp_sum_f64(int N, const double* x, double* result, void* ws, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* PCG / FPCG fused vector kernels (libs/linearSolver)                  */
+/* ------------------------------------------------------------------ */
+/* Device-resident scalar block used by the fused PCG kernels (doubles):
+ *  [0] rdotz1  [1] rdotz2  [2] alpha  [3] beta  [4] pAp  [5] rdotr  [6] zdotAp  [7] TOL */
+enum { LIBP_PCG_RDOTZ1 = 0, LIBP_PCG_RDOTZ2, LIBP_PCG_ALPHA, LIBP_PCG_BETA, LIBP_PCG_PAP,
+       LIBP_PCG_RDOTR, LIBP_PCG_ZDOTAP, LIBP_PCG_TOL, LIBP_PCG_NSCALARS };
+
+/* Replaces updatePCG (libs/linearSolver/okl/linearSolverUpdatePCG.okl:29-75) and the
+ * host sum of pcg::UpdatePCG (linearSolverPCG.cpp:153-171): alpha = s[RDOTZ1]/s[PAP]
+ * is formed on the device, then x += alpha*p, r -= alpha*Ap, s[RDOTR] = r.r. */
+int libp_pcg_update_f64(int N, const double* p, const double* Ap, double* x, double* r, double* scalars,
+                        void* ws, void* stream);
+/* r.z and (flexible) z.Ap, then beta and p = z + beta*p, as in
+ * linearSolverPCG.cpp:113-127 (iter==0 gives beta=0). */
+int libp_pcg_rdotz_f64(int N, const double* r, const double* z, const double* Ap, int flexible, double* scalars,
+                       void* ws, void* stream);
+int libp_pcg_update_p_f64(int N, int first_iter, const double* z, double* p, const double* scalars, void* stream);
+/* p.Ap into scalars[PAP]. */
+int libp_pcg_pap_f64(int N, const double* p, const double* Ap, double* scalars, void* ws, void* stream);
+/* Jacobi preconditioner z = invDiag.*r fused with r.z (JacobiPrecon::Operator,
+ * solvers/elliptic/src/ellipticPreconJacobi.cpp:42-51 followed by the r.z dot). */
+int libp_pcg_jacobi_rdotz_f64(int N, const double* invDiag, const double* r, double* z, const double* Ap,
+                              int flexible, double* scalars, void* ws, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* p-multigrid transfers and smoother pieces                            */
+/* ------------------------------------------------------------------ */
+/* Replaces ellipticPartialPreconCoarsenHex3D (okl/ellipticPreconCoarsenHex3D.okl:209-294),
+ * called by MGLevel::coarsen (src/ellipticPreconMultiGridLevel.cpp:41-82):
+ * qc[e*NpC+..] = (P^T x P^T x P^T) qf_L, qf_L gathered through the fine GlobalToLocal.
+ * P is NqF x NqC row-major (P[f*NqC+c]). */
+int libp_mg_coarsen_hex_f64(int Nf, int Nc, int Nelements, const int* elementList, const int* GlobalToLocal,
+                            const double* P, const double* qf, double* qc, void* stream);
+int libp_mg_coarsen_hex_f32(int Nf, int Nc, int Nelements, const int* elementList, const int* GlobalToLocal,
+                            const float* P, const float* qf, float* qc, void* stream);
+/* Replaces ellipticPartialPreconProlongateHex3D (okl/ellipticPreconProlongateHex3D.okl:216-301),
+ * called by MGLevel::prolongate (ellipticPreconMultiGridLevel.cpp:84-125). */
+int libp_mg_prolong_hex_f64(int Nf, int Nc, int Nelements, const int* elementList, const int* GlobalToLocalC,
+                            const double* P, const double* qc, double* qf, void* stream);
+int libp_mg_prolong_hex_f32(int Nf, int Nc, int Nelements, const int* elementList, const int* GlobalToLocalC,
+                            const float* P, const float* qc, float* qf, void* stream);
+/* Chebyshev recurrence steps of MGLevel::smoothChebyshev (ellipticPreconMultiGridLevel.cpp:154-206):
+ * cheb_start: RES = invDiag.*(r - Ax) (Ax may be NULL: x is zero), d = invTheta*RES, and, when
+ *             xIsZero, X = d;
+ * cheb_step:  RES -= invDiag.*Ad, d = c1*RES + c2*d, X += d (fused).
+ * Both FP32 (the smoother precision) and FP64. */
+int libp_cheb_start_f32(int N, const float* invDiag, const float* r, const float* Ax, float invTheta, float* RES,
+                        float* d, float* X, int xIsZero, void* stream);
+int libp_cheb_step_f32(int N, const float* invDiag, const float* Ad, float c1, float c2, float* RES, float* d,
+                       float* X, void* stream);
+int libp_cheb_start_f64(int N, const double* invDiag, const double* r, const double* Ax, double invTheta,
+                        double* RES, double* d, double* X, int xIsZero, void* stream);
+int libp_cheb_step_f64(int N, const double* invDiag, const double* Ad, double c1, double c2, double* RES,
+                       double* d, double* X, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Setup kernels (mesh geometry, operator diagonal, right-hand side)    */
+/* ------------------------------------------------------------------ */
+/* Physical nodes and volume geometric factors of hex elements: restates
+ * meshPhysicalNodesHex3D.cpp:31-112 (trilinear map of EX/EY/EZ [e][8]) followed by
+ * the optional curved map x += amp*sin(2 pi y)*sin(2 pi z) (no reference
+ * counterpart), and meshGeometricFactorsHex3D.cpp:31-193 for ggeo and wJ.
+ * r1 = GLL nodes (Nq), w1 = GLL weights (Nq), D as above. *badElement receives
+ * 1 + the largest element index whose Jacobian fell below 1e-12 (0 if none);
+ * it must be zero on entry (device int). */
+int libp_geofacs_hex_f64(int N, int Nelements, const double* EX, const double* EY, const double* EZ,
+                         const double* r1, const double* w1, const double* D, double curveAmp, double* x,
+                         double* y, double* z, double* ggeo, double* wJ, int* badElement, void* stream);
+/* Local (E-vector) operator diagonal of BuildOperatorDiagonalContinuousHex3D
+ * (src/ellipticBuildOperatorDiagonal.cpp:998-1057); mapB is the elliptic node flag
+ * (1 = Dirichlet). The caller gathers it with libp_ogs_gather_f64 (Trans). */
+int libp_diag_hex_local_f64(int N, int Nelements, const double* ggeo, const double* wJ, const double* D,
+                            double lambda, const int* mapB, int allNeumann, double neumannShift, double* diagL,
+                            void* stream);
+/* Forcing of ellipticRhsHex3D.okl:28-51 for the built-in data sets, and the Dirichlet
+ * data used by ellipticRhsBCHex3D / ellipticAddBCHex3D:
+ *   dataKind 0: data/ellipticSine3D.h (f=(3pi^2+lambda) sin sin sin, u_b = sin sin sin)
+ *   dataKind 1: seeded 64-mode Fourier forcing (coeffs: 64 x {kx,ky,kz,amp,phase}), u_b = 0.
+ * rhsL = wJ*f; ubL = (mapB==1) ? u_b : 0. */
+int libp_rhs_hex_f64(int Nelements, int Np, int dataKind, const double* coeffs, const double* x, const double* y,
+                     const double* z, const double* wJ, double lambda, const int* mapB, double* rhsL, double* ubL,
+                     void* stream);
+/* xL[n] = u_b(x,y,z) where mapB==1 (ellipticAddBCHex3D.okl:27-47). */
+int libp_addbc_hex_f64(int NL, int dataKind, const double* coeffs, const double* x, const double* y, const double* z,
+                       const int* mapB, double* xL, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Learned per-element smoother (no reference counterpart)              */
+/* ------------------------------------------------------------------ */
+/* BF16-weight MLP, FP32 accumulate, hand-written MFMA GEMM tiles for gfx950:
+ * Y[M,Nout] = act(X[M,K] * W[K,Nout] + b), with X, Y FP32 and W BF16 (row-major).
+ * act: 0 = identity, 1 = ReLU. M is any size; K and Nout multiples of 32. */
+int libp_mlp_linear_fwd(int M, int K, int Nout, const float* X, const unsigned short* Wbf16, const float* b,
+                        int act, float* Y, void* stream);
+/* Backward of the same layer: given dY (FP32) and the layer output Y (for the ReLU
+ * mask), computes dX = (dY.*mask) W^T (if dX != NULL), dW = X^T (dY.*mask) and
+ * db = colsum(dY.*mask) in FP32. */
+int libp_mlp_linear_bwd(int M, int K, int Nout, const float* X, const unsigned short* Wbf16, const float* Y,
+                        const float* dY, int act, float* dX, float* dW, float* db, void* stream);
+/* Adam update of FP32 master weights with BF16 copy: m,v moments, step t (1-based). */
+int libp_adam_step(int n, float* master, unsigned short* wbf16, const float* grad, float* m, float* v, float lr,
+                   float beta1, float beta2, float eps, int t, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Solver handle: the C++ elliptic_t driven through the C ABI           */
+/* ------------------------------------------------------------------ */
+/* Creates platform + mesh + elliptic_t (+ preconditioner) from .rc text in the
+ * reference format (libs/core/settings.cpp:134-214). Returns NULL on error. */
+void* libp_elliptic_create(const char* rc_text);
+void libp_elliptic_free(void* h);
+/* Scalar facts about the handle: "Nelements","N","Np","Ngather","Nmasked",... */
+long libp_elliptic_geti(void* h, const char* name);
+double libp_elliptic_getd(void* h, const char* name);
+/* Device pointer of a named array of the fine level ("ggeo","wJ","GlobalToLocal",
+ * "rowStartsT","colIdsT","rowStartsN","colIdsN","x","y","z","D","invDiag",...),
+ * with its length and dtype (0 f64, 1 i32, 2 i64, 3 f32). Host arrays from setup
+ * ("globalIds","mapB","blockRowStartsT","blockRowStartsN","EToV") are returned
+ * as host pointers with *on_host = 1. */
+void* libp_elliptic_array(void* h, const char* name, long* length, int* dtype, int* on_host);
+/* The HIP stream the handle launches on. */
+void* libp_elliptic_stream(void* h);
+/* Aq = A q on gathered vectors (elliptic_t::Operator). */
+int libp_elliptic_operator(void* h, const double* q, double* Aq);
+/* Partial Ax only (E-vector out). */
+int libp_elliptic_partial_ax(void* h, const double* q, double* AqL);
+/* Apply the configured preconditioner: z = M^{-1} r. */
+int libp_elliptic_precon(void* h, const double* r, double* z);
+/* elliptic_t::Run: rhs, solve, boundary fill, solution norm. Results in out[]:
+ * [0] solution norm, [1] iterations, [2] solve seconds, [3] final |r|, [4] |r0|,
+ * [5] max nodal error vs the manufactured solution (sine data only, else NaN). */
+int libp_elliptic_run(void* h, double* out, int nout);
+/* Copy of the solution E-vector (host buffer of Nelements*Np doubles) from the last run. */
+int libp_elliptic_solution(void* h, double* xL_host);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

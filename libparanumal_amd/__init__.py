@@ -1,0 +1,222 @@
+"""libparanumal_amd: MI355X (gfx950) build of libParanumal's C0 hex elliptic path.
+
+The product is the in-tree shared library ``libp_amd.so`` (HIP kernels, the
+C++ elliptic_t/pcg/p-multigrid host layer and its C ABI, declared in
+``include/libp_amd.h``) plus the ``ellipticMain`` driver. This module only
+binds that C ABI with ctypes for Python callers; it has no compute path of its
+own and raises if the library is missing.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libp_amd.so")
+MAIN_PATH = os.path.join(_HERE, "ellipticMain")
+
+_lib = None
+
+c_int, c_long, c_double, c_float, c_void_p, c_char_p = (
+    ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_float, ctypes.c_void_p, ctypes.c_char_p)
+
+# name -> (restype, argtypes) for every entry point of include/libp_amd.h
+SIGNATURES = {
+    "libp_status_string": (c_char_p, [c_int]),
+    "libp_last_error": (c_char_p, []),
+    "libp_ax_hex_partial_f64": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double,
+                                        c_void_p, c_void_p, c_void_p]),
+    "libp_ax_hex_partial_f32": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float,
+                                        c_void_p, c_void_p, c_void_p]),
+    "libp_ax_hex_local_f64": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p,
+                                      c_void_p]),
+    "libp_ax_hex_partial_mfma_n15": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double,
+                                             c_void_p, c_void_p, c_void_p]),
+    "libp_ogs_gather_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "libp_ogs_gather_f32": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "libp_ogs_scatter_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "libp_ogs_scatter_i32": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "libp_axpy_f64": (c_int, [c_int, c_double, c_void_p, c_double, c_void_p, c_void_p]),
+    "libp_axpy_f32": (c_int, [c_int, c_float, c_void_p, c_float, c_void_p, c_void_p]),
+    "libp_zaxpy_f64": (c_int, [c_int, c_double, c_void_p, c_double, c_void_p, c_void_p, c_void_p]),
+    "libp_amx_f64": (c_int, [c_int, c_double, c_void_p, c_void_p, c_void_p]),
+    "libp_amx_f32": (c_int, [c_int, c_float, c_void_p, c_void_p, c_void_p]),
+    "libp_amxpy_f64": (c_int, [c_int, c_double, c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
+    "libp_amxpy_f32": (c_int, [c_int, c_float, c_void_p, c_void_p, c_float, c_void_p, c_void_p]),
+    "libp_set_f64": (c_int, [c_int, c_double, c_void_p, c_void_p]),
+    "libp_add_f64": (c_int, [c_int, c_double, c_void_p, c_void_p]),
+    "libp_scale_f64": (c_int, [c_int, c_double, c_void_p, c_void_p]),
+    "libp_convert_f64_to_f32": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
+    "libp_convert_f32_to_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
+    "libp_reduce_ws_bytes": (c_long, []),
+    "libp_inner_prod_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "libp_weighted_inner_prod_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "libp_norm2sq_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "libp_sum_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "libp_pcg_update_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "libp_pcg_rdotz_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "libp_pcg_update_p_f64": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "libp_pcg_pap_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "libp_pcg_jacobi_rdotz_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                                          c_void_p]),
+    "libp_mg_coarsen_hex_f64": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                        c_void_p]),
+    "libp_mg_coarsen_hex_f32": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                        c_void_p]),
+    "libp_mg_prolong_hex_f64": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                        c_void_p]),
+    "libp_mg_prolong_hex_f32": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                        c_void_p]),
+    "libp_cheb_start_f32": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_void_p, c_void_p,
+                                    c_int, c_void_p]),
+    "libp_cheb_step_f32": (c_int, [c_int, c_void_p, c_void_p, c_float, c_float, c_void_p, c_void_p, c_void_p,
+                                   c_void_p]),
+    "libp_cheb_start_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p,
+                                    c_int, c_void_p]),
+    "libp_cheb_step_f64": (c_int, [c_int, c_void_p, c_void_p, c_double, c_double, c_void_p, c_void_p, c_void_p,
+                                   c_void_p]),
+    "libp_geofacs_hex_f64": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_void_p]),
+    "libp_diag_hex_local_f64": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_int,
+                                        c_double, c_void_p, c_void_p]),
+    "libp_rhs_hex_f64": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double,
+                                 c_void_p, c_void_p, c_void_p, c_void_p]),
+    "libp_addbc_hex_f64": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                   c_void_p]),
+    "libp_mlp_linear_fwd": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    "libp_mlp_linear_bwd": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
+                                    c_void_p, c_void_p, c_void_p]),
+    "libp_adam_step": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_float, c_float,
+                               c_float, c_int, c_void_p]),
+    "libp_elliptic_create": (c_void_p, [c_char_p]),
+    "libp_elliptic_free": (None, [c_void_p]),
+    "libp_elliptic_geti": (c_long, [c_void_p, c_char_p]),
+    "libp_elliptic_getd": (c_double, [c_void_p, c_char_p]),
+    "libp_elliptic_array": (c_void_p, [c_void_p, c_char_p, ctypes.POINTER(c_long), ctypes.POINTER(c_int),
+                                       ctypes.POINTER(c_int)]),
+    "libp_elliptic_stream": (c_void_p, [c_void_p]),
+    "libp_elliptic_operator": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "libp_elliptic_partial_ax": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "libp_elliptic_precon": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "libp_elliptic_run": (c_int, [c_void_p, ctypes.POINTER(c_double), c_int]),
+    "libp_elliptic_solution": (c_int, [c_void_p, c_void_p]),
+}
+
+
+class LibpError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded libp_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LibpError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                            "or `make -C libparanumal_amd`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status, what=""):
+    if status != 0:
+        L = lib()
+        raise LibpError(f"{what}: {L.libp_status_string(status).decode()} ({L.libp_last_error().decode()})")
+
+
+def settings_text(settings):
+    """Render a dict of settings in the .rc format (test/test.py:84-92)."""
+    return "".join(f"[{k}]\n{v}\n\n" for k, v in settings.items())
+
+
+_DTYPES = {0: "float64", 1: "int32", 2: "int64", 3: "float32"}
+
+
+class Elliptic:
+    """elliptic_t built from settings, driven through the C ABI."""
+
+    def __init__(self, settings):
+        text = settings if isinstance(settings, str) else settings_text(settings)
+        self._h = lib().libp_elliptic_create(text.encode())
+        if not self._h:
+            raise LibpError("libp_elliptic_create: " + lib().libp_last_error().decode())
+
+    def close(self):
+        if self._h:
+            lib().libp_elliptic_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def geti(self, name):
+        return int(lib().libp_elliptic_geti(self._h, name.encode()))
+
+    def getd(self, name):
+        return float(lib().libp_elliptic_getd(self._h, name.encode()))
+
+    @property
+    def stream(self):
+        return lib().libp_elliptic_stream(self._h)
+
+    def array_info(self, name):
+        n, dt, host = c_long(), c_int(), c_int()
+        p = lib().libp_elliptic_array(self._h, name.encode(), ctypes.byref(n), ctypes.byref(dt), ctypes.byref(host))
+        if not p:
+            raise LibpError(f"no array named {name}")
+        return p, n.value, _DTYPES[dt.value], bool(host.value)
+
+    def host_array(self, name):
+        """Copy of a named array as a numpy array (device arrays are copied back)."""
+        import numpy as np
+        p, n, dt, on_host = self.array_info(name)
+        out = np.empty(n, dtype=dt)
+        if on_host:
+            ctypes.memmove(out.ctypes.data, p, out.nbytes)
+        else:
+            hip_d2h(out.ctypes.data, p, out.nbytes, self.stream)
+        return out
+
+    def operator(self, q, Aq):
+        check(lib().libp_elliptic_operator(self._h, q.data_ptr(), Aq.data_ptr()), "operator")
+
+    def partial_ax(self, q, AqL):
+        check(lib().libp_elliptic_partial_ax(self._h, q.data_ptr(), AqL.data_ptr()), "partial_ax")
+
+    def precon(self, r, z):
+        check(lib().libp_elliptic_precon(self._h, r.data_ptr(), z.data_ptr()), "precon")
+
+    def run(self):
+        out = (c_double * 6)()
+        check(lib().libp_elliptic_run(self._h, out, 6), "run")
+        return {"norm": out[0], "iterations": int(out[1]), "seconds": out[2], "r_final": out[3], "r0": out[4],
+                "error_l2": out[5]}
+
+    def solution(self):
+        import numpy as np
+        x = np.empty(self.geti("Nelements") * self.geti("Np"))
+        check(lib().libp_elliptic_solution(self._h, x.ctypes.data), "solution")
+        return x
+
+
+def hip_d2h(dst, src, nbytes, stream=None):
+    """Synchronise `stream` and copy device memory to host with the HIP runtime
+    that libp_amd.so already loaded."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipStreamSynchronize.restype = c_int
+    hip.hipStreamSynchronize.argtypes = [c_void_p]
+    hip.hipMemcpy.restype = c_int
+    hip.hipMemcpy.argtypes = [c_void_p, c_void_p, ctypes.c_size_t, c_int]
+    if stream:
+        if hip.hipStreamSynchronize(c_void_p(stream)) != 0:
+            raise LibpError("hipStreamSynchronize failed")
+    st = hip.hipMemcpy(c_void_p(dst), c_void_p(src), ctypes.c_size_t(nbytes), 2)  # hipMemcpyDeviceToHost
+    if st != 0:
+        raise LibpError(f"hipMemcpy failed ({st})")

@@ -1,0 +1,200 @@
+// C ABI over elliptic_t: lets Python (ctypes), bench drivers and foreign
+// callers build and drive the C++ solver without C++ types at the boundary.
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "elliptic.hpp"
+
+namespace libp_amd {
+void set_last_error(const char* where, const char* what);
+}
+
+namespace libp {
+int multigridCoarseIterations(MultiGridPrecon& mg);
+}
+
+using namespace libp;
+
+namespace {
+struct Handle {
+  settings_t settings;
+  platform_t platform;
+  mesh_t mesh;
+  std::shared_ptr<elliptic_t> elliptic;
+  ellipticRunResult_t last;
+};
+
+template <typename F>
+int guarded(const char* where, F f) {
+  try {
+    f();
+    return LIBP_OK;
+  } catch (std::exception& e) {
+    libp_amd::set_last_error(where, e.what());
+    return LIBP_ERR_SETUP;
+  }
+}
+}  // namespace
+
+extern "C" void* libp_elliptic_create(const char* rc_text) {
+  Handle* h = nullptr;
+  int st = guarded("libp_elliptic_create", [&]() {
+    LIBP_ABORT("null settings text", rc_text == nullptr);
+    std::unique_ptr<Handle> hh(new Handle());
+    hh->settings = ellipticDefaultSettings();
+    hh->settings.readSettingsFromText(rc_text);
+    hh->platform = platform_t(hh->settings);
+    hh->mesh.Setup(hh->platform, hh->settings, comm_t());
+    double lambda = 0.0;
+    hh->settings.getSetting("LAMBDA", lambda);
+    memory<int> BCType(3);
+    BCType[0] = 0;
+    BCType[1] = 1;
+    BCType[2] = 2;
+    hh->elliptic = std::make_shared<elliptic_t>(hh->platform, hh->mesh, hh->settings, lambda, 3, BCType);
+    hh->platform.finish();
+    h = hh.release();
+  });
+  return st == LIBP_OK ? h : nullptr;
+}
+
+extern "C" void libp_elliptic_free(void* p) { delete (Handle*)p; }
+
+extern "C" long libp_elliptic_geti(void* p, const char* name) {
+  Handle* h = (Handle*)p;
+  if (!h || !name) return -1;
+  const std::string n(name);
+  elliptic_t& e = *h->elliptic;
+  if (n == "Nelements") return e.mesh.Nelements;
+  if (n == "N") return e.mesh.N;
+  if (n == "Np") return e.mesh.Np;
+  if (n == "Nq") return e.mesh.Nq;
+  if (n == "Ngather") return e.Ndofs;
+  if (n == "Nmasked") return e.Nmasked;
+  if (n == "allNeumann") return e.allNeumann;
+  if (n == "NrowBlocksT") return e.ogsMasked.NrowBlocksT;
+  if (n == "NrowBlocksN") return e.ogsMasked.NrowBlocksN;
+  if (n == "lastIterations") return h->last.iterations;
+  if (n == "coarseIterations") {
+    if (auto* mg = dynamic_cast<MultiGridPrecon*>(e.precon.get())) return multigridCoarseIterations(*mg);
+    return -1;
+  }
+  return -1;
+}
+
+extern "C" double libp_elliptic_getd(void* p, const char* name) {
+  Handle* h = (Handle*)p;
+  if (!h || !name) return NAN;
+  const std::string n(name);
+  elliptic_t& e = *h->elliptic;
+  if (n == "lambda") return e.lambda;
+  if (n == "allNeumannScale") return e.allNeumannScale;
+  if (n == "curveAmp") return e.mesh.curveAmp;
+  return NAN;
+}
+
+extern "C" void* libp_elliptic_array(void* p, const char* name, long* length, int* dtype, int* on_host) {
+  Handle* h = (Handle*)p;
+  if (!h || !name) return nullptr;
+  const std::string n(name);
+  elliptic_t& e = *h->elliptic;
+  mesh_t& m = e.mesh;
+  *on_host = 0;
+  auto dev = [&](auto& mem, int dt) -> void* {
+    *length = (long)mem.length();
+    *dtype = dt;
+    return (void*)mem.ptr();
+  };
+  auto host = [&](auto& mem, int dt) -> void* {
+    *length = (long)mem.length();
+    *dtype = dt;
+    *on_host = 1;
+    return (void*)mem.ptr();
+  };
+  if (n == "ggeo") return dev(m.o_ggeo, 0);
+  if (n == "wJ") return dev(m.o_wJ, 0);
+  if (n == "x") return dev(m.o_x, 0);
+  if (n == "y") return dev(m.o_y, 0);
+  if (n == "z") return dev(m.o_z, 0);
+  if (n == "D") return dev(m.o_D, 0);
+  if (n == "gllz") return dev(m.o_gllz, 0);
+  if (n == "gllw") return dev(m.o_gllw, 0);
+  if (n == "weightG") return dev(e.o_weightG, 0);
+  if (n == "GlobalToLocal") return dev(e.o_GlobalToLocal, 1);
+  if (n == "rowStartsT") return dev(e.ogsMasked.o_rowStartsT, 1);
+  if (n == "colIdsT") return dev(e.ogsMasked.o_colIdsT, 1);
+  if (n == "rowStartsN") return dev(e.ogsMasked.o_rowStartsN, 1);
+  if (n == "colIdsN") return dev(e.ogsMasked.o_colIdsN, 1);
+  if (n == "mapBDevice") return dev(e.o_mapB, 1);
+  if (n == "invDiag") {
+    if (auto* j = dynamic_cast<JacobiPrecon*>(e.precon.get())) return dev(j->o_invDiagA, 0);
+    return nullptr;
+  }
+  if (n == "globalIds") return host(m.globalIds, 2);
+  if (n == "maskedGlobalIds") return host(e.maskedGlobalIds, 2);
+  if (n == "meshMapB") return host(m.mapB, 1);
+  if (n == "mapB") return host(e.mapB, 1);
+  if (n == "EToV") return host(m.EToV, 2);
+  if (n == "blockRowStartsT") return host(e.ogsMasked.blockRowStartsT, 1);
+  if (n == "blockRowStartsN") return host(e.ogsMasked.blockRowStartsN, 1);
+  return nullptr;
+}
+
+extern "C" void* libp_elliptic_stream(void* p) {
+  Handle* h = (Handle*)p;
+  return h ? (void*)h->platform.stream : nullptr;
+}
+
+extern "C" int libp_elliptic_operator(void* p, const double* q, double* Aq) {
+  Handle* h = (Handle*)p;
+  return guarded("libp_elliptic_operator", [&]() {
+    LIBP_ABORT("null handle/pointer", !h || !q || !Aq);
+    const dlong N = h->elliptic->Ndofs;
+    deviceMemory<dfloat> o_q = deviceMemory<dfloat>::wrap(const_cast<double*>(q), N);
+    deviceMemory<dfloat> o_Aq = deviceMemory<dfloat>::wrap(Aq, N);
+    h->elliptic->Operator(o_q, o_Aq);
+  });
+}
+
+extern "C" int libp_elliptic_partial_ax(void* p, const double* q, double* AqL) {
+  Handle* h = (Handle*)p;
+  return guarded("libp_elliptic_partial_ax", [&]() {
+    LIBP_ABORT("null handle/pointer", !h || !q || !AqL);
+    elliptic_t& e = *h->elliptic;
+    deviceMemory<dfloat> o_q = deviceMemory<dfloat>::wrap(const_cast<double*>(q), e.Ndofs);
+    deviceMemory<dfloat> o_AqL = deviceMemory<dfloat>::wrap(AqL, (size_t)e.mesh.Nelements * e.mesh.Np);
+    e.PartialAx(o_q, o_AqL);
+  });
+}
+
+extern "C" int libp_elliptic_precon(void* p, const double* r, double* z) {
+  Handle* h = (Handle*)p;
+  return guarded("libp_elliptic_precon", [&]() {
+    LIBP_ABORT("null handle/pointer", !h || !r || !z);
+    const dlong N = h->elliptic->Ndofs;
+    deviceMemory<dfloat> o_r = deviceMemory<dfloat>::wrap(const_cast<double*>(r), N);
+    deviceMemory<dfloat> o_z = deviceMemory<dfloat>::wrap(z, N);
+    h->elliptic->precon.Operator(o_r, o_z);
+  });
+}
+
+extern "C" int libp_elliptic_run(void* p, double* out, int nout) {
+  Handle* h = (Handle*)p;
+  return guarded("libp_elliptic_run", [&]() {
+    LIBP_ABORT("null handle", !h);
+    h->last = h->elliptic->Run();
+    const double v[6] = {h->last.norm, (double)h->last.iterations, h->last.solveSeconds,
+                         h->last.rFinal, h->last.r0, h->last.maxError};
+    for (int i = 0; i < nout && i < 6; ++i) out[i] = v[i];
+  });
+}
+
+extern "C" int libp_elliptic_solution(void* p, double* xL_host) {
+  Handle* h = (Handle*)p;
+  return guarded("libp_elliptic_solution", [&]() {
+    LIBP_ABORT("no solution available", !h || !h->elliptic->o_xL_last.isInitialized());
+    h->elliptic->o_xL_last.copyTo(xL_host, h->elliptic->o_xL_last.length(), 0, h->platform.stream);
+    h->platform.finish();
+  });
+}

@@ -1,0 +1,191 @@
+// Core runtime implementation: settings and the HIP platform.
+#include "core.hpp"
+
+#include <cctype>
+#include <iostream>
+
+namespace libp {
+
+void settings_t::newSetting(const std::string& name, const std::string& val, const std::string& description,
+                            const std::vector<std::string>& options) {
+  LIBP_ABORT("Setting with name: [" << name << "] already exists.", settings.count(name));
+  settings[name] = setting_t{name, val, description, options};
+  insertOrder.push_back(name);
+}
+
+// setting_t::updateVal (settings.cpp:41-59): options, when given, are exhaustive.
+void settings_t::changeSetting(const std::string& name, const std::string& newVal) {
+  auto it = settings.find(name);
+  LIBP_ABORT("Setting with name: [" << name << "] does not exist.", it == settings.end());
+  setting_t& s = it->second;
+  if (!s.options.empty()) {
+    bool ok = false;
+    for (auto& o : s.options) ok = ok || (o == newVal);
+    LIBP_ABORT("Value: \"" << newVal << "\" not valid for setting " << name, !ok);
+  }
+  s.val = newVal;
+}
+
+std::string settings_t::getSetting(const std::string& name) const {
+  auto it = settings.find(name);
+  LIBP_ABORT("Unable to find setting: [" << name << "]", it == settings.end());
+  return it->second.val;
+}
+
+bool settings_t::compareSetting(const std::string& name, const std::string& token) const {
+  return getSetting(name).find(token) != std::string::npos;
+}
+
+// Line grammar of settings_t::readSettingsFromFile (settings.cpp:134-214):
+// '#' ends the line, "[NAME]" starts a setting, all other non-space characters
+// accumulate into the value; a pair is emitted when both parts are non-empty.
+std::vector<std::pair<std::string, std::string>> settings_t::parseText(const std::string& text) {
+  std::vector<std::pair<std::string, std::string>> out;
+  std::istringstream in(text);
+  std::string line, name, val;
+  while (std::getline(in, line)) {
+    LIBP_ABORT("Line in settings file is too long: " << line, line.size() > 512);
+    for (size_t i = 0; i < line.size(); ++i) {
+      const char c = line[i];
+      if (c == '#') break;
+      if (c == '[') {
+        if (!name.empty() && !val.empty()) {
+          out.emplace_back(name, val);
+          name.clear();
+          val.clear();
+        }
+        name.clear();
+        val.clear();
+        ++i;
+        while (i < line.size() && line[i] != ']') name += line[i++];
+      } else if (!isspace((unsigned char)c)) {
+        val += c;
+      }
+    }
+    if (!name.empty() && !val.empty()) {
+      out.emplace_back(name, val);
+      name.clear();
+      val.clear();
+    }
+  }
+  return out;
+}
+
+// ellipticSettings_t::parseFromFile (ellipticSettings.cpp:127-148): every key
+// must already exist.
+void settings_t::readSettingsFromText(const std::string& text) {
+  for (auto& kv : parseText(text)) {
+    LIBP_ABORT("Unknown setting: [" << kv.first << "] requested", !hasSetting(kv.first));
+    changeSetting(kv.first, kv.second);
+  }
+}
+
+void settings_t::report(std::ostream& os) const {
+  for (auto& n : insertOrder) os << "[" << n << "] " << settings.at(n).val << "\n";
+}
+
+settings_t ellipticDefaultSettings() {
+  settings_t s;
+  // platformSettings_t (libs/core/platformSettings.cpp:31-56)
+  s.newSetting("FORMAT", "2.0", "Setting file version number", {"2.0"});
+  s.newSetting("THREAD MODEL", "HIP", "Parallel execution platform", {"Serial", "OpenMP", "CUDA", "HIP", "OpenCL"});
+  s.newSetting("PLATFORM NUMBER", "0");
+  s.newSetting("DEVICE NUMBER", "0");
+  s.newSetting("CACHE DIR", ".");
+  // meshSettings_t (libs/mesh/meshSettings.cpp:32-91)
+  s.newSetting("MESH FILE", "BOX");
+  s.newSetting("MESH DIMENSION", "2", "Dimension of input mesh", {"2", "3"});
+  s.newSetting("ELEMENT TYPE", "3", "Type of mesh elements (number of edges)", {"3", "4", "6", "12"});
+  s.newSetting("ELEMENT MAP", "ISOPARAMETRIC", "Element map", {"ISOPARAMETRIC", "AFFINE"});
+  s.newSetting("BOX DIMX", "10");
+  s.newSetting("BOX DIMY", "10");
+  s.newSetting("BOX DIMZ", "10");
+  s.newSetting("BOX NX", "10");
+  s.newSetting("BOX NY", "10");
+  s.newSetting("BOX NZ", "10");
+  s.newSetting("BOX GLOBAL NX", "0");
+  s.newSetting("BOX GLOBAL NY", "0");
+  s.newSetting("BOX GLOBAL NZ", "0");
+  s.newSetting("BOX BOUNDARY FLAG", "1");
+  s.newSetting("POLYNOMIAL DEGREE", "4", "Degree of polynomial finite element space",
+               {"1", "2", "3", "4", "5", "6", "7", "8", "9", "10", "11", "12", "13", "14", "15"});
+  // paradogs settings accepted for file compatibility (meshSettings.cpp:90)
+  s.newSetting("PARADOGS PARTITIONING", "INERTIAL");
+  // this build: smooth node perturbation x += A sin(2 pi y) sin(2 pi z)
+  s.newSetting("BOX CURVED MAP AMPLITUDE", "0", "Amplitude of the curved box map (0 = trilinear box)");
+  // ellipticAddRunSettings (ellipticSettings.cpp:38-54)
+  s.newSetting("DATA FILE", "data/ellipticSine2D.h", "Boundary and Initial conditions header");
+  s.newSetting("LAMBDA", "1.0");
+  s.newSetting("OUTPUT TO FILE", "FALSE", "", {"TRUE", "FALSE"});
+  s.newSetting("OUTPUT FILE NAME", "elliptic");
+  // ellipticAddSettings (ellipticSettings.cpp:56-103)
+  s.newSetting("DISCRETIZATION", "CONTINUOUS", "", {"CONTINUOUS", "IPDG"});
+  s.newSetting("LINEAR SOLVER", "PCG", "", {"PCG", "FPCG", "NBPCG", "NBFPCG", "PGMRES", "PMINRES"});
+  s.newSetting("LINEAR SOLVER STOPPING CRITERION", "ABS/REL-INITRESID", "",
+               {"ABS/REL-INITRESID", "ABS/REL-RHS-2NORM", "RELATIVE"});
+  s.newSetting("PRECONDITIONER", "NONE", "",
+               {"NONE", "JACOBI", "MASSMATRIX", "PARALMOND", "MULTIGRID", "SEMFEM", "OAS"});
+  s.newSetting("MULTIGRID COARSENING", "HALFDOFS", "", {"ALLDEGREES", "HALFDEGREES", "HALFDOFS", "CUSTOM"});
+  s.newSetting("MULTIGRID SMOOTHER", "CHEBYSHEV", "", {"DAMPEDJACOBI", "CHEBYSHEV", "LEARNED"});
+  s.newSetting("MULTIGRID CHEBYSHEV DEGREE", "2");
+  s.newSetting("VERBOSE", "FALSE", "", {"TRUE", "FALSE"});
+  // parAlmond (libs/parAlmond/parAlmondSettings.cpp:33-60)
+  s.newSetting("PARALMOND CYCLE", "VCYCLE", "", {"VCYCLE", "KCYCLE", "EXACT"});
+  s.newSetting("PARALMOND STRENGTH", "SYMMETRIC", "", {"RUGESTUBEN", "SYMMETRIC"});
+  s.newSetting("PARALMOND AGGREGATION", "SMOOTHED", "", {"SMOOTHED", "UNSMOOTHED"});
+  s.newSetting("PARALMOND SMOOTHER", "CHEBYSHEV", "", {"DAMPEDJACOBI", "CHEBYSHEV"});
+  s.newSetting("PARALMOND CHEBYSHEV DEGREE", "2");
+  // this build: p-MG degree list, precision, coarse Jacobi-PCG and solver knobs
+  s.newSetting("MULTIGRID DEGREES", "", "Comma-separated degrees for MULTIGRID COARSENING CUSTOM, e.g. 7,3,1");
+  s.newSetting("MULTIGRID PRECISION", "FP32", "Precision of p-MG smoothing and transfers", {"FP32", "FP64"});
+  s.newSetting("MULTIGRID COARSE TOLERANCE", "1e-2", "Relative tolerance of the coarse Jacobi-PCG");
+  s.newSetting("MULTIGRID COARSE MAXIT", "200");
+  s.newSetting("MULTIGRID LEARNED WEIGHTS", "", "Weight file of the learned fine smoother");
+  s.newSetting("LINEAR SOLVER TOLERANCE", "1e-8");
+  s.newSetting("MAXIMUM ITERATIONS", "5000");
+  s.newSetting("DATA SEED", "1", "Seed of the Fourier forcing data set");
+  return s;
+}
+
+platform_t::platform_t(const settings_t& s) {
+  int dev = 0;
+  s.getSetting("DEVICE NUMBER", dev);
+  int ndev = 0;
+  LIBP_HIP(hipGetDeviceCount(&ndev));
+  LIBP_ABORT("No HIP device available", ndev <= 0);
+  LIBP_ABORT("DEVICE NUMBER " << dev << " out of range", dev < 0 || dev >= ndev);
+  device = dev;
+  LIBP_HIP(hipSetDevice(device));
+  hipStream_t st;
+  LIBP_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  double* pinned = nullptr;
+  LIBP_HIP(hipHostMalloc((void**)&pinned, 16 * sizeof(double), hipHostMallocDefault));
+  struct State {
+    hipStream_t st = nullptr;
+    double* pinned = nullptr;
+    ~State() {
+      if (st) (void)hipStreamDestroy(st);
+      if (pinned) (void)hipHostFree(pinned);
+    }
+  };
+  auto stt = std::make_shared<State>();
+  stt->st = st;
+  stt->pinned = pinned;
+  state = stt;
+  stream = st;
+  h_scalars = pinned;
+  o_reduceWS = deviceMemory<char>::alloc((size_t)libp_reduce_ws_bytes());
+  LIBP_HIP(hipMemsetAsync(o_reduceWS.ptr(), 0, o_reduceWS.length(), stream));
+  o_scalars = deviceMemory<double>::alloc(64);
+  LIBP_HIP(hipMemsetAsync(o_scalars.ptr(), 0, 64 * sizeof(double), stream));
+  finish();
+}
+
+const double* platform_t::readScalars(const double* dptr, int count) const {
+  LIBP_ABORT("readScalars: at most 16 values", count > 16);
+  LIBP_HIP(hipMemcpyAsync(h_scalars, dptr, count * sizeof(double), hipMemcpyDeviceToHost, stream));
+  finish();
+  return h_scalars;
+}
+
+}  // namespace libp

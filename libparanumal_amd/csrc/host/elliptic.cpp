@@ -1,0 +1,297 @@
+// elliptic_t for the C0 hex path on one MI355X GPU.
+#include "elliptic.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+
+namespace libp {
+
+void elliptic_t::Setup(platform_t& _platform, mesh_t& _mesh, settings_t& _settings, dfloat _lambda,
+                       const int _NBCTypes, const memory<int> _BCType, std::shared_ptr<randStream_t> _rng) {
+  platform = _platform;
+  mesh = _mesh;
+  comm = _mesh.comm;
+  settings = _settings;
+  lambda = _lambda;
+  rng = _rng ? _rng : std::make_shared<randStream_t>(1);
+  disc_c0 = settings.compareSetting("DISCRETIZATION", "CONTINUOUS");
+  LIBP_ABORT("Only DISCRETIZATION CONTINUOUS is supported on this path", !disc_c0);
+  NBCTypes = _NBCTypes;
+  BCType = _BCType.clone();
+  BoundarySetup();
+  SetupPrecon();
+}
+
+void elliptic_t::SetupPrecon() {
+  if (settings.compareSetting("PRECONDITIONER", "JACOBI"))
+    precon.Setup<JacobiPrecon>(*this);
+  else if (settings.compareSetting("PRECONDITIONER", "MULTIGRID"))
+    precon.Setup<MultiGridPrecon>(*this);
+  else if (settings.compareSetting("PRECONDITIONER", "NONE"))
+    precon.Setup<IdentityPrecon>(Ndofs, platform);
+  else
+    LIBP_FORCE_ABORT("PRECONDITIONER " << settings.getSetting("PRECONDITIONER") << " is not supported on this path");
+}
+
+void elliptic_t::BoundarySetup() {
+  const dlong NL = mesh.Nelements * mesh.Np;
+  allNeumann = (lambda == 0) ? 1 : 0;
+  allNeumannPenalty = 1.;
+  // node-wise flags (ellipticBoundarySetup.cpp:53-65); box faces all carry one flag,
+  // so the face-wise EToB translation only matters through allNeumann
+  if (!mesh.periodic && mesh.boundaryFlag > 0) {
+    LIBP_ABORT("BOX BOUNDARY FLAG outside BCType table", mesh.boundaryFlag >= NBCTypes);
+    if (BCType[mesh.boundaryFlag] != 2) allNeumann = 0;
+  }
+  Nmasked = 0;
+  mapB.malloc(NL, 0);
+  for (dlong n = 0; n < NL; ++n) {
+    const int bc = mesh.mapB[n];
+    if (bc > 0) {
+      mapB[n] = BCType[bc];
+      if (mapB[n] == 1) Nmasked++;
+    }
+  }
+  maskIds.malloc(Nmasked);
+  Nmasked = 0;
+  for (dlong n = 0; n < NL; ++n)
+    if (mapB[n] == 1) maskIds[Nmasked++] = n;
+  o_mapB = platform.malloc(mapB);
+
+  maskedGlobalIds = mesh.globalIds.clone();
+  for (dlong n = 0; n < Nmasked; ++n) maskedGlobalIds[maskIds[n]] = 0;
+  ogsMasked.Setup(NL, maskedGlobalIds, true, *rng, platform);
+  allNeumannScale = 1. / std::sqrt((dfloat)ogsMasked.NgatherGlobal);
+
+  // inverse multiplicity (ellipticBoundarySetup.cpp:102-114): the gather of
+  // ones is exactly the row length
+  weightG.malloc(ogsMasked.Ngather);
+  for (dlong r = 0; r < ogsMasked.Ngather; ++r) {
+    const double cnt = ogsMasked.rowStartsT[r + 1] - ogsMasked.rowStartsT[r];
+    weightG[r] = (cnt > 0.0) ? 1. / cnt : 0.0;
+  }
+  o_weightG = platform.malloc(weightG);
+  o_weightG32 = deviceMemory<float>();
+
+  ogsMasked.SetupGlobalToLocalMapping(GlobalToLocal);
+  o_GlobalToLocal = platform.malloc(GlobalToLocal);
+  o_AqL = platform.malloc<dfloat>(NL);
+  o_AqL32 = deviceMemory<float>();
+  Ndofs = ogsMasked.Ngather * Nfields;
+  Nhalo = 0;
+}
+
+void elliptic_t::PartialAx(deviceMemory<dfloat>& o_q, deviceMemory<dfloat>& o_AqL_) {
+  LIBP_CALL(libp_ax_hex_partial_f64(mesh.N, mesh.NlocalGatherElements, nullptr, o_GlobalToLocal.ptr(),
+                                    mesh.o_wJ.ptr(), mesh.o_ggeo.ptr(), mesh.o_D.ptr(), lambda, o_q.ptr(),
+                                    o_AqL_.ptr(), platform.s()));
+}
+
+void elliptic_t::Operator(deviceMemory<dfloat>& o_q, deviceMemory<dfloat>& o_Aq) {
+  PartialAx(o_q, o_AqL);
+  ogsMasked.Gather(o_Aq, o_AqL, 1, ogs::Add, ogs::Trans);
+}
+
+void elliptic_t::OperatorF32(deviceMemory<float>& o_q, deviceMemory<float>& o_Aq) {
+  mesh.MakeFP32();
+  if (!o_AqL32.isInitialized()) o_AqL32 = platform.malloc<float>((size_t)mesh.Nelements * mesh.Np);
+  LIBP_CALL(libp_ax_hex_partial_f32(mesh.N, mesh.NlocalGatherElements, nullptr, o_GlobalToLocal.ptr(),
+                                    mesh.o_wJ32.ptr(), mesh.o_ggeo32.ptr(), mesh.o_D32.ptr(), (float)lambda,
+                                    o_q.ptr(), o_AqL32.ptr(), platform.s()));
+  ogsMasked.Gather(o_Aq, o_AqL32, 1, ogs::Add, ogs::Trans);
+}
+
+void elliptic_t::BuildOperatorDiagonal(deviceMemory<dfloat>& o_diagA) {
+  const dlong NL = mesh.Nelements * mesh.Np;
+  deviceMemory<dfloat> o_diagL = platform.malloc<dfloat>(NL);
+  const double shift = allNeumannPenalty * allNeumannScale * allNeumannScale;
+  LIBP_CALL(libp_diag_hex_local_f64(mesh.N, mesh.Nelements, mesh.o_ggeo.ptr(), mesh.o_wJ.ptr(), mesh.o_D.ptr(),
+                                    lambda, o_mapB.ptr(), allNeumann, shift, o_diagL.ptr(), platform.s()));
+  ogsMasked.Gather(o_diagA, o_diagL, 1, ogs::Add, ogs::Trans);
+  platform.finish();
+}
+
+void elliptic_t::ZeroMean(deviceMemory<dfloat>& o_q) {
+  double* tmp = platform.o_scalars.ptr() + 8;
+  LIBP_CALL(libp_sum_f64(Ndofs, o_q.ptr(), tmp, platform.o_reduceWS.ptr(), platform.s()));
+  double qmean = platform.readScalars(tmp, 1)[0];
+  qmean *= allNeumannScale * allNeumannScale;
+  LIBP_CALL(libp_add_f64(Ndofs, -qmean, o_q.ptr(), platform.s()));
+}
+
+elliptic_t elliptic_t::SetupNewDegree(mesh_t& meshC) {
+  if (meshC.N == mesh.N) return *this;
+  elliptic_t e = *this;
+  e.mesh = meshC;
+  e.BoundarySetup();
+  e.precon = precon_t();
+  return e;
+}
+
+int elliptic_t::dataKind() const {
+  const std::string f = settings.getSetting("DATA FILE");
+  if (f.find("Fourier3D") != std::string::npos) return 1;
+  LIBP_ABORT("DATA FILE " << f << " is not supported (ellipticSine3D.h or ellipticFourier3D)",
+             f.find("Sine3D") == std::string::npos);
+  return 0;
+}
+
+// 64-mode forcing sum_m a_m sin(pi k_m.x + phi_m): k in {1..4}^3, a = +-1/|k|^2,
+// phi uniform in [0,2pi), drawn with erand48 from [DATA SEED].
+deviceMemory<dfloat> elliptic_t::dataCoefficients() {
+  if (dataKind() != 1) return deviceMemory<dfloat>();
+  long seed = 1;
+  settings.getSetting("DATA SEED", seed);
+  unsigned short xs[3] = {0x330E, (unsigned short)(seed & 0xffff), (unsigned short)((seed >> 16) & 0xffff)};
+  std::vector<double> c(64 * 5);
+  for (int m = 0; m < 64; ++m) {
+    const int kx = 1 + (int)(4 * erand48(xs));
+    const int ky = 1 + (int)(4 * erand48(xs));
+    const int kz = 1 + (int)(4 * erand48(xs));
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    const double sg = (erand48(xs) < 0.5) ? -1.0 : 1.0;
+    c[5 * m + 0] = kx;
+    c[5 * m + 1] = ky;
+    c[5 * m + 2] = kz;
+    c[5 * m + 3] = sg / k2;
+    c[5 * m + 4] = 2 * M_PI * erand48(xs);
+  }
+  return platform.malloc(c);
+}
+
+int elliptic_t::Solve(linearSolver_t& linearSolver, deviceMemory<dfloat>& o_x, deviceMemory<dfloat>& o_r,
+                      const dfloat tol, const int MAXIT, const int verbose) {
+  if (allNeumann) ZeroMean(o_r);
+  return linearSolver.Solve(*this, precon, o_x, o_r, tol, MAXIT, verbose);
+}
+
+ellipticRunResult_t elliptic_t::Run(int verboseOverride) {
+  ellipticRunResult_t res;
+  const hlong NglobalDofs = ogsMasked.NgatherGlobal * Nfields;
+  LIBP_ABORT("LINEAR SOLVER " << settings.getSetting("LINEAR SOLVER") << " is not supported on this path",
+             !settings.compareSetting("LINEAR SOLVER", "PCG") || settings.compareSetting("LINEAR SOLVER", "NBPCG") ||
+                 settings.compareSetting("LINEAR SOLVER", "NBFPCG"));
+  linearSolver_t linearSolver;
+  linearSolver.Setup<LinearSolver::pcg>(Ndofs, Nhalo, platform, settings, comm);
+  linearSolver.get()->pureRelative = settings.compareSetting("LINEAR SOLVER STOPPING CRITERION", "RELATIVE") &&
+                                     !settings.compareSetting("LINEAR SOLVER STOPPING CRITERION", "ABS/REL");
+
+  const int kind = dataKind();
+  deviceMemory<dfloat> o_coeffs = dataCoefficients();
+  const dlong NL = mesh.Nelements * mesh.Np;
+  deviceMemory<dfloat> o_rL = platform.malloc<dfloat>(NL), o_xL = platform.malloc<dfloat>(NL);
+  deviceMemory<dfloat> o_ubL = platform.malloc<dfloat>(NL), o_AubL = platform.malloc<dfloat>(NL);
+  LIBP_CALL(libp_set_f64(NL, 0.0, o_xL.ptr(), platform.s()));
+  // forcing (ellipticRhsHex3D.okl) and Dirichlet lift (ellipticRhsBCHex3D.okl: rhs -= A_L u_b)
+  LIBP_CALL(libp_rhs_hex_f64(mesh.Nelements, mesh.Np, kind, o_coeffs.ptr(), mesh.o_x.ptr(), mesh.o_y.ptr(),
+                             mesh.o_z.ptr(), mesh.o_wJ.ptr(), lambda, o_mapB.ptr(), o_rL.ptr(), o_ubL.ptr(),
+                             platform.s()));
+  LIBP_CALL(libp_ax_hex_local_f64(mesh.N, mesh.Nelements, mesh.o_wJ.ptr(), mesh.o_ggeo.ptr(), mesh.o_D.ptr(), lambda,
+                                  o_ubL.ptr(), o_AubL.ptr(), platform.s()));
+  LIBP_CALL(libp_axpy_f64(NL, -1.0, o_AubL.ptr(), 1.0, o_rL.ptr(), platform.s()));
+
+  deviceMemory<dfloat> o_r = platform.malloc<dfloat>(Ndofs + Nhalo), o_x = platform.malloc<dfloat>(Ndofs + Nhalo);
+  ogsMasked.Gather(o_r, o_rL, 1, ogs::Add, ogs::Trans);
+  ogsMasked.Gather(o_x, o_xL, 1, ogs::Add, ogs::NoTrans);
+
+  int maxIter = 5000;
+  settings.getSetting("MAXIMUM ITERATIONS", maxIter);
+  double tol = 1e-8;
+  settings.getSetting("LINEAR SOLVER TOLERANCE", tol);
+  const int verbose = (verboseOverride >= 0) ? verboseOverride : (settings.compareSetting("VERBOSE", "TRUE") ? 1 : 0);
+
+  platform.finish();
+  const auto t0 = std::chrono::steady_clock::now();
+  const int iter = Solve(linearSolver, o_x, o_r, tol, maxIter, verbose);
+  ogsMasked.Scatter(o_xL, o_x, 1, ogs::NoTrans);
+  LIBP_CALL(libp_addbc_hex_f64(NL, kind, o_coeffs.ptr(), mesh.o_x.ptr(), mesh.o_y.ptr(), mesh.o_z.ptr(),
+                               o_mapB.ptr(), o_xL.ptr(), platform.s()));
+  platform.finish();
+  const auto t1 = std::chrono::steady_clock::now();
+  const double elapsed = std::chrono::duration<double>(t1 - t0).count();
+  if (verbose)
+    printf("%d, %lld, %g, %d, %g, %g; global: N, dofs, elapsed, iterations, time per node, nodes*iterations/time %s\n",
+           mesh.N, (long long)NglobalDofs, elapsed, iter, elapsed / NglobalDofs, NglobalDofs * ((dfloat)iter / elapsed),
+           settings.getSetting("PRECONDITIONER").c_str());
+
+  // Solution norm sqrt(xL' M xL) (ellipticRun.cpp:238-247)
+  double* tmp = platform.o_scalars.ptr() + 8;
+  LIBP_CALL(libp_weighted_inner_prod_f64(NL, mesh.o_wJ.ptr(), o_xL.ptr(), o_xL.ptr(), tmp, platform.o_reduceWS.ptr(),
+                                         platform.s()));
+  res.norm = std::sqrt(platform.readScalars(tmp, 1)[0]);
+  if (kind == 0) {
+    // L2 error against the manufactured solution: u_b evaluated at every node
+    memory<int> all(NL, 1);
+    deviceMemory<int> o_all = platform.malloc(all);
+    LIBP_CALL(libp_addbc_hex_f64(NL, 0, nullptr, mesh.o_x.ptr(), mesh.o_y.ptr(), mesh.o_z.ptr(), o_all.ptr(),
+                                 o_ubL.ptr(), platform.s()));
+    LIBP_CALL(libp_axpy_f64(NL, 1.0, o_xL.ptr(), -1.0, o_ubL.ptr(), platform.s()));
+    LIBP_CALL(libp_weighted_inner_prod_f64(NL, mesh.o_wJ.ptr(), o_ubL.ptr(), o_ubL.ptr(), tmp,
+                                           platform.o_reduceWS.ptr(), platform.s()));
+    res.maxError = std::sqrt(platform.readScalars(tmp, 1)[0]);
+  }
+  if (comm.rank() == 0 && verbose) printf("Solution norm = %17.15lg\n", res.norm);
+  res.iterations = iter;
+  res.solveSeconds = elapsed;
+  res.r0 = std::sqrt(linearSolver.get()->stats.rdotr0);
+  res.rFinal = std::sqrt(std::max(linearSolver.get()->stats.rdotrFinal, 0.0));
+  o_xL_last = o_xL;
+  return res;
+}
+
+/* JacobiPrecon (ellipticPreconJacobi.cpp:30-51). */
+JacobiPrecon::JacobiPrecon(elliptic_t& _elliptic) : elliptic(_elliptic) {
+  deviceMemory<dfloat> o_diagA = elliptic.platform.malloc<dfloat>(elliptic.Ndofs);
+  elliptic.BuildOperatorDiagonal(o_diagA);
+  std::vector<double> d(elliptic.Ndofs);
+  o_diagA.copyTo(d.data(), d.size(), 0, elliptic.platform.stream);
+  elliptic.platform.finish();
+  for (auto& v : d) v = 1.0 / v;
+  o_invDiagA = elliptic.platform.malloc(d);
+}
+
+void JacobiPrecon::Operator(deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_Mr) {
+  LIBP_CALL(libp_amxpy_f64(elliptic.Ndofs, 1.0, o_invDiagA.ptr(), o_r.ptr(), 0.0, o_Mr.ptr(), elliptic.platform.s()));
+  if (elliptic.allNeumann) elliptic.ZeroMean(o_Mr);
+}
+
+std::vector<int> multigridDegrees(const settings_t& settings, int N) {
+  std::vector<int> deg;
+  if (settings.compareSetting("MULTIGRID COARSENING", "CUSTOM")) {
+    std::string v = settings.getSetting("MULTIGRID DEGREES");
+    size_t p = 0;
+    while (p < v.size()) {
+      size_t q = v.find(',', p);
+      if (q == std::string::npos) q = v.size();
+      deg.push_back(std::atoi(v.substr(p, q - p).c_str()));
+      p = q + 1;
+    }
+    LIBP_ABORT("MULTIGRID DEGREES must start at the POLYNOMIAL DEGREE and end at 1",
+               deg.empty() || deg.front() != N || deg.back() != 1);
+    for (size_t i = 1; i < deg.size(); ++i) LIBP_ABORT("MULTIGRID DEGREES must decrease", deg[i] >= deg[i - 1]);
+    return deg;
+  }
+  int Nf = N, Nc = N, NpFine = (N + 1) * (N + 1) * (N + 1), NpCoarse = NpFine;
+  deg.push_back(N);
+  while (Nc > 1) {
+    if (settings.compareSetting("MULTIGRID COARSENING", "ALLDEGREES")) {
+      Nc = Nf - 1;
+    } else if (settings.compareSetting("MULTIGRID COARSENING", "HALFDEGREES")) {
+      Nc = std::max(1, (Nf + 1) / 2);
+    } else {
+      while (NpCoarse > NpFine / 2 && Nc > 1) {
+        Nc--;
+        NpCoarse = (Nc + 1) * (Nc + 1) * (Nc + 1);
+      }
+    }
+    NpCoarse = (Nc + 1) * (Nc + 1) * (Nc + 1);
+    deg.push_back(Nc);
+    Nf = Nc;
+    NpFine = NpCoarse;
+  }
+  return deg;
+}
+
+}  // namespace libp

@@ -1,0 +1,270 @@
+// Hex box mesh: structured connectivity in closed form, geometry on the GPU.
+#include "mesh.hpp"
+
+#include <cmath>
+
+namespace libp {
+
+static void legendre(int N, double x, double& P, double& dP) {
+  // P_N(x) and P_N'(x) by the three-term recurrence
+  double p0 = 1.0, p1 = x, d0 = 0.0, d1 = 1.0;
+  if (N == 0) {
+    P = 1.0;
+    dP = 0.0;
+    return;
+  }
+  for (int n = 1; n < N; ++n) {
+    const double p2 = ((2 * n + 1) * x * p1 - n * p0) / (n + 1);
+    const double d2 = d0 + (2 * n + 1) * p1;
+    p0 = p1;
+    p1 = p2;
+    d0 = d1;
+    d1 = d2;
+  }
+  P = p1;
+  dP = d1;
+}
+
+void gllNodesWeights(int N, std::vector<double>& r, std::vector<double>& w) {
+  r.assign(N + 1, 0.0);
+  w.assign(N + 1, 0.0);
+  r[0] = -1.0;
+  r[N] = 1.0;
+  // interior nodes: roots of P_N'; Newton on q(x) = P_N'(x) with
+  // q'(x) = (2x P_N' - N(N+1) P_N)/(1-x^2), from Chebyshev-Lobatto guesses.
+  for (int i = 1; 2 * i <= N; ++i) {
+    double x = -std::cos(M_PI * i / N);
+    for (int it = 0; it < 100; ++it) {
+      double P, dP;
+      legendre(N, x, P, dP);
+      const double ddP = (2 * x * dP - N * (N + 1) * P) / (1 - x * x);
+      const double dx = dP / ddP;
+      x -= dx;
+      if (std::fabs(dx) < 1e-17) break;
+    }
+    r[i] = x;
+    r[N - i] = -x;
+  }
+  if (N % 2 == 0) r[N / 2] = 0.0;
+  for (int i = 0; i <= N; ++i) {
+    double P, dP;
+    legendre(N, r[i], P, dP);
+    w[i] = 2.0 / (N * (N + 1) * P * P);
+  }
+}
+
+void gllDmatrix(int N, const std::vector<double>& r, std::vector<double>& D) {
+  const int Nq = N + 1;
+  D.assign(Nq * Nq, 0.0);
+  if (N == 0) return;
+  std::vector<double> P(Nq);
+  for (int i = 0; i < Nq; ++i) {
+    double dP;
+    legendre(N, r[i], P[i], dP);
+  }
+  for (int i = 0; i < Nq; ++i)
+    for (int j = 0; j < Nq; ++j) {
+      if (i != j) D[i * Nq + j] = P[i] / (P[j] * (r[i] - r[j]));
+    }
+  D[0] = -0.25 * N * (N + 1);
+  D[N * Nq + N] = 0.25 * N * (N + 1);
+}
+
+void degreeRaiseMatrix(int Nc, int Nf, std::vector<double>& P) {
+  std::vector<double> rc, wc, rf, wf;
+  gllNodesWeights(Nc, rc, wc);
+  gllNodesWeights(Nf, rf, wf);
+  const int NqC = Nc + 1, NqF = Nf + 1;
+  P.assign(NqF * NqC, 0.0);
+  for (int f = 0; f < NqF; ++f)
+    for (int c = 0; c < NqC; ++c) {
+      double l = 1.0;
+      for (int k = 0; k < NqC; ++k)
+        if (k != c) l *= (rf[f] - rc[k]) / (rc[c] - rc[k]);
+      P[f * NqC + c] = l;
+    }
+}
+
+void mesh_t::Setup(platform_t& _platform, settings_t& _settings, comm_t _comm) {
+  platform = _platform;
+  settings = _settings;
+  comm = _comm;
+  LIBP_ABORT("Only BOX meshes are supported", !settings.compareSetting("MESH FILE", "BOX"));
+  LIBP_ABORT("Only hexahedral meshes (ELEMENT TYPE 12) are supported",
+             !settings.compareSetting("ELEMENT TYPE", "12"));
+  LIBP_ABORT("Only 3-D meshes are supported", !settings.compareSetting("MESH DIMENSION", "3"));
+  SetupBox();
+  int NN;
+  settings.getSetting("POLYNOMIAL DEGREE", NN);
+  ReferenceNodes(NN);
+  ConnectNodes();
+  GeometricFactors();
+  GatherScatterSetup();
+}
+
+void mesh_t::SetupBox() {
+  hlong GX, GY, GZ;
+  settings.getSetting("BOX GLOBAL NX", GX);
+  settings.getSetting("BOX GLOBAL NY", GY);
+  settings.getSetting("BOX GLOBAL NZ", GZ);
+  settings.getSetting("BOX NX", nx);
+  settings.getSetting("BOX NY", ny);
+  settings.getSetting("BOX NZ", nz);
+  if (GX * GY * GZ > 0) {  // global sizes override local ones (meshSetupBoxHex3D.cpp:59-65)
+    nx = (dlong)GX;
+    ny = (dlong)GY;
+    nz = (dlong)GZ;
+  }
+  LIBP_ABORT("BOX NX/NY/NZ must be positive", nx <= 0 || ny <= 0 || nz <= 0);
+  settings.getSetting("BOX BOUNDARY FLAG", boundaryFlag);
+  periodic = (boundaryFlag == -1);
+  LIBP_ABORT("periodic boxes need at least 2 elements per direction", periodic && (nx < 2 || ny < 2 || nz < 2));
+  LIBP_ABORT("BOX BOUNDARY FLAG must be -1, 0, 1 or 2", boundaryFlag < -1 || boundaryFlag > 2);
+  double DIMX, DIMY, DIMZ;
+  settings.getSetting("BOX DIMX", DIMX);
+  settings.getSetting("BOX DIMY", DIMY);
+  settings.getSetting("BOX DIMZ", DIMZ);
+  settings.getSetting("BOX CURVED MAP AMPLITUDE", curveAmp);
+  const double dx = DIMX / nx, dy = DIMY / ny, dz = DIMZ / nz;
+  const double X0 = -DIMX / 2.0, Y0 = -DIMY / 2.0, Z0 = -DIMZ / 2.0;
+  const hlong NnX = periodic ? nx : nx + 1, NnY = periodic ? ny : ny + 1, NnZ = periodic ? nz : nz + 1;
+  Nelements = nx * ny * nz;
+  EToV.malloc((size_t)Nelements * Nverts);
+  EX.malloc((size_t)Nelements * Nverts);
+  EY.malloc((size_t)Nelements * Nverts);
+  EZ.malloc((size_t)Nelements * Nverts);
+  // vertex order of the reference hex (meshSetupBoxHex3D.cpp:126-152)
+  static const int vi[8] = {0, 1, 1, 0, 0, 1, 1, 0}, vj[8] = {0, 0, 1, 1, 0, 0, 1, 1}, vk[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+#pragma omp parallel for collapse(3)
+  for (dlong k = 0; k < nz; ++k)
+    for (dlong j = 0; j < ny; ++j)
+      for (dlong i = 0; i < nx; ++i) {
+        const size_t e = i + (size_t)j * nx + (size_t)k * nx * ny;
+        for (int v = 0; v < 8; ++v) {
+          const hlong ii = (i + vi[v]) % NnX, jj = (j + vj[v]) % NnY, kk = (k + vk[v]) % NnZ;
+          EToV[e * 8 + v] = ii + jj * NnX + kk * NnX * NnY;
+          EX[e * 8 + v] = X0 + dx * i + (vi[v] ? dx : 0.0);
+          EY[e * 8 + v] = Y0 + dy * j + (vj[v] ? dy : 0.0);
+          EZ[e * 8 + v] = Z0 + dz * k + (vk[v] ? dz : 0.0);
+        }
+      }
+  o_EX = platform.malloc(EX);
+  o_EY = platform.malloc(EY);
+  o_EZ = platform.malloc(EZ);
+}
+
+void mesh_t::ReferenceNodes(int NN) {
+  LIBP_ABORT("POLYNOMIAL DEGREE must be in [1,15]", NN < 1 || NN > 15);
+  N = NN;
+  Nq = N + 1;
+  Nfp = Nq * Nq;
+  Np = Nq * Nq * Nq;
+  gllNodesWeights(N, gllz, gllw);
+  gllDmatrix(N, gllz, D);
+  o_gllz = platform.malloc(gllz);
+  o_gllw = platform.malloc(gllw);
+  o_D = platform.malloc(D);
+}
+
+/* Node numbering. The reference labels every local node 1+(local index) and
+ * propagates the minimum across face traces to a fixed point
+ * (meshConnectNodes.cpp:32-117); on a structured box the fixed point is known:
+ * a node's id is 1 + the smallest local index among its copies. Copies of a
+ * lattice point lie in the elements of a product set of per-direction
+ * candidates, so the smallest copy takes, in each direction, the lowest
+ * element containing the point. mapB: the smallest positive boundary flag of
+ * the boundary faces touching the node, else -1 (meshConnectNodes.cpp:52-67);
+ * every box face carries the same flag. */
+void mesh_t::ConnectNodes() {
+  const size_t NL = (size_t)Nelements * Np;
+  globalIds.malloc(NL);
+  mapB.malloc(NL);
+  const hlong LX = (hlong)nx * N, LY = (hlong)ny * N, LZ = (hlong)nz * N;
+  auto lowest = [&](hlong X, hlong L, dlong nE, dlong& em, int& im) {
+    if (periodic) X %= L;
+    if (X == 0) {
+      em = 0;
+    } else if (X % N == 0) {
+      em = (dlong)(X / N - 1);
+    } else {
+      em = (dlong)(X / N);
+    }
+    (void)nE;
+    im = (int)(X - (hlong)em * N);
+  };
+  const bool flagged = !periodic && boundaryFlag > 0;
+#pragma omp parallel for collapse(3)
+  for (dlong ez = 0; ez < nz; ++ez)
+    for (dlong ey = 0; ey < ny; ++ey)
+      for (dlong ex = 0; ex < nx; ++ex) {
+        const size_t e = ex + (size_t)ey * nx + (size_t)ez * nx * ny;
+        for (int k = 0; k < Nq; ++k)
+          for (int j = 0; j < Nq; ++j)
+            for (int i = 0; i < Nq; ++i) {
+              const hlong X = (hlong)ex * N + i, Y = (hlong)ey * N + j, Z = (hlong)ez * N + k;
+              dlong exm, eym, ezm;
+              int im, jm, km;
+              lowest(X, LX, nx, exm, im);
+              lowest(Y, LY, ny, eym, jm);
+              lowest(Z, LZ, nz, ezm, km);
+              const hlong em = exm + (hlong)eym * nx + (hlong)ezm * nx * ny;
+              const size_t id = e * Np + i + j * Nq + k * Nq * Nq;
+              globalIds[id] = 1 + em * Np + im + jm * Nq + km * Nq * Nq;
+              const bool onB = (X == 0 || X == LX || Y == 0 || Y == LY || Z == 0 || Z == LZ);
+              mapB[id] = (flagged && onB) ? boundaryFlag : -1;
+            }
+      }
+}
+
+void mesh_t::GeometricFactors() {
+  const size_t NL = (size_t)Nelements * Np;
+  o_x = platform.malloc<double>(NL);
+  o_y = platform.malloc<double>(NL);
+  o_z = platform.malloc<double>(NL);
+  o_ggeo = platform.malloc<double>(NL * 6);
+  o_wJ = platform.malloc<double>(NL);
+  deviceMemory<int> o_bad = platform.malloc<int>(1);
+  LIBP_HIP(hipMemsetAsync(o_bad.ptr(), 0, sizeof(int), platform.stream));
+  LIBP_CALL(libp_geofacs_hex_f64(N, Nelements, o_EX.ptr(), o_EY.ptr(), o_EZ.ptr(), o_gllz.ptr(), o_gllw.ptr(),
+                                 o_D.ptr(), curveAmp, o_x.ptr(), o_y.ptr(), o_z.ptr(), o_ggeo.ptr(), o_wJ.ptr(),
+                                 o_bad.ptr(), platform.s()));
+  int bad = 0;
+  o_bad.copyTo(&bad, 1, 0, platform.stream);
+  platform.finish();
+  LIBP_ABORT("Negative J found at element " << bad - 1, bad > 0);
+  o_ggeo32 = deviceMemory<float>();
+  o_wJ32 = deviceMemory<float>();
+}
+
+void mesh_t::GatherScatterSetup() {
+  NlocalGatherElements = Nelements;
+  NglobalGatherElements = 0;
+  localGatherElementList.malloc(Nelements);
+  for (dlong e = 0; e < Nelements; ++e) localGatherElementList[e] = e;
+}
+
+void mesh_t::MakeFP32() {
+  if (o_ggeo32.isInitialized()) return;
+  const size_t NL = (size_t)Nelements * Np;
+  o_ggeo32 = platform.malloc<float>(NL * 6);
+  o_wJ32 = platform.malloc<float>(NL);
+  o_D32 = platform.malloc<float>((size_t)Nq * Nq);
+  LIBP_CALL(libp_convert_f64_to_f32((int)(NL * 6), o_ggeo.ptr(), o_ggeo32.ptr(), platform.s()));
+  LIBP_CALL(libp_convert_f64_to_f32((int)NL, o_wJ.ptr(), o_wJ32.ptr(), platform.s()));
+  LIBP_CALL(libp_convert_f64_to_f32(Nq * Nq, o_D.ptr(), o_D32.ptr(), platform.s()));
+}
+
+mesh_t mesh_t::SetupNewDegree(int Nf) {
+  if (Nf == N) return *this;
+  mesh_t m = *this;
+  m.o_ggeo32 = deviceMemory<float>();
+  m.o_wJ32 = deviceMemory<float>();
+  m.o_D32 = deviceMemory<float>();
+  m.ReferenceNodes(Nf);
+  m.ConnectNodes();
+  m.GeometricFactors();
+  m.GatherScatterSetup();
+  return m;
+}
+
+}  // namespace libp

@@ -1,0 +1,532 @@
+// p-multigrid preconditioner (MultiGridPrecon, MGLevel, multigrid_t V-cycle)
+// with the smoother and transfers in FP32 or FP64 and a Jacobi-PCG base level.
+//
+// Reference: solvers/elliptic/src/ellipticPreconMultiGrid.cpp:31-154,
+// ellipticPreconMultiGridLevel.cpp:30-473, libs/parAlmond/parAlmondVcycle.cpp:34-60,
+// and the multigridLevel ABC of include/parAlmond.hpp:70-91.
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <type_traits>
+
+#include "elliptic.hpp"
+#include "learned.hpp"
+
+namespace libp {
+
+/* Eigenvalues of a k x k upper Hessenberg matrix (column-major H[i+j*k]) by
+ * complex single-shift QR with deflation; stands in for dgeev in
+ * ellipticPreconMultiGridLevel.cpp:456-470. */
+static std::vector<std::complex<double>> hessenbergEigenvalues(int n, const std::vector<double>& H) {
+  typedef std::complex<double> cd;
+  std::vector<cd> A(n * n);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) A[i * n + j] = H[i + j * n];
+  double scale = 0;
+  for (auto& a : A) scale = std::max(scale, std::abs(a));
+  if (scale == 0) scale = 1;
+  std::vector<cd> ev, gc(n), gs(n);
+  int hi = n - 1, its = 0;
+  while (hi >= 0) {
+    if (hi == 0) {
+      ev.push_back(A[0]);
+      break;
+    }
+    int l = hi;
+    for (; l > 0; --l) {
+      double d = std::abs(A[(l - 1) * n + l - 1]) + std::abs(A[l * n + l]);
+      if (d == 0) d = scale;
+      if (std::abs(A[l * n + l - 1]) <= 1e-16 * d) {
+        A[l * n + l - 1] = 0;
+        break;
+      }
+    }
+    if (l == hi) {
+      ev.push_back(A[hi * n + hi]);
+      --hi;
+      its = 0;
+      continue;
+    }
+    LIBP_ABORT("Hessenberg QR did not converge", ++its > 1000);
+    const cd a = A[(hi - 1) * n + hi - 1], b = A[(hi - 1) * n + hi], c = A[hi * n + hi - 1], d = A[hi * n + hi];
+    const cd m = 0.5 * (a + d), disc = std::sqrt(0.25 * (a - d) * (a - d) + b * c);
+    cd mu = (std::abs(m + disc - d) < std::abs(m - disc - d)) ? m + disc : m - disc;
+    if (its % 11 == 10) mu = d + std::abs(c);
+    for (int i = l; i <= hi; ++i) A[i * n + i] -= mu;
+    for (int i = l; i < hi; ++i) {
+      const cd x = A[i * n + i], y = A[(i + 1) * n + i];
+      const double r = std::sqrt(std::norm(x) + std::norm(y));
+      gc[i] = (r == 0) ? cd(1) : x / r;
+      gs[i] = (r == 0) ? cd(0) : y / r;
+      for (int j = i; j <= hi; ++j) {
+        const cd u = A[i * n + j], v = A[(i + 1) * n + j];
+        A[i * n + j] = std::conj(gc[i]) * u + std::conj(gs[i]) * v;
+        A[(i + 1) * n + j] = -gs[i] * u + gc[i] * v;
+      }
+    }
+    for (int i = l; i < hi; ++i)
+      for (int r = l; r <= std::min(i + 2, hi); ++r) {
+        const cd u = A[r * n + i], v = A[r * n + i + 1];
+        A[r * n + i] = u * gc[i] + v * gs[i];
+        A[r * n + i + 1] = -u * std::conj(gs[i]) + v * std::conj(gc[i]);
+      }
+    for (int i = l; i <= hi; ++i) A[i * n + i] += mu;
+  }
+  return ev;
+}
+
+namespace {
+
+template <typename T>
+struct prec;
+template <>
+struct prec<double> {
+  static int axpy(int N, double a, const double* x, double b, double* y, void* s) { return libp_axpy_f64(N, a, x, b, y, s); }
+  static int amxpy(int N, double a, const double* A, const double* x, double b, double* y, void* s) {
+    return libp_amxpy_f64(N, a, A, x, b, y, s);
+  }
+  static int coarsen(int Nf, int Nc, int E, const int* g2l, const double* P, const double* in, double* out, void* s) {
+    return libp_mg_coarsen_hex_f64(Nf, Nc, E, nullptr, g2l, P, in, out, s);
+  }
+  static int prolong(int Nf, int Nc, int E, const int* g2l, const double* P, const double* in, double* out, void* s) {
+    return libp_mg_prolong_hex_f64(Nf, Nc, E, nullptr, g2l, P, in, out, s);
+  }
+  static int chebStart(int N, const double* iD, const double* r, const double* Ax, double it, double* R, double* d,
+                       double* X, int z, void* s) {
+    return libp_cheb_start_f64(N, iD, r, Ax, it, R, d, X, z, s);
+  }
+  static int chebStep(int N, const double* iD, const double* Ad, double c1, double c2, double* R, double* d, double* X,
+                      void* s) {
+    return libp_cheb_step_f64(N, iD, Ad, c1, c2, R, d, X, s);
+  }
+  static void op(elliptic_t& e, deviceMemory<double>& x, deviceMemory<double>& y) { e.Operator(x, y); }
+  static deviceMemory<double> fromDouble(platform_t& p, deviceMemory<double>& v, dlong) { return v; }
+};
+template <>
+struct prec<float> {
+  static int axpy(int N, float a, const float* x, float b, float* y, void* s) { return libp_axpy_f32(N, a, x, b, y, s); }
+  static int amxpy(int N, float a, const float* A, const float* x, float b, float* y, void* s) {
+    return libp_amxpy_f32(N, a, A, x, b, y, s);
+  }
+  static int coarsen(int Nf, int Nc, int E, const int* g2l, const float* P, const float* in, float* out, void* s) {
+    return libp_mg_coarsen_hex_f32(Nf, Nc, E, nullptr, g2l, P, in, out, s);
+  }
+  static int prolong(int Nf, int Nc, int E, const int* g2l, const float* P, const float* in, float* out, void* s) {
+    return libp_mg_prolong_hex_f32(Nf, Nc, E, nullptr, g2l, P, in, out, s);
+  }
+  static int chebStart(int N, const float* iD, const float* r, const float* Ax, float it, float* R, float* d,
+                       float* X, int z, void* s) {
+    return libp_cheb_start_f32(N, iD, r, Ax, it, R, d, X, z, s);
+  }
+  static int chebStep(int N, const float* iD, const float* Ad, float c1, float c2, float* R, float* d, float* X,
+                      void* s) {
+    return libp_cheb_step_f32(N, iD, Ad, c1, c2, R, d, X, s);
+  }
+  static void op(elliptic_t& e, deviceMemory<float>& x, deviceMemory<float>& y) { e.OperatorF32(x, y); }
+  static deviceMemory<float> fromDouble(platform_t& p, deviceMemory<double>& v, dlong n) {
+    deviceMemory<float> f = p.malloc<float>(n);
+    LIBP_CALL(libp_convert_f64_to_f32(n, v.ptr(), f.ptr(), p.s()));
+    return f;
+  }
+};
+
+// Scratch shared by all levels (the reference keeps it static in MGLevel,
+// ellipticPreconMultiGridLevel.cpp:208-221).
+template <typename T>
+struct mgScratch {
+  deviceMemory<T> RES, Ad, d, transfer;
+  void ensure(platform_t& p, dlong Ncols, dlong NL) {
+    if ((dlong)RES.length() < Ncols) {
+      RES = p.malloc<T>(Ncols);
+      Ad = p.malloc<T>(Ncols);
+      d = p.malloc<T>(Ncols);
+    }
+    if ((dlong)transfer.length() < NL) transfer = p.malloc<T>(NL);
+  }
+};
+
+}  // namespace
+
+namespace parAlmond {
+
+/* include/parAlmond.hpp:70-91, templated on the preconditioner precision. */
+template <typename T>
+class multigridLevel {
+ public:
+  dlong Nrows = 0, Ncols = 0;
+  virtual void Operator(deviceMemory<T>& o_X, deviceMemory<T>& o_Ax) = 0;
+  virtual void smooth(deviceMemory<T>& o_RHS, deviceMemory<T>& o_X, bool x_is_zero) = 0;
+  virtual void residual(deviceMemory<T>& o_RHS, deviceMemory<T>& o_X, deviceMemory<T>& o_RES) = 0;
+  virtual void coarsen(deviceMemory<T>& o_X, deviceMemory<T>& o_Cx) = 0;
+  virtual void prolongate(deviceMemory<T>& o_X, deviceMemory<T>& o_Px) = 0;
+  virtual void Report() = 0;
+  virtual ~multigridLevel() {}
+};
+
+}  // namespace parAlmond
+
+template <typename T>
+class MGLevel : public parAlmond::multigridLevel<T> {
+ public:
+  std::shared_ptr<elliptic_t> elliptic, ellipticC;
+  platform_t platform;
+  int Nf = 0, Nc = 0;
+  enum { JACOBI = 0, CHEBYSHEV = 1, LEARNED = 2 } stype = CHEBYSHEV;
+  int ChebyshevIterations = 2;
+  double lambda0 = 0, lambda1 = 0;
+  deviceMemory<double> o_invDiagA64;
+  deviceMemory<T> o_invDiagA, o_P, o_weightG;
+  std::shared_ptr<mgScratch<T>> scratch;
+  std::shared_ptr<learnedSmoother_t> learned;  // fine-level learned smoother (FP32 only)
+
+  MGLevel(std::shared_ptr<elliptic_t> el, int _Nc, unsigned short* drandState, std::shared_ptr<mgScratch<T>> sc)
+      : elliptic(el), platform(el->platform), Nf(el->mesh.N), Nc(_Nc), scratch(sc) {
+    this->Nrows = el->Ndofs;
+    this->Ncols = el->Ndofs + el->Nhalo;
+    SetupSmoother(drandState);
+    std::vector<double> P;
+    degreeRaiseMatrix(Nc, Nf, P);
+    deviceMemory<double> o_P64 = platform.malloc(P);
+    o_P = prec<T>::fromDouble(platform, o_P64, (dlong)P.size());
+    o_weightG = prec<T>::fromDouble(platform, el->o_weightG, el->Ndofs);
+    scratch->ensure(platform, this->Ncols, el->mesh.Nelements * el->mesh.Np);
+  }
+
+  // MGLevel::SetupSmoother (ellipticPreconMultiGridLevel.cpp:345-384)
+  void SetupSmoother(unsigned short* drandState) {
+    const dlong N = elliptic->Ndofs;
+    deviceMemory<double> o_diag = platform.malloc<double>(N);
+    elliptic->BuildOperatorDiagonal(o_diag);
+    std::vector<double> inv(N);
+    o_diag.copyTo(inv.data(), N, 0, platform.stream);
+    platform.finish();
+    for (auto& v : inv) v = 1.0 / v;
+    o_invDiagA64 = platform.malloc(inv);
+    const settings_t& s = elliptic->settings;
+    if (s.compareSetting("MULTIGRID SMOOTHER", "CHEBYSHEV") || s.compareSetting("MULTIGRID SMOOTHER", "LEARNED")) {
+      stype = CHEBYSHEV;
+      s.getSetting("MULTIGRID CHEBYSHEV DEGREE", ChebyshevIterations);
+      const double rho = maxEigSmoothAx(drandState);
+      lambda1 = rho;
+      lambda0 = rho / 10.;
+    } else {
+      stype = JACOBI;
+      const double rho = maxEigSmoothAx(drandState);
+      lambda0 = (4. / 3.) / rho;
+      for (auto& v : inv) v *= lambda0;
+      o_invDiagA64.copyFrom(inv.data(), N, 0, platform.stream);
+    }
+    o_invDiagA = prec<T>::fromDouble(platform, o_invDiagA64, N);
+    platform.finish();
+  }
+
+  // MGLevel::maxEigSmoothAx (ellipticPreconMultiGridLevel.cpp:393-473), in FP64.
+  double maxEigSmoothAx(unsigned short* xs) {
+    const dlong N = elliptic->Ndofs;
+    int k = 10;
+    if (k > N) k = N;
+    std::vector<double> H(k * k, 0.0), Vx(N);
+    std::vector<deviceMemory<double>> V(k + 1);
+    for (int i = 0; i <= k; ++i) V[i] = platform.malloc<double>(N);
+    deviceMemory<double> o_AVx = platform.malloc<double>(N);
+    for (dlong i = 0; i < N; ++i) Vx[i] = erand48(xs);  // the drand48() stream seeded by srand48(0)
+    deviceMemory<double> o_Vx = platform.malloc(Vx);
+    void* st = platform.s();
+    void* ws = platform.o_reduceWS.ptr();
+    double* tmp = platform.o_scalars.ptr() + 12;
+    LIBP_CALL(libp_norm2sq_f64(N, o_Vx.ptr(), tmp, ws, st));
+    const double norm_vo = std::sqrt(platform.readScalars(tmp, 1)[0]);
+    LIBP_CALL(libp_axpy_f64(N, 1. / norm_vo, o_Vx.ptr(), 0.0, V[0].ptr(), st));
+    for (int j = 0; j < k; ++j) {
+      elliptic->Operator(V[j], o_AVx);
+      LIBP_CALL(libp_amxpy_f64(N, 1.0, o_invDiagA64.ptr(), o_AVx.ptr(), 0.0, V[j + 1].ptr(), st));
+      for (int i = 0; i <= j; ++i) {
+        LIBP_CALL(libp_inner_prod_f64(N, V[i].ptr(), V[j + 1].ptr(), tmp, ws, st));
+        const double hij = platform.readScalars(tmp, 1)[0];
+        LIBP_CALL(libp_axpy_f64(N, -hij, V[i].ptr(), 1.0, V[j + 1].ptr(), st));
+        H[i + j * k] = hij;
+      }
+      if (j + 1 < k) {
+        LIBP_CALL(libp_norm2sq_f64(N, V[j + 1].ptr(), tmp, ws, st));
+        const double norm_vj = std::sqrt(platform.readScalars(tmp, 1)[0]);
+        LIBP_CALL(libp_scale_f64(N, 1.0 / norm_vj, V[j + 1].ptr(), st));
+        H[j + 1 + j * k] = norm_vj;
+      }
+    }
+    double rho = 0;
+    for (auto& l : hessenbergEigenvalues(k, H)) rho = std::max(rho, std::abs(l));
+    return rho;
+  }
+
+  void Operator(deviceMemory<T>& o_X, deviceMemory<T>& o_Ax) override { prec<T>::op(*elliptic, o_X, o_Ax); }
+
+  void residual(deviceMemory<T>& o_RHS, deviceMemory<T>& o_X, deviceMemory<T>& o_RES) override {
+    Operator(o_X, o_RES);
+    LIBP_CALL(prec<T>::axpy(elliptic->Ndofs, T(1), o_RHS.ptr(), T(-1), o_RES.ptr(), platform.s()));
+  }
+
+  // MGLevel::coarsen (ellipticPreconMultiGridLevel.cpp:41-82)
+  void coarsen(deviceMemory<T>& o_X, deviceMemory<T>& o_Rx) override {
+    deviceMemory<T>& o_wx = scratch->RES;
+    deviceMemory<T>& o_RxL = scratch->transfer;
+    LIBP_CALL(prec<T>::amxpy(elliptic->Ndofs, T(1), o_weightG.ptr(), o_X.ptr(), T(0), o_wx.ptr(), platform.s()));
+    LIBP_CALL(prec<T>::coarsen(Nf, Nc, elliptic->mesh.Nelements, elliptic->o_GlobalToLocal.ptr(), o_P.ptr(),
+                               o_wx.ptr(), o_RxL.ptr(), platform.s()));
+    ellipticC->ogsMasked.Gather(o_Rx, o_RxL, 1, ogs::Add, ogs::Trans);
+  }
+
+  // MGLevel::prolongate (ellipticPreconMultiGridLevel.cpp:84-125)
+  void prolongate(deviceMemory<T>& o_X, deviceMemory<T>& o_Px) override {
+    deviceMemory<T>& o_PxG = scratch->RES;
+    deviceMemory<T>& o_PxL = scratch->transfer;
+    LIBP_CALL(prec<T>::prolong(Nf, Nc, ellipticC->mesh.Nelements, ellipticC->o_GlobalToLocal.ptr(), o_P.ptr(),
+                               o_X.ptr(), o_PxL.ptr(), platform.s()));
+    elliptic->ogsMasked.Gather(o_PxG, o_PxL, 1, ogs::Add, ogs::NoTrans);
+    LIBP_CALL(prec<T>::axpy(elliptic->Ndofs, T(1), o_PxG.ptr(), T(1), o_Px.ptr(), platform.s()));
+  }
+
+  void smooth(deviceMemory<T>& o_RHS, deviceMemory<T>& o_X, bool x_is_zero) override {
+    if (learned && learned->active) {
+      smoothLearned(o_RHS, o_X, x_is_zero);
+    } else if (stype == JACOBI) {
+      smoothJacobi(o_RHS, o_X, x_is_zero);
+    } else {
+      smoothChebyshev(o_RHS, o_X, x_is_zero);
+    }
+  }
+
+  void smoothJacobi(deviceMemory<T>& o_r, deviceMemory<T>& o_X, bool xIsZero) {
+    const dlong N = elliptic->Ndofs;
+    if (xIsZero) {
+      LIBP_CALL(prec<T>::amxpy(N, T(1), o_invDiagA.ptr(), o_r.ptr(), T(0), o_X.ptr(), platform.s()));
+      return;
+    }
+    deviceMemory<T>& o_RES = scratch->RES;
+    Operator(o_X, o_RES);
+    LIBP_CALL(prec<T>::axpy(N, T(1), o_r.ptr(), T(-1), o_RES.ptr(), platform.s()));
+    LIBP_CALL(prec<T>::amxpy(N, T(1), o_invDiagA.ptr(), o_RES.ptr(), T(1), o_X.ptr(), platform.s()));
+  }
+
+  // MGLevel::smoothChebyshev (ellipticPreconMultiGridLevel.cpp:154-206) with
+  // the vector updates fused: start = {RES, d, first x update}, each step =
+  // {RES -= invD Ad, d = c1 RES + c2 d, x += d}.
+  void smoothChebyshev(deviceMemory<T>& o_r, deviceMemory<T>& o_X, bool xIsZero) {
+    const dlong N = elliptic->Ndofs;
+    const double theta = 0.5 * (lambda1 + lambda0);
+    const double delta = 0.5 * (lambda1 - lambda0);
+    const double invTheta = 1.0 / theta;
+    const double sigma = theta / delta;
+    double rho_n = 1. / sigma, rho_np1;
+    deviceMemory<T>& o_RES = scratch->RES;
+    deviceMemory<T>& o_Ad = scratch->Ad;
+    deviceMemory<T>& o_d = scratch->d;
+    if (xIsZero) {
+      LIBP_CALL(prec<T>::chebStart(N, o_invDiagA.ptr(), o_r.ptr(), nullptr, (T)invTheta, o_RES.ptr(), o_d.ptr(),
+                                   o_X.ptr(), 1, platform.s()));
+    } else {
+      Operator(o_X, o_Ad);
+      LIBP_CALL(prec<T>::chebStart(N, o_invDiagA.ptr(), o_r.ptr(), o_Ad.ptr(), (T)invTheta, o_RES.ptr(), o_d.ptr(),
+                                   o_X.ptr(), 0, platform.s()));
+    }
+    for (int k = 0; k < ChebyshevIterations; k++) {
+      Operator(o_d, o_Ad);
+      rho_np1 = 1.0 / (2. * sigma - rho_n);
+      const double rhoDivDelta = 2.0 * rho_np1 / delta;
+      LIBP_CALL(prec<T>::chebStep(N, o_invDiagA.ptr(), o_Ad.ptr(), (T)rhoDivDelta, (T)(rho_np1 * rho_n),
+                                  o_RES.ptr(), o_d.ptr(), o_X.ptr(), platform.s()));
+      rho_n = rho_np1;
+    }
+  }
+
+  void smoothLearned(deviceMemory<T>& o_r, deviceMemory<T>& o_X, bool xIsZero);
+
+  void Report() override {
+    printf("|    pMG     |    %10lld  |   Matrix-free   |   %s|  Degree %2d\n", (long long)this->Nrows,
+           stype == CHEBYSHEV ? "Chebyshev       " : "Damped Jacobi   ", Nf);
+  }
+};
+
+template <>
+void MGLevel<float>::smoothLearned(deviceMemory<float>& o_r, deviceMemory<float>& o_X, bool xIsZero) {
+  learned->smooth(*elliptic, o_r, o_X, xIsZero, o_invDiagA, scratch->RES, scratch->Ad);
+}
+template <>
+void MGLevel<double>::smoothLearned(deviceMemory<double>& o_r, deviceMemory<double>& o_X, bool xIsZero) {
+  smoothChebyshev(o_r, o_X, xIsZero);
+}
+
+/* Base level: FP64 Jacobi-PCG on the N=1 operator (the composition the p-MG
+ * configuration asks for; the reference ends in parAlmond AMG instead,
+ * ellipticPreconMultiGrid.cpp:133-153). Tolerance is relative to the
+ * incoming residual; a zero tolerance runs MAXIT iterations. */
+class coarseJacobiPCG_t {
+ public:
+  std::shared_ptr<elliptic_t> elliptic;
+  std::shared_ptr<LinearSolver::pcg> solver;
+  std::shared_ptr<JacobiPrecon> jacobi;
+  deviceMemory<double> o_x, o_r;
+  double tol = 1e-2;
+  int maxit = 200;
+  int lastIterations = 0;
+  explicit coarseJacobiPCG_t(std::shared_ptr<elliptic_t> el) : elliptic(el) {
+    elliptic->settings.getSetting("MULTIGRID COARSE TOLERANCE", tol);
+    elliptic->settings.getSetting("MULTIGRID COARSE MAXIT", maxit);
+    jacobi = std::make_shared<JacobiPrecon>(*elliptic);
+    settings_t s = elliptic->settings;
+    s.changeSetting("LINEAR SOLVER", "PCG");
+    s.changeSetting("LINEAR SOLVER STOPPING CRITERION", "RELATIVE");
+    solver = std::make_shared<LinearSolver::pcg>(elliptic->Ndofs, 0, elliptic->platform, s, elliptic->comm);
+    solver->pureRelative = true;
+    o_x = elliptic->platform.malloc<double>(elliptic->Ndofs);
+    o_r = elliptic->platform.malloc<double>(elliptic->Ndofs);
+  }
+  template <typename T>
+  void solve(deviceMemory<T>& o_rhs, deviceMemory<T>& o_X);
+};
+
+template <>
+void coarseJacobiPCG_t::solve<double>(deviceMemory<double>& o_rhs, deviceMemory<double>& o_X) {
+  platform_t& p = elliptic->platform;
+  const dlong N = elliptic->Ndofs;
+  LIBP_HIP(hipMemcpyAsync(o_r.ptr(), o_rhs.ptr(), N * sizeof(double), hipMemcpyDeviceToDevice, p.stream));
+  LIBP_CALL(libp_set_f64(N, 0.0, o_X.ptr(), p.s()));
+  lastIterations = solver->Solve(*elliptic, *jacobi, o_X, o_r, tol, maxit, 0);
+}
+template <>
+void coarseJacobiPCG_t::solve<float>(deviceMemory<float>& o_rhs, deviceMemory<float>& o_X) {
+  platform_t& p = elliptic->platform;
+  const dlong N = elliptic->Ndofs;
+  LIBP_CALL(libp_convert_f32_to_f64(N, o_rhs.ptr(), o_r.ptr(), p.s()));
+  LIBP_CALL(libp_set_f64(N, 0.0, o_x.ptr(), p.s()));
+  lastIterations = solver->Solve(*elliptic, *jacobi, o_x, o_r, tol, maxit, 0);
+  LIBP_CALL(libp_convert_f64_to_f32(N, o_x.ptr(), o_X.ptr(), p.s()));
+}
+
+class MultiGridPreconBase {
+ public:
+  virtual void Operator(deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_Mr) = 0;
+  virtual ~MultiGridPreconBase() {}
+};
+
+template <typename T>
+class multigrid_t : public MultiGridPreconBase {
+ public:
+  elliptic_t& fine;
+  std::vector<std::shared_ptr<elliptic_t>> ellipticLevels;
+  std::vector<std::shared_ptr<MGLevel<T>>> levels;
+  std::shared_ptr<coarseJacobiPCG_t> coarse;
+  std::vector<deviceMemory<T>> o_rhs, o_x;
+  deviceMemory<T> o_res, o_r0, o_z0;
+  std::shared_ptr<mgScratch<T>> scratch = std::make_shared<mgScratch<T>>();
+
+  explicit multigrid_t(elliptic_t& _fine) : fine(_fine) {
+    std::vector<int> deg = multigridDegrees(fine.settings, fine.mesh.N);
+    LIBP_ABORT("p-multigrid needs at least two degrees", deg.size() < 2);
+    unsigned short xs[3] = {0x330E, 0, 0};  // srand48(0): parAlmondKernels.cpp:55-58 on rank 0
+    auto el0 = std::make_shared<elliptic_t>(fine);  // SetupNewDegree(N) returns the fine operator
+    ellipticLevels.push_back(el0);
+    for (size_t l = 0; l + 1 < deg.size(); ++l) {
+      auto lev = std::make_shared<MGLevel<T>>(ellipticLevels[l], deg[l + 1], xs, scratch);
+      levels.push_back(lev);
+      mesh_t meshC = fine.mesh.SetupNewDegree(deg[l + 1]);
+      auto elC = std::make_shared<elliptic_t>(fine.SetupNewDegree(meshC));
+      ellipticLevels.push_back(elC);
+      lev->ellipticC = elC;
+    }
+    coarse = std::make_shared<coarseJacobiPCG_t>(ellipticLevels.back());
+    o_rhs.resize(ellipticLevels.size());
+    o_x.resize(ellipticLevels.size());
+    dlong maxN = 0;
+    for (size_t l = 0; l < ellipticLevels.size(); ++l) {
+      const dlong N = ellipticLevels[l]->Ndofs;
+      maxN = std::max(maxN, N);
+      if (l > 0) {
+        o_rhs[l] = fine.platform.malloc<T>(N);
+        o_x[l] = fine.platform.malloc<T>(N);
+      }
+    }
+    o_res = fine.platform.malloc<T>(maxN);
+    if (std::is_same<T, float>::value) {
+      o_r0 = fine.platform.malloc<T>(fine.Ndofs);
+      o_z0 = fine.platform.malloc<T>(fine.Ndofs);
+    }
+    if (fine.settings.compareSetting("MULTIGRID SMOOTHER", "LEARNED")) attachLearned();
+    if (fine.settings.compareSetting("VERBOSE", "TRUE")) {
+      printf("p-multigrid levels (%s):\n", std::is_same<T, float>::value ? "FP32" : "FP64");
+      for (auto& l : levels) l->Report();
+      printf("|   coarse   |    %10lld  |  Jacobi-PCG FP64 |  Degree %2d\n", (long long)ellipticLevels.back()->Ndofs,
+             ellipticLevels.back()->mesh.N);
+    }
+  }
+
+  void attachLearned();
+
+  // parAlmondVcycle.cpp:34-60
+  void vcycle(size_t k, deviceMemory<T>& o_RHS, deviceMemory<T>& o_X) {
+    if (k + 1 == ellipticLevels.size()) {
+      coarse->solve<T>(o_RHS, o_X);
+      return;
+    }
+    MGLevel<T>& level = *levels[k];
+    level.smooth(o_RHS, o_X, true);
+    level.residual(o_RHS, o_X, o_res);
+    level.coarsen(o_res, o_rhs[k + 1]);
+    vcycle(k + 1, o_rhs[k + 1], o_x[k + 1]);
+    level.prolongate(o_x[k + 1], o_X);
+    level.smooth(o_RHS, o_X, false);
+  }
+
+  void Operator(deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_Mr) override;
+};
+
+template <>
+void multigrid_t<double>::Operator(deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_Mr) {
+  vcycle(0, o_r, o_Mr);
+}
+template <>
+void multigrid_t<float>::Operator(deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_Mr) {
+  const dlong N = fine.Ndofs;
+  LIBP_CALL(libp_convert_f64_to_f32(N, o_r.ptr(), o_r0.ptr(), fine.platform.s()));
+  vcycle(0, o_r0, o_z0);
+  LIBP_CALL(libp_convert_f32_to_f64(N, o_z0.ptr(), o_Mr.ptr(), fine.platform.s()));
+}
+
+template <>
+void multigrid_t<float>::attachLearned() {
+  std::string path = fine.settings.getSetting("MULTIGRID LEARNED WEIGHTS");
+  auto ls = std::make_shared<learnedSmoother_t>(*ellipticLevels[0]);
+  if (!path.empty()) ls->load(path);
+  levels[0]->learned = ls;
+}
+template <>
+void multigrid_t<double>::attachLearned() {
+  LIBP_FORCE_ABORT("MULTIGRID SMOOTHER LEARNED needs MULTIGRID PRECISION FP32");
+}
+
+MultiGridPrecon::MultiGridPrecon(elliptic_t& _elliptic) : elliptic(_elliptic) {
+  if (elliptic.settings.compareSetting("MULTIGRID PRECISION", "FP64"))
+    impl = std::make_shared<multigrid_t<double>>(elliptic);
+  else
+    impl = std::make_shared<multigrid_t<float>>(elliptic);
+}
+
+// MultiGridPrecon::Operator (ellipticPreconMultiGrid.cpp:31-38)
+void MultiGridPrecon::Operator(deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_Mr) {
+  impl->Operator(o_r, o_Mr);
+  if (elliptic.allNeumann) elliptic.ZeroMean(o_Mr);
+}
+
+learnedSmoother_t* multigridLearned(MultiGridPrecon& mg) {
+  auto* m = dynamic_cast<multigrid_t<float>*>(mg.impl.get());
+  if (!m || m->levels.empty()) return nullptr;
+  return m->levels[0]->learned.get();
+}
+
+int multigridCoarseIterations(MultiGridPrecon& mg) {
+  if (auto* m = dynamic_cast<multigrid_t<float>*>(mg.impl.get())) return m->coarse->lastIterations;
+  if (auto* m = dynamic_cast<multigrid_t<double>*>(mg.impl.get())) return m->coarse->lastIterations;
+  return -1;
+}
+
+}  // namespace libp

@@ -1,0 +1,83 @@
+// (F)PCG on the device (libs/linearSolver/linearSolverPCG.cpp:35-171).
+#include <cmath>
+#include <cstdio>
+
+#include "solver.hpp"
+
+namespace libp {
+namespace LinearSolver {
+
+pcg::pcg(dlong _N, dlong _Nhalo, platform_t& _platform, settings_t& _settings, comm_t _comm)
+    : linearSolverBase_t(_N, _Nhalo, _platform, _settings, _comm) {
+  flexible = settings.compareSetting("LINEAR SOLVER", "FPCG");
+  const dlong Ntotal = N + Nhalo;
+  o_p = platform.malloc<dfloat>(Ntotal);
+  o_z = platform.malloc<dfloat>(Ntotal);
+  o_Ax = platform.malloc<dfloat>(Ntotal);
+  o_Ap = platform.malloc<dfloat>(Ntotal);
+  o_s = platform.malloc<dfloat>(LIBP_PCG_NSCALARS);
+  // zero work vectors (linearSolverPCG.cpp:45-50 fills them with 0)
+  LIBP_CALL(libp_set_f64(Ntotal, 0.0, o_p.ptr(), platform.s()));
+  LIBP_CALL(libp_set_f64(Ntotal, 0.0, o_z.ptr(), platform.s()));
+  LIBP_CALL(libp_set_f64(Ntotal, 0.0, o_Ax.ptr(), platform.s()));
+  LIBP_CALL(libp_set_f64(Ntotal, 0.0, o_Ap.ptr(), platform.s()));
+}
+
+int pcg::Solve(operator_t& linearOperator, operator_t& precon, deviceMemory<dfloat>& o_x, deviceMemory<dfloat>& o_r,
+               const dfloat tol, const int MAXIT, const int verbose) {
+  void* st = platform.s();
+  void* ws = platform.o_reduceWS.ptr();
+  double* s = o_s.ptr();
+  double* tmp = platform.o_scalars.ptr();
+  stats = solveStats_t();
+  LIBP_CALL(libp_set_f64(LIBP_PCG_NSCALARS, 0.0, s, st));
+
+  dfloat TOL = 0.0;
+  const bool rhsNorm = settings.compareSetting("LINEAR SOLVER STOPPING CRITERION", "ABS/REL-RHS-2NORM");
+  if (rhsNorm) {
+    LIBP_CALL(libp_norm2sq_f64(N, o_r.ptr(), tmp, ws, st));
+    const double nb2 = platform.readScalars(tmp, 1)[0];
+    TOL = std::max(tol * tol * nb2, tol * tol);
+  }
+
+  linearOperator.Operator(o_x, o_Ax);
+  LIBP_CALL(libp_axpy_f64(N, -1.0, o_Ax.ptr(), 1.0, o_r.ptr(), st));
+  LIBP_CALL(libp_norm2sq_f64(N, o_r.ptr(), tmp, ws, st));
+  double rdotr0 = platform.readScalars(tmp, 1)[0];
+  if (!rhsNorm) TOL = pureRelative ? tol * tol * rdotr0 : std::max(tol * tol * rdotr0, tol * tol);
+  stats.rdotr0 = rdotr0;
+  if (recordHistory) stats.resHistory.push_back(std::sqrt(rdotr0));
+  if (verbose) printf("PCG: initial res norm %12.12f \n", std::sqrt(rdotr0));
+
+  diagonalPrecon_t* jac = dynamic_cast<diagonalPrecon_t*>(&precon);
+  if (jac && jac->needsZeroMean()) jac = nullptr;
+
+  int iter;
+  for (iter = 0; iter < MAXIT; ++iter) {
+    if (((iter == 0) && (rdotr0 == 0.0)) || ((iter > 0) && (rdotr0 <= TOL))) break;
+    if (jac) {
+      LIBP_CALL(libp_pcg_jacobi_rdotz_f64(N, jac->invDiagonal().ptr(), o_r.ptr(), o_z.ptr(), o_Ap.ptr(), flexible, s,
+                                          ws, st));
+    } else {
+      precon.Operator(o_r, o_z);
+      LIBP_CALL(libp_pcg_rdotz_f64(N, o_r.ptr(), o_z.ptr(), o_Ap.ptr(), flexible, s, ws, st));
+    }
+    LIBP_CALL(libp_pcg_update_p_f64(N, iter == 0, o_z.ptr(), o_p.ptr(), s, st));
+    linearOperator.Operator(o_p, o_Ap);
+    LIBP_CALL(libp_pcg_pap_f64(N, o_p.ptr(), o_Ap.ptr(), s, ws, st));
+    LIBP_CALL(libp_pcg_update_f64(N, o_p.ptr(), o_Ap.ptr(), o_x.ptr(), o_r.ptr(), s, ws, st));
+    const double* h = platform.readScalars(s, LIBP_PCG_NSCALARS);
+    rdotr0 = h[LIBP_PCG_RDOTR];
+    if (recordHistory) stats.resHistory.push_back(std::sqrt(std::max(rdotr0, 0.0)));
+    if (verbose) {
+      if (rdotr0 < 0) printf("WARNING CG: rdotr = %17.15lf\n", rdotr0);
+      printf("CG: it %d, r norm %12.12le, alpha = %le \n", iter + 1, std::sqrt(rdotr0), h[LIBP_PCG_ALPHA]);
+    }
+  }
+  stats.iterations = iter;
+  stats.rdotrFinal = rdotr0;
+  return iter;
+}
+
+}  // namespace LinearSolver
+}  // namespace libp

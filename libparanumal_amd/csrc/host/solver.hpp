@@ -1,0 +1,106 @@
+// operator_t / precon_t / linearSolver_t and the (F)PCG solver.
+// Interfaces follow include/operator.hpp:35-40, include/precon.hpp:36-69 and
+// include/linearSolver.hpp:41-96 so elliptic_t, the preconditioners and
+// callers written against them keep their shape.
+#pragma once
+
+#include "core.hpp"
+
+namespace libp {
+
+class operator_t {
+ public:
+  virtual void Operator(deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_Mr) = 0;
+  virtual ~operator_t() {}
+};
+
+// A preconditioner that is a diagonal scaling (Jacobi) exposes its inverse
+// diagonal so the solver can fuse z = invDiag.*r into the r.z reduction.
+class diagonalPrecon_t : public operator_t {
+ public:
+  virtual const deviceMemory<dfloat>& invDiagonal() const = 0;
+  virtual bool needsZeroMean() const { return false; }
+};
+
+class precon_t : public operator_t {
+  std::shared_ptr<operator_t> precon;
+
+ public:
+  template <class Precon, class... Args>
+  void Setup(Args&&... args) {
+    precon = std::make_shared<Precon>(std::forward<Args>(args)...);
+  }
+  void Operator(deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_Mr) override { precon->Operator(o_r, o_Mr); }
+  operator_t* get() const { return precon.get(); }
+  bool isInitialized() const { return (bool)precon; }
+};
+
+class IdentityPrecon : public operator_t {
+  dlong N;
+  platform_t platform;
+
+ public:
+  IdentityPrecon(dlong _N, platform_t& _platform) : N(_N), platform(_platform) {}
+  void Operator(deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_Mr) override {
+    LIBP_HIP(hipMemcpyAsync(o_Mr.ptr(), o_r.ptr(), sizeof(dfloat) * N, hipMemcpyDeviceToDevice, platform.stream));
+  }
+};
+
+struct solveStats_t {
+  int iterations = 0;
+  double rdotr0 = 0, rdotrFinal = 0;
+  std::vector<double> resHistory;  // |r| after each iteration (index 0: initial)
+};
+
+class linearSolverBase_t {
+ public:
+  platform_t platform;
+  settings_t settings;
+  comm_t comm;
+  dlong N = 0, Nhalo = 0;
+  solveStats_t stats;
+  bool pureRelative = false;  // stop at |r| <= tol |r0| (no absolute floor)
+  bool recordHistory = false;
+
+  linearSolverBase_t(dlong _N, dlong _Nhalo, platform_t& _platform, settings_t& _settings, comm_t _comm)
+      : platform(_platform), settings(_settings), comm(_comm), N(_N), Nhalo(_Nhalo) {}
+  virtual int Solve(operator_t& linearOperator, operator_t& precon, deviceMemory<dfloat>& o_x,
+                    deviceMemory<dfloat>& o_rhs, const dfloat tol, const int MAXIT, const int verbose) = 0;
+  virtual ~linearSolverBase_t() {}
+};
+
+class linearSolver_t {
+  std::shared_ptr<linearSolverBase_t> ls;
+
+ public:
+  template <class Solver>
+  void Setup(dlong N, dlong Nhalo, platform_t& platform, settings_t& settings, comm_t comm) {
+    ls = std::make_shared<Solver>(N, Nhalo, platform, settings, comm);
+  }
+  int Solve(operator_t& A, operator_t& M, deviceMemory<dfloat>& o_x, deviceMemory<dfloat>& o_rhs, const dfloat tol,
+            const int MAXIT, const int verbose) {
+    return ls->Solve(A, M, o_x, o_rhs, tol, MAXIT, verbose);
+  }
+  linearSolverBase_t* get() const { return ls.get(); }
+};
+
+namespace LinearSolver {
+
+/* (F)PCG with device-resident scalars (libs/linearSolver/linearSolverPCG.cpp).
+ * alpha and beta are formed by the reduction kernels on the device; the host
+ * reads r.r once per iteration for the stopping test. Flexible when the
+ * LINEAR SOLVER setting contains "FPCG" (linearSolverPCG.cpp:43). */
+class pcg : public linearSolverBase_t {
+  bool flexible = false;
+  deviceMemory<dfloat> o_p, o_z, o_Ax, o_Ap;
+  deviceMemory<dfloat> o_s;  // LIBP_PCG_NSCALARS device scalars
+
+ public:
+  pcg(dlong N, dlong Nhalo, platform_t& platform, settings_t& settings, comm_t comm);
+  int Solve(operator_t& linearOperator, operator_t& precon, deviceMemory<dfloat>& o_x, deviceMemory<dfloat>& o_r,
+            const dfloat tol, const int MAXIT, const int verbose) override;
+  void setFlexible(bool f) { flexible = f; }
+};
+
+}  // namespace LinearSolver
+}  // namespace libp

@@ -1,0 +1,175 @@
+// Matrix-free C0 hex stiffness + mass operator for gfx950.
+//
+// Thread layout: one Nq x Nq thread tile per element, each thread owning the
+// k-pencil q(i,j,0:N) in registers (the structure of
+// solvers/elliptic/okl/ellipticAxHex3D.okl:156-295). Differences that matter on
+// CDNA4:
+//  * several small elements share a 64-lane wavefront (NE elements per block),
+//    so N<=4 levels do not idle most of the lanes;
+//  * an explicit waves-per-EU target bounds the register budget, so the
+//    compiler keeps enough wavefronts resident to cover HBM latency;
+//  * the E-vector result is written with non-temporal stores (it is consumed
+//    by the gather kernel, not re-read by this one).
+#include "kcommon.hpp"
+
+namespace libp_amd {
+
+template <int Nq>
+struct AxShape {
+  static constexpr int NqNq = Nq * Nq;
+  static constexpr int Np = Nq * Nq * Nq;
+  static constexpr int NE = (NqNq >= 64) ? 1 : (64 / NqNq);
+  static constexpr int threads = NE * NqNq;
+  // occupancy target: bounds the register budget so that the compiler
+  // re-reads loop-invariant LDS operands instead of pinning them in VGPRs
+  static constexpr int waves = (Nq <= 8) ? 4 : 2;
+};
+
+// MODE 0: q is a gathered vector indexed through GlobalToLocal (-1 -> 0).
+// MODE 1: q is an E-vector (element-local input, no gather).
+// Every element slot of a block is a real element: the launcher runs the
+// whole blocks first and the remainder with NE=1 (element offset e0). A
+// clamped/predicated tail inside this kernel defeats the compiler's uniform
+// address analysis and roughly quadruples register use.
+template <int Nq, int NE, typename T, int MODE>
+__global__ void __launch_bounds__(NE * Nq * Nq) __attribute__((amdgpu_waves_per_eu(AxShape<Nq>::waves, 8)))
+    ax_hex_kernel(const int e0, const int* __restrict__ elementList, const int* __restrict__ G2L,
+                  const T* __restrict__ wJ, const T* __restrict__ ggeo, const T* __restrict__ D, const T lambda,
+                  const T* __restrict__ q, T* __restrict__ Aq) {
+  using S = AxShape<Nq>;
+  constexpr int NqNq = S::NqNq, Np = S::Np, threads = NE * NqNq;
+  __shared__ T s_D[Nq][Nq + 1];
+  __shared__ T s_q[NE][Nq][Nq + 1];
+  __shared__ T s_Gqr[NE][Nq][Nq + 1];
+  __shared__ T s_Gqs[NE][Nq][Nq + 1];
+
+  const int t = threadIdx.x;
+  const int es = (NE == 1) ? 0 : t / NqNq;
+  const int ij = (NE == 1) ? t : t - es * NqNq;
+  const int i = ij % Nq;
+  const int j = ij / Nq;
+  const int eIdx = e0 + blockIdx.x * NE + es;
+  const int element = elementList ? elementList[eIdx] : eIdx;
+
+  for (int n = t; n < NqNq; n += threads) s_D[n / Nq][n % Nq] = D[n];
+
+  T r_q[Nq], r_Aq[Nq];
+  const size_t base = (size_t)element * Np + ij;
+#pragma unroll
+  for (int k = 0; k < Nq; ++k) {
+    if (MODE == 0) {
+      const int id = G2L[base + k * NqNq];
+      r_q[k] = (id >= 0) ? q[id] : T(0);
+    } else {
+      r_q[k] = q[base + k * NqNq];
+    }
+    r_Aq[k] = 0;
+  }
+  const T* gb = ggeo + (size_t)element * 6 * Np + ij;
+  const T* wb = wJ + (size_t)element * Np + ij;
+  __syncthreads();
+
+#pragma unroll
+  for (int k = 0; k < Nq; ++k) {
+    const T G00 = gb[0 * Np + k * NqNq], G01 = gb[1 * Np + k * NqNq], G02 = gb[2 * Np + k * NqNq];
+    const T G11 = gb[3 * Np + k * NqNq], G12 = gb[4 * Np + k * NqNq], G22 = gb[5 * Np + k * NqNq];
+    const T GwJ = wb[k * NqNq];
+
+    s_q[es][j][i] = r_q[k];
+    T qt = 0;
+#pragma unroll
+    for (int m = 0; m < Nq; ++m) qt += s_D[k][m] * r_q[m];
+    __syncthreads();
+
+    T qr = 0, qs = 0;
+#pragma unroll
+    for (int m = 0; m < Nq; ++m) {
+      qr += s_D[i][m] * s_q[es][j][m];
+      qs += s_D[j][m] * s_q[es][m][i];
+    }
+    s_Gqs[es][j][i] = G01 * qr + G11 * qs + G12 * qt;
+    s_Gqr[es][j][i] = G00 * qr + G01 * qs + G02 * qt;
+    const T Gqt = G02 * qr + G12 * qs + G22 * qt;
+    T Auk = GwJ * lambda * r_q[k];
+    __syncthreads();
+
+#pragma unroll
+    for (int m = 0; m < Nq; ++m) {
+      Auk += s_D[m][j] * s_Gqs[es][m][i];
+      r_Aq[m] += s_D[k][m] * Gqt;
+      Auk += s_D[m][i] * s_Gqr[es][j][m];
+    }
+    r_Aq[k] += Auk;
+  }
+
+  T* out = Aq + base;
+#pragma unroll
+  for (int k = 0; k < Nq; ++k) __builtin_nontemporal_store(r_Aq[k], out + k * NqNq);
+}
+
+template <int Nq, typename T, int MODE>
+static int launch_ax(int Nelements, const int* elist, const int* g2l, const T* wJ, const T* ggeo, const T* D,
+                     T lambda, const T* q, T* Aq, hipStream_t st) {
+  using S = AxShape<Nq>;
+  if (Nelements <= 0) return LIBP_OK;
+  const int full = Nelements / S::NE;
+  if (full > 0)
+    hipLaunchKernelGGL((ax_hex_kernel<Nq, S::NE, T, MODE>), dim3(full), dim3(S::threads), 0, st, 0, elist, g2l,
+                       wJ, ggeo, D, lambda, q, Aq);
+  const int rest = Nelements - full * S::NE;
+  if (S::NE > 1 && rest > 0)
+    hipLaunchKernelGGL((ax_hex_kernel<Nq, 1, T, MODE>), dim3(rest), dim3(S::NqNq), 0, st, full * S::NE, elist,
+                       g2l, wJ, ggeo, D, lambda, q, Aq);
+  return launch_status("ax_hex");
+}
+
+template <typename T, int MODE>
+static int dispatch_ax(int N, int Nelements, const int* elist, const int* g2l, const T* wJ, const T* ggeo,
+                       const T* D, T lambda, const T* q, T* Aq, void* stream) {
+  hipStream_t st = as_stream(stream);
+  if (!wJ || !ggeo || !D || !q || !Aq || (MODE == 0 && !g2l)) return arg_error("ax_hex", "null pointer");
+  switch (N) {
+#define LIBP_AX_CASE(NN) \
+  case NN:               \
+    return launch_ax<NN + 1, T, MODE>(Nelements, elist, g2l, wJ, ggeo, D, lambda, q, Aq, st);
+    LIBP_AX_CASE(1)
+    LIBP_AX_CASE(2)
+    LIBP_AX_CASE(3)
+    LIBP_AX_CASE(4)
+    LIBP_AX_CASE(5)
+    LIBP_AX_CASE(6)
+    LIBP_AX_CASE(7)
+    LIBP_AX_CASE(8)
+    LIBP_AX_CASE(9)
+    LIBP_AX_CASE(10)
+    LIBP_AX_CASE(11)
+    LIBP_AX_CASE(12)
+    LIBP_AX_CASE(13)
+    LIBP_AX_CASE(14)
+    LIBP_AX_CASE(15)
+#undef LIBP_AX_CASE
+    default:
+      return arg_error("ax_hex", "polynomial degree must be in [1,15]");
+  }
+}
+
+}  // namespace libp_amd
+
+using namespace libp_amd;
+
+extern "C" int libp_ax_hex_partial_f64(int N, int Nelements, const int* elementList, const int* GlobalToLocal,
+                                       const double* wJ, const double* ggeo, const double* D, double lambda,
+                                       const double* q, double* AqL, void* stream) {
+  return dispatch_ax<double, 0>(N, Nelements, elementList, GlobalToLocal, wJ, ggeo, D, lambda, q, AqL, stream);
+}
+
+extern "C" int libp_ax_hex_partial_f32(int N, int Nelements, const int* elementList, const int* GlobalToLocal,
+                                       const float* wJ, const float* ggeo, const float* D, float lambda,
+                                       const float* q, float* AqL, void* stream) {
+  return dispatch_ax<float, 0>(N, Nelements, elementList, GlobalToLocal, wJ, ggeo, D, lambda, q, AqL, stream);
+}
+
+extern "C" int libp_ax_hex_local_f64(int N, int Nelements, const double* wJ, const double* ggeo, const double* D,
+                                     double lambda, const double* qL, double* AqL, void* stream) {
+  return dispatch_ax<double, 1>(N, Nelements, nullptr, nullptr, wJ, ggeo, D, lambda, qL, AqL, stream);
+}

@@ -1,0 +1,378 @@
+// BLAS-1, deterministic reductions to device scalars, fused PCG/FPCG vector
+// kernels and Chebyshev smoother steps, FP64 and FP32.
+//
+// Reductions: each block reduces a fixed grid-stride slice (wavefront shuffles,
+// then one LDS step across the block's waves) and stores a partial; the last
+// block to finish (ticket counter) sums the partials in block order and writes
+// the scalar. The block count depends only on N, so results are reproducible
+// run to run. The counter is reset by the last block, so no memset is needed
+// between calls (safe under stream capture).
+#include "kcommon.hpp"
+
+namespace libp_amd {
+
+constexpr int RBS = 256;       // reduction block size
+constexpr int RMAXB = 1024;    // max reduction blocks
+constexpr int RNPART = 3;      // partial arrays available (fused multi-dot)
+
+struct ReduceWS {
+  double part[RNPART][RMAXB];
+  unsigned int counter;
+};
+
+inline unsigned reduce_blocks(long N) {
+  long b = (N + RBS * 4 - 1) / (RBS * 4);
+  if (b < 1) b = 1;
+  if (b > RMAXB) b = RMAXB;
+  return (unsigned)b;
+}
+
+// Block-level sum; result valid in thread 0.
+__device__ inline double block_sum(double v, double* s_w) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) s_w[w] = v;
+  __syncthreads();
+  double r = 0;
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; ++i) r += s_w[i];
+  }
+  __syncthreads();
+  return r;
+}
+
+// Stores partials[k][blockIdx] for k < NV; returns true in thread 0 of the last block.
+template <int NV>
+__device__ inline bool finish_partials(ReduceWS* ws, const double (&v)[NV], double* s_w, bool& last) {
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double b = block_sum(v[k], s_w);
+    if (threadIdx.x == 0) ws->part[k][blockIdx.x] = b;
+  }
+  __shared__ bool s_last;
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned ticket = atomicAdd(&ws->counter, 1u);
+    s_last = (ticket == gridDim.x - 1);
+  }
+  __syncthreads();
+  last = s_last;
+  return last;
+}
+
+template <int NV>
+__device__ inline void sum_partials(ReduceWS* ws, double (&out)[NV]) {
+  // executed by the whole last block: each wave lane sums a strided subset in
+  // a fixed order, then a fixed tree; deterministic for a given grid size.
+  __shared__ double s_w[16];
+  __threadfence();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double acc = 0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) acc += __builtin_nontemporal_load(&ws->part[k][b]);
+    out[k] = block_sum(acc, s_w);
+  }
+  if (threadIdx.x == 0) ws->counter = 0;
+}
+
+/* ------------------------------ reductions ------------------------------ */
+template <int KIND>  // 0 x.y, 1 x.x, 2 sum x, 3 w.x.y
+__global__ void __launch_bounds__(RBS) reduce_kernel(const int N, const double* __restrict__ w,
+                                                     const double* __restrict__ x, const double* __restrict__ y,
+                                                     double* __restrict__ result, ReduceWS* __restrict__ ws) {
+  __shared__ double s_w[16];
+  double acc = 0;
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
+    if (KIND == 0) acc += x[n] * y[n];
+    if (KIND == 1) acc += x[n] * x[n];
+    if (KIND == 2) acc += x[n];
+    if (KIND == 3) acc += w[n] * x[n] * y[n];
+  }
+  double v[1] = {acc};
+  bool last;
+  if (finish_partials<1>(ws, v, s_w, last)) {}
+  if (last) {
+    double out[1];
+    sum_partials<1>(ws, out);
+    if (threadIdx.x == 0) *result = out[0];
+  }
+}
+
+template <int KIND>
+static int reduce(int N, const double* w, const double* x, const double* y, double* result, void* ws, void* stream) {
+  if (!x || !result || !ws) return arg_error("reduce", "null pointer");
+  hipLaunchKernelGGL(reduce_kernel<KIND>, dim3(reduce_blocks(N)), dim3(RBS), 0, as_stream(stream), N, w, x, y,
+                     result, (ReduceWS*)ws);
+  return launch_status("reduce");
+}
+
+/* -------------------------------- BLAS-1 -------------------------------- */
+template <typename T>
+__global__ void __launch_bounds__(256) axpy_kernel(const int N, const T alpha, const T* __restrict__ x, const T beta,
+                                                   T* __restrict__ y) {
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x)
+    y[n] = (beta != T(0)) ? alpha * x[n] + beta * y[n] : alpha * x[n];
+}
+template <typename T>
+__global__ void __launch_bounds__(256) zaxpy_kernel(const int N, const T alpha, const T* __restrict__ x, const T beta,
+                                                    const T* __restrict__ y, T* __restrict__ z) {
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x)
+    z[n] = alpha * x[n] + beta * y[n];
+}
+template <typename T>
+__global__ void __launch_bounds__(256) amx_kernel(const int N, const T alpha, const T* __restrict__ a,
+                                                  T* __restrict__ x) {
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x)
+    x[n] = alpha * a[n] * x[n];
+}
+template <typename T>
+__global__ void __launch_bounds__(256) amxpy_kernel(const int N, const T alpha, const T* __restrict__ a,
+                                                    const T* __restrict__ x, const T beta, T* __restrict__ y) {
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x)
+    y[n] = (beta != T(0)) ? alpha * a[n] * x[n] + beta * y[n] : alpha * a[n] * x[n];
+}
+template <int OP>  // 0 set, 1 add, 2 scale
+__global__ void __launch_bounds__(256) scalar_kernel(const int N, const double alpha, double* __restrict__ x) {
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
+    if (OP == 0) x[n] = alpha;
+    if (OP == 1) x[n] += alpha;
+    if (OP == 2) x[n] *= alpha;
+  }
+}
+template <typename S, typename D>
+__global__ void __launch_bounds__(256) convert_kernel(const int N, const S* __restrict__ x, D* __restrict__ y) {
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) y[n] = (D)x[n];
+}
+
+#define LIBP_LAUNCH_1D(kern, N, ...)                                                                      \
+  do {                                                                                                    \
+    if ((N) <= 0) return LIBP_OK;                                                                         \
+    hipLaunchKernelGGL(kern, dim3(stream_blocks((N), 256)), dim3(256), 0, as_stream(stream), __VA_ARGS__); \
+    return launch_status(#kern);                                                                          \
+  } while (0)
+
+/* ---------------------------- PCG fused kernels ---------------------------- */
+// scalars: [0] rdotz1 [1] rdotz2 [2] alpha [3] beta [4] pAp [5] rdotr [6] zdotAp [7] TOL
+
+// z = invDiag.*r (if invDiag) ; r.z ; (flexible) z.Ap ; then beta.
+template <bool JAC>
+__global__ void __launch_bounds__(RBS) pcg_rdotz_kernel(const int N, const double* __restrict__ invDiag,
+                                                        const double* __restrict__ r, double* __restrict__ z,
+                                                        const double* __restrict__ Ap, const int flexible,
+                                                        double* __restrict__ s, ReduceWS* __restrict__ ws) {
+  __shared__ double s_w[16];
+  double a0 = 0, a1 = 0;
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
+    const double rn = r[n];
+    double zn;
+    if (JAC) {
+      zn = 1.0 * invDiag[n] * rn;
+      z[n] = zn;
+    } else {
+      zn = z[n];
+    }
+    a0 += rn * zn;
+    if (flexible) a1 += zn * Ap[n];
+  }
+  double v[2] = {a0, a1};
+  bool last;
+  finish_partials<2>(ws, v, s_w, last);
+  if (last) {
+    double out[2];
+    sum_partials<2>(ws, out);
+    if (threadIdx.x == 0) {
+      // linearSolverPCG.cpp:116-124 ; beta of the first iteration is handled by the p update
+      const double rdotz1_old = s[0];
+      s[1] = rdotz1_old;
+      s[0] = out[0];
+      s[6] = out[1];
+      s[3] = flexible ? (-s[2] * out[1] / rdotz1_old) : (out[0] / rdotz1_old);
+    }
+  }
+}
+
+// p = z + beta*p  (axpy(N, 1, z, beta, p); beta==0 on the first iteration: p = z)
+__global__ void __launch_bounds__(256) pcg_update_p_kernel(const int N, const int first, const double* __restrict__ z,
+                                                           double* __restrict__ p, const double* __restrict__ s) {
+  const double beta = first ? 0.0 : s[3];
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x)
+    p[n] = (beta != 0) ? 1.0 * z[n] + beta * p[n] : 1.0 * z[n];
+}
+
+__global__ void __launch_bounds__(RBS) pcg_pap_kernel(const int N, const double* __restrict__ p,
+                                                      const double* __restrict__ Ap, double* __restrict__ s,
+                                                      ReduceWS* __restrict__ ws) {
+  __shared__ double s_w[16];
+  double a0 = 0;
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) a0 += p[n] * Ap[n];
+  double v[1] = {a0};
+  bool last;
+  finish_partials<1>(ws, v, s_w, last);
+  if (last) {
+    double out[1];
+    sum_partials<1>(ws, out);
+    if (threadIdx.x == 0) {
+      s[4] = out[0];
+      s[2] = s[0] / out[0];  // alpha = rdotz1/pAp
+    }
+  }
+}
+
+// x += alpha p ; r -= alpha Ap ; r.r   (linearSolverUpdatePCG.okl:29-75)
+__global__ void __launch_bounds__(RBS) pcg_update_kernel(const int N, const double* __restrict__ p,
+                                                         const double* __restrict__ Ap, double* __restrict__ x,
+                                                         double* __restrict__ r, double* __restrict__ s,
+                                                         ReduceWS* __restrict__ ws) {
+  __shared__ double s_w[16];
+  const double alpha = s[2];
+  double a0 = 0;
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
+    double rn = r[n];
+    x[n] += alpha * p[n];
+    rn -= alpha * Ap[n];
+    a0 += rn * rn;
+    r[n] = rn;
+  }
+  double v[1] = {a0};
+  bool last;
+  finish_partials<1>(ws, v, s_w, last);
+  if (last) {
+    double out[1];
+    sum_partials<1>(ws, out);
+    if (threadIdx.x == 0) s[5] = out[0];
+  }
+}
+
+/* ------------------------------ Chebyshev ------------------------------ */
+// RES = invDiag.*(r - Ax) [Ax==NULL: invDiag.*r] ; d = invTheta*RES ; X = d (xIsZero) or X += d
+template <typename T>
+__global__ void __launch_bounds__(256) cheb_start_kernel(const int N, const T* __restrict__ invDiag,
+                                                         const T* __restrict__ r, const T* __restrict__ Ax,
+                                                         const T invTheta, T* __restrict__ RES, T* __restrict__ d,
+                                                         T* __restrict__ X, const int xIsZero) {
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
+    const T res = Ax ? T(1) * invDiag[n] * (T(1) * r[n] - T(1) * Ax[n]) : T(1) * invDiag[n] * r[n];
+    const T dn = invTheta * res;
+    RES[n] = res;
+    d[n] = dn;
+    X[n] = xIsZero ? dn : X[n] + dn;
+  }
+}
+// RES -= invDiag.*Ad ; d = c1*RES + c2*d ; X += d
+template <typename T>
+__global__ void __launch_bounds__(256) cheb_step_kernel(const int N, const T* __restrict__ invDiag,
+                                                        const T* __restrict__ Ad, const T c1, const T c2,
+                                                        T* __restrict__ RES, T* __restrict__ d, T* __restrict__ X) {
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
+    const T res = T(-1) * invDiag[n] * Ad[n] + RES[n];
+    const T dn = c1 * res + c2 * d[n];
+    RES[n] = res;
+    d[n] = dn;
+    X[n] = X[n] + dn;
+  }
+}
+
+}  // namespace libp_amd
+
+using namespace libp_amd;
+
+extern "C" long libp_reduce_ws_bytes(void) { return (long)sizeof(ReduceWS); }
+
+extern "C" int libp_inner_prod_f64(int N, const double* x, const double* y, double* result, void* ws, void* stream) {
+  return reduce<0>(N, nullptr, x, y, result, ws, stream);
+}
+extern "C" int libp_weighted_inner_prod_f64(int N, const double* w, const double* x, const double* y, double* result,
+                                            void* ws, void* stream) {
+  return reduce<3>(N, w, x, y, result, ws, stream);
+}
+extern "C" int libp_norm2sq_f64(int N, const double* x, double* result, void* ws, void* stream) {
+  return reduce<1>(N, nullptr, x, nullptr, result, ws, stream);
+}
+extern "C" int libp_sum_f64(int N, const double* x, double* result, void* ws, void* stream) {
+  return reduce<2>(N, nullptr, x, nullptr, result, ws, stream);
+}
+
+extern "C" int libp_axpy_f64(int N, double alpha, const double* x, double beta, double* y, void* stream) {
+  LIBP_LAUNCH_1D(axpy_kernel<double>, N, N, alpha, x, beta, y);
+}
+extern "C" int libp_axpy_f32(int N, float alpha, const float* x, float beta, float* y, void* stream) {
+  LIBP_LAUNCH_1D(axpy_kernel<float>, N, N, alpha, x, beta, y);
+}
+extern "C" int libp_zaxpy_f64(int N, double alpha, const double* x, double beta, const double* y, double* z,
+                              void* stream) {
+  LIBP_LAUNCH_1D(zaxpy_kernel<double>, N, N, alpha, x, beta, y, z);
+}
+extern "C" int libp_amx_f64(int N, double alpha, const double* a, double* x, void* stream) {
+  LIBP_LAUNCH_1D(amx_kernel<double>, N, N, alpha, a, x);
+}
+extern "C" int libp_amx_f32(int N, float alpha, const float* a, float* x, void* stream) {
+  LIBP_LAUNCH_1D(amx_kernel<float>, N, N, alpha, a, x);
+}
+extern "C" int libp_amxpy_f64(int N, double alpha, const double* a, const double* x, double beta, double* y,
+                              void* stream) {
+  LIBP_LAUNCH_1D(amxpy_kernel<double>, N, N, alpha, a, x, beta, y);
+}
+extern "C" int libp_amxpy_f32(int N, float alpha, const float* a, const float* x, float beta, float* y, void* stream) {
+  LIBP_LAUNCH_1D(amxpy_kernel<float>, N, N, alpha, a, x, beta, y);
+}
+extern "C" int libp_set_f64(int N, double alpha, double* x, void* stream) {
+  LIBP_LAUNCH_1D(scalar_kernel<0>, N, N, alpha, x);
+}
+extern "C" int libp_add_f64(int N, double alpha, double* x, void* stream) {
+  LIBP_LAUNCH_1D(scalar_kernel<1>, N, N, alpha, x);
+}
+extern "C" int libp_scale_f64(int N, double alpha, double* x, void* stream) {
+  LIBP_LAUNCH_1D(scalar_kernel<2>, N, N, alpha, x);
+}
+extern "C" int libp_convert_f64_to_f32(int N, const double* x, float* y, void* stream) {
+  LIBP_LAUNCH_1D((convert_kernel<double, float>), N, N, x, y);
+}
+extern "C" int libp_convert_f32_to_f64(int N, const float* x, double* y, void* stream) {
+  LIBP_LAUNCH_1D((convert_kernel<float, double>), N, N, x, y);
+}
+
+extern "C" int libp_pcg_rdotz_f64(int N, const double* r, const double* z, const double* Ap, int flexible,
+                                  double* scalars, void* ws, void* stream) {
+  hipLaunchKernelGGL(pcg_rdotz_kernel<false>, dim3(reduce_blocks(N)), dim3(RBS), 0, as_stream(stream), N, nullptr,
+                     r, const_cast<double*>(z), Ap, flexible, scalars, (ReduceWS*)ws);
+  return launch_status("pcg_rdotz");
+}
+extern "C" int libp_pcg_jacobi_rdotz_f64(int N, const double* invDiag, const double* r, double* z, const double* Ap,
+                                         int flexible, double* scalars, void* ws, void* stream) {
+  hipLaunchKernelGGL(pcg_rdotz_kernel<true>, dim3(reduce_blocks(N)), dim3(RBS), 0, as_stream(stream), N, invDiag, r,
+                     z, Ap, flexible, scalars, (ReduceWS*)ws);
+  return launch_status("pcg_jacobi_rdotz");
+}
+extern "C" int libp_pcg_update_p_f64(int N, int first_iter, const double* z, double* p, const double* scalars,
+                                     void* stream) {
+  LIBP_LAUNCH_1D(pcg_update_p_kernel, N, N, first_iter, z, p, scalars);
+}
+extern "C" int libp_pcg_pap_f64(int N, const double* p, const double* Ap, double* scalars, void* ws, void* stream) {
+  hipLaunchKernelGGL(pcg_pap_kernel, dim3(reduce_blocks(N)), dim3(RBS), 0, as_stream(stream), N, p, Ap, scalars,
+                     (ReduceWS*)ws);
+  return launch_status("pcg_pap");
+}
+extern "C" int libp_pcg_update_f64(int N, const double* p, const double* Ap, double* x, double* r, double* scalars,
+                                   void* ws, void* stream) {
+  hipLaunchKernelGGL(pcg_update_kernel, dim3(reduce_blocks(N)), dim3(RBS), 0, as_stream(stream), N, p, Ap, x, r,
+                     scalars, (ReduceWS*)ws);
+  return launch_status("pcg_update");
+}
+
+extern "C" int libp_cheb_start_f32(int N, const float* invDiag, const float* r, const float* Ax, float invTheta,
+                                   float* RES, float* d, float* X, int xIsZero, void* stream) {
+  LIBP_LAUNCH_1D(cheb_start_kernel<float>, N, N, invDiag, r, Ax, invTheta, RES, d, X, xIsZero);
+}
+extern "C" int libp_cheb_step_f32(int N, const float* invDiag, const float* Ad, float c1, float c2, float* RES,
+                                  float* d, float* X, void* stream) {
+  LIBP_LAUNCH_1D(cheb_step_kernel<float>, N, N, invDiag, Ad, c1, c2, RES, d, X);
+}
+extern "C" int libp_cheb_start_f64(int N, const double* invDiag, const double* r, const double* Ax, double invTheta,
+                                   double* RES, double* d, double* X, int xIsZero, void* stream) {
+  LIBP_LAUNCH_1D(cheb_start_kernel<double>, N, N, invDiag, r, Ax, invTheta, RES, d, X, xIsZero);
+}
+extern "C" int libp_cheb_step_f64(int N, const double* invDiag, const double* Ad, double c1, double c2, double* RES,
+                                  double* d, double* X, void* stream) {
+  LIBP_LAUNCH_1D(cheb_step_kernel<double>, N, N, invDiag, Ad, c1, c2, RES, d, X);
+}

@@ -1,0 +1,309 @@
+// CPU oracle (test infrastructure only): C entry points for the Python tests
+// and for bench.py's CPU baseline. Nothing in the product links this library.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "oracle.hpp"
+
+namespace oracle {
+std::vector<std::complex<double>> hessenbergEigs(int n, const std::vector<double>& Hcm);
+}
+
+using namespace oracle;
+
+namespace {
+struct Handle {
+  Settings s;
+  GlibcRand rng;
+  Drand48 drng;
+  Mesh mesh;
+  Elliptic el;
+  std::unique_ptr<MultiGrid> mg;
+  std::vector<double> invDiag;
+  std::vector<double> scratch;
+  std::string err;
+};
+thread_local std::string g_err;
+
+template <typename T>
+long copy_out(const std::vector<T>& v, void* out) {
+  if (out) std::memcpy(out, v.data(), v.size() * sizeof(T));
+  return (long)v.size();
+}
+}  // namespace
+
+extern "C" {
+
+const char* orc_last_error() { return g_err.c_str(); }
+
+void* orc_create(const char* rc_text) {
+  try {
+    Handle* h = new Handle();
+    h->s = Settings::defaults();
+    Settings user;
+    user.parseText(rc_text);
+    for (auto& k : user.order) h->s.set(k, user.get(k));
+    h->mesh.Setup(h->s);
+    h->el.Setup(h->mesh, h->s, h->s.getd("LAMBDA"), h->rng);
+    return h;
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+
+void orc_free(void* p) { delete (Handle*)p; }
+
+long orc_geti(void* p, const char* name) {
+  Handle* h = (Handle*)p;
+  std::string n(name);
+  if (n == "Nelements") return h->mesh.Nelements;
+  if (n == "N") return h->mesh.N;
+  if (n == "Np") return h->mesh.Np;
+  if (n == "Nq") return h->mesh.Nq;
+  if (n == "Ngather") return h->el.Ndofs;
+  if (n == "Nmasked") return h->el.Nmasked;
+  if (n == "NrowBlocksT") return h->el.ogsMasked.gatherLocal.NrowBlocksT;
+  if (n == "NrowBlocksN") return h->el.ogsMasked.gatherLocal.NrowBlocksN;
+  if (n == "allNeumann") return h->el.allNeumann;
+  return -1;
+}
+
+double orc_getd(void* p, const char* name) {
+  Handle* h = (Handle*)p;
+  std::string n(name);
+  if (n == "lambda") return h->el.lambda;
+  if (n == "allNeumannScale") return h->el.allNeumannScale;
+  return NAN;
+}
+
+// Copies the named array into out (if non-null) and returns its length.
+// dtype: 0 float64, 1 int32, 2 int64.
+long orc_array(void* p, const char* name, void* out, int* dtype) {
+  Handle* h = (Handle*)p;
+  std::string n(name);
+  const Mesh& m = h->mesh;
+  const Elliptic& e = h->el;
+  const OgsOperator& G = e.ogsMasked.gatherLocal;
+  *dtype = 0;
+  if (n == "x") return copy_out(m.x, out);
+  if (n == "y") return copy_out(m.y, out);
+  if (n == "z") return copy_out(m.z, out);
+  if (n == "ggeo") return copy_out(m.ggeo, out);
+  if (n == "wJ") return copy_out(m.wJ, out);
+  if (n == "D") return copy_out(m.D, out);
+  if (n == "gllz") return copy_out(m.gllz, out);
+  if (n == "gllw") return copy_out(m.gllw, out);
+  if (n == "weightG") return copy_out(e.weightG, out);
+  *dtype = 2;
+  if (n == "globalIds") return copy_out(m.globalIds, out);
+  if (n == "maskedGlobalIds") return copy_out(e.maskedGlobalIds, out);
+  if (n == "EToV") return copy_out(m.EToV, out);
+  *dtype = 1;
+  if (n == "mapB") return copy_out(e.mapB, out);
+  if (n == "meshMapB") return copy_out(m.mapB, out);
+  if (n == "EToE") return copy_out(m.EToE, out);
+  if (n == "EToB") return copy_out(m.EToB, out);
+  if (n == "vmapM") return copy_out(m.vmapM, out);
+  if (n == "vmapP") return copy_out(m.vmapP, out);
+  if (n == "GlobalToLocal") return copy_out(e.GlobalToLocal, out);
+  if (n == "rowStartsT") return copy_out(G.rowStartsT, out);
+  if (n == "colIdsT") return copy_out(G.colIdsT, out);
+  if (n == "rowStartsN") return copy_out(G.rowStartsN, out);
+  if (n == "colIdsN") return copy_out(G.colIdsN, out);
+  if (n == "blockRowStartsT") return copy_out(G.blockRowStartsT, out);
+  if (n == "blockRowStartsN") return copy_out(G.blockRowStartsN, out);
+  if (n == "faceNodes") return copy_out(m.faceNodes, out);
+  *dtype = -1;
+  return -1;
+}
+
+void orc_partial_ax(void* p, const double* q, double* AqL) { ((Handle*)p)->el.PartialAx(q, AqL); }
+
+void orc_operator(void* p, const double* q, double* Aq) {
+  Handle* h = (Handle*)p;
+  h->el.Operator(q, Aq, h->scratch);
+}
+
+void orc_gather(void* p, const double* v, double* gv, int trans) {
+  ((Handle*)p)->el.ogsMasked.gatherLocal.GatherAdd(gv, v, trans != 0);
+}
+
+void orc_scatter(void* p, const double* gv, double* v) {
+  ((Handle*)p)->el.ogsMasked.gatherLocal.Scatter(v, gv, false);
+}
+
+void orc_diagonal(void* p, double* diag) {
+  std::vector<double> d;
+  ((Handle*)p)->el.BuildOperatorDiagonal(d);
+  std::memcpy(diag, d.data(), d.size() * sizeof(double));
+}
+
+void orc_rhs_local(void* p, double* rL) {
+  std::vector<double> r;
+  ((Handle*)p)->el.Rhs(r);
+  std::memcpy(rL, r.data(), r.size() * sizeof(double));
+}
+
+double orc_inner(long N, const double* x, const double* y) { return linalg_innerProd((dlong)N, x, y); }
+
+// Jacobi-PCG (or PCG without preconditioner when jacobi==0) on the handle's
+// operator. Returns the iteration count; x and r are updated in place.
+int orc_pcg(void* p, double* x, double* r, double tol, int maxit, int jacobi, int pureRelative, double* rdotr_final) {
+  Handle* h = (Handle*)p;
+  struct A_ : OpFn {
+    Handle* h;
+    void apply(const double* a, double* b) override { h->el.Operator(a, b, h->scratch); }
+  } A;
+  A.h = h;
+  struct M_ : OpFn {
+    std::vector<double>* inv;
+    dlong N;
+    void apply(const double* a, double* b) override {
+      if (inv)
+        for (dlong n = 0; n < N; ++n) b[n] = 1.0 * (*inv)[n] * a[n];
+      else
+        for (dlong n = 0; n < N; ++n) b[n] = a[n];
+    }
+  } M;
+  M.N = h->el.Ndofs;
+  M.inv = nullptr;
+  if (jacobi) {
+    std::vector<double> d;
+    h->el.BuildOperatorDiagonal(d);
+    h->invDiag.resize(d.size());
+    for (size_t n = 0; n < d.size(); ++n) h->invDiag[n] = 1.0 / d[n];
+    M.inv = &h->invDiag;
+  }
+  PCGResult res = pcg_solve(h->el.Ndofs, A, M, x, r, tol, maxit, false, false, pureRelative != 0);
+  if (rdotr_final) *rdotr_final = res.rdotrFinal;
+  return res.iters;
+}
+
+// Multigrid preconditioner on the handle (degrees from the settings).
+int orc_mg_setup(void* p) {
+  Handle* h = (Handle*)p;
+  try {
+    h->mg.reset(new MultiGrid());
+    if (h->s.has("MULTIGRID COARSE TOLERANCE")) h->mg->coarseTol = h->s.getd("MULTIGRID COARSE TOLERANCE");
+    if (h->s.has("MULTIGRID COARSE MAXIT")) h->mg->coarseMaxIt = h->s.geti("MULTIGRID COARSE MAXIT");
+    // same schedule logic as elliptic_run
+    std::vector<int> deg;
+    if (h->s.compare("MULTIGRID COARSENING", "CUSTOM")) {
+      std::string v = h->s.get("MULTIGRID DEGREES");
+      size_t a = 0;
+      while (a < v.size()) {
+        size_t b = v.find(',', a);
+        if (b == std::string::npos) b = v.size();
+        deg.push_back(std::stoi(v.substr(a, b - a)));
+        a = b + 1;
+      }
+    } else {
+      throw std::runtime_error("orc_mg_setup expects MULTIGRID COARSENING CUSTOM");
+    }
+    h->mg->Setup(h->el, h->rng, h->drng, deg);
+    return (int)h->mg->levels.size();
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+void orc_mg_apply(void* p, const double* r, double* z) { ((Handle*)p)->mg->apply(r, z); }
+
+double orc_mg_level_d(void* p, int l, const char* name) {
+  Handle* h = (Handle*)p;
+  std::string n(name);
+  if (n == "lambda1") return h->mg->mg[l].lambda1;
+  if (n == "lambda0") return h->mg->mg[l].lambda0;
+  return NAN;
+}
+
+// End-to-end run of a settings text. Returns the solution norm (NaN on error).
+double orc_run(const char* rc_text, int* iters, long* ngather) {
+  try {
+    Settings s = Settings::defaults();
+    Settings user;
+    user.parseText(rc_text);
+    for (auto& k : user.order) s.set(k, user.get(k));
+    RunResult r = elliptic_run(s, 0);
+    if (iters) *iters = r.iters;
+    if (ngather) *ngather = r.Ngather;
+    return r.norm;
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return NAN;
+  }
+}
+
+void orc_gll(int N, double* x, double* w) {
+  std::vector<double> xx, ww;
+  JacobiGLL(N, xx, ww);
+  std::memcpy(x, xx.data(), sizeof(double) * (N + 1));
+  std::memcpy(w, ww.data(), sizeof(double) * (N + 1));
+}
+
+void orc_dmatrix(int N, double* D) {
+  std::vector<double> x, w, d;
+  JacobiGLL(N, x, w);
+  Dmatrix1D(N, x, d);
+  std::memcpy(D, d.data(), sizeof(double) * d.size());
+}
+
+void orc_interp(int Nc, int Nf, double* P) {
+  std::vector<double> p;
+  DegreeRaiseMatrix1D(Nc, Nf, p);
+  std::memcpy(P, p.data(), sizeof(double) * p.size());
+}
+
+void orc_rand_seq(int n, int* out) {
+  GlibcRand r;
+  for (int i = 0; i < n; ++i) out[i] = r.next();
+}
+
+void orc_drand48_seq(long seed, int n, double* out) {
+  Drand48 d;
+  d.seed(seed);
+  for (int i = 0; i < n; ++i) out[i] = d.next();
+}
+
+int orc_hess_eigs(int n, const double* Hcm, double* re, double* im) {
+  try {
+    std::vector<double> H(Hcm, Hcm + n * n);
+    auto ev = hessenbergEigs(n, H);
+    for (int i = 0; i < (int)ev.size(); ++i) {
+      re[i] = ev[i].real();
+      im[i] = ev[i].imag();
+    }
+    return (int)ev.size();
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+// CPU baseline: time `reps` applications of the partial Ax on the handle's
+// mesh. Returns the median seconds per application.
+double orc_time_partial_ax(void* p, int reps) {
+  Handle* h = (Handle*)p;
+  const size_t NL = (size_t)h->mesh.Nelements * h->mesh.Np;
+  std::vector<double> q(h->el.Ndofs), AqL(NL);
+  for (size_t i = 0; i < q.size(); ++i) q[i] = std::sin(0.37 * (double)i);
+  std::vector<double> t;
+  for (int r = 0; r < reps; ++r) {
+    auto a = std::chrono::steady_clock::now();
+    h->el.PartialAx(q.data(), AqL.data());
+    auto b = std::chrono::steady_clock::now();
+    t.push_back(std::chrono::duration<double>(b - a).count());
+  }
+  std::sort(t.begin(), t.end());
+  return t[t.size() / 2];
+}
+
+}  // extern "C"

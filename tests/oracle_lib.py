@@ -1,0 +1,162 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
+
+Test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's CPU
+baseline load this. The oracle restates the reference's Serial-mode path; see
+oracle/oracle.hpp.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
+
+c_int, c_long, c_double, c_void_p, c_char_p = ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_void_p, ctypes.c_char_p
+_lib = None
+_DT = {0: np.float64, 1: np.int32, 2: np.int64}
+
+
+def build():
+    subprocess.run(["make", "-C", ORACLE_DIR, "-s"], check=True)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            build()
+        L = ctypes.CDLL(ORACLE_SO)
+        sig = {
+            "orc_last_error": (c_char_p, []),
+            "orc_create": (c_void_p, [c_char_p]),
+            "orc_free": (None, [c_void_p]),
+            "orc_geti": (c_long, [c_void_p, c_char_p]),
+            "orc_getd": (c_double, [c_void_p, c_char_p]),
+            "orc_array": (c_long, [c_void_p, c_char_p, c_void_p, ctypes.POINTER(c_int)]),
+            "orc_partial_ax": (None, [c_void_p, c_void_p, c_void_p]),
+            "orc_operator": (None, [c_void_p, c_void_p, c_void_p]),
+            "orc_gather": (None, [c_void_p, c_void_p, c_void_p, c_int]),
+            "orc_scatter": (None, [c_void_p, c_void_p, c_void_p]),
+            "orc_diagonal": (None, [c_void_p, c_void_p]),
+            "orc_rhs_local": (None, [c_void_p, c_void_p]),
+            "orc_inner": (c_double, [c_long, c_void_p, c_void_p]),
+            "orc_pcg": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_int, c_int, c_int,
+                                ctypes.POINTER(c_double)]),
+            "orc_mg_setup": (c_int, [c_void_p]),
+            "orc_mg_apply": (None, [c_void_p, c_void_p, c_void_p]),
+            "orc_mg_level_d": (c_double, [c_void_p, c_int, c_char_p]),
+            "orc_run": (c_double, [c_char_p, ctypes.POINTER(c_int), ctypes.POINTER(c_long)]),
+            "orc_gll": (None, [c_int, c_void_p, c_void_p]),
+            "orc_dmatrix": (None, [c_int, c_void_p]),
+            "orc_interp": (None, [c_int, c_int, c_void_p]),
+            "orc_rand_seq": (None, [c_int, c_void_p]),
+            "orc_drand48_seq": (None, [c_long, c_int, c_void_p]),
+            "orc_hess_eigs": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
+            "orc_time_partial_ax": (c_double, [c_void_p, c_int]),
+        }
+        for n, (r, a) in sig.items():
+            f = getattr(L, n)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def rc(settings):
+    return "".join(f"[{k}]\n{v}\n\n" for k, v in settings.items())
+
+
+class OracleElliptic:
+    def __init__(self, settings):
+        self.L = load()
+        text = settings if isinstance(settings, str) else rc(settings)
+        self.h = self.L.orc_create(text.encode())
+        if not self.h:
+            raise RuntimeError("oracle create failed: " + self.L.orc_last_error().decode())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_free(self.h)
+            self.h = None
+
+    def geti(self, name):
+        return int(self.L.orc_geti(self.h, name.encode()))
+
+    def array(self, name):
+        dt = c_int()
+        n = self.L.orc_array(self.h, name.encode(), None, ctypes.byref(dt))
+        if n < 0:
+            raise KeyError(name)
+        out = np.empty(n, dtype=_DT[dt.value])
+        self.L.orc_array(self.h, name.encode(), out.ctypes.data, ctypes.byref(dt))
+        return out
+
+    def partial_ax(self, q):
+        NL = self.geti("Nelements") * self.geti("Np")
+        out = np.empty(NL)
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        self.L.orc_partial_ax(self.h, q.ctypes.data, out.ctypes.data)
+        return out
+
+    def operator(self, q):
+        out = np.empty(self.geti("Ngather"))
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        self.L.orc_operator(self.h, q.ctypes.data, out.ctypes.data)
+        return out
+
+    def gather(self, vL, trans=True):
+        out = np.empty(self.geti("Ngather"))
+        vL = np.ascontiguousarray(vL, dtype=np.float64)
+        self.L.orc_gather(self.h, vL.ctypes.data, out.ctypes.data, 1 if trans else 0)
+        return out
+
+    def scatter(self, g):
+        NL = self.geti("Nelements") * self.geti("Np")
+        out = np.zeros(NL)
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        self.L.orc_scatter(self.h, g.ctypes.data, out.ctypes.data)
+        return out
+
+    def diagonal(self):
+        out = np.empty(self.geti("Ngather"))
+        self.L.orc_diagonal(self.h, out.ctypes.data)
+        return out
+
+    def rhs_local(self):
+        NL = self.geti("Nelements") * self.geti("Np")
+        out = np.empty(NL)
+        self.L.orc_rhs_local(self.h, out.ctypes.data)
+        return out
+
+    def pcg(self, x, r, tol, maxit, jacobi=True, pure_relative=False):
+        x = np.ascontiguousarray(x, dtype=np.float64).copy()
+        r = np.ascontiguousarray(r, dtype=np.float64).copy()
+        rd = c_double()
+        it = self.L.orc_pcg(self.h, x.ctypes.data, r.ctypes.data, tol, maxit, int(jacobi), int(pure_relative),
+                            ctypes.byref(rd))
+        return it, x, r, rd.value
+
+    def mg_setup(self):
+        n = self.L.orc_mg_setup(self.h)
+        if n < 0:
+            raise RuntimeError(self.L.orc_last_error().decode())
+        return n
+
+    def mg_apply(self, r):
+        z = np.zeros(self.geti("Ngather"))
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        self.L.orc_mg_apply(self.h, r.ctypes.data, z.ctypes.data)
+        return z
+
+
+def run(settings):
+    L = load()
+    it, ng = c_int(), c_long()
+    text = settings if isinstance(settings, str) else rc(settings)
+    norm = L.orc_run(text.encode(), ctypes.byref(it), ctypes.byref(ng))
+    if norm != norm:
+        raise RuntimeError(L.orc_last_error().decode())
+    return norm, it.value, ng.value

@@ -1,0 +1,261 @@
+"""GPU parity tests: the HIP path (through the C ABI) against the CPU oracle."""
+import numpy as np
+import pytest
+
+from tests import oracle_lib
+from tests.oracle_lib import OracleElliptic
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def L():
+    import libparanumal_amd
+    return libparanumal_amd.lib()
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+def box(n, N, flag=1, curve=0.0, lam=1.0, **extra):
+    s = {"MESH FILE": "BOX", "MESH DIMENSION": 3, "ELEMENT TYPE": 12, "BOX NX": n, "BOX NY": n + 1, "BOX NZ": n - 1,
+         "BOX DIMX": 2, "BOX DIMY": 2, "BOX DIMZ": 2, "BOX BOUNDARY FLAG": flag, "POLYNOMIAL DEGREE": N,
+         "LAMBDA": lam, "DATA FILE": "data/ellipticSine3D.h", "BOX CURVED MAP AMPLITUDE": curve}
+    s.update(extra)
+    return s
+
+
+def relerr(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+# ---------------------------------------------------------------------------
+# Ax kernel in isolation: oracle arrays in, compare to the oracle's OKL restatement
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 7, 8, 11, 15])
+def test_ax_kernel_f64(N):
+    n = 3 if N <= 8 else 2
+    o = OracleElliptic(box(n, N, curve=0.05, lam=0.7))
+    rng = np.random.default_rng(N)
+    q = rng.uniform(-1, 1, o.geti("Ngather"))
+    ref = o.partial_ax(q)
+    E = o.geti("Nelements")
+    g2l, wJ, ggeo, D = dev(o.array("GlobalToLocal")), dev(o.array("wJ")), dev(o.array("ggeo")), dev(o.array("D"))
+    qd = dev(q)
+    out = torch.full((ref.size,), np.nan, dtype=torch.float64, device="cuda")
+    st = L().libp_ax_hex_partial_f64(N, E, None, g2l.data_ptr(), wJ.data_ptr(), ggeo.data_ptr(), D.data_ptr(), 0.7,
+                                     qd.data_ptr(), out.data_ptr(), stream())
+    assert st == 0
+    torch.cuda.synchronize()
+    assert relerr(out.cpu().numpy(), ref) < 1e-12
+
+
+def test_ax_kernel_element_list_and_tail():
+    # odd element count (tail launch for NE>1) and an explicit element list
+    N = 2
+    o = OracleElliptic(box(4, N, lam=1.3))
+    q = np.random.default_rng(1).uniform(-1, 1, o.geti("Ngather"))
+    ref = o.partial_ax(q)
+    E = o.geti("Nelements")
+    Np = (N + 1) ** 3
+    sub = np.arange(E - 3, dtype=np.int32)[::-1].copy()  # all but the last 3, reversed
+    args = [dev(o.array(k)) for k in ("GlobalToLocal", "wJ", "ggeo", "D")]
+    qd = dev(q)
+    out = torch.zeros(ref.size, dtype=torch.float64, device="cuda")
+    subd = dev(sub)
+    st = L().libp_ax_hex_partial_f64(N, len(sub), subd.data_ptr(), *[a.data_ptr() for a in args], 1.3, qd.data_ptr(),
+                                     out.data_ptr(), stream())
+    assert st == 0
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().reshape(E, Np)
+    refe = ref.reshape(E, Np)
+    assert relerr(got[sub], refe[sub]) < 1e-12
+    assert np.all(got[E - 3:] == 0)
+
+
+def test_ax_kernel_f32():
+    N = 7
+    o = OracleElliptic(box(3, N, curve=0.05))
+    q = np.random.default_rng(2).uniform(-1, 1, o.geti("Ngather"))
+    ref = o.partial_ax(q)
+    E = o.geti("Nelements")
+    f = lambda k: dev(o.array(k).astype(np.float32))
+    g2l = dev(o.array("GlobalToLocal"))
+    wJ, ggeo, D = f("wJ"), f("ggeo"), f("D")
+    qd = dev(q.astype(np.float32))
+    out = torch.zeros(ref.size, dtype=torch.float32, device="cuda")
+    st = L().libp_ax_hex_partial_f32(N, E, None, g2l.data_ptr(), wJ.data_ptr(), ggeo.data_ptr(), D.data_ptr(), 1.0,
+                                     qd.data_ptr(), out.data_ptr(), stream())
+    assert st == 0
+    torch.cuda.synchronize()
+    assert relerr(out.cpu().numpy().astype(np.float64), ref) < 2e-5
+
+
+def test_ax_bad_degree_rejected():
+    st = L().libp_ax_hex_partial_f64(16, 1, None, None, None, None, None, 0.0, None, None, None)
+    assert st != 0
+
+
+# ---------------------------------------------------------------------------
+# gather / scatter: bit-exact
+# ---------------------------------------------------------------------------
+def test_gather_bitexact():
+    o = OracleElliptic(box(3, 4, curve=0.03))
+    rng = np.random.default_rng(5)
+    vL = rng.standard_normal(o.geti("Nelements") * o.geti("Np"))
+    for trans, rsn, cin in ((True, "rowStartsT", "colIdsT"), (False, "rowStartsN", "colIdsN")):
+        ref = o.gather(vL, trans)
+        rs, ci = dev(o.array(rsn)), dev(o.array(cin))
+        v = dev(vL)
+        gv = torch.zeros(ref.size, dtype=torch.float64, device="cuda")
+        assert L().libp_ogs_gather_f64(ref.size, rs.data_ptr(), ci.data_ptr(), v.data_ptr(), gv.data_ptr(),
+                                       stream()) == 0
+        torch.cuda.synchronize()
+        assert np.array_equal(gv.cpu().numpy(), ref)
+
+
+def test_scatter_bitexact():
+    o = OracleElliptic(box(3, 3))
+    g = np.random.default_rng(6).standard_normal(o.geti("Ngather"))
+    ref = o.scatter(g)
+    rs, ci = dev(o.array("rowStartsT")), dev(o.array("colIdsT"))
+    gd = dev(g)
+    v = torch.zeros(ref.size, dtype=torch.float64, device="cuda")
+    assert L().libp_ogs_scatter_f64(g.size, rs.data_ptr(), ci.data_ptr(), gd.data_ptr(), v.data_ptr(), stream()) == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(v.cpu().numpy(), ref)
+
+
+# ---------------------------------------------------------------------------
+# product setup vs oracle: index maps bit-exact, geometry to rounding
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n,N,flag,curve", [(3, 4, 1, 0.0), (4, 2, 1, 0.05), (3, 3, -1, 0.0), (2, 7, 1, 0.05),
+                                            (3, 1, 1, 0.0)])
+def test_setup_parity(n, N, flag, curve):
+    import libparanumal_amd as lp
+    s = box(n, N, flag, curve, PRECONDITIONER="JACOBI")
+    o = OracleElliptic(s)
+    p = lp.Elliptic(s)
+    assert p.geti("Ngather") == o.geti("Ngather")
+    for name in ("globalIds", "maskedGlobalIds", "GlobalToLocal", "rowStartsT", "colIdsT", "rowStartsN", "colIdsN",
+                 "blockRowStartsT", "blockRowStartsN", "mapB"):
+        a, b = p.host_array(name), o.array(name)
+        assert a.dtype.itemsize == b.dtype.itemsize or name in ("mapB",)
+        assert np.array_equal(a.astype(np.int64), b.astype(np.int64)), name
+    for name, tol in (("x", 1e-14), ("y", 1e-14), ("z", 1e-14), ("wJ", 1e-13), ("ggeo", 1e-12), ("D", 1e-13),
+                      ("gllz", 1e-15), ("gllw", 1e-14), ("weightG", 0.0)):
+        a, b = p.host_array(name), o.array(name)
+        assert relerr(a, b) <= tol, name
+    assert relerr(1.0 / p.host_array("invDiag"), o.diagonal()) < 1e-12
+
+
+def test_operator_parity():
+    import libparanumal_amd as lp
+    s = box(3, 5, 1, 0.05, lam=0.3)
+    o = OracleElliptic(s)
+    p = lp.Elliptic(s)
+    q = np.random.default_rng(7).uniform(-1, 1, o.geti("Ngather"))
+    ref = o.operator(q)
+    qd = dev(q)
+    Aq = torch.zeros(ref.size, dtype=torch.float64, device="cuda")
+    p.operator(qd, Aq)
+    torch.cuda.synchronize()
+    lp.hip_d2h(0, 0, 0, p.stream) if False else None
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipStreamSynchronize(ctypes.c_void_p(p.stream))
+    assert relerr(Aq.cpu().numpy(), ref) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# end-to-end: the reference's known answers and iteration parity
+# ---------------------------------------------------------------------------
+REF = {"MESH FILE": "BOX", "MESH DIMENSION": 3, "ELEMENT TYPE": 12, "BOX NX": 10, "BOX NY": 10, "BOX NZ": 10,
+       "BOX DIMX": 1, "BOX DIMY": 1, "BOX DIMZ": 1, "BOX BOUNDARY FLAG": 1, "POLYNOMIAL DEGREE": 4,
+       "DATA FILE": "data/ellipticSine3D.h", "DISCRETIZATION": "CONTINUOUS", "LINEAR SOLVER": "PCG",
+       "PRECONDITIONER": "NONE"}
+
+
+@pytest.mark.parametrize("precon,flag,refnorm", [("NONE", 1, 0.353553390458384), ("JACOBI", 1, 0.353553400508458),
+                                                 ("NONE", -1, 0.059540839002614)])
+def test_run_known_answers(precon, flag, refnorm):
+    import libparanumal_amd as lp
+    s = dict(REF, PRECONDITIONER=precon, **{"BOX BOUNDARY FLAG": flag})
+    onorm, oit, _ = oracle_lib.run(s)
+    r = lp.Elliptic(s).run()
+    assert abs(r["norm"] - refnorm) < 1e-5          # harness tolerance (test/test.py:63)
+    assert abs(r["norm"] - onorm) < 1e-10           # against the oracle
+    assert abs(r["iterations"] - oit) <= 1          # PCG iteration parity
+
+
+def test_pcg_jacobi_c1_iteration_parity():
+    # configuration 1: 8^3 box on [-1,1]^3, N=4, lambda=0, Dirichlet, PCG+Jacobi
+    import libparanumal_amd as lp
+    s = dict(REF, PRECONDITIONER="JACOBI", LAMBDA=0, **{"BOX NX": 8, "BOX NY": 8, "BOX NZ": 8, "BOX DIMX": 2,
+                                                         "BOX DIMY": 2, "BOX DIMZ": 2})
+    onorm, oit, ng = oracle_lib.run(s)
+    assert ng == 29791
+    r = lp.Elliptic(s).run()
+    assert abs(r["iterations"] - oit) <= 1
+    assert abs(r["norm"] - onorm) < 1e-9
+    assert r["error_l2"] < 1e-3
+
+
+# ---------------------------------------------------------------------------
+# p-multigrid
+# ---------------------------------------------------------------------------
+def mg_settings(prec, **kw):
+    s = box(3, 7, 1, 0.05, lam=1.0, PRECONDITIONER="MULTIGRID", **{"LINEAR SOLVER": "FPCG",
+        "MULTIGRID COARSENING": "CUSTOM", "MULTIGRID DEGREES": "7,3,1", "MULTIGRID PRECISION": prec,
+        "MULTIGRID COARSE TOLERANCE": 0.0, "MULTIGRID COARSE MAXIT": 15})
+    s.update(kw)
+    return s
+
+
+def test_mg_vcycle_fp64_vs_oracle():
+    import libparanumal_amd as lp
+    s = mg_settings("FP64")
+    o = OracleElliptic(s)
+    o.mg_setup()
+    p = lp.Elliptic(s)
+    r = np.random.default_rng(11).standard_normal(o.geti("Ngather"))
+    ref = o.mg_apply(r)
+    rd = dev(r)
+    z = torch.zeros_like(rd)
+    p.precon(rd, z)
+    import ctypes
+    ctypes.CDLL("libamdhip64.so").hipStreamSynchronize(ctypes.c_void_p(p.stream))
+    assert relerr(z.cpu().numpy(), ref) < 1e-9
+
+
+def test_mg_vcycle_fp32_close_to_fp64_oracle():
+    import libparanumal_amd as lp
+    s = mg_settings("FP32")
+    o = OracleElliptic(s)
+    o.mg_setup()
+    p = lp.Elliptic(s)
+    r = np.random.default_rng(12).standard_normal(o.geti("Ngather"))
+    ref = o.mg_apply(r)
+    rd = dev(r)
+    z = torch.zeros_like(rd)
+    p.precon(rd, z)
+    import ctypes
+    ctypes.CDLL("libamdhip64.so").hipStreamSynchronize(ctypes.c_void_p(p.stream))
+    assert relerr(z.cpu().numpy(), ref) < 1e-4
+
+
+@pytest.mark.parametrize("prec", ["FP32", "FP64"])
+def test_fpcg_pmg_solve_reaches_relative_tolerance(prec):
+    import libparanumal_amd as lp
+    s = mg_settings(prec, **{"LINEAR SOLVER STOPPING CRITERION": "RELATIVE", "MULTIGRID COARSE TOLERANCE": 1e-2,
+                             "MULTIGRID COARSE MAXIT": 100, "BOX NX": 4})
+    r = lp.Elliptic(s).run()
+    assert r["r_final"] <= 1e-8 * r["r0"] * (1 + 1e-12)
+    assert r["iterations"] < 40
+    assert r["error_l2"] < 1e-4

@@ -118,7 +118,10 @@ void mesh_t::SetupBox() {
   LIBP_ABORT("BOX NX/NY/NZ must be positive", nx <= 0 || ny <= 0 || nz <= 0);
   settings.getSetting("BOX BOUNDARY FLAG", boundaryFlag);
   periodic = (boundaryFlag == -1);
-  LIBP_ABORT("periodic boxes need at least 2 elements per direction", periodic && (nx < 2 || ny < 2 || nz < 2));
+  // With two elements in a periodic direction both elements carry the same
+  // eight vertex ids, so face matching by vertex set (meshConnect.cpp:85-110)
+  // cannot tell their faces apart; the reference then merges unrelated nodes.
+  LIBP_ABORT("periodic boxes need at least 3 elements per direction", periodic && (nx < 3 || ny < 3 || nz < 3));
   LIBP_ABORT("BOX BOUNDARY FLAG must be -1, 0, 1 or 2", boundaryFlag < -1 || boundaryFlag > 2);
   double DIMX, DIMY, DIMZ;
   settings.getSetting("BOX DIMX", DIMX);

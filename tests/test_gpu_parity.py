@@ -135,7 +135,7 @@ def test_scatter_bitexact():
 # ---------------------------------------------------------------------------
 # product setup vs oracle: index maps bit-exact, geometry to rounding
 # ---------------------------------------------------------------------------
-@pytest.mark.parametrize("n,N,flag,curve", [(3, 4, 1, 0.0), (4, 2, 1, 0.05), (3, 3, -1, 0.0), (2, 7, 1, 0.05),
+@pytest.mark.parametrize("n,N,flag,curve", [(3, 4, 1, 0.0), (4, 2, 1, 0.05), (4, 3, -1, 0.0), (2, 7, 1, 0.05),
                                             (3, 1, 1, 0.0)])
 def test_setup_parity(n, N, flag, curve):
     import libparanumal_amd as lp
@@ -258,4 +258,7 @@ def test_fpcg_pmg_solve_reaches_relative_tolerance(prec):
     r = lp.Elliptic(s).run()
     assert r["r_final"] <= 1e-8 * r["r0"] * (1 + 1e-12)
     assert r["iterations"] < 40
-    assert r["error_l2"] < 1e-4
+    # same discrete solution as the all-FP64 oracle (Jacobi-PCG to a tight tolerance)
+    onorm, _, _ = oracle_lib.run(dict(s, PRECONDITIONER="JACOBI", **{"LINEAR SOLVER": "PCG",
+                                                                      "LINEAR SOLVER TOLERANCE": 1e-12}))
+    assert abs(r["norm"] - onorm) < 1e-8 * onorm

@@ -60,10 +60,13 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nx", type=int, default=48)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also time a 1 GiB device-to-device copy (achievable HBM bandwidth; "
+                         "the reference kernel for scaling FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -132,6 +135,22 @@ def main():
         "gather": {"ms": ms_gather, "GBs": bytes_gather / (ms_gather * 1e-3) / 1e9},
         "setup_seconds": setup_s,
     }
+    if args.copy_probe:
+        nbytes = 1 << 30
+        src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").uniform_()
+        dst = torch.empty_like(src)
+        cs = torch.cuda.current_stream()
+        for _ in range(3):
+            dst.copy_(src)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(cs)
+        for _ in range(10):
+            dst.copy_(src)
+        b.record(cs)
+        b.synchronize()
+        ms = a.elapsed_time(b) / 10
+        result["hbm_copy_probe"] = {"bytes_moved": 2 * nbytes, "ms": ms, "GBs": 2 * nbytes / (ms * 1e-3) / 1e9}
+        del src, dst
     if not args.no_solve:
         r = el.run()
         result["solve"] = {"solver": "PCG+Jacobi, stop |r|<=max(1e-8|r0|,1e-8)", "iterations": r["iterations"],

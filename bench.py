@@ -32,6 +32,62 @@ def workload_settings(nx=48, N=7, precon="JACOBI"):
             "LINEAR SOLVER": "PCG", "PRECONDITIONER": precon}
 
 
+def pmg_settings(nx=32, precision="FP32", smoother="CHEBYSHEV", lam=1, data="data/ellipticSine3D.h", seed=1):
+    """Configuration 3 (and 5 with the Fourier data): curved 32^3 box, N=7, FPCG with p-MG 7->3->1,
+    degree-2 Chebyshev in `precision`, FP64 Jacobi-PCG coarse level, pure relative 1e-8 stop."""
+    return {"MESH FILE": "BOX", "MESH DIMENSION": 3, "ELEMENT TYPE": 12, "BOX NX": nx, "BOX NY": nx, "BOX NZ": nx,
+            "BOX DIMX": 2, "BOX DIMY": 2, "BOX DIMZ": 2, "BOX BOUNDARY FLAG": 1, "POLYNOMIAL DEGREE": 7,
+            "BOX CURVED MAP AMPLITUDE": 0.05, "LAMBDA": lam, "DATA FILE": data, "DATA SEED": seed,
+            "DISCRETIZATION": "CONTINUOUS", "LINEAR SOLVER": "FPCG", "PRECONDITIONER": "MULTIGRID",
+            "MULTIGRID COARSENING": "CUSTOM", "MULTIGRID DEGREES": "7,3,1", "MULTIGRID SMOOTHER": smoother,
+            "MULTIGRID CHEBYSHEV DEGREE": 2, "MULTIGRID PRECISION": precision,
+            "MULTIGRID COARSE TOLERANCE": 1e-2, "MULTIGRID COARSE MAXIT": 200,
+            "LINEAR SOLVER STOPPING CRITERION": "RELATIVE", "LINEAR SOLVER TOLERANCE": 1e-8}
+
+
+def pmg_solve(lp, settings, reps=2):
+    """Time to 1e-8 relative residual (best of `reps` solves on one setup) and the solution E-vector."""
+    el = lp.Elliptic(settings)
+    best = None
+    for _ in range(reps):
+        r = el.run()
+        if best is None or r["seconds"] < best["seconds"]:
+            best = r
+    x = el.solution()
+    info = {"iterations": best["iterations"], "seconds": best["seconds"], "r0": best["r0"],
+            "r_final": best["r_final"], "solution_norm": best["norm"], "gathered_dofs": el.geti("Ngather"),
+            "coarse_iterations_last": el.geti("coarseIterations")}
+    el.close()
+    return info, x
+
+
+def ax_rate(lp, settings, steps=20, warmup=5):
+    """GDOF/s and algorithmic HBM rate of the partial Ax for a settings dict."""
+    import torch
+    el = lp.Elliptic(settings)
+    E, Np, Ng, N = el.geti("Nelements"), el.geti("Np"), el.geti("Ngather"), el.geti("N")
+    NL = E * Np
+    stream = torch.cuda.ExternalStream(el.stream)
+    q = torch.rand(Ng, dtype=torch.float64, device="cuda")
+    AqL = torch.empty(NL, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    for _ in range(warmup):
+        el.partial_ax(q, AqL)
+    stream.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(steps):
+        el.partial_ax(q, AqL)
+    b.record(stream)
+    b.synchronize()
+    ms = a.elapsed_time(b) / steps
+    el.close()
+    flops = NL * (12 * (N + 1) + 18)
+    return {"ms": ms, "gdofs": Ng / (ms * 1e-3) / 1e9, "GBs": (68 * NL + 8 * Ng) / (ms * 1e-3) / 1e9,
+            "fp64_TFs": flops / (ms * 1e-3) / 1e12, "fp64_frac_of_peak": flops / (ms * 1e-3) / 78.6e12,
+            "elements": E, "N": N, "gathered_dofs": Ng}
+
+
 def cpu_baseline(reps=20):
     """Oracle partial Ax (serial, 1 thread) on a 16^3 x N=7 sample of the workload."""
     from tests import oracle_lib
@@ -64,6 +120,7 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nx", type=int, default=48)
+    ap.add_argument("--no-extra", action="store_true", help="skip the p-MG (32^3) and N=15 (16^3) measurements")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also time a 1 GiB device-to-device copy (achievable HBM bandwidth; "
                          "the reference kernel for scaling FETCH_SIZE/WRITE_SIZE counters)")
@@ -157,6 +214,21 @@ def main():
                            "seconds": r["seconds"], "r0": r["r0"], "r_final": r["r_final"],
                            "error_l2_vs_exact": r["error_l2"], "solution_norm": r["norm"]}
     el.close()
+    if not args.no_extra:
+        # configuration 4: 16^3 elements, N=15, curved, lambda=1
+        n15 = {"MESH FILE": "BOX", "MESH DIMENSION": 3, "ELEMENT TYPE": 12, "BOX NX": 16, "BOX NY": 16,
+               "BOX NZ": 16, "BOX DIMX": 2, "BOX DIMY": 2, "BOX DIMZ": 2, "BOX BOUNDARY FLAG": 1,
+               "POLYNOMIAL DEGREE": 15, "BOX CURVED MAP AMPLITUDE": 0.05, "LAMBDA": 1,
+               "DATA FILE": "data/ellipticSine3D.h", "PRECONDITIONER": "JACOBI"}
+        result["ax_n15"] = ax_rate(lp, n15)
+        # configuration 3: FPCG + p-MG 7->3->1 in FP32 and FP64, same problem
+        import numpy as np
+        info32, x32 = pmg_solve(lp, pmg_settings(precision="FP32"))
+        info64, x64 = pmg_solve(lp, pmg_settings(precision="FP64"))
+        diff = float(np.max(np.abs(x32 - x64)) / np.max(np.abs(x64)))
+        result["pmg_c3"] = {"fp32_smoother": info32, "fp64_smoother": info64,
+                            "max_rel_diff_fp32_vs_fp64_solution": diff,
+                            "stop": "|r| <= 1e-8 |r0| (pure relative)"}
     if not args.no_cpu:
         try:
             result["cpu_baseline"] = cpu_baseline()

@@ -84,6 +84,12 @@ void elliptic_t::BoundarySetup() {
 }
 
 void elliptic_t::PartialAx(deviceMemory<dfloat>& o_q, deviceMemory<dfloat>& o_AqL_) {
+  if (mesh.N == 15) {  // 16x16 contractions on FP64 MFMA
+    LIBP_CALL(libp_ax_hex_partial_mfma_n15(mesh.NlocalGatherElements, nullptr, o_GlobalToLocal.ptr(), mesh.o_wJ.ptr(),
+                                           mesh.o_ggeo.ptr(), mesh.o_D.ptr(), lambda, o_q.ptr(), o_AqL_.ptr(),
+                                           platform.s()));
+    return;
+  }
   LIBP_CALL(libp_ax_hex_partial_f64(mesh.N, mesh.NlocalGatherElements, nullptr, o_GlobalToLocal.ptr(),
                                     mesh.o_wJ.ptr(), mesh.o_ggeo.ptr(), mesh.o_D.ptr(), lambda, o_q.ptr(),
                                     o_AqL_.ptr(), platform.s()));

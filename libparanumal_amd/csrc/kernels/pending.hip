@@ -2,11 +2,6 @@
 // LIBP_ERR_UNSUPPORTED instead of computing anything.
 #include "kcommon.hpp"
 
-extern "C" int libp_ax_hex_partial_mfma_n15(int, const int*, const int*, const double*, const double*, const double*,
-                                            double, const double*, double*, void*) {
-  libp_amd::set_last_error("ax_hex_partial_mfma_n15", "not built");
-  return LIBP_ERR_UNSUPPORTED;
-}
 extern "C" int libp_mlp_linear_fwd(int, int, int, const float*, const unsigned short*, const float*, int, float*,
                                    void*) {
   libp_amd::set_last_error("mlp_linear_fwd", "not built");

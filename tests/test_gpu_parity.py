@@ -56,6 +56,24 @@ def test_ax_kernel_f64(N):
     assert relerr(out.cpu().numpy(), ref) < 1e-12
 
 
+@pytest.mark.parametrize("use_list", [False, True])
+def test_ax_mfma_n15(use_list):
+    o = OracleElliptic(box(2, 15, curve=0.05, lam=0.9))
+    rng = np.random.default_rng(15)
+    q = rng.uniform(-1, 1, o.geti("Ngather"))
+    ref = o.partial_ax(q)
+    E = o.geti("Nelements")
+    g2l, wJ, ggeo, D = dev(o.array("GlobalToLocal")), dev(o.array("wJ")), dev(o.array("ggeo")), dev(o.array("D"))
+    qd = dev(q)
+    out = torch.full((ref.size,), np.nan, dtype=torch.float64, device="cuda")
+    lst = dev(np.arange(E, dtype=np.int32)[::-1].copy()) if use_list else None
+    st = L().libp_ax_hex_partial_mfma_n15(E, lst.data_ptr() if use_list else None, g2l.data_ptr(), wJ.data_ptr(),
+                                          ggeo.data_ptr(), D.data_ptr(), 0.9, qd.data_ptr(), out.data_ptr(), stream())
+    assert st == 0
+    torch.cuda.synchronize()
+    assert relerr(out.cpu().numpy(), ref) < 1e-12
+
+
 def test_ax_kernel_element_list_and_tail():
     # odd element count (tail launch for NE>1) and an explicit element list
     N = 2

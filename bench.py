@@ -57,6 +57,10 @@ def pmg_solve(lp, settings, reps=2):
     info = {"iterations": best["iterations"], "seconds": best["seconds"], "r0": best["r0"],
             "r_final": best["r_final"], "solution_norm": best["norm"], "gathered_dofs": el.geti("Ngather"),
             "coarse_iterations_last": el.geti("coarseIterations")}
+    if settings.get("MULTIGRID SMOOTHER") == "LEARNED":
+        li = el.learned_info()
+        info["learned"] = {"active": bool(li["active"]), "fallback": bool(li["fallback"]),
+                           "rejected_steps": int(li["rejected_steps"]), "status": el.learned_status()}
     el.close()
     return info, x
 
@@ -110,6 +114,45 @@ def cpu_baseline(reps=20):
     return {"value": ng / t / 1e9, "unit": "GDOF/s", "cores": 1, "kind": "port",
             "sample": f"oracle partial Ax, 16^3 elements N=7 ({ng} gathered dofs), median of {reps} applications, "
                       f"serial (1 thread) on {cpu}; setup {setup:.1f}s"}
+
+
+LEARNED_WEIGHTS = os.path.join(ROOT, "libparanumal_amd", "weights", "learned_hex_n7.bin")
+
+
+def learned_ab(lp, nx=32, seed=1):
+    """Configuration 5: time to 1e-8 (pure relative) with the learned fine smoother
+    against the plain FP32 Chebyshev smoother, same problem (curved nx^3, N=7,
+    lambda=0, Fourier forcing of a test seed the network was not trained on),
+    and each solution's distance to an all-FP64 solve to 1e-12."""
+    import numpy as np
+    if not os.path.exists(LEARNED_WEIGHTS):
+        return {"error": "no weight file " + os.path.relpath(LEARNED_WEIGHTS, ROOT)}
+    base = pmg_settings(nx=nx, lam=0, data="ellipticFourier3D", seed=seed)
+    plain, xp = pmg_solve(lp, base)
+    learned, xl = pmg_solve(lp, dict(base, **{"MULTIGRID SMOOTHER": "LEARNED",
+                                              "MULTIGRID LEARNED WEIGHTS": LEARNED_WEIGHTS}))
+    ref, xr = pmg_solve(lp, dict(base, **{"MULTIGRID PRECISION": "FP64", "LINEAR SOLVER TOLERANCE": 1e-12}), reps=1)
+    scale = float(np.max(np.abs(xr)))
+    return {"workload": f"curved {nx}^3 box, N=7, lambda=0, Fourier forcing seed {seed} (training seeds 101-104, "
+                        f"201-202), FPCG + p-MG 7,3,1 FP32, stop |r| <= 1e-8 |r0|",
+            "plain_chebyshev": plain, "learned": learned,
+            "speedup_learned_vs_plain": plain["seconds"] / learned["seconds"],
+            "max_rel_error_vs_fp64_plain": float(np.max(np.abs(xp - xr))) / scale,
+            "max_rel_error_vs_fp64_learned": float(np.max(np.abs(xl - xr))) / scale,
+            "fp64_reference": {"iterations": ref["iterations"], "seconds": ref["seconds"]}}
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ax_hex_n7_48cubed_traffic.json")
+
+
+def measured_traffic(nx):
+    """HBM bytes per Ax launch from the committed FETCH_SIZE/WRITE_SIZE summary
+    (two rocprofv3 --pmc runs of this script, calibrated by a 1 GiB copy), for
+    the 48^3 workload it was measured on; None otherwise."""
+    if nx != 48 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        return json.load(f)["traffic_bytes"]
 
 
 def main():
@@ -186,7 +229,7 @@ def main():
         "config": {"workload": f"Poisson hex box {args.nx}^3 elements, N=7, affine, Dirichlet, FP64 partial Ax",
                    "elements": E, "N": 7, "local_nodes": NL, "gathered_dofs": Ng},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs / HBM_PEAK_GBS, "traffic": None},
+                     "frac": gbs / HBM_PEAK_GBS, "traffic": measured_traffic(args.nx)},
         "ax_gflops": flops_ax / (ms_ax * 1e-3) / 1e9,
         "operator": {"ms": ms_op, "gdofs": Ng / (ms_op * 1e-3) / 1e9},
         "gather": {"ms": ms_gather, "GBs": bytes_gather / (ms_gather * 1e-3) / 1e9},
@@ -229,6 +272,8 @@ def main():
         result["pmg_c3"] = {"fp32_smoother": info32, "fp64_smoother": info64,
                             "max_rel_diff_fp32_vs_fp64_solution": diff,
                             "stop": "|r| <= 1e-8 |r0| (pure relative)"}
+        # configuration 5: learned fine smoother against plain FP32 Chebyshev
+        result["pmg_c5_learned"] = learned_ab(lp)
     if not args.no_cpu:
         try:
             result["cpu_baseline"] = cpu_baseline()

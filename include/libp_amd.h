@@ -120,9 +120,14 @@ int libp_sum_f64(int N, const double* x, double* result, void* ws, void* stream)
 /* PCG / FPCG fused vector kernels (libs/linearSolver)                  */
 /* ------------------------------------------------------------------ */
 /* Device-resident scalar block used by the fused PCG kernels (doubles):
- *  [0] rdotz1  [1] rdotz2  [2] alpha  [3] beta  [4] pAp  [5] rdotr  [6] zdotAp  [7] TOL */
+ *  [0] rdotz1  [1] rdotz2  [2] alpha  [3] beta  [4] pAp  [5] rdotr  [6] zdotAp  [7] TOL
+ *  [8] STOP  [9] ITERS.
+ * libp_pcg_update_f64 counts ITERS and sets STOP once rdotr <= TOL (TOL > 0
+ * only); every fused PCG kernel returns at once while STOP is set. Iterations
+ * can so be queued in chunks and the host read the scalars once per chunk,
+ * with the same arithmetic as a convergence test after every iteration. */
 enum { LIBP_PCG_RDOTZ1 = 0, LIBP_PCG_RDOTZ2, LIBP_PCG_ALPHA, LIBP_PCG_BETA, LIBP_PCG_PAP,
-       LIBP_PCG_RDOTR, LIBP_PCG_ZDOTAP, LIBP_PCG_TOL, LIBP_PCG_NSCALARS };
+       LIBP_PCG_RDOTR, LIBP_PCG_ZDOTAP, LIBP_PCG_TOL, LIBP_PCG_STOP, LIBP_PCG_ITERS, LIBP_PCG_NSCALARS };
 
 /* Replaces updatePCG (libs/linearSolver/okl/linearSolverUpdatePCG.okl:29-75) and the
  * host sum of pcg::UpdatePCG (linearSolverPCG.cpp:153-171): alpha = s[RDOTZ1]/s[PAP]
@@ -206,19 +211,63 @@ int libp_addbc_hex_f64(int NL, int dataKind, const double* coeffs, const double*
 /* ------------------------------------------------------------------ */
 /* Learned per-element smoother (no reference counterpart)              */
 /* ------------------------------------------------------------------ */
-/* BF16-weight MLP, FP32 accumulate, hand-written MFMA GEMM tiles for gfx950:
- * Y[M,Nout] = act(X[M,K] * W[K,Nout] + b), with X, Y FP32 and W BF16 (row-major).
- * act: 0 = identity, 1 = ReLU. M is any size; K and Nout multiples of 32. */
-int libp_mlp_linear_fwd(int M, int K, int Nout, const float* X, const unsigned short* Wbf16, const float* b,
-                        int act, float* Y, void* stream);
-/* Backward of the same layer: given dY (FP32) and the layer output Y (for the ReLU
- * mask), computes dX = (dY.*mask) W^T (if dX != NULL), dW = X^T (dY.*mask) and
- * db = colsum(dY.*mask) in FP32. */
-int libp_mlp_linear_bwd(int M, int K, int Nout, const float* X, const unsigned short* Wbf16, const float* Y,
-                        const float* dY, int act, float* dX, float* dW, float* db, void* stream);
-/* Adam update of FP32 master weights with BF16 copy: m,v moments, step t (1-based). */
+/* None of these entry points has a counterpart in the reference: the
+ * reference has no learned components. The network is the per-element MLP
+ * nIn = Np + 6*Nfp -> 256 -> 256 -> Np with ReLU, BF16 weights and FP32
+ * accumulation on v_mfma_f32_32x32x16_bf16. Matrices are row-major with the
+ * given leading dimensions (in elements); BF16 values are passed as
+ * unsigned short bit patterns. */
+
+/* One layer forward: Y[M x Nout] = act(X[M x K] Wt^T + b), X BF16 [M][ldx],
+ * Wt BF16 [Nout][ldw] (PyTorch's Linear.weight layout), b FP32 [Nout] or NULL.
+ * relu = 1: Y is BF16 = bf16(max(., 0)); relu = 0: Y is FP32 (no activation). */
+int libp_mlp_linear_fwd(int M, int K, int Nout, const unsigned short* X, long ldx, const unsigned short* Wt,
+                        long ldw, const float* b, int relu, void* Y, long ldy, void* stream);
+/* Backward through the weights: dX[M x K] = bf16((dY Wt) .* (H > 0)), dY BF16
+ * [M][lddy], H BF16 [M][ldh] (the layer input after its ReLU) or NULL for no mask. */
+int libp_mlp_linear_bwd_data(int M, int K, int Nout, const unsigned short* dY, long lddy, const unsigned short* Wt,
+                             long ldw, const unsigned short* H, long ldh, unsigned short* dX, long lddx,
+                             void* stream);
+/* Weight gradients: dWt[Nout x K] = dY^T X (FP32, [Nout][K] dense) and
+ * db[Nout] = column sums of dY (FP32, or NULL). Split-K over M with FP32
+ * partials summed in a fixed order; ws holds libp_mlp_ws_bytes(M, K, Nout). */
+int libp_mlp_linear_bwd_weight(int M, int K, int Nout, const unsigned short* dY, long lddy, const unsigned short* X,
+                               long ldx, float* dWt, float* db, void* ws, void* stream);
+long libp_mlp_ws_bytes(int M, int K, int Nout);
+/* Adam (Kingma & Ba 2015) on FP32 masters, writing a BF16 copy (wbf16 may be
+ * NULL); step counts from 1. */
 int libp_adam_step(int n, float* master, unsigned short* wbf16, const float* grad, float* m, float* v, float lr,
-                   float beta1, float beta2, float eps, int t, void* stream);
+                   float beta1, float beta2, float eps, int step, void* stream);
+int libp_f32_to_bf16(long n, const float* x, unsigned short* y, void* stream);
+
+/* Network input: X[e][0:Np] = z at the element's nodes, X[e][Np:Np+Ntr] = z at
+ * traceIds[e][:] (gathered ids, -1 = none), z = invDiag .* r, each row divided
+ * by its RMS, which is written to scale[e] (0 for an all-zero row). */
+int libp_learned_features_f32(int Nelements, int Np, int Ntr, const int* GlobalToLocal, const int* traceIds,
+                              const float* r, const float* invDiag, unsigned short* X, long ldx, float* scale,
+                              void* stream);
+/* c[g] = (1/|row g|) sum_{j in row g} scale[j/Np] Y[j/Np][j%Np] + omega invDiag[g] r[g],
+ * rows of the masked gather (rowStarts, colIds); omega is a device pointer (or NULL = 0). */
+int libp_learned_assemble_f32(int Nrows, const int* rowStarts, const int* colIds, int Np, const float* Y, long ldy,
+                              const float* scale, const float* omega, const float* r, const float* invDiag, float* c,
+                              void* stream);
+/* Energy-optimal step: sc[0] = c.res, sc[1] = c.Ac, sc[2] = alpha = sc[0]/sc[1]
+ * (0, and sc[3] += 1, if not finite or c.Ac <= 0). ws: libp_reduce_ws_bytes(). */
+int libp_learned_alpha_f32(int N, const float* c, const float* res, const float* Ac, double* sc, void* ws,
+                           void* stream);
+/* x = (xIsZero ? 0 : x) + sc[2] c; res -= sc[2] Ac when res != NULL. */
+int libp_learned_update_f32(int N, int xIsZero, const double* sc, const float* c, const float* Ac, float* x,
+                            float* res, void* stream);
+/* Training loss J = d.Ad / *nrm of one sample (d = target - c, Ad = A d) and
+ * gc = dJ/dc = -2 Ad / *nrm; stat[0] = J, stat[1] += J; *gradOmega = gc.(invDiag.*r). */
+int libp_learned_loss_f32(int N, const float* d, const float* Ad, const float* r, const float* invDiag,
+                          const double* nrm, float* gc, double* stat, float* gradOmega, void* ws, void* stream);
+/* dY[e][n] = bf16(gc[g] scale[e] invLen[g]) at g = GlobalToLocal[e*Np+n] >= 0, else 0. */
+int libp_learned_scatter_grad_f32(int Nelements, int Np, const int* GlobalToLocal, const float* gc,
+                                  const float* invLen, const float* scale, unsigned short* dY, long lddy,
+                                  void* stream);
+/* invLen[g] = 1 / (rowStarts[g+1] - rowStarts[g]). */
+int libp_inv_row_length_f32(int Nrows, const int* rowStarts, float* invLen, void* stream);
 
 /* ------------------------------------------------------------------ */
 /* Solver handle: the C++ elliptic_t driven through the C ABI           */
@@ -250,6 +299,31 @@ int libp_elliptic_precon(void* h, const double* r, double* z);
 int libp_elliptic_run(void* h, double* out, int nout);
 /* Copy of the solution E-vector (host buffer of Nelements*Np doubles) from the last run. */
 int libp_elliptic_solution(void* h, double* xL_host);
+
+/* Learned fine smoother of a handle built with MULTIGRID SMOOTHER LEARNED and
+ * MULTIGRID PRECISION FP32 (no reference counterpart).
+ * Training: options "key=value" separated by blanks (seeds=101,102
+ * daggerSeeds=201 steps=3000 dagger=1 daggerSteps=1500 lr=1e-3 targetTol=1e-9
+ * initSeed=1 logEvery=250 maxRecord=24 verbose=1). Returns the number of loss
+ * values (mean loss per logEvery steps, up to nloss copied), or -status. */
+int libp_elliptic_learned_train(void* h, const char* options, double* loss, int nloss);
+/* Weight files: header + FP32 parameters + FNV-1a hash; a rejected file leaves
+ * the Chebyshev smoother in use and returns LIBP_ERR_SETUP (reason in
+ * libp_last_error and libp_elliptic_learned_status). */
+int libp_elliptic_learned_load(void* h, const char* path);
+int libp_elliptic_learned_save(void* h, const char* path);
+/* out: [0] active, [1] fallen back to Chebyshev, [2] rejected steps, [3] parameter
+ * count, [4] last step length alpha, [5] last training loss, [6] input width, [7] hidden width */
+int libp_elliptic_learned_info(void* h, double* out, int nout);
+const char* libp_elliptic_learned_status(void* h);
+/* FP32 parameter vector (host buffers of the parameter count). */
+int libp_elliptic_learned_get_params(void* h, float* theta, long n);
+int libp_elliptic_learned_set_params(void* h, const float* theta, long n, int activate);
+/* Network correction c(r) of a device FP32 gathered residual (before the step length). */
+int libp_elliptic_learned_correction(void* h, const float* r, float* c);
+/* Training loss of one sample (device FP32 r, target eh; nrm = eh.A eh) and its
+ * gradient, left in the handle's "learnedGrad" device array; no update. */
+int libp_elliptic_learned_loss_grad(void* h, const float* r, const float* eh, double nrm, double* loss);
 
 #ifdef __cplusplus
 }

@@ -82,11 +82,27 @@ SIGNATURES = {
                                  c_void_p, c_void_p, c_void_p, c_void_p]),
     "libp_addbc_hex_f64": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                    c_void_p]),
-    "libp_mlp_linear_fwd": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
-    "libp_mlp_linear_bwd": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
-                                    c_void_p, c_void_p, c_void_p]),
+    "libp_mlp_linear_fwd": (c_int, [c_int, c_int, c_int, c_void_p, c_long, c_void_p, c_long, c_void_p, c_int,
+                                    c_void_p, c_long, c_void_p]),
+    "libp_mlp_linear_bwd_data": (c_int, [c_int, c_int, c_int, c_void_p, c_long, c_void_p, c_long, c_void_p, c_long,
+                                         c_void_p, c_long, c_void_p]),
+    "libp_mlp_linear_bwd_weight": (c_int, [c_int, c_int, c_int, c_void_p, c_long, c_void_p, c_long, c_void_p,
+                                           c_void_p, c_void_p, c_void_p]),
+    "libp_mlp_ws_bytes": (c_long, [c_int, c_int, c_int]),
     "libp_adam_step": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_float, c_float,
                                c_float, c_int, c_void_p]),
+    "libp_f32_to_bf16": (c_int, [c_long, c_void_p, c_void_p, c_void_p]),
+    "libp_learned_features_f32": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                          c_long, c_void_p, c_void_p]),
+    "libp_learned_assemble_f32": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_long, c_void_p, c_void_p,
+                                          c_void_p, c_void_p, c_void_p, c_void_p]),
+    "libp_learned_alpha_f32": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "libp_learned_update_f32": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "libp_learned_loss_f32": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                      c_void_p, c_void_p, c_void_p]),
+    "libp_learned_scatter_grad_f32": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_long,
+                                              c_void_p]),
+    "libp_inv_row_length_f32": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
     "libp_elliptic_create": (c_void_p, [c_char_p]),
     "libp_elliptic_free": (None, [c_void_p]),
     "libp_elliptic_geti": (c_long, [c_void_p, c_char_p]),
@@ -97,6 +113,15 @@ SIGNATURES = {
     "libp_elliptic_operator": (c_int, [c_void_p, c_void_p, c_void_p]),
     "libp_elliptic_partial_ax": (c_int, [c_void_p, c_void_p, c_void_p]),
     "libp_elliptic_precon": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "libp_elliptic_learned_train": (c_int, [c_void_p, c_char_p, ctypes.POINTER(c_double), c_int]),
+    "libp_elliptic_learned_load": (c_int, [c_void_p, c_char_p]),
+    "libp_elliptic_learned_save": (c_int, [c_void_p, c_char_p]),
+    "libp_elliptic_learned_info": (c_int, [c_void_p, ctypes.POINTER(c_double), c_int]),
+    "libp_elliptic_learned_status": (c_char_p, [c_void_p]),
+    "libp_elliptic_learned_get_params": (c_int, [c_void_p, c_void_p, c_long]),
+    "libp_elliptic_learned_set_params": (c_int, [c_void_p, c_void_p, c_long, c_int]),
+    "libp_elliptic_learned_correction": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "libp_elliptic_learned_loss_grad": (c_int, [c_void_p, c_void_p, c_void_p, c_double, ctypes.POINTER(c_double)]),
     "libp_elliptic_run": (c_int, [c_void_p, ctypes.POINTER(c_double), c_int]),
     "libp_elliptic_solution": (c_int, [c_void_p, c_void_p]),
 }
@@ -204,6 +229,56 @@ class Elliptic:
         x = np.empty(self.geti("Nelements") * self.geti("Np"))
         check(lib().libp_elliptic_solution(self._h, x.ctypes.data), "solution")
         return x
+
+    # learned fine smoother (MULTIGRID SMOOTHER LEARNED, MULTIGRID PRECISION FP32)
+    def learned_train(self, **options):
+        """Train the learned smoother; options as in libp_elliptic_learned_train.
+        Returns the mean loss of each logged block of steps."""
+        text = " ".join(f"{k}={','.join(map(str, v)) if isinstance(v, (list, tuple)) else v}"
+                        for k, v in options.items())
+        out = (c_double * 4096)()
+        n = lib().libp_elliptic_learned_train(self._h, text.encode(), out, 4096)
+        if n < 0:
+            check(-n, "learned_train")
+        return [out[i] for i in range(min(n, 4096))]
+
+    def learned_load(self, path):
+        """True when the weights were accepted; False (Chebyshev stays in use) otherwise."""
+        return lib().libp_elliptic_learned_load(self._h, str(path).encode()) == 0
+
+    def learned_save(self, path):
+        check(lib().libp_elliptic_learned_save(self._h, str(path).encode()), "learned_save")
+
+    def learned_info(self):
+        out = (c_double * 8)()
+        check(lib().libp_elliptic_learned_info(self._h, out, 8), "learned_info")
+        keys = ("active", "fallback", "rejected_steps", "parameters", "alpha", "loss", "inputs", "hidden")
+        return {k: out[i] for i, k in enumerate(keys)}
+
+    def learned_status(self):
+        return lib().libp_elliptic_learned_status(self._h).decode()
+
+    def learned_params(self):
+        import numpy as np
+        n = int(self.learned_info()["parameters"])
+        th = np.empty(n, dtype=np.float32)
+        check(lib().libp_elliptic_learned_get_params(self._h, th.ctypes.data, n), "learned_get_params")
+        return th
+
+    def learned_set_params(self, theta, activate=True):
+        import numpy as np
+        th = np.ascontiguousarray(theta, dtype=np.float32)
+        check(lib().libp_elliptic_learned_set_params(self._h, th.ctypes.data, th.size, int(activate)),
+              "learned_set_params")
+
+    def learned_correction(self, r, c):
+        check(lib().libp_elliptic_learned_correction(self._h, r.data_ptr(), c.data_ptr()), "learned_correction")
+
+    def learned_loss_grad(self, r, eh, nrm):
+        loss = c_double()
+        check(lib().libp_elliptic_learned_loss_grad(self._h, r.data_ptr(), eh.data_ptr(), float(nrm),
+                                                    ctypes.byref(loss)), "learned_loss_grad")
+        return loss.value
 
 
 def hip_d2h(dst, src, nbytes, stream=None):

@@ -4,14 +4,13 @@
 #include <cstring>
 #include <string>
 
+#include <sstream>
+
 #include "elliptic.hpp"
+#include "learned.hpp"
 
 namespace libp_amd {
 void set_last_error(const char* where, const char* what);
-}
-
-namespace libp {
-int multigridCoarseIterations(MultiGridPrecon& mg);
 }
 
 using namespace libp;
@@ -131,6 +130,20 @@ extern "C" void* libp_elliptic_array(void* p, const char* name, long* length, in
     if (auto* j = dynamic_cast<JacobiPrecon*>(e.precon.get())) return dev(j->o_invDiagA, 0);
     return nullptr;
   }
+  if (auto* mg = dynamic_cast<MultiGridPrecon*>(e.precon.get())) {
+    static thread_local deviceMemory<float> keep;
+    if (n == "learnedInvDiag") {
+      keep = multigridFineInvDiag32(*mg);
+      return keep.ptr() ? dev(keep, 3) : nullptr;
+    }
+    if (learnedSmoother_t* L = multigridLearned(*mg)) {
+      if (n == "learnedTheta") return dev(L->o_theta, 3);
+      if (n == "learnedTrace") return dev(L->o_trace, 1);
+      if (n == "learnedGrad") return dev(L->o_grad, 3);
+      if (n == "learnedScale") return dev(L->o_scale, 3);
+      if (n == "learnedY") return dev(L->o_Y, 3);
+    }
+  }
   if (n == "globalIds") return host(m.globalIds, 2);
   if (n == "maskedGlobalIds") return host(e.maskedGlobalIds, 2);
   if (n == "meshMapB") return host(m.mapB, 1);
@@ -196,5 +209,145 @@ extern "C" int libp_elliptic_solution(void* p, double* xL_host) {
     LIBP_ABORT("no solution available", !h || !h->elliptic->o_xL_last.isInitialized());
     h->elliptic->o_xL_last.copyTo(xL_host, h->elliptic->o_xL_last.length(), 0, h->platform.stream);
     h->platform.finish();
+  });
+}
+
+/* ------------------------- learned fine smoother ------------------------- */
+namespace {
+learnedSmoother_t* learnedOf(Handle* h) {
+  LIBP_ABORT("null handle", !h);
+  auto* mg = dynamic_cast<MultiGridPrecon*>(h->elliptic->precon.get());
+  LIBP_ABORT("the handle has no p-MG preconditioner", !mg);
+  learnedSmoother_t* L = multigridLearned(*mg);
+  LIBP_ABORT("the handle has no learned smoother (MULTIGRID SMOOTHER LEARNED, MULTIGRID PRECISION FP32)", !L);
+  return L;
+}
+}  // namespace
+
+extern "C" int libp_elliptic_learned_train(void* p, const char* options, double* loss, int nloss) {
+  Handle* h = (Handle*)p;
+  int n = 0;
+  int st = guarded("libp_elliptic_learned_train", [&]() {
+    learnedOf(h);
+    learnedTrainOptions opt;
+    int verbose = 0;
+    auto list = [](const std::string& v) {
+      std::vector<long> out;
+      std::stringstream ss(v);
+      std::string item;
+      while (std::getline(ss, item, ',')) out.push_back(std::stol(item));
+      return out;
+    };
+    std::stringstream ss(options ? options : "");
+    std::string kv;
+    while (ss >> kv) {
+      const size_t eq = kv.find('=');
+      LIBP_ABORT("training option without '=': " << kv, eq == std::string::npos);
+      const std::string k = kv.substr(0, eq), v = kv.substr(eq + 1);
+      if (k == "seeds") opt.seeds = list(v);
+      else if (k == "daggerSeeds") opt.daggerSeeds = list(v);
+      else if (k == "steps") opt.steps = std::stoi(v);
+      else if (k == "dagger") opt.dagger = std::stoi(v);
+      else if (k == "daggerSteps") opt.daggerSteps = std::stoi(v);
+      else if (k == "lr") opt.lr = std::stof(v);
+      else if (k == "targetTol") opt.targetTol = std::stod(v);
+      else if (k == "initSeed") opt.initSeed = std::stoul(v);
+      else if (k == "logEvery") opt.logEvery = std::stoi(v);
+      else if (k == "maxRecord") opt.maxRecordPerSolve = std::stoi(v);
+      else if (k == "verbose") verbose = std::stoi(v);
+      else LIBP_FORCE_ABORT("unknown training option " << k);
+    }
+    auto* mg = dynamic_cast<MultiGridPrecon*>(h->elliptic->precon.get());
+    std::vector<double> hist = multigridTrainLearned(*mg, opt, verbose);
+    for (size_t i = 0; i < hist.size() && (int)i < nloss; ++i) loss[i] = hist[i];
+    n = (int)hist.size();
+  });
+  return st == LIBP_OK ? n : -st;
+}
+
+extern "C" int libp_elliptic_learned_load(void* p, const char* path) {
+  Handle* h = (Handle*)p;
+  return guarded("libp_elliptic_learned_load", [&]() {
+    learnedSmoother_t* L = learnedOf(h);
+    LIBP_ABORT(L->status, !L->load(path ? path : ""));
+  });
+}
+
+extern "C" int libp_elliptic_learned_save(void* p, const char* path) {
+  Handle* h = (Handle*)p;
+  return guarded("libp_elliptic_learned_save", [&]() { learnedOf(h)->save(path ? path : ""); });
+}
+
+/* out: [0] active, [1] fallback, [2] rejected steps, [3] parameters, [4] last
+ * step alpha, [5] last training loss, [6] input width, [7] hidden width */
+extern "C" int libp_elliptic_learned_info(void* p, double* out, int nout) {
+  Handle* h = (Handle*)p;
+  return guarded("libp_elliptic_learned_info", [&]() {
+    learnedSmoother_t* L = learnedOf(h);
+    std::vector<double> s = L->stats();
+    const double v[8] = {(double)L->active, (double)L->fallback, s[3], (double)L->nParams, s[2], s[4],
+                         (double)L->nIn, (double)L->H};
+    for (int i = 0; i < nout && i < 8; ++i) out[i] = v[i];
+  });
+}
+
+extern "C" const char* libp_elliptic_learned_status(void* p) {
+  static thread_local std::string msg;
+  Handle* h = (Handle*)p;
+  int st = guarded("libp_elliptic_learned_status", [&]() { msg = learnedOf(h)->status; });
+  return st == LIBP_OK ? msg.c_str() : "no learned smoother";
+}
+
+extern "C" int libp_elliptic_learned_get_params(void* p, float* theta, long n) {
+  Handle* h = (Handle*)p;
+  return guarded("libp_elliptic_learned_get_params", [&]() {
+    learnedSmoother_t* L = learnedOf(h);
+    LIBP_ABORT("parameter count " << n << " != " << L->nParams, (size_t)n != L->nParams || !theta);
+    std::vector<float> th = L->parameters();
+    std::memcpy(theta, th.data(), n * sizeof(float));
+  });
+}
+
+extern "C" int libp_elliptic_learned_set_params(void* p, const float* theta, long n, int activate) {
+  Handle* h = (Handle*)p;
+  return guarded("libp_elliptic_learned_set_params", [&]() {
+    learnedSmoother_t* L = learnedOf(h);
+    LIBP_ABORT("parameter count " << n << " != " << L->nParams, (size_t)n != L->nParams || !theta);
+    L->setParameters(std::vector<float>(theta, theta + n));
+    L->active = activate != 0;
+    L->fallback = false;
+    L->status = activate ? "parameters set" : "parameters set (inactive)";
+  });
+}
+
+/* c = c(r) for a device FP32 gathered residual r (the network correction,
+ * before the energy-optimal step). */
+extern "C" int libp_elliptic_learned_correction(void* p, const float* r, float* c) {
+  Handle* h = (Handle*)p;
+  return guarded("libp_elliptic_learned_correction", [&]() {
+    learnedSmoother_t* L = learnedOf(h);
+    LIBP_ABORT("null pointer", !r || !c);
+    auto* mg = dynamic_cast<MultiGridPrecon*>(h->elliptic->precon.get());
+    deviceMemory<float> iD = multigridFineInvDiag32(*mg);
+    L->correction(r, iD.ptr());
+    LIBP_HIP(hipMemcpyAsync(c, L->o_c.ptr(), L->Ndofs * sizeof(float), hipMemcpyDeviceToDevice, h->platform.stream));
+    h->platform.finish();
+  });
+}
+
+/* Loss of one sample (device FP32 r and target eh, nrm = eh.A eh) and its
+ * gradient (left in the "learnedGrad" array); no parameter update. */
+extern "C" int libp_elliptic_learned_loss_grad(void* p, const float* r, const float* eh, double nrm, double* loss) {
+  Handle* h = (Handle*)p;
+  return guarded("libp_elliptic_learned_loss_grad", [&]() {
+    learnedSmoother_t* L = learnedOf(h);
+    LIBP_ABORT("null pointer", !r || !eh || !loss);
+    auto* mg = dynamic_cast<MultiGridPrecon*>(h->elliptic->precon.get());
+    deviceMemory<float> iD = multigridFineInvDiag32(*mg);
+    deviceMemory<double> o_n = h->platform.malloc<double>(1);
+    o_n.copyFrom(&nrm, 1, 0, h->platform.stream);
+    auto A = [mg](deviceMemory<float>& x, deviceMemory<float>& y) { multigridApplyFineOperator32(*mg, x, y); };
+    L->trainStep(r, eh, o_n.ptr(), iD.ptr(), A, 0.f, false);
+    *loss = L->stats()[4];
   });
 }

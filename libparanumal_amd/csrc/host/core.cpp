@@ -140,7 +140,12 @@ settings_t ellipticDefaultSettings() {
   s.newSetting("MULTIGRID PRECISION", "FP32", "Precision of p-MG smoothing and transfers", {"FP32", "FP64"});
   s.newSetting("MULTIGRID COARSE TOLERANCE", "1e-2", "Relative tolerance of the coarse Jacobi-PCG");
   s.newSetting("MULTIGRID COARSE MAXIT", "200");
+  s.newSetting("MULTIGRID COARSE CHECK EVERY", "8",
+               "Coarse Jacobi-PCG: iterations queued between host convergence reads (same iterates as 1)");
   s.newSetting("MULTIGRID LEARNED WEIGHTS", "", "Weight file of the learned fine smoother");
+  s.newSetting("MULTIGRID LEARNED MODE", "REPLACE",
+               "REPLACE: network pre- and post-smoothing; PRE: network pre-smoothing, Chebyshev post-smoothing",
+               {"REPLACE", "PRE"});
   s.newSetting("LINEAR SOLVER TOLERANCE", "1e-8");
   s.newSetting("MAXIMUM ITERATIONS", "5000");
   s.newSetting("DATA SEED", "1", "Seed of the Fourier forcing data set");

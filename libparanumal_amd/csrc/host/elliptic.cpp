@@ -147,9 +147,13 @@ int elliptic_t::dataKind() const {
 // 64-mode forcing sum_m a_m sin(pi k_m.x + phi_m): k in {1..4}^3, a = +-1/|k|^2,
 // phi uniform in [0,2pi), drawn with erand48 from [DATA SEED].
 deviceMemory<dfloat> elliptic_t::dataCoefficients() {
-  if (dataKind() != 1) return deviceMemory<dfloat>();
   long seed = 1;
   settings.getSetting("DATA SEED", seed);
+  return dataCoefficients(seed);
+}
+
+deviceMemory<dfloat> elliptic_t::dataCoefficients(long seed) {
+  if (dataKind() != 1) return deviceMemory<dfloat>();
   unsigned short xs[3] = {0x330E, (unsigned short)(seed & 0xffff), (unsigned short)((seed >> 16) & 0xffff)};
   std::vector<double> c(64 * 5);
   for (int m = 0; m < 64; ++m) {
@@ -171,6 +175,22 @@ int elliptic_t::Solve(linearSolver_t& linearSolver, deviceMemory<dfloat>& o_x, d
                       const dfloat tol, const int MAXIT, const int verbose) {
   if (allNeumann) ZeroMean(o_r);
   return linearSolver.Solve(*this, precon, o_x, o_r, tol, MAXIT, verbose);
+}
+
+void elliptic_t::BuildRhs(long seed, deviceMemory<dfloat>& o_r) {
+  const int kind = dataKind();
+  deviceMemory<dfloat> o_coeffs = dataCoefficients(seed);
+  const dlong NL = mesh.Nelements * mesh.Np;
+  deviceMemory<dfloat> o_rL = platform.malloc<dfloat>(NL), o_ubL = platform.malloc<dfloat>(NL);
+  deviceMemory<dfloat> o_AubL = platform.malloc<dfloat>(NL);
+  LIBP_CALL(libp_rhs_hex_f64(mesh.Nelements, mesh.Np, kind, o_coeffs.ptr(), mesh.o_x.ptr(), mesh.o_y.ptr(),
+                             mesh.o_z.ptr(), mesh.o_wJ.ptr(), lambda, o_mapB.ptr(), o_rL.ptr(), o_ubL.ptr(),
+                             platform.s()));
+  LIBP_CALL(libp_ax_hex_local_f64(mesh.N, mesh.Nelements, mesh.o_wJ.ptr(), mesh.o_ggeo.ptr(), mesh.o_D.ptr(), lambda,
+                                  o_ubL.ptr(), o_AubL.ptr(), platform.s()));
+  LIBP_CALL(libp_axpy_f64(NL, -1.0, o_AubL.ptr(), 1.0, o_rL.ptr(), platform.s()));
+  ogsMasked.Gather(o_r, o_rL, 1, ogs::Add, ogs::Trans);
+  platform.finish();
 }
 
 ellipticRunResult_t elliptic_t::Run(int verboseOverride) {

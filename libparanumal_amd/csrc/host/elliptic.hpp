@@ -76,6 +76,9 @@ class elliptic_t : public operator_t {
   // Data set of the DATA FILE setting: kind (0 sine, 1 Fourier) and device coefficients
   int dataKind() const;
   deviceMemory<dfloat> dataCoefficients();
+  deviceMemory<dfloat> dataCoefficients(long seed);
+  // gathered right-hand side (forcing + Dirichlet lift) of the data with the given seed
+  void BuildRhs(long seed, deviceMemory<dfloat>& o_r);
 
   deviceMemory<dfloat> o_xL_last;  // solution E-vector of the last Run
 };
@@ -97,7 +100,39 @@ class MultiGridPrecon : public operator_t {
   std::shared_ptr<MultiGridPreconBase> impl;
   explicit MultiGridPrecon(elliptic_t& elliptic);
   void Operator(deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_Mr) override;
+  void solveStarting() override;
+  void iterationDone(int it, double rdotr) override;
 };
+
+class learnedSmoother_t;
+// Fine-level learned smoother of an FP32 p-MG preconditioner (null otherwise).
+learnedSmoother_t* multigridLearned(MultiGridPrecon& mg);
+int multigridCoarseIterations(MultiGridPrecon& mg);
+// FP32 inverse diagonal and operator of the fine p-MG level (FP32 preconditioner only)
+deviceMemory<float> multigridFineInvDiag32(MultiGridPrecon& mg);
+void multigridApplyFineOperator32(MultiGridPrecon& mg, deviceMemory<float>& x, deviceMemory<float>& y);
+
+/* Training of the learned fine smoother (no reference counterpart). Pairs
+ * (r, eh) come from FPCG solves preconditioned by this p-MG with the Chebyshev
+ * smoother, one per seed of `seeds` (Fourier forcing of that seed): r is each
+ * smoother input residual (outer residuals, and the residuals after the
+ * coarse correction), scaled to unit norm; eh = A^{-1} r - P A_1^{-1} R r is
+ * the error left after an exact coarse-grid correction. `dagger` further
+ * rounds add pairs from solves that use the network being trained. Returns
+ * the mean loss of each block of `logEvery` steps. */
+struct learnedTrainOptions {
+  std::vector<long> seeds{101, 102, 103, 104};
+  std::vector<long> daggerSeeds{201, 202};
+  int steps = 3000;
+  int dagger = 1;
+  int daggerSteps = 1500;
+  float lr = 1e-3f;
+  double targetTol = 1e-9;
+  unsigned long initSeed = 1;
+  int logEvery = 250;
+  int maxRecordPerSolve = 24;
+};
+std::vector<double> multigridTrainLearned(MultiGridPrecon& mg, const learnedTrainOptions& opt, int verbose);
 
 // Degree schedule of MultiGridPrecon (ellipticPreconMultiGrid.cpp:45-131) plus CUSTOM.
 std::vector<int> multigridDegrees(const settings_t& settings, int N);

@@ -1,43 +1,93 @@
 // Learned per-element smoother for the fine p-MG level (no reference counterpart).
 #pragma once
 
+#include <functional>
+#include <string>
+
 #include "core.hpp"
 
 namespace libp {
 
 class elliptic_t;
 
-/* A per-element MLP maps the element residual (Np values) and the six
- * face-neighbour residual traces (6 x Nfp) to an element error correction;
- * corrections are weighted by inverse multiplicity and assembled with the
- * gather-scatter. The weights are BF16 with FP32 masters; inference and
- * training run on the project's own MFMA kernels (libp_mlp_*). Used in place
- * of the Chebyshev pre/post smoothing on the fine level, with Chebyshev as
- * the fallback when no weights are loaded or the weights are rejected. */
+/* A per-element MLP (nIn = Np + 6 Nfp -> 256 -> 256 -> Np, ReLU, BF16 weights
+ * with FP32 masters, FP32 accumulation on MFMA) maps the Jacobi-scaled
+ * residual of an element and of the six neighbour layers next to its faces to
+ * an element error correction. Element corrections are averaged over the
+ * copies of each node and a learned multiple omega of the Jacobi correction
+ * is added:
+ *   c(r) = avg_e[s_e N(z_e / s_e)] + omega D^{-1} r,  z = D^{-1} r,
+ * with s_e the RMS of the element's input. Each application takes the
+ * energy-optimal step x += alpha c, alpha = c.res / c.Ac, which never raises
+ * the A-norm error; a non-finite or non-positive c.Ac rejects the step.
+ *
+ * Parameters are one FP32 vector theta (layout below), written to disk with a
+ * header and an FNV-1a hash. Training minimises the energy norm
+ *   J = (eh - c(r)).A(eh - c(r)) / eh.A eh
+ * over residual / target pairs, where eh is the part of the error A^{-1} r that
+ * the coarse-grid correction leaves (computed by the multigrid trainer).
+ *
+ * theta: W1 [H][nIn], b1 [H], W2 [H][H], b2 [H], W3 [Np][H], b3 [Np], omega. */
 class learnedSmoother_t {
  public:
-  bool active = false;
-  int Np = 0, Nfp = 0, nIn = 0, nHidden = 256, nOut = 0;
-  int chebSteps = 0;  // Chebyshev sweeps applied after the network correction
+  typedef std::function<void(deviceMemory<float>&, deviceMemory<float>&)> op32_t;
+
   elliptic_t& elliptic;
-  // layer l: W[l] (BF16, nIn_l x nOut_l row-major), b[l] (FP32)
-  std::vector<deviceMemory<unsigned short>> W;
-  std::vector<deviceMemory<float>> b;
-  std::vector<deviceMemory<float>> master;  // FP32 masters of W (training)
-  deviceMemory<int> o_traceIds;             // [E][6*Nfp] local id of the neighbour node, -1 at the boundary
-  deviceMemory<float> o_feat, o_h1, o_h2, o_out, o_rL;
+  platform_t platform;
+  int Np = 0, Nfp = 0, nIn = 0, H = 256, nOut = 0;
+  dlong E = 0, Ndofs = 0;
+  size_t oW1 = 0, ob1 = 0, oW2 = 0, ob2 = 0, oW3 = 0, ob3 = 0, oOm = 0, nParams = 0;
+
+  bool active = false;    // weights loaded and accepted
+  bool fallback = false;  // switched to Chebyshev at run time
+  std::string status = "no weights";
+  int rejectedSteps = 0;
+
+  deviceMemory<int> o_trace;    // [E][6 Nfp] gathered ids of the neighbour traces (-1: none)
+  deviceMemory<float> o_invLen;  // [Ndofs] 1 / multiplicity
+  deviceMemory<float> o_theta, o_grad, o_m, o_v;
+  deviceMemory<unsigned short> o_thetaBf;
+  deviceMemory<unsigned short> o_X, o_H1, o_H2, o_dY, o_dH1, o_dH2;
+  deviceMemory<float> o_scale, o_Y, o_c, o_Ac, o_res, o_gc, o_d, o_Ad;
+  deviceMemory<double> o_sc;  // [0..3] step statistics, [4] last loss, [5] running loss sum
+  deviceMemory<char> o_ws, o_rws;
+  int adamStep = 0;
+  double monitorRejected = 0;
+  std::vector<double> monitorHist;
 
   explicit learnedSmoother_t(elliptic_t& el);
-  void load(const std::string& path);
+
+  // trace map from the box structure; checked against the global node ids
+  void setupMaps();
+  void initRandom(unsigned long seed);
+  // false (and status set) when the file is missing, malformed, of another
+  // shape, fails its hash or holds non-finite values
+  bool load(const std::string& path);
   void save(const std::string& path) const;
-  void initRandom(unsigned seed);
-  // x += S(r - A x) style smoothing step with the network (see learned.cpp)
-  void smooth(elliptic_t& el, deviceMemory<float>& o_r, deviceMemory<float>& o_X, bool xIsZero,
-              deviceMemory<float>& o_invDiag, deviceMemory<float>& o_s1, deviceMemory<float>& o_s2);
-  // network features of a gathered FP32 residual: [E][nIn]
-  void features(deviceMemory<float>& o_r);
-  // forward pass on o_feat; output correction in o_out ([E][Np])
-  void forward(int E);
+  std::vector<float> parameters() const;
+  void setParameters(const std::vector<float>& th);
+
+  // c = c(r) in o_c (features, three layers, assembly)
+  void correction(const float* r, const float* invDiag);
+  // pre-smoothing: x = alpha c(r); res = r - alpha A c when res != null (the
+  // V-cycle residual comes free with the step)
+  void smoothPre(deviceMemory<float>& o_r, deviceMemory<float>& o_x, deviceMemory<float>* o_resOut,
+                 const float* invDiag, const op32_t& A);
+  // post-smoothing: res = r - A x, x += alpha c(res)
+  void smoothPost(deviceMemory<float>& o_r, deviceMemory<float>& o_x, const float* invDiag, const op32_t& A);
+
+  // loss and gradient of one sample (r, target eh: FP32, gathered; *nrm = eh.A eh on
+  // the device) into o_grad, then one Adam step when update is set
+  void trainStep(const float* r, const float* eh, const double* nrm, const float* invDiag, const op32_t& A, float lr,
+                 bool update = true);
+  // run-time acceptance: the network stays in use while FPCG keeps
+  // converging (finite residuals, no rejected step, and no two consecutive
+  // iterations with |r| reduced by less than 10%); otherwise Chebyshev takes over
+  void monitorStart();
+  void monitorIteration(int it, double rdotr);
+  // statistics: [0] c.res, [1] c.Ac, [2] alpha, [3] rejected steps, [4] last loss, [5] loss sum
+  std::vector<double> stats() const;
+  void resetLossSum();
 };
 
 }  // namespace libp

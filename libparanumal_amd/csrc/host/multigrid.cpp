@@ -8,6 +8,8 @@
 #include <complex>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
+#include <random>
 #include <type_traits>
 
 #include "elliptic.hpp"
@@ -179,6 +181,9 @@ class MGLevel : public parAlmond::multigridLevel<T> {
   deviceMemory<T> o_invDiagA, o_P, o_weightG;
   std::shared_ptr<mgScratch<T>> scratch;
   std::shared_ptr<learnedSmoother_t> learned;  // fine-level learned smoother (FP32 only)
+  bool learnedPostToo = true;                  // MULTIGRID LEARNED MODE REPLACE (else PRE)
+  std::vector<deviceMemory<T>>* record = nullptr;  // smoother input residuals are copied here when set
+  size_t recordMax = 0;
 
   MGLevel(std::shared_ptr<elliptic_t> el, int _Nc, unsigned short* drandState, std::shared_ptr<mgScratch<T>> sc)
       : elliptic(el), platform(el->platform), Nf(el->mesh.N), Nc(_Nc), scratch(sc) {
@@ -286,9 +291,34 @@ class MGLevel : public parAlmond::multigridLevel<T> {
     LIBP_CALL(prec<T>::axpy(elliptic->Ndofs, T(1), o_PxG.ptr(), T(1), o_Px.ptr(), platform.s()));
   }
 
+  bool learnedUsable() const { return learned && learned->active && !learned->fallback; }
+
+  // copy of the residual the smoother is about to see (training data)
+  void recordInput(deviceMemory<T>& o_RHS, deviceMemory<T>& o_X, bool x_is_zero) {
+    if (record->size() >= recordMax) return;
+    const dlong N = elliptic->Ndofs;
+    deviceMemory<T> v = platform.malloc<T>(N);
+    if (x_is_zero) {
+      LIBP_HIP(hipMemcpyAsync(v.ptr(), o_RHS.ptr(), N * sizeof(T), hipMemcpyDeviceToDevice, platform.stream));
+    } else {
+      residual(o_RHS, o_X, v);
+    }
+    record->push_back(v);
+  }
+
+  // Pre-smoothing that also leaves the V-cycle residual in o_RES (the learned
+  // step computes it on the way); false when the caller must do both itself.
+  bool smoothPreResidual(deviceMemory<T>& o_RHS, deviceMemory<T>& o_X, deviceMemory<T>& o_RES) {
+    if (!learnedUsable()) return false;
+    if (record) recordInput(o_RHS, o_X, true);
+    smoothLearned(o_RHS, o_X, true, &o_RES);
+    return true;
+  }
+
   void smooth(deviceMemory<T>& o_RHS, deviceMemory<T>& o_X, bool x_is_zero) override {
-    if (learned && learned->active) {
-      smoothLearned(o_RHS, o_X, x_is_zero);
+    if (record) recordInput(o_RHS, o_X, x_is_zero);
+    if (learnedUsable() && (x_is_zero || learnedPostToo)) {
+      smoothLearned(o_RHS, o_X, x_is_zero, nullptr);
     } else if (stype == JACOBI) {
       smoothJacobi(o_RHS, o_X, x_is_zero);
     } else {
@@ -339,7 +369,7 @@ class MGLevel : public parAlmond::multigridLevel<T> {
     }
   }
 
-  void smoothLearned(deviceMemory<T>& o_r, deviceMemory<T>& o_X, bool xIsZero);
+  void smoothLearned(deviceMemory<T>& o_r, deviceMemory<T>& o_X, bool xIsZero, deviceMemory<T>* o_RES);
 
   void Report() override {
     printf("|    pMG     |    %10lld  |   Matrix-free   |   %s|  Degree %2d\n", (long long)this->Nrows,
@@ -348,12 +378,18 @@ class MGLevel : public parAlmond::multigridLevel<T> {
 };
 
 template <>
-void MGLevel<float>::smoothLearned(deviceMemory<float>& o_r, deviceMemory<float>& o_X, bool xIsZero) {
-  learned->smooth(*elliptic, o_r, o_X, xIsZero, o_invDiagA, scratch->RES, scratch->Ad);
+void MGLevel<float>::smoothLearned(deviceMemory<float>& o_r, deviceMemory<float>& o_X, bool xIsZero,
+                                   deviceMemory<float>* o_RES) {
+  auto A = [this](deviceMemory<float>& x, deviceMemory<float>& y) { Operator(x, y); };
+  if (xIsZero) {
+    learned->smoothPre(o_r, o_X, o_RES, o_invDiagA.ptr(), A);
+  } else {
+    learned->smoothPost(o_r, o_X, o_invDiagA.ptr(), A);
+  }
 }
 template <>
-void MGLevel<double>::smoothLearned(deviceMemory<double>& o_r, deviceMemory<double>& o_X, bool xIsZero) {
-  smoothChebyshev(o_r, o_X, xIsZero);
+void MGLevel<double>::smoothLearned(deviceMemory<double>&, deviceMemory<double>&, bool, deviceMemory<double>*) {
+  LIBP_FORCE_ABORT("the learned smoother runs in FP32 only");
 }
 
 /* Base level: FP64 Jacobi-PCG on the N=1 operator (the composition the p-MG
@@ -378,6 +414,7 @@ class coarseJacobiPCG_t {
     s.changeSetting("LINEAR SOLVER STOPPING CRITERION", "RELATIVE");
     solver = std::make_shared<LinearSolver::pcg>(elliptic->Ndofs, 0, elliptic->platform, s, elliptic->comm);
     solver->pureRelative = true;
+    elliptic->settings.getSetting("MULTIGRID COARSE CHECK EVERY", solver->checkEvery);
     o_x = elliptic->platform.malloc<double>(elliptic->Ndofs);
     o_r = elliptic->platform.malloc<double>(elliptic->Ndofs);
   }
@@ -461,6 +498,7 @@ class multigrid_t : public MultiGridPreconBase {
   }
 
   void attachLearned();
+  std::vector<double> trainLearned(const learnedTrainOptions& opt, int verbose);
 
   // parAlmondVcycle.cpp:34-60
   void vcycle(size_t k, deviceMemory<T>& o_RHS, deviceMemory<T>& o_X) {
@@ -469,8 +507,10 @@ class multigrid_t : public MultiGridPreconBase {
       return;
     }
     MGLevel<T>& level = *levels[k];
-    level.smooth(o_RHS, o_X, true);
-    level.residual(o_RHS, o_X, o_res);
+    if (!level.smoothPreResidual(o_RHS, o_X, o_res)) {
+      level.smooth(o_RHS, o_X, true);
+      level.residual(o_RHS, o_X, o_res);
+    }
     level.coarsen(o_res, o_rhs[k + 1]);
     vcycle(k + 1, o_rhs[k + 1], o_x[k + 1]);
     level.prolongate(o_x[k + 1], o_X);
@@ -496,8 +536,10 @@ template <>
 void multigrid_t<float>::attachLearned() {
   std::string path = fine.settings.getSetting("MULTIGRID LEARNED WEIGHTS");
   auto ls = std::make_shared<learnedSmoother_t>(*ellipticLevels[0]);
-  if (!path.empty()) ls->load(path);
+  if (!path.empty() && !ls->load(path) && fine.settings.compareSetting("VERBOSE", "TRUE"))
+    printf("learned smoother %s; smoothing with Chebyshev\n", ls->status.c_str());
   levels[0]->learned = ls;
+  levels[0]->learnedPostToo = !fine.settings.compareSetting("MULTIGRID LEARNED MODE", "PRE");
 }
 template <>
 void multigrid_t<double>::attachLearned() {
@@ -517,10 +559,189 @@ void MultiGridPrecon::Operator(deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& 
   if (elliptic.allNeumann) elliptic.ZeroMean(o_Mr);
 }
 
+void MultiGridPrecon::solveStarting() {
+  if (learnedSmoother_t* L = multigridLearned(*this)) L->monitorStart();
+}
+
+void MultiGridPrecon::iterationDone(int it, double rdotr) {
+  if (learnedSmoother_t* L = multigridLearned(*this)) L->monitorIteration(it, rdotr);
+}
+
+namespace {
+
+// FPCG preconditioner: the FP32 V-cycle started at level k of a multigrid.
+class levelVcycle_t : public operator_t {
+ public:
+  multigrid_t<float>& mg;
+  size_t k;
+  levelVcycle_t(multigrid_t<float>& m, size_t kk) : mg(m), k(kk) {}
+  void Operator(deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_z) override {
+    if (k == 0) {
+      mg.Operator(o_r, o_z);
+      return;
+    }
+    void* s = mg.fine.platform.s();
+    const dlong N = mg.ellipticLevels[k]->Ndofs;
+    LIBP_CALL(libp_convert_f64_to_f32(N, o_r.ptr(), mg.o_rhs[k].ptr(), s));
+    mg.vcycle(k, mg.o_rhs[k], mg.o_x[k]);
+    LIBP_CALL(libp_convert_f32_to_f64(N, mg.o_x[k].ptr(), o_z.ptr(), s));
+  }
+};
+
+// x = A^{-1} b by FPCG from a zero guess to a relative tolerance; b is kept
+int fpcgSolve(elliptic_t& el, operator_t& M, deviceMemory<dfloat>& o_x, deviceMemory<dfloat>& o_b,
+              deviceMemory<dfloat>& o_work, double tol, int maxit) {
+  settings_t s = el.settings;
+  s.changeSetting("LINEAR SOLVER", "FPCG");
+  LinearSolver::pcg solver(el.Ndofs, el.Nhalo, el.platform, s, el.comm);
+  solver.pureRelative = true;
+  LIBP_CALL(libp_set_f64(el.Ndofs, 0.0, o_x.ptr(), el.platform.s()));
+  LIBP_HIP(hipMemcpyAsync(o_work.ptr(), o_b.ptr(), el.Ndofs * sizeof(double), hipMemcpyDeviceToDevice,
+                          el.platform.stream));
+  return solver.Solve(el, M, o_x, o_work, tol, maxit, 0);
+}
+
+}  // namespace
+
+template <>
+std::vector<double> multigrid_t<float>::trainLearned(const learnedTrainOptions& opt, int verbose) {
+  LIBP_ABORT("training needs MULTIGRID SMOOTHER LEARNED", levels.empty() || !levels[0]->learned);
+  LIBP_ABORT("training needs at least two p-MG levels above the coarse solve", ellipticLevels.size() < 3);
+  learnedSmoother_t& L = *levels[0]->learned;
+  MGLevel<float>& lev0 = *levels[0];
+  elliptic_t& el0 = *ellipticLevels[0];
+  elliptic_t& el1 = *ellipticLevels[1];
+  platform_t& p = fine.platform;
+  void* st = p.s();
+  const dlong N0 = el0.Ndofs, N1 = el1.Ndofs;
+  levelVcycle_t M0(*this, 0), M1(*this, 1);
+  deviceMemory<double> b = p.malloc<double>(N0), x = p.malloc<double>(N0), w0 = p.malloc<double>(N0);
+  deviceMemory<double> r64 = p.malloc<double>(N0), e64 = p.malloc<double>(N0), t64 = p.malloc<double>(N0);
+  deviceMemory<double> rc64 = p.malloc<double>(N1), ec64 = p.malloc<double>(N1), w1 = p.malloc<double>(N1);
+  deviceMemory<float> ec32 = p.malloc<float>(N1), t32 = p.malloc<float>(N0);
+  double* tmp = p.o_scalars.ptr() + 20;
+  void* ws = p.o_reduceWS.ptr();
+  auto A0 = [&](deviceMemory<float>& a, deviceMemory<float>& y) { lev0.Operator(a, y); };
+
+  struct Pair {
+    deviceMemory<float> r, eh;
+  };
+  std::vector<Pair> pairs;
+  std::vector<double> nrms;
+
+  auto collect = [&](const std::vector<long>& seeds, bool useNet) {
+    for (long seed : seeds) {
+      std::vector<deviceMemory<float>> rec;
+      L.active = useNet;
+      L.fallback = false;
+      lev0.record = &rec;
+      lev0.recordMax = opt.maxRecordPerSolve;
+      el0.BuildRhs(seed, b);
+      const int its = fpcgSolve(el0, M0, x, b, w0, 1e-8, 60);
+      lev0.record = nullptr;
+      L.active = false;  // targets are computed with the Chebyshev cycle
+      for (auto& v : rec) {
+        LIBP_CALL(libp_convert_f32_to_f64(N0, v.ptr(), r64.ptr(), st));
+        LIBP_CALL(libp_norm2sq_f64(N0, r64.ptr(), tmp, ws, st));
+        const double nr = std::sqrt(p.readScalars(tmp, 1)[0]);
+        if (!(nr > 0) || !std::isfinite(nr)) continue;
+        LIBP_CALL(libp_scale_f64(N0, 1.0 / nr, r64.ptr(), st));
+        Pair pr;
+        pr.r = p.malloc<float>(N0);
+        pr.eh = p.malloc<float>(N0);
+        LIBP_CALL(libp_convert_f64_to_f32(N0, r64.ptr(), pr.r.ptr(), st));
+        // e = A^{-1} r
+        fpcgSolve(el0, M0, e64, r64, w0, opt.targetTol, 200);
+        // ec = A_1^{-1} R r
+        lev0.coarsen(pr.r, o_rhs[1]);
+        LIBP_CALL(libp_convert_f32_to_f64(N1, o_rhs[1].ptr(), rc64.ptr(), st));
+        fpcgSolve(el1, M1, ec64, rc64, w1, opt.targetTol, 200);
+        // eh = e - P ec
+        LIBP_CALL(libp_convert_f64_to_f32(N0, e64.ptr(), pr.eh.ptr(), st));
+        LIBP_CALL(libp_convert_f64_to_f32(N1, ec64.ptr(), ec32.ptr(), st));
+        LIBP_CALL(libp_axpy_f32(N1, 0.f, ec32.ptr(), -1.f, ec32.ptr(), st));
+        lev0.prolongate(ec32, pr.eh);
+        // nrm = eh.A eh
+        A0(pr.eh, t32);
+        LIBP_CALL(libp_convert_f32_to_f64(N0, pr.eh.ptr(), t64.ptr(), st));
+        LIBP_CALL(libp_convert_f32_to_f64(N0, t32.ptr(), e64.ptr(), st));
+        LIBP_CALL(libp_inner_prod_f64(N0, t64.ptr(), e64.ptr(), tmp, ws, st));
+        const double nrm = p.readScalars(tmp, 1)[0];
+        if (!(nrm > 0) || !std::isfinite(nrm)) continue;
+        pairs.push_back(pr);
+        nrms.push_back(nrm);
+      }
+      if (verbose)
+        printf("learned smoother: seed %ld (%s smoother) FPCG %d iterations, %zu pairs in total\n", seed,
+               useNet ? "learned" : "Chebyshev", its, pairs.size());
+    }
+  };
+
+  std::vector<double> hist;
+  std::mt19937_64 gen(opt.initSeed);
+  deviceMemory<double> o_nrm;
+  auto train = [&](int steps, float lr0) {
+    o_nrm = p.malloc(nrms);
+    std::vector<size_t> order(pairs.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    L.resetLossSum();
+    for (int s = 0; s < steps; ++s) {
+      if (s % order.size() == 0) std::shuffle(order.begin(), order.end(), gen);
+      const size_t i = order[s % order.size()];
+      const float lr = (float)(lr0 * (0.02 + 0.98 * 0.5 * (1.0 + std::cos(M_PI * s / steps))));
+      L.trainStep(pairs[i].r.ptr(), pairs[i].eh.ptr(), o_nrm.ptr() + i, lev0.o_invDiagA.ptr(), A0, lr);
+      if ((s + 1) % opt.logEvery == 0) {
+        hist.push_back(L.stats()[5] / opt.logEvery);
+        L.resetLossSum();
+        if (verbose) printf("learned smoother: step %d loss %.4f\n", s + 1, hist.back());
+      }
+    }
+  };
+
+  L.initRandom(opt.initSeed);
+  collect(opt.seeds, false);
+  LIBP_ABORT("learned smoother: no training pairs", pairs.empty());
+  train(opt.steps, opt.lr);
+  for (int round = 0; round < opt.dagger; ++round) {
+    std::vector<long> ds;
+    for (long sd : opt.daggerSeeds) ds.push_back(sd + 1000L * round);
+    collect(ds, true);
+    train(opt.daggerSteps, opt.lr * 0.5f);
+  }
+  L.active = true;
+  L.fallback = false;
+  L.status = "trained";
+  return hist;
+}
+
+template <>
+std::vector<double> multigrid_t<double>::trainLearned(const learnedTrainOptions&, int) {
+  LIBP_FORCE_ABORT("MULTIGRID SMOOTHER LEARNED needs MULTIGRID PRECISION FP32");
+  return {};
+}
+
+std::vector<double> multigridTrainLearned(MultiGridPrecon& mg, const learnedTrainOptions& opt, int verbose) {
+  auto* m = dynamic_cast<multigrid_t<float>*>(mg.impl.get());
+  LIBP_ABORT("training needs MULTIGRID PRECISION FP32", !m);
+  return m->trainLearned(opt, verbose);
+}
+
 learnedSmoother_t* multigridLearned(MultiGridPrecon& mg) {
   auto* m = dynamic_cast<multigrid_t<float>*>(mg.impl.get());
   if (!m || m->levels.empty()) return nullptr;
   return m->levels[0]->learned.get();
+}
+
+deviceMemory<float> multigridFineInvDiag32(MultiGridPrecon& mg) {
+  auto* m = dynamic_cast<multigrid_t<float>*>(mg.impl.get());
+  if (!m || m->levels.empty()) return deviceMemory<float>();
+  return m->levels[0]->o_invDiagA;
+}
+
+void multigridApplyFineOperator32(MultiGridPrecon& mg, deviceMemory<float>& x, deviceMemory<float>& y) {
+  auto* m = dynamic_cast<multigrid_t<float>*>(mg.impl.get());
+  LIBP_ABORT("needs an FP32 p-MG preconditioner", !m || m->levels.empty());
+  m->levels[0]->Operator(x, y);
 }
 
 int multigridCoarseIterations(MultiGridPrecon& mg) {

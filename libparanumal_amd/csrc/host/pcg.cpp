@@ -1,4 +1,5 @@
 // (F)PCG on the device (libs/linearSolver/linearSolverPCG.cpp:35-171).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 
@@ -49,8 +50,42 @@ int pcg::Solve(operator_t& linearOperator, operator_t& precon, deviceMemory<dflo
   if (recordHistory) stats.resHistory.push_back(std::sqrt(rdotr0));
   if (verbose) printf("PCG: initial res norm %12.12f \n", std::sqrt(rdotr0));
 
+  precon.solveStarting();
   diagonalPrecon_t* jac = dynamic_cast<diagonalPrecon_t*>(&precon);
   if (jac && jac->needsZeroMean()) jac = nullptr;
+
+  if (checkEvery > 1) {
+    // chunked: the kernels stop themselves once rdotr <= TOL (device STOP flag)
+    int iter = 0;
+    if (rdotr0 != 0.0) {
+      const double tolv = TOL > 0 ? TOL : 1e-300;
+      LIBP_CALL(libp_set_f64(1, tolv, s + LIBP_PCG_TOL, st));
+      while (iter < MAXIT) {
+        const int chunk = std::min(checkEvery, MAXIT - iter);
+        for (int c = 0; c < chunk; ++c) {
+          const int it = iter + c;
+          if (jac) {
+            LIBP_CALL(libp_pcg_jacobi_rdotz_f64(N, jac->invDiagonal().ptr(), o_r.ptr(), o_z.ptr(), o_Ap.ptr(),
+                                                flexible, s, ws, st));
+          } else {
+            precon.Operator(o_r, o_z);
+            LIBP_CALL(libp_pcg_rdotz_f64(N, o_r.ptr(), o_z.ptr(), o_Ap.ptr(), flexible, s, ws, st));
+          }
+          LIBP_CALL(libp_pcg_update_p_f64(N, it == 0, o_z.ptr(), o_p.ptr(), s, st));
+          linearOperator.Operator(o_p, o_Ap);
+          LIBP_CALL(libp_pcg_pap_f64(N, o_p.ptr(), o_Ap.ptr(), s, ws, st));
+          LIBP_CALL(libp_pcg_update_f64(N, o_p.ptr(), o_Ap.ptr(), o_x.ptr(), o_r.ptr(), s, ws, st));
+        }
+        const double* h = platform.readScalars(s, LIBP_PCG_NSCALARS);
+        rdotr0 = h[LIBP_PCG_RDOTR];
+        iter = (int)h[LIBP_PCG_ITERS];
+        if (h[LIBP_PCG_STOP] != 0.0) break;
+      }
+    }
+    stats.iterations = iter;
+    stats.rdotrFinal = rdotr0;
+    return iter;
+  }
 
   int iter;
   for (iter = 0; iter < MAXIT; ++iter) {
@@ -68,6 +103,7 @@ int pcg::Solve(operator_t& linearOperator, operator_t& precon, deviceMemory<dflo
     LIBP_CALL(libp_pcg_update_f64(N, o_p.ptr(), o_Ap.ptr(), o_x.ptr(), o_r.ptr(), s, ws, st));
     const double* h = platform.readScalars(s, LIBP_PCG_NSCALARS);
     rdotr0 = h[LIBP_PCG_RDOTR];
+    precon.iterationDone(iter, rdotr0);
     if (recordHistory) stats.resHistory.push_back(std::sqrt(std::max(rdotr0, 0.0)));
     if (verbose) {
       if (rdotr0 < 0) printf("WARNING CG: rdotr = %17.15lf\n", rdotr0);

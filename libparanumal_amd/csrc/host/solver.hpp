@@ -11,6 +11,13 @@ namespace libp {
 class operator_t {
  public:
   virtual void Operator(deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_Mr) = 0;
+  // Solver notifications (no-ops by default): a solve starts; iteration `it`
+  // ended with residual norm^2 rdotr (already on the host).
+  virtual void solveStarting() {}
+  virtual void iterationDone(int it, double rdotr) {
+    (void)it;
+    (void)rdotr;
+  }
   virtual ~operator_t() {}
 };
 
@@ -31,6 +38,12 @@ class precon_t : public operator_t {
     precon = std::make_shared<Precon>(std::forward<Args>(args)...);
   }
   void Operator(deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_Mr) override { precon->Operator(o_r, o_Mr); }
+  void solveStarting() override {
+    if (precon) precon->solveStarting();
+  }
+  void iterationDone(int it, double rdotr) override {
+    if (precon) precon->iterationDone(it, rdotr);
+  }
   operator_t* get() const { return precon.get(); }
   bool isInitialized() const { return (bool)precon; }
 };
@@ -60,6 +73,7 @@ class linearSolverBase_t {
   dlong N = 0, Nhalo = 0;
   solveStats_t stats;
   bool pureRelative = false;  // stop at |r| <= tol |r0| (no absolute floor)
+  int checkEvery = 1;         // host convergence test every k iterations (the device stops itself)
   bool recordHistory = false;
 
   linearSolverBase_t(dlong _N, dlong _Nhalo, platform_t& _platform, settings_t& _settings, comm_t _comm)

@@ -1,0 +1,275 @@
+// Kernels of the learned per-element smoother around the MLP GEMMs (no
+// reference counterpart): network features, assembly of the element
+// corrections, the energy-optimal step that safeguards each application, and
+// the loss / gradient kernels of training.
+//
+// Network input of element e (nIn = Np + 6*Nfp values): z = D^{-1} r at the
+// element's own nodes (local order) then at the six neighbour traces (the
+// layer of neighbour nodes next to each shared face, ordered by this element's
+// face nodes), zero where there is no node (Dirichlet or no neighbour). The
+// input is divided by its RMS s_e, and the output y_e is multiplied by s_e:
+// the map r -> correction is positively homogeneous of degree one.
+//
+// Correction at gathered node g (rows of the masked gather):
+//   c[g] = (1/|row g|) sum_{copies j} s_{e(j)} y[j] + omega D^{-1}_g r[g].
+#include "reduce.hpp"
+
+namespace libp_amd {
+
+typedef unsigned short u16;
+
+__device__ inline u16 f2bf_l(float f) {
+  unsigned u = __builtin_bit_cast(unsigned, f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (u16)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (u16)(u >> 16);
+}
+
+// One wavefront per element, four per block. NPER = ceil(nIn/64) values per lane.
+template <int NPER>
+__global__ void __launch_bounds__(256) learned_features_kernel(const int E, const int Np, const int Ntr,
+                                                               const int* __restrict__ G2L,
+                                                               const int* __restrict__ traceIds,
+                                                               const float* __restrict__ r,
+                                                               const float* __restrict__ invD, u16* __restrict__ X,
+                                                               const long ldx, float* __restrict__ scale) {
+  const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int l = threadIdx.x & 63;
+  if (e >= E) return;
+  const int nIn = Np + Ntr;
+  float z[NPER];
+  double ss = 0;
+#pragma unroll
+  for (int i = 0; i < NPER; ++i) {
+    const int f = l + 64 * i;
+    int g = -1;
+    if (f < Np) g = G2L[(long)e * Np + f];
+    else if (f < nIn) g = traceIds[(long)e * Ntr + (f - Np)];
+    z[i] = (g >= 0) ? invD[g] * r[g] : 0.f;
+    ss += (double)z[i] * z[i];
+  }
+  ss = wave_sum(ss);
+  const float s = (float)sqrt(ss / nIn);
+  const float inv = (s > 0.f) ? 1.f / s : 0.f;
+#pragma unroll
+  for (int i = 0; i < NPER; ++i) {
+    const int f = l + 64 * i;
+    if (f < nIn) X[(long)e * ldx + f] = f2bf_l(z[i] * inv);
+  }
+  if (l == 0) scale[e] = s;
+}
+
+// c[g] = invLen * sum_j s_e Y[j] + omega * invD[g] r[g]; omega read from device memory
+__global__ void __launch_bounds__(256) learned_assemble_kernel(const int Nrows, const int* __restrict__ rowStarts,
+                                                               const int* __restrict__ colIds, const int Np,
+                                                               const float* __restrict__ Y, const long ldy,
+                                                               const float* __restrict__ scale,
+                                                               const float* __restrict__ omega,
+                                                               const float* __restrict__ r,
+                                                               const float* __restrict__ invD, float* __restrict__ c) {
+  const float om = omega ? *omega : 0.f;
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < Nrows; g += gridDim.x * blockDim.x) {
+    const int a = rowStarts[g], b = rowStarts[g + 1];
+    float acc = 0.f;
+    for (int j = a; j < b; ++j) {
+      const int id = colIds[j];
+      const int e = id / Np, n = id - e * Np;
+      acc += scale[e] * Y[(long)e * ldy + n];
+    }
+    c[g] = (b > a ? acc / (float)(b - a) : 0.f) + om * invD[g] * r[g];
+  }
+}
+
+/* Energy-optimal step along the correction c for the residual res:
+ *   alpha = (c . res) / (c . A c).
+ * sc[0] = c.res, sc[1] = c.Ac, sc[2] = alpha, sc[3] += 1 when the step is
+ * rejected (non-finite or c.Ac <= 0: alpha = 0 then). */
+__global__ void __launch_bounds__(RBS) learned_alpha_kernel(const int N, const float* __restrict__ c,
+                                                            const float* __restrict__ res,
+                                                            const float* __restrict__ Ac, double* __restrict__ sc,
+                                                            ReduceWS* __restrict__ ws) {
+  __shared__ double s_w[16];
+  double a0 = 0, a1 = 0;
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
+    const double cn = c[n];
+    a0 += cn * res[n];
+    a1 += cn * Ac[n];
+  }
+  double v[2] = {a0, a1};
+  bool last;
+  finish_partials<2>(ws, v, s_w, last);
+  if (last) {
+    double out[2];
+    sum_partials<2>(ws, out);
+    if (threadIdx.x == 0) {
+      sc[0] = out[0];
+      sc[1] = out[1];
+      const double alpha = out[0] / out[1];
+      const bool ok = (out[1] > 0.0) && __builtin_isfinite(alpha) && __builtin_isfinite(out[0]);
+      sc[2] = ok ? alpha : 0.0;
+      if (!ok) sc[3] += 1.0;
+    }
+  }
+}
+
+// x = (xIsZero ? 0 : x) + alpha c ; res -= alpha Ac (when res != null)
+__global__ void __launch_bounds__(256) learned_update_kernel(const int N, const int xIsZero,
+                                                             const double* __restrict__ sc,
+                                                             const float* __restrict__ c,
+                                                             const float* __restrict__ Ac, float* __restrict__ x,
+                                                             float* __restrict__ res) {
+  const float alpha = (float)sc[2];
+  // a rejected step (alpha = 0) leaves x and res untouched even where c is not finite
+  const bool take = alpha != 0.f;
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
+    const float xn = xIsZero ? 0.f : x[n];
+    x[n] = take ? xn + alpha * c[n] : xn;
+    if (res && take) res[n] -= alpha * Ac[n];
+  }
+}
+
+/* Training loss of one sample and its gradient with respect to c, with
+ * d = eh - c and Ad = A d (A symmetric):
+ *   J = d.Ad / nrm,  dJ/dc = -2 Ad / nrm,  dJ/domega = sum_g dJ/dc[g] invD[g] r[g].
+ * Writes gc, stat[0] = J, stat[1] += J, grad[ofsOmega] = dJ/domega. */
+__global__ void __launch_bounds__(RBS) learned_loss_kernel(const int N, const float* __restrict__ d,
+                                                           const float* __restrict__ Ad,
+                                                           const float* __restrict__ r,
+                                                           const float* __restrict__ invD,
+                                                           const double* __restrict__ nrm, float* __restrict__ gc,
+                                                           double* __restrict__ stat, float* __restrict__ gradOmega,
+                                                           ReduceWS* __restrict__ ws) {
+  __shared__ double s_w[16];
+  const double scaleG = -2.0 / *nrm;
+  double a0 = 0, a1 = 0;
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
+    const double adn = Ad[n];
+    const double g = scaleG * adn;
+    gc[n] = (float)g;
+    a0 += (double)d[n] * adn;
+    a1 += g * (double)invD[n] * (double)r[n];
+  }
+  double v[2] = {a0, a1};
+  bool last;
+  finish_partials<2>(ws, v, s_w, last);
+  if (last) {
+    double out[2];
+    sum_partials<2>(ws, out);
+    if (threadIdx.x == 0) {
+      const double J = out[0] / *nrm;
+      stat[0] = J;
+      stat[1] += J;
+      if (gradOmega) *gradOmega = (float)out[1];
+    }
+  }
+}
+
+// dY[e][n] = bf16(gc[g] * s_e * invLen[g]) at g = G2L[e*Np+n] >= 0, else 0
+__global__ void __launch_bounds__(256) learned_scatter_grad_kernel(const int E, const int Np,
+                                                                   const int* __restrict__ G2L,
+                                                                   const float* __restrict__ gc,
+                                                                   const float* __restrict__ invLen,
+                                                                   const float* __restrict__ scale,
+                                                                   u16* __restrict__ dY, const long lddy) {
+  const long NL = (long)E * Np;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < NL; i += (long)gridDim.x * blockDim.x) {
+    const int e = (int)(i / Np), n = (int)(i - (long)e * Np);
+    const int g = G2L[i];
+    const float v = (g >= 0) ? gc[g] * scale[e] * invLen[g] : 0.f;
+    dY[(long)e * lddy + n] = f2bf_l(v);
+  }
+}
+
+// invLen[g] = 1 / (rowStarts[g+1] - rowStarts[g])
+__global__ void __launch_bounds__(256) inv_row_length_kernel(const int Nrows, const int* __restrict__ rowStarts,
+                                                             float* __restrict__ invLen) {
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < Nrows; g += gridDim.x * blockDim.x) {
+    const int len = rowStarts[g + 1] - rowStarts[g];
+    invLen[g] = len > 0 ? 1.f / (float)len : 0.f;
+  }
+}
+
+}  // namespace libp_amd
+
+using namespace libp_amd;
+
+extern "C" int libp_learned_features_f32(int Nelements, int Np, int Ntr, const int* GlobalToLocal,
+                                         const int* traceIds, const float* r, const float* invDiag,
+                                         unsigned short* X, long ldx, float* scale, void* stream) {
+  if (Nelements <= 0) return LIBP_OK;
+  if (!GlobalToLocal || (Ntr > 0 && !traceIds) || !r || !invDiag || !X || !scale)
+    return arg_error("learned_features", "null pointer");
+  const int nIn = Np + Ntr;
+  if (Np <= 0 || Ntr < 0 || ldx < nIn) return arg_error("learned_features", "bad size");
+  const int nper = (nIn + 63) / 64;
+  dim3 grid((Nelements + 3) / 4), block(256);
+  hipStream_t s = as_stream(stream);
+#define LF(NP)                                                                                                    \
+  case NP:                                                                                                        \
+    hipLaunchKernelGGL(learned_features_kernel<NP>, grid, block, 0, s, Nelements, Np, Ntr, GlobalToLocal,        \
+                       traceIds, r, invDiag, X, ldx, scale);                                                      \
+    break;
+  switch (nper) {
+    LF(1) LF(2) LF(3) LF(4) LF(5) LF(6) LF(7) LF(8) LF(9) LF(10) LF(11) LF(12) LF(13) LF(14) LF(15) LF(16)
+    default:
+      return arg_error("learned_features", "more than 1024 features per element");
+  }
+#undef LF
+  return launch_status("learned_features");
+}
+
+extern "C" int libp_learned_assemble_f32(int Nrows, const int* rowStarts, const int* colIds, int Np, const float* Y,
+                                         long ldy, const float* scale, const float* omega, const float* r,
+                                         const float* invDiag, float* c, void* stream) {
+  if (Nrows <= 0) return LIBP_OK;
+  if (!rowStarts || !colIds || !Y || !scale || !r || !invDiag || !c) return arg_error("learned_assemble", "null pointer");
+  if (Np <= 0 || ldy < Np) return arg_error("learned_assemble", "bad size");
+  hipLaunchKernelGGL(learned_assemble_kernel, dim3(stream_blocks(Nrows, 256)), dim3(256), 0, as_stream(stream), Nrows,
+                     rowStarts, colIds, Np, Y, ldy, scale, omega, r, invDiag, c);
+  return launch_status("learned_assemble");
+}
+
+extern "C" int libp_learned_alpha_f32(int N, const float* c, const float* res, const float* Ac, double* sc, void* ws,
+                                      void* stream) {
+  if (!c || !res || !Ac || !sc || !ws) return arg_error("learned_alpha", "null pointer");
+  hipLaunchKernelGGL(learned_alpha_kernel, dim3(reduce_blocks(N)), dim3(RBS), 0, as_stream(stream), N, c, res, Ac, sc,
+                     (ReduceWS*)ws);
+  return launch_status("learned_alpha");
+}
+
+extern "C" int libp_learned_update_f32(int N, int xIsZero, const double* sc, const float* c, const float* Ac, float* x,
+                                       float* res, void* stream) {
+  if (N <= 0) return LIBP_OK;
+  if (!sc || !c || !x || (res && !Ac)) return arg_error("learned_update", "null pointer");
+  hipLaunchKernelGGL(learned_update_kernel, dim3(stream_blocks(N, 256)), dim3(256), 0, as_stream(stream), N, xIsZero, sc,
+                     c, Ac, x, res);
+  return launch_status("learned_update");
+}
+
+extern "C" int libp_learned_loss_f32(int N, const float* d, const float* Ad, const float* r, const float* invDiag,
+                                     const double* nrm, float* gc, double* stat, float* gradOmega, void* ws,
+                                     void* stream) {
+  if (!d || !Ad || !r || !invDiag || !nrm || !gc || !stat || !ws) return arg_error("learned_loss", "null pointer");
+  hipLaunchKernelGGL(learned_loss_kernel, dim3(reduce_blocks(N)), dim3(RBS), 0, as_stream(stream), N, d, Ad, r, invDiag,
+                     nrm, gc, stat, gradOmega, (ReduceWS*)ws);
+  return launch_status("learned_loss");
+}
+
+extern "C" int libp_learned_scatter_grad_f32(int Nelements, int Np, const int* GlobalToLocal, const float* gc,
+                                             const float* invLen, const float* scale, unsigned short* dY, long lddy,
+                                             void* stream) {
+  if (Nelements <= 0) return LIBP_OK;
+  if (!GlobalToLocal || !gc || !invLen || !scale || !dY) return arg_error("learned_scatter_grad", "null pointer");
+  if (lddy < Np) return arg_error("learned_scatter_grad", "bad size");
+  hipLaunchKernelGGL(learned_scatter_grad_kernel, dim3(stream_blocks((long)Nelements * Np, 256)), dim3(256), 0,
+                     as_stream(stream), Nelements, Np, GlobalToLocal, gc, invLen, scale, dY, lddy);
+  return launch_status("learned_scatter_grad");
+}
+
+extern "C" int libp_inv_row_length_f32(int Nrows, const int* rowStarts, float* invLen, void* stream) {
+  if (Nrows <= 0) return LIBP_OK;
+  if (!rowStarts || !invLen) return arg_error("inv_row_length", "null pointer");
+  hipLaunchKernelGGL(inv_row_length_kernel, dim3(stream_blocks(Nrows, 256)), dim3(256), 0, as_stream(stream), Nrows,
+                     rowStarts, invLen);
+  return launch_status("inv_row_length");
+}

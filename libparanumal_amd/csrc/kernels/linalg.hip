@@ -1,80 +1,10 @@
 // BLAS-1, deterministic reductions to device scalars, fused PCG/FPCG vector
 // kernels and Chebyshev smoother steps, FP64 and FP32.
 //
-// Reductions: each block reduces a fixed grid-stride slice (wavefront shuffles,
-// then one LDS step across the block's waves) and stores a partial; the last
-// block to finish (ticket counter) sums the partials in block order and writes
-// the scalar. The block count depends only on N, so results are reproducible
-// run to run. The counter is reset by the last block, so no memset is needed
-// between calls (safe under stream capture).
-#include "kcommon.hpp"
+// Reductions use the deterministic two-stage scheme of reduce.hpp.
+#include "reduce.hpp"
 
 namespace libp_amd {
-
-constexpr int RBS = 256;       // reduction block size
-constexpr int RMAXB = 1024;    // max reduction blocks
-constexpr int RNPART = 3;      // partial arrays available (fused multi-dot)
-
-struct ReduceWS {
-  double part[RNPART][RMAXB];
-  unsigned int counter;
-};
-
-inline unsigned reduce_blocks(long N) {
-  long b = (N + RBS * 4 - 1) / (RBS * 4);
-  if (b < 1) b = 1;
-  if (b > RMAXB) b = RMAXB;
-  return (unsigned)b;
-}
-
-// Block-level sum; result valid in thread 0.
-__device__ inline double block_sum(double v, double* s_w) {
-  v = wave_sum(v);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) s_w[w] = v;
-  __syncthreads();
-  double r = 0;
-  if (threadIdx.x == 0) {
-    const int nw = (blockDim.x + 63) >> 6;
-    for (int i = 0; i < nw; ++i) r += s_w[i];
-  }
-  __syncthreads();
-  return r;
-}
-
-// Stores partials[k][blockIdx] for k < NV; returns true in thread 0 of the last block.
-template <int NV>
-__device__ inline bool finish_partials(ReduceWS* ws, const double (&v)[NV], double* s_w, bool& last) {
-#pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    double b = block_sum(v[k], s_w);
-    if (threadIdx.x == 0) ws->part[k][blockIdx.x] = b;
-  }
-  __shared__ bool s_last;
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned ticket = atomicAdd(&ws->counter, 1u);
-    s_last = (ticket == gridDim.x - 1);
-  }
-  __syncthreads();
-  last = s_last;
-  return last;
-}
-
-template <int NV>
-__device__ inline void sum_partials(ReduceWS* ws, double (&out)[NV]) {
-  // executed by the whole last block: each wave lane sums a strided subset in
-  // a fixed order, then a fixed tree; deterministic for a given grid size.
-  __shared__ double s_w[16];
-  __threadfence();
-#pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    double acc = 0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) acc += __builtin_nontemporal_load(&ws->part[k][b]);
-    out[k] = block_sum(acc, s_w);
-  }
-  if (threadIdx.x == 0) ws->counter = 0;
-}
 
 /* ------------------------------ reductions ------------------------------ */
 template <int KIND>  // 0 x.y, 1 x.x, 2 sum x, 3 w.x.y
@@ -162,6 +92,7 @@ __global__ void __launch_bounds__(RBS) pcg_rdotz_kernel(const int N, const doubl
                                                         const double* __restrict__ Ap, const int flexible,
                                                         double* __restrict__ s, ReduceWS* __restrict__ ws) {
   __shared__ double s_w[16];
+  if (s[8] != 0.0) return;
   double a0 = 0, a1 = 0;
   for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
     const double rn = r[n];
@@ -195,6 +126,7 @@ __global__ void __launch_bounds__(RBS) pcg_rdotz_kernel(const int N, const doubl
 // p = z + beta*p  (axpy(N, 1, z, beta, p); beta==0 on the first iteration: p = z)
 __global__ void __launch_bounds__(256) pcg_update_p_kernel(const int N, const int first, const double* __restrict__ z,
                                                            double* __restrict__ p, const double* __restrict__ s) {
+  if (s[8] != 0.0) return;
   const double beta = first ? 0.0 : s[3];
   for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x)
     p[n] = (beta != 0) ? 1.0 * z[n] + beta * p[n] : 1.0 * z[n];
@@ -204,6 +136,7 @@ __global__ void __launch_bounds__(RBS) pcg_pap_kernel(const int N, const double*
                                                       const double* __restrict__ Ap, double* __restrict__ s,
                                                       ReduceWS* __restrict__ ws) {
   __shared__ double s_w[16];
+  if (s[8] != 0.0) return;
   double a0 = 0;
   for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) a0 += p[n] * Ap[n];
   double v[1] = {a0};
@@ -225,6 +158,7 @@ __global__ void __launch_bounds__(RBS) pcg_update_kernel(const int N, const doub
                                                          double* __restrict__ r, double* __restrict__ s,
                                                          ReduceWS* __restrict__ ws) {
   __shared__ double s_w[16];
+  if (s[8] != 0.0) return;  // converged in an earlier iteration of this chunk
   const double alpha = s[2];
   double a0 = 0;
   for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
@@ -240,7 +174,11 @@ __global__ void __launch_bounds__(RBS) pcg_update_kernel(const int N, const doub
   if (last) {
     double out[1];
     sum_partials<1>(ws, out);
-    if (threadIdx.x == 0) s[5] = out[0];
+    if (threadIdx.x == 0) {
+      s[5] = out[0];
+      s[9] += 1.0;
+      if (s[7] > 0.0 && out[0] <= s[7]) s[8] = 1.0;
+    }
   }
 }
 

@@ -1,0 +1,90 @@
+"""Train the learned fine smoother of the p-MG preconditioner on the GPU and
+write its weight file and a manifest next to it.
+
+    python -m libparanumal_amd.train_learned [--nx 32] [--out libparanumal_amd/weights/learned_hex_n7.bin]
+
+Workload: curved hex box nx^3 on [-1,1]^3 (x += 0.05 sin(2 pi y) sin(2 pi z)),
+N = 7, lambda = 0, Dirichlet, seeded 64-mode Fourier forcing; FPCG with p-MG
+7 -> 3 -> 1, FP32 smoothing, FP64 Jacobi-PCG coarse level. Training pairs come
+from solves with the training seeds (default 101-104, then 201-202 with the
+network in the cycle); the benchmark's test seed (1) is not among them.
+Everything runs through the library's own kernels (libp_mlp_*, libp_adam_step,
+libp_learned_*).
+"""
+import argparse
+import hashlib
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+DEFAULT_OUT = os.path.join(ROOT, "libparanumal_amd", "weights", "learned_hex_n7.bin")
+
+
+def settings(nx=32, smoother="LEARNED", seed=1, precision="FP32", weights=None):
+    s = {"MESH FILE": "BOX", "MESH DIMENSION": 3, "ELEMENT TYPE": 12, "BOX NX": nx, "BOX NY": nx, "BOX NZ": nx,
+         "BOX DIMX": 2, "BOX DIMY": 2, "BOX DIMZ": 2, "BOX BOUNDARY FLAG": 1, "POLYNOMIAL DEGREE": 7,
+         "BOX CURVED MAP AMPLITUDE": 0.05, "LAMBDA": 0, "DATA FILE": "ellipticFourier3D", "DATA SEED": seed,
+         "DISCRETIZATION": "CONTINUOUS", "LINEAR SOLVER": "FPCG", "PRECONDITIONER": "MULTIGRID",
+         "MULTIGRID COARSENING": "CUSTOM", "MULTIGRID DEGREES": "7,3,1", "MULTIGRID SMOOTHER": smoother,
+         "MULTIGRID CHEBYSHEV DEGREE": 2, "MULTIGRID PRECISION": precision,
+         "MULTIGRID COARSE TOLERANCE": 1e-2, "MULTIGRID COARSE MAXIT": 200,
+         "LINEAR SOLVER STOPPING CRITERION": "RELATIVE", "LINEAR SOLVER TOLERANCE": 1e-8}
+    if weights:
+        s["MULTIGRID LEARNED WEIGHTS"] = weights
+    return s
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--nx", type=int, default=32)
+    ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--seeds", default="101,102,103,104")
+    ap.add_argument("--dagger-seeds", default="201,202")
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--dagger", type=int, default=1)
+    ap.add_argument("--dagger-steps", type=int, default=1500)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--init-seed", type=int, default=1)
+    ap.add_argument("--mode", default="REPLACE", choices=["REPLACE", "PRE"])
+    a = ap.parse_args(argv)
+
+    import libparanumal_amd as lp
+    s = settings(a.nx)
+    s["MULTIGRID LEARNED MODE"] = a.mode
+    t0 = time.time()
+    el = lp.Elliptic(s)
+    setup = time.time() - t0
+    opts = dict(seeds=a.seeds, daggerSeeds=a.dagger_seeds, steps=a.steps, dagger=a.dagger,
+                daggerSteps=a.dagger_steps, lr=a.lr, initSeed=a.init_seed, logEvery=250, verbose=1)
+    t0 = time.time()
+    hist = el.learned_train(**opts)
+    train_s = time.time() - t0
+    os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+    el.learned_save(a.out)
+    info = el.learned_info()
+    el.close()
+    with open(a.out, "rb") as f:
+        sha = hashlib.sha256(f.read()).hexdigest()
+    manifest = {
+        "weights": os.path.basename(a.out), "sha256": sha, "parameters": int(info["parameters"]),
+        "network": {"inputs": int(info["inputs"]), "hidden": [int(info["hidden"])] * 2, "outputs": 512,
+                    "activation": "relu", "weights": "bf16 (fp32 masters)", "accumulate": "fp32"},
+        "workload": {k: s[k] for k in ("BOX NX", "POLYNOMIAL DEGREE", "BOX CURVED MAP AMPLITUDE", "LAMBDA",
+                                       "DATA FILE", "MULTIGRID DEGREES", "MULTIGRID PRECISION")},
+        "mode": a.mode, "training": opts, "loss_per_250_steps": hist,
+        "setup_seconds": setup, "training_seconds": train_s,
+    }
+    with open(os.path.splitext(a.out)[0] + ".json", "w") as f:
+        json.dump(manifest, f, indent=1)
+    print(json.dumps({"loss_first": hist[0] if hist else None, "loss_last": hist[-1] if hist else None,
+                      "training_seconds": train_s, "out": a.out}))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -306,4 +306,31 @@ double orc_time_partial_ax(void* p, int reps) {
   return t[t.size() / 2];
 }
 
+/* ---- learned smoother oracle ---- */
+long orc_learned_trace(void* p, int* out) {
+  Handle* h = (Handle*)p;
+  std::vector<int> t;
+  LearnedTraceIds(h->el, t);
+  if (out) std::memcpy(out, t.data(), sizeof(int) * t.size());
+  return (long)t.size();
+}
+
+double orc_learned_loss_grad(void* p, int H, const double* theta, const double* r, const double* eh, double nrm,
+                             const double* invD, int bf, double* grad, double* c) {
+  Handle* h = (Handle*)p;
+  std::vector<int> t;
+  LearnedTraceIds(h->el, t);
+  return LearnedLossGrad(h->el, t, H, theta, r, eh, nrm, invD, bf != 0, grad, c);
+}
+
+void orc_learned_forward(void* p, int H, const double* theta, const double* r, const double* invD, int bf, double* X,
+                         double* scale, double* Y, double* c) {
+  Handle* h = (Handle*)p;
+  std::vector<int> t;
+  LearnedTraceIds(h->el, t);
+  LearnedForward(h->el, t, H, theta, r, invD, bf != 0, X, scale, Y, c);
+}
+
+double orc_bf16r(double x) { return bf16r(x); }
+
 }  // extern "C"

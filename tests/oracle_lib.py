@@ -56,6 +56,12 @@ def load():
             "orc_drand48_seq": (None, [c_long, c_int, c_void_p]),
             "orc_hess_eigs": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
             "orc_time_partial_ax": (c_double, [c_void_p, c_int]),
+            "orc_learned_trace": (c_long, [c_void_p, c_void_p]),
+            "orc_learned_loss_grad": (c_double, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p,
+                                                 c_int, c_void_p, c_void_p]),
+            "orc_learned_forward": (None, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                                           c_void_p, c_void_p]),
+            "orc_bf16r": (c_double, [c_double]),
         }
         for n, (r, a) in sig.items():
             f = getattr(L, n)
@@ -150,6 +156,57 @@ class OracleElliptic:
         r = np.ascontiguousarray(r, dtype=np.float64)
         self.L.orc_mg_apply(self.h, r.ctypes.data, z.ctypes.data)
         return z
+
+
+def learned_dims(Np, Nfp, H):
+    """Offsets of the learned smoother's parameter vector (W1, b1, W2, b2, W3, b3, omega)."""
+    nIn, nOut = Np + 6 * Nfp, Np
+    o = {"W1": 0}
+    o["b1"] = o["W1"] + H * nIn
+    o["W2"] = o["b1"] + H
+    o["b2"] = o["W2"] + H * H
+    o["W3"] = o["b2"] + H
+    o["b3"] = o["W3"] + nOut * H
+    o["omega"] = o["b3"] + nOut
+    o["n"] = o["omega"] + 1
+    o["nIn"], o["nOut"], o["H"] = nIn, nOut, H
+    return o
+
+
+class OracleLearned:
+    """The learned smoother's CPU restatement on an oracle elliptic handle."""
+
+    def __init__(self, el, H):
+        self.el, self.L, self.H = el, el.L, H
+        self.Np, self.E, self.Ng = el.geti("Np"), el.geti("Nelements"), el.geti("Ngather")
+        self.Nfp = round(self.Np ** (2.0 / 3.0))
+        self.dims = learned_dims(self.Np, self.Nfp, H)
+
+    def trace(self):
+        n = self.L.orc_learned_trace(self.el.h, None)
+        out = np.empty(n, dtype=np.int32)
+        self.L.orc_learned_trace(self.el.h, out.ctypes.data)
+        return out
+
+    def forward(self, theta, r, invD, bf16=False):
+        d = self.dims
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        invD = np.ascontiguousarray(invD, dtype=np.float64)
+        X = np.empty(self.E * d["nIn"]); s = np.empty(self.E); Y = np.empty(self.E * d["nOut"]); c = np.empty(self.Ng)
+        self.L.orc_learned_forward(self.el.h, self.H, th.ctypes.data, r.ctypes.data, invD.ctypes.data, int(bf16),
+                                   X.ctypes.data, s.ctypes.data, Y.ctypes.data, c.ctypes.data)
+        return X.reshape(self.E, -1), s, Y.reshape(self.E, -1), c
+
+    def loss_grad(self, theta, r, eh, nrm, invD, bf16=False, grad=True):
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        eh = np.ascontiguousarray(eh, dtype=np.float64)
+        invD = np.ascontiguousarray(invD, dtype=np.float64)
+        g = np.empty(self.dims["n"]) if grad else None
+        J = self.L.orc_learned_loss_grad(self.el.h, self.H, th.ctypes.data, r.ctypes.data, eh.ctypes.data, nrm,
+                                         invD.ctypes.data, int(bf16), g.ctypes.data if grad else None, None)
+        return J, g
 
 
 def run(settings):

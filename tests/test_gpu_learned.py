@@ -1,0 +1,304 @@
+"""GPU tests of the learned fine smoother: the BF16 MFMA GEMM kernels against
+numpy, the network features / correction / loss gradient against the CPU
+oracle (which rounds to BF16 where the kernels store BF16), training on the
+GPU, use inside FPCG + p-MG, the weight file checks and the Chebyshev
+fallback."""
+import os
+
+import numpy as np
+import pytest
+
+from tests.oracle_lib import OracleElliptic, OracleLearned
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def lp():
+    import libparanumal_amd
+    return libparanumal_amd
+
+
+def L():
+    return lp().lib()
+
+
+def stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+def sync():
+    torch.cuda.synchronize()
+
+
+def to_bf16(x):
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return u.astype(np.uint16)
+
+
+def from_bf16(h):
+    return (h.astype(np.uint32) << 16).view(np.float32)
+
+
+def dev(a):
+    a = np.ascontiguousarray(a)
+    if a.dtype == np.uint16:
+        return torch.from_numpy(a.view(np.int16)).cuda()
+    return torch.from_numpy(a).cuda()
+
+
+def host_bf16(t):
+    return t.cpu().numpy().view(np.uint16)
+
+
+def relnorm(a, b):
+    return float(np.linalg.norm(np.asarray(a, dtype=np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+# ---------------------------------------------------------------------------
+# GEMM kernels
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("M,K,N", [(300, 160, 96), (257, 150, 130), (1024, 896, 256)])
+def test_mlp_forward_against_numpy(M, K, N):
+    print(f"mlp fwd M={M} K={K} N={N}", flush=True)
+    rng = np.random.default_rng(M + K + N)
+    Xb = to_bf16(rng.standard_normal((M, K)))
+    Wb = to_bf16(rng.standard_normal((N, K)) / np.sqrt(K))
+    b = rng.standard_normal(N).astype(np.float32)
+    ref = from_bf16(Xb).astype(np.float64) @ from_bf16(Wb).astype(np.float64).T + b
+    X, W, bd = dev(Xb), dev(Wb), dev(b)
+    Y = torch.empty((M, N), dtype=torch.float32, device="cuda")
+    lp().check(L().libp_mlp_linear_fwd(M, K, N, X.data_ptr(), K, W.data_ptr(), K, bd.data_ptr(), 0, Y.data_ptr(), N,
+                                       stream()))
+    sync()
+    assert relnorm(Y.cpu().numpy(), ref) < 1e-5
+    H = torch.empty((M, N), dtype=torch.int16, device="cuda")
+    lp().check(L().libp_mlp_linear_fwd(M, K, N, X.data_ptr(), K, W.data_ptr(), K, bd.data_ptr(), 1, H.data_ptr(), N,
+                                       stream()))
+    sync()
+    got = from_bf16(host_bf16(H))
+    assert relnorm(got, np.maximum(ref, 0)) < 5e-3
+    assert (got[ref < -1e-2] == 0).all()
+
+
+@pytest.mark.parametrize("M,K,N", [(300, 160, 96), (3000, 256, 512)])
+def test_mlp_backward_against_numpy(M, K, N):
+    print(f"mlp bwd M={M} K={K} N={N}", flush=True)
+    rng = np.random.default_rng(7 + M)
+    dYb = to_bf16(rng.standard_normal((M, N)))
+    Wb = to_bf16(rng.standard_normal((N, K)) / np.sqrt(K))
+    Hb = to_bf16(np.maximum(rng.standard_normal((M, K)), 0))
+    Xb = to_bf16(rng.standard_normal((M, K)))
+    dY, W, H, X = (from_bf16(a).astype(np.float64) for a in (dYb, Wb, Hb, Xb))
+    # dX = (dY W) .* (H > 0)
+    ref = (dY @ W) * (H > 0)
+    dXd = torch.empty((M, K), dtype=torch.int16, device="cuda")
+    g_dY, g_W, g_H, g_X = dev(dYb), dev(Wb), dev(Hb), dev(Xb)
+    lp().check(L().libp_mlp_linear_bwd_data(M, K, N, g_dY.data_ptr(), N, g_W.data_ptr(), K, g_H.data_ptr(), K,
+                                            dXd.data_ptr(), K, stream()))
+    # dWt = dY^T X, db = colsum dY
+    ws = torch.empty(L().libp_mlp_ws_bytes(M, K, N), dtype=torch.uint8, device="cuda")
+    dW = torch.empty((N, K), dtype=torch.float32, device="cuda")
+    db = torch.empty(N, dtype=torch.float32, device="cuda")
+    lp().check(L().libp_mlp_linear_bwd_weight(M, K, N, g_dY.data_ptr(), N, g_X.data_ptr(), K, dW.data_ptr(),
+                                              db.data_ptr(), ws.data_ptr(), stream()))
+    sync()
+    assert relnorm(from_bf16(host_bf16(dXd)), ref) < 5e-3
+    assert relnorm(dW.cpu().numpy(), dY.T @ X) < 1e-5
+    assert relnorm(db.cpu().numpy(), dY.sum(0)) < 1e-5
+
+
+def test_adam_step_against_numpy():
+    rng = np.random.default_rng(3)
+    n = 10007
+    p, g = rng.standard_normal(n).astype(np.float32), rng.standard_normal(n).astype(np.float32)
+    m, v = (0.1 * rng.standard_normal(n)).astype(np.float32), np.abs(rng.standard_normal(n)).astype(np.float32)
+    lr, b1, b2, eps, t = 1e-3, 0.9, 0.999, 1e-8, 5
+    m2 = b1 * m + (1 - b1) * g
+    v2 = b2 * v + (1 - b2) * g * g
+    p2 = p - lr * (m2 / (1 - b1 ** t)) / (np.sqrt(v2 / (1 - b2 ** t)) + eps)
+    P, G, Mm, V = dev(p), dev(g), dev(m), dev(v)
+    B = torch.empty(n, dtype=torch.int16, device="cuda")
+    lp().check(L().libp_adam_step(n, P.data_ptr(), B.data_ptr(), G.data_ptr(), Mm.data_ptr(), V.data_ptr(), lr, b1, b2,
+                                  eps, t, stream()))
+    sync()
+    assert np.allclose(P.cpu().numpy(), p2, rtol=1e-6, atol=1e-7)
+    assert np.array_equal(host_bf16(B), to_bf16(P.cpu().numpy()))
+
+
+# ---------------------------------------------------------------------------
+# features, correction and gradient against the oracle
+# ---------------------------------------------------------------------------
+def box(n, N, curve=0.05, lam=0.0, **extra):
+    s = {"MESH FILE": "BOX", "MESH DIMENSION": 3, "ELEMENT TYPE": 12, "BOX NX": n, "BOX NY": n + 1, "BOX NZ": n - 1,
+         "BOX DIMX": 2, "BOX DIMY": 2, "BOX DIMZ": 2, "BOX BOUNDARY FLAG": 1, "POLYNOMIAL DEGREE": N,
+         "LAMBDA": lam, "DATA FILE": "data/ellipticSine3D.h", "BOX CURVED MAP AMPLITUDE": curve}
+    s.update(extra)
+    return s
+
+
+def learned_settings(n, N, degrees, **kw):
+    s = box(n, N, **{"LINEAR SOLVER": "FPCG", "PRECONDITIONER": "MULTIGRID", "MULTIGRID COARSENING": "CUSTOM",
+                     "MULTIGRID DEGREES": degrees, "MULTIGRID PRECISION": "FP32", "MULTIGRID SMOOTHER": "LEARNED"})
+    s.update(kw)
+    return s
+
+
+def random_theta(d, seed):
+    rng = np.random.default_rng(seed)
+    th = np.zeros(d["n"], dtype=np.float32)
+    H, nIn, nOut = d["H"], d["nIn"], d["nOut"]
+    th[d["W1"]:d["b1"]] = rng.uniform(-1, 1, H * nIn) * np.sqrt(6 / nIn)
+    th[d["b1"]:d["W2"]] = rng.uniform(-0.1, 0.1, H)
+    th[d["W2"]:d["b2"]] = rng.uniform(-1, 1, H * H) * np.sqrt(6 / H)
+    th[d["b2"]:d["W3"]] = rng.uniform(-0.1, 0.1, H)
+    th[d["W3"]:d["b3"]] = rng.uniform(-1, 1, nOut * H) * np.sqrt(3 / H)
+    th[d["b3"]:d["omega"]] = rng.uniform(-0.1, 0.1, nOut)
+    th[d["omega"]] = 0.4
+    return th
+
+
+@pytest.fixture(scope="module")
+def pipeline():
+    s = learned_settings(4, 3, "3,1")
+    p = lp().Elliptic(s)
+    o = OracleElliptic(box(4, 3))
+    ol = OracleLearned(o, 256)
+    th = random_theta(ol.dims, 5)
+    p.learned_set_params(th, activate=False)
+    invD = p.host_array("learnedInvDiag")
+    return p, o, ol, th, invD
+
+
+def test_learned_trace_bitexact(pipeline):
+    p, o, ol, th, invD = pipeline
+    assert np.array_equal(p.host_array("learnedTrace"), ol.trace())
+
+
+def test_learned_correction_vs_oracle(pipeline):
+    p, o, ol, th, invD = pipeline
+    r = np.random.default_rng(21).standard_normal(ol.Ng).astype(np.float32)
+    rd = dev(r)
+    c = torch.empty_like(rd)
+    p.learned_correction(rd, c)
+    X, s, Y, cref = ol.forward(th, r, invD, bf16=True)
+    assert relnorm(p.host_array("learnedScale"), s) < 1e-6
+    assert relnorm(p.host_array("learnedY"), Y.reshape(-1)) < 2e-2
+    assert relnorm(c.cpu().numpy(), cref) < 2e-2
+    # unrounded oracle: BF16 storage costs well under a percent here
+    _, _, _, c64 = ol.forward(th, r, invD, bf16=False)
+    assert relnorm(c.cpu().numpy(), c64) < 5e-2
+
+
+def test_learned_gradient_vs_oracle(pipeline):
+    p, o, ol, th, invD = pipeline
+    rng = np.random.default_rng(22)
+    r = rng.standard_normal(ol.Ng).astype(np.float32)
+    eh = rng.standard_normal(ol.Ng).astype(np.float32)
+    nrm = float(eh.astype(np.float64) @ o.operator(eh.astype(np.float64)))
+    J = p.learned_loss_grad(dev(r), dev(eh), nrm)
+    Jref, gref = ol.loss_grad(th, r, eh, nrm, invD, bf16=True)
+    assert abs(J - Jref) < 1e-4 * abs(Jref)
+    g = p.host_array("learnedGrad")
+    d = ol.dims
+    for blk, (a, b) in {"W1": (d["W1"], d["b1"]), "b1": (d["b1"], d["W2"]), "W2": (d["W2"], d["b2"]),
+                        "b2": (d["b2"], d["W3"]), "W3": (d["W3"], d["b3"]), "b3": (d["b3"], d["omega"]),
+                        "omega": (d["omega"], d["n"])}.items():
+        err = relnorm(g[a:b], gref[a:b])
+        print(f"gradient block {blk}: rel err {err:.2e}", flush=True)
+        assert err < 3e-2, blk
+
+
+# ---------------------------------------------------------------------------
+# training, solves, weight files, fallback
+# ---------------------------------------------------------------------------
+def solve_settings(**kw):
+    s = learned_settings(4, 7, "7,3,1", **{"DATA FILE": "ellipticFourier3D", "DATA SEED": 7,
+                                            "LINEAR SOLVER STOPPING CRITERION": "RELATIVE",
+                                            "MULTIGRID COARSE TOLERANCE": 1e-2, "MULTIGRID COARSE MAXIT": 100,
+                                            "MAXIMUM ITERATIONS": 200})
+    s.update(kw)
+    return s
+
+
+@pytest.fixture(scope="module")
+def trained(tmp_path_factory):
+    p = lp().Elliptic(solve_settings())
+    th0 = p.learned_params()
+    print("training on seeds 101,102", flush=True)
+    hist = p.learned_train(seeds=[101, 102], daggerSeeds=[201], steps=400, dagger=1, daggerSteps=200, lr=1e-3,
+                           logEvery=100, verbose=1)
+    path = tmp_path_factory.mktemp("w") / "learned.bin"
+    p.learned_save(path)
+    return p, th0, hist, path
+
+
+def test_training_lowers_the_loss(trained):
+    p, th0, hist, path = trained
+    print("loss history", hist, flush=True)
+    assert len(hist) == 6
+    assert hist[3] < 0.8 * hist[0]
+    assert np.isfinite(hist).all()
+
+
+def test_learned_pmg_solve_and_gain_over_untrained(trained):
+    p, th0, hist, path = trained
+    r = p.run()
+    info = p.learned_info()
+    print("learned", r, info, p.learned_status(), flush=True)
+    assert r["r_final"] <= 1e-8 * r["r0"] * (1 + 1e-12)
+    # the same V-cycle with the untrained network (initially omega D^-1 r, a damped Jacobi step)
+    q = lp().Elliptic(solve_settings())
+    q.learned_set_params(th0, activate=True)
+    r0 = q.run()
+    print("untrained", r0, q.learned_info(), q.learned_status(), flush=True)
+    assert r0["r_final"] <= 1e-8 * r0["r0"] * (1 + 1e-12)
+    if not info["fallback"]:
+        assert r["iterations"] < r0["iterations"]
+    # same discrete solution as the Chebyshev smoother
+    c = lp().Elliptic(solve_settings(**{"MULTIGRID SMOOTHER": "CHEBYSHEV"})).run()
+    assert abs(r["norm"] - c["norm"]) < 1e-7 * c["norm"]
+
+
+def test_weight_file_round_trip_and_hash(trained, tmp_path):
+    p, th0, hist, path = trained
+    q = lp().Elliptic(solve_settings())
+    assert q.learned_load(path), q.learned_status()
+    assert np.array_equal(q.learned_params(), p.learned_params())
+    assert q.learned_info()["active"] == 1
+    bad = tmp_path / "corrupt.bin"
+    data = bytearray(open(path, "rb").read())
+    data[100] ^= 0x40
+    open(bad, "wb").write(bytes(data))
+    q2 = lp().Elliptic(solve_settings())
+    assert not q2.learned_load(bad)
+    assert "hash" in q2.learned_status()
+    assert q2.learned_info()["active"] == 0
+    r = q2.run()  # Chebyshev smoothing
+    assert r["r_final"] <= 1e-8 * r["r0"] * (1 + 1e-12)
+
+
+def test_settings_file_loads_weights(trained):
+    p, th0, hist, path = trained
+    q = lp().Elliptic(solve_settings(**{"MULTIGRID LEARNED WEIGHTS": str(path)}))
+    assert q.learned_info()["active"] == 1
+    assert np.array_equal(q.learned_params(), p.learned_params())
+
+
+def test_nan_injection_falls_back_to_chebyshev(trained):
+    p, th0, hist, path = trained
+    th = p.learned_params().copy()
+    from tests.oracle_lib import learned_dims
+    dims = learned_dims(512, 64, 256)
+    th[dims["b3"] + 5] = np.nan
+    q = lp().Elliptic(solve_settings())
+    q.learned_set_params(th, activate=True)
+    r = q.run()
+    info = q.learned_info()
+    print("nan injection", r, info, q.learned_status(), flush=True)
+    assert info["fallback"] == 1 and info["rejected_steps"] >= 1
+    assert "fallback" in q.learned_status()
+    assert r["r_final"] <= 1e-8 * r["r0"] * (1 + 1e-12)
+    assert np.isfinite(r["norm"])

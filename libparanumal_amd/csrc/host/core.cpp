@@ -144,8 +144,9 @@ settings_t ellipticDefaultSettings() {
                "Coarse Jacobi-PCG: iterations queued between host convergence reads (same iterates as 1)");
   s.newSetting("MULTIGRID LEARNED WEIGHTS", "", "Weight file of the learned fine smoother");
   s.newSetting("MULTIGRID LEARNED MODE", "REPLACE",
-               "REPLACE: network pre- and post-smoothing; PRE: network pre-smoothing, Chebyshev post-smoothing",
-               {"REPLACE", "PRE"});
+               "REPLACE: network pre- and post-smoothing; PRE: network pre-smoothing, Chebyshev post-smoothing; "
+               "AUGMENT: Chebyshev smoothing plus one network step after the pre-smoothing",
+               {"REPLACE", "PRE", "AUGMENT"});
   s.newSetting("LINEAR SOLVER TOLERANCE", "1e-8");
   s.newSetting("MAXIMUM ITERATIONS", "5000");
   s.newSetting("DATA SEED", "1", "Seed of the Fourier forcing data set");

@@ -256,6 +256,15 @@ void learnedSmoother_t::smoothPre(deviceMemory<float>& o_r, deviceMemory<float>&
   LIBP_CALL(libp_learned_update_f32(Ndofs, 1, o_sc.ptr(), o_c.ptr(), o_Ac.ptr(), o_x.ptr(), res, s));
 }
 
+void learnedSmoother_t::augment(deviceMemory<float>& o_x, deviceMemory<float>& o_resIO, const float* invDiag,
+                                const op32_t& A) {
+  void* s = platform.s();
+  correction(o_resIO.ptr(), invDiag);
+  A(o_c, o_Ac);
+  LIBP_CALL(libp_learned_alpha_f32(Ndofs, o_c.ptr(), o_resIO.ptr(), o_Ac.ptr(), o_sc.ptr(), o_rws.ptr(), s));
+  LIBP_CALL(libp_learned_update_f32(Ndofs, 0, o_sc.ptr(), o_c.ptr(), o_Ac.ptr(), o_x.ptr(), o_resIO.ptr(), s));
+}
+
 void learnedSmoother_t::smoothPost(deviceMemory<float>& o_r, deviceMemory<float>& o_x, const float* invDiag,
                                    const op32_t& A) {
   void* s = platform.s();
@@ -303,7 +312,8 @@ void learnedSmoother_t::trainStep(const float* r, const float* eh, const double*
 
 void learnedSmoother_t::monitorStart() {
   monitorHist.clear();
-  if (!active || fallback) return;
+  if (!active) return;
+  fallback = false;  // every solve starts with the network
   monitorRejected = stats()[3];
 }
 

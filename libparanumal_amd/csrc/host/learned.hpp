@@ -73,6 +73,8 @@ class learnedSmoother_t {
   // V-cycle residual comes free with the step)
   void smoothPre(deviceMemory<float>& o_r, deviceMemory<float>& o_x, deviceMemory<float>* o_resOut,
                  const float* invDiag, const op32_t& A);
+  // one network step on a given residual: x += alpha c(res), res -= alpha A c
+  void augment(deviceMemory<float>& o_x, deviceMemory<float>& o_res, const float* invDiag, const op32_t& A);
   // post-smoothing: res = r - A x, x += alpha c(res)
   void smoothPost(deviceMemory<float>& o_r, deviceMemory<float>& o_x, const float* invDiag, const op32_t& A);
 
@@ -80,9 +82,10 @@ class learnedSmoother_t {
   // the device) into o_grad, then one Adam step when update is set
   void trainStep(const float* r, const float* eh, const double* nrm, const float* invDiag, const op32_t& A, float lr,
                  bool update = true);
-  // run-time acceptance: the network stays in use while FPCG keeps
-  // converging (finite residuals, no rejected step, and no two consecutive
-  // iterations with |r| reduced by less than 10%); otherwise Chebyshev takes over
+  // run-time acceptance: in each solve the network stays in use while FPCG
+  // keeps converging (finite residuals, no rejected step, and no two
+  // consecutive iterations with |r| reduced by less than 10%); otherwise
+  // Chebyshev takes over for the rest of that solve
   void monitorStart();
   void monitorIteration(int it, double rdotr);
   // statistics: [0] c.res, [1] c.Ac, [2] alpha, [3] rejected steps, [4] last loss, [5] loss sum

@@ -181,7 +181,10 @@ class MGLevel : public parAlmond::multigridLevel<T> {
   deviceMemory<T> o_invDiagA, o_P, o_weightG;
   std::shared_ptr<mgScratch<T>> scratch;
   std::shared_ptr<learnedSmoother_t> learned;  // fine-level learned smoother (FP32 only)
-  bool learnedPostToo = true;                  // MULTIGRID LEARNED MODE REPLACE (else PRE)
+  // MULTIGRID LEARNED MODE: REPLACE (network pre- and post-smoothing), PRE (network
+  // pre-smoothing, Chebyshev post) or AUGMENT (Chebyshev pre-smoothing followed by
+  // one network step on its residual, Chebyshev post)
+  enum { MODE_REPLACE = 0, MODE_PRE = 1, MODE_AUGMENT = 2 } learnedMode = MODE_REPLACE;
   std::vector<deviceMemory<T>>* record = nullptr;  // smoother input residuals are copied here when set
   size_t recordMax = 0;
 
@@ -309,15 +312,30 @@ class MGLevel : public parAlmond::multigridLevel<T> {
   // Pre-smoothing that also leaves the V-cycle residual in o_RES (the learned
   // step computes it on the way); false when the caller must do both itself.
   bool smoothPreResidual(deviceMemory<T>& o_RHS, deviceMemory<T>& o_X, deviceMemory<T>& o_RES) {
+    if (learned && learnedMode == MODE_AUGMENT && (learnedUsable() || record)) {
+      smoothChebyshev(o_RHS, o_X, true);
+      residual(o_RHS, o_X, o_RES);
+      if (record && record->size() < recordMax) {
+        deviceMemory<T> v = platform.malloc<T>(elliptic->Ndofs);
+        LIBP_HIP(hipMemcpyAsync(v.ptr(), o_RES.ptr(), elliptic->Ndofs * sizeof(T), hipMemcpyDeviceToDevice,
+                                platform.stream));
+        record->push_back(v);
+      }
+      if (learnedUsable()) augmentLearned(o_X, o_RES);
+      return true;
+    }
     if (!learnedUsable()) return false;
     if (record) recordInput(o_RHS, o_X, true);
     smoothLearned(o_RHS, o_X, true, &o_RES);
     return true;
   }
 
+  // x += alpha c(res), res -= alpha A c (AUGMENT mode)
+  void augmentLearned(deviceMemory<T>& o_X, deviceMemory<T>& o_RES);
+
   void smooth(deviceMemory<T>& o_RHS, deviceMemory<T>& o_X, bool x_is_zero) override {
-    if (record) recordInput(o_RHS, o_X, x_is_zero);
-    if (learnedUsable() && (x_is_zero || learnedPostToo)) {
+    if (record && !(learnedMode == MODE_AUGMENT && learned)) recordInput(o_RHS, o_X, x_is_zero);
+    if (learnedUsable() && (x_is_zero || learnedMode == MODE_REPLACE) && learnedMode != MODE_AUGMENT) {
       smoothLearned(o_RHS, o_X, x_is_zero, nullptr);
     } else if (stype == JACOBI) {
       smoothJacobi(o_RHS, o_X, x_is_zero);
@@ -389,6 +407,15 @@ void MGLevel<float>::smoothLearned(deviceMemory<float>& o_r, deviceMemory<float>
 }
 template <>
 void MGLevel<double>::smoothLearned(deviceMemory<double>&, deviceMemory<double>&, bool, deviceMemory<double>*) {
+  LIBP_FORCE_ABORT("the learned smoother runs in FP32 only");
+}
+template <>
+void MGLevel<float>::augmentLearned(deviceMemory<float>& o_X, deviceMemory<float>& o_RES) {
+  auto A = [this](deviceMemory<float>& x, deviceMemory<float>& y) { Operator(x, y); };
+  learned->augment(o_X, o_RES, o_invDiagA.ptr(), A);
+}
+template <>
+void MGLevel<double>::augmentLearned(deviceMemory<double>&, deviceMemory<double>&) {
   LIBP_FORCE_ABORT("the learned smoother runs in FP32 only");
 }
 
@@ -539,7 +566,9 @@ void multigrid_t<float>::attachLearned() {
   if (!path.empty() && !ls->load(path) && fine.settings.compareSetting("VERBOSE", "TRUE"))
     printf("learned smoother %s; smoothing with Chebyshev\n", ls->status.c_str());
   levels[0]->learned = ls;
-  levels[0]->learnedPostToo = !fine.settings.compareSetting("MULTIGRID LEARNED MODE", "PRE");
+  if (fine.settings.compareSetting("MULTIGRID LEARNED MODE", "PRE")) levels[0]->learnedMode = MGLevel<float>::MODE_PRE;
+  if (fine.settings.compareSetting("MULTIGRID LEARNED MODE", "AUGMENT"))
+    levels[0]->learnedMode = MGLevel<float>::MODE_AUGMENT;
 }
 template <>
 void multigrid_t<double>::attachLearned() {

@@ -50,7 +50,7 @@ def main(argv=None):
     ap.add_argument("--dagger-steps", type=int, default=1500)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--init-seed", type=int, default=1)
-    ap.add_argument("--mode", default="REPLACE", choices=["REPLACE", "PRE"])
+    ap.add_argument("--mode", default="REPLACE", choices=["REPLACE", "PRE", "AUGMENT"])
     a = ap.parse_args(argv)
 
     import libparanumal_amd as lp

@@ -146,6 +146,48 @@ int libp_pcg_pap_f64(int N, const double* p, const double* Ap, double* scalars, 
 int libp_pcg_jacobi_rdotz_f64(int N, const double* invDiag, const double* r, double* z, const double* Ap,
                               int flexible, double* scalars, void* ws, void* stream);
 
+/* One Jacobi-PCG iteration on the N=1 hex operator in three launches (no
+ * single reference counterpart: it fuses the iteration of
+ * linearSolverPCG.cpp:107-171 with JacobiPrecon::Operator,
+ * ellipticPreconJacobi.cpp:42-51, and the N=1 partial Ax + gather of
+ * ellipticOperator.cpp:29-107). Used for the p-MG coarse level. Before the
+ * first call: scalars zeroed, scalars[TOL] set, z = invDiag.*r and
+ * scalars[RDOTZ1] = r.z (libp_pcg_jacobi_rdotz_f64). first = 1 on the first
+ * iteration. AqL holds Nelements*8 doubles. Stops itself via scalars[STOP]. */
+int libp_pcg_jacobi_hex1_iteration_f64(int Nelements, const int* GlobalToLocal, const double* wJ, const double* ggeo,
+                                       const double* D, double lambda, int Nrows, const int* rowStarts,
+                                       const int* colIds, const double* invDiag, int first, double* x, double* r,
+                                       double* z, double* p, double* Ap, double* AqL, double* scalars, void* ws,
+                                       void* stream);
+
+/* Non-blocking solvers (libs/linearSolver/linearSolverNBPCG.cpp,
+ * linearSolverNBFPCG.cpp): fused update + multi-dot kernels replacing
+ * update1NBPCG/update2NBPCG (okl/linearSolverUpdateNBPCG.okl) and
+ * update0NBFPCG/update1NBFPCG (okl/linearSolverUpdateNBFPCG.okl) together
+ * with the host sums and scalar recurrences that follow them. The scalars stay
+ * on the device:
+ *   NBPCG  [0] gamma=r.z [1] previous gamma [2] alpha [3] beta [4] delta=p.s [5] r.r [6] z.z
+ *   NBFPCG [0] gamma=u.r [1] delta=u.w [2] eta [3] alpha [4] beta [5] r.r
+ * nbpcg_update1: p = z + beta p, s = Z + beta s, delta = p.s, alpha = gamma/delta.
+ * nbpcg_update2: r -= alpha s, z -= alpha S, r.z/z.z/r.r, beta = gamma/previous gamma
+ *   (first = 1: alpha = beta = 0, linearSolverNBPCG.cpp:86-90).
+ * nbfpcg_update0: u.r, u.w, r.r; eta = delta; alpha = gamma/eta.
+ * nbfpcg_update1: x += alpha p, r -= alpha s, u -= alpha q, w -= alpha z; u.r, u.s,
+ *   u.w, r.r; beta = -u.s/eta; eta = delta - beta^2 eta; alpha = gamma/eta.
+ * nbfpcg_update2: p = u + beta p, s = w + beta s, q = m + beta q, z = n + beta z
+ *   (linearSolverNBFPCG.cpp:155-165). axpy_dev: y += (*alpha) x. */
+int libp_nbpcg_update1_f64(int N, const double* z, const double* Z, double* p, double* s, double* scalars, void* ws,
+                           void* stream);
+int libp_nbpcg_update2_f64(int N, int first, const double* s, const double* S, double* r, double* z, double* scalars,
+                           void* ws, void* stream);
+int libp_axpy_dev_f64(int N, const double* alpha, const double* x, double* y, void* stream);
+int libp_nbfpcg_update0_f64(int N, const double* u, const double* r, const double* w, double* scalars, void* ws,
+                            void* stream);
+int libp_nbfpcg_update1_f64(int N, const double* p, const double* s, const double* q, const double* z, double* x,
+                            double* r, double* u, double* w, double* scalars, void* ws, void* stream);
+int libp_nbfpcg_update2_f64(int N, const double* u, const double* w, const double* m, const double* n, double* p,
+                            double* s, double* q, double* z, const double* scalars, void* stream);
+
 /* ------------------------------------------------------------------ */
 /* p-multigrid transfers and smoother pieces                            */
 /* ------------------------------------------------------------------ */

@@ -82,6 +82,14 @@ SIGNATURES = {
                                  c_void_p, c_void_p, c_void_p, c_void_p]),
     "libp_addbc_hex_f64": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                    c_void_p]),
+    "libp_pcg_jacobi_hex1_iteration_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int,
+                                                   c_void_p, c_void_p, c_void_p, c_int] + [c_void_p] * 9),
+    "libp_nbpcg_update1_f64": (c_int, [c_int] + [c_void_p] * 7),
+    "libp_nbpcg_update2_f64": (c_int, [c_int, c_int] + [c_void_p] * 7),
+    "libp_axpy_dev_f64": (c_int, [c_int] + [c_void_p] * 4),
+    "libp_nbfpcg_update0_f64": (c_int, [c_int] + [c_void_p] * 6),
+    "libp_nbfpcg_update1_f64": (c_int, [c_int] + [c_void_p] * 11),
+    "libp_nbfpcg_update2_f64": (c_int, [c_int] + [c_void_p] * 10),
     "libp_mlp_linear_fwd": (c_int, [c_int, c_int, c_int, c_void_p, c_long, c_void_p, c_long, c_void_p, c_int,
                                     c_void_p, c_long, c_void_p]),
     "libp_mlp_linear_bwd_data": (c_int, [c_int, c_int, c_int, c_void_p, c_long, c_void_p, c_long, c_void_p, c_long,

@@ -140,6 +140,8 @@ settings_t ellipticDefaultSettings() {
   s.newSetting("MULTIGRID PRECISION", "FP32", "Precision of p-MG smoothing and transfers", {"FP32", "FP64"});
   s.newSetting("MULTIGRID COARSE TOLERANCE", "1e-2", "Relative tolerance of the coarse Jacobi-PCG");
   s.newSetting("MULTIGRID COARSE MAXIT", "200");
+  s.newSetting("MULTIGRID COARSE FUSED", "TRUE", "Coarse Jacobi-PCG at N=1 in three fused kernels per iteration",
+               {"TRUE", "FALSE"});
   s.newSetting("MULTIGRID COARSE CHECK EVERY", "8",
                "Coarse Jacobi-PCG: iterations queued between host convergence reads (same iterates as 1)");
   s.newSetting("MULTIGRID LEARNED WEIGHTS", "", "Weight file of the learned fine smoother");

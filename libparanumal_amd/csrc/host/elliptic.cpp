@@ -196,11 +196,17 @@ void elliptic_t::BuildRhs(long seed, deviceMemory<dfloat>& o_r) {
 ellipticRunResult_t elliptic_t::Run(int verboseOverride) {
   ellipticRunResult_t res;
   const hlong NglobalDofs = ogsMasked.NgatherGlobal * Nfields;
-  LIBP_ABORT("LINEAR SOLVER " << settings.getSetting("LINEAR SOLVER") << " is not supported on this path",
-             !settings.compareSetting("LINEAR SOLVER", "PCG") || settings.compareSetting("LINEAR SOLVER", "NBPCG") ||
-                 settings.compareSetting("LINEAR SOLVER", "NBFPCG"));
+  // solver choice in the order of ellipticRun.cpp:41-51 (compareSetting is a substring test)
   linearSolver_t linearSolver;
-  linearSolver.Setup<LinearSolver::pcg>(Ndofs, Nhalo, platform, settings, comm);
+  if (settings.compareSetting("LINEAR SOLVER", "NBPCG")) {
+    linearSolver.Setup<LinearSolver::nbpcg>(Ndofs, Nhalo, platform, settings, comm);
+  } else if (settings.compareSetting("LINEAR SOLVER", "NBFPCG")) {
+    linearSolver.Setup<LinearSolver::nbfpcg>(Ndofs, Nhalo, platform, settings, comm);
+  } else if (settings.compareSetting("LINEAR SOLVER", "PCG")) {
+    linearSolver.Setup<LinearSolver::pcg>(Ndofs, Nhalo, platform, settings, comm);
+  } else {
+    LIBP_FORCE_ABORT("LINEAR SOLVER " << settings.getSetting("LINEAR SOLVER") << " is not supported on this path");
+  }
   linearSolver.get()->pureRelative = settings.compareSetting("LINEAR SOLVER STOPPING CRITERION", "RELATIVE") &&
                                      !settings.compareSetting("LINEAR SOLVER STOPPING CRITERION", "ABS/REL");
 

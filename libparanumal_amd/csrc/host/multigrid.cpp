@@ -444,16 +444,68 @@ class coarseJacobiPCG_t {
     elliptic->settings.getSetting("MULTIGRID COARSE CHECK EVERY", solver->checkEvery);
     o_x = elliptic->platform.malloc<double>(elliptic->Ndofs);
     o_r = elliptic->platform.malloc<double>(elliptic->Ndofs);
+    // fused three-kernel iterations on an N=1 level (not with the all-Neumann mean removal)
+    std::string fused = "TRUE";
+    elliptic->settings.getSetting("MULTIGRID COARSE FUSED", fused);
+    fusedHex1 = elliptic->mesh.N == 1 && !elliptic->allNeumann && fused == "TRUE";
   }
+  bool fusedHex1 = false;
+  deviceMemory<double> o_s, o_z, o_p, o_Ap, o_AqL;
+  int solveFusedHex1(deviceMemory<double>& o_X);
   template <typename T>
   void solve(deviceMemory<T>& o_rhs, deviceMemory<T>& o_X);
 };
+
+/* Jacobi-PCG from x = 0 with the N=1 operator fused into three kernels per
+ * iteration (libp_pcg_jacobi_hex1_iteration_f64): the same iterates and
+ * stopping rule as LinearSolver::pcg with JacobiPrecon on this level. */
+int coarseJacobiPCG_t::solveFusedHex1(deviceMemory<double>& o_X) {
+  platform_t& p = elliptic->platform;
+  void* st = p.s();
+  void* ws = p.o_reduceWS.ptr();
+  const dlong N = elliptic->Ndofs;
+  const mesh_t& m = elliptic->mesh;
+  if (!o_s.ptr()) {
+    o_s = p.malloc<double>(LIBP_PCG_NSCALARS);
+    o_z = p.malloc<double>(N);
+    o_p = p.malloc<double>(N);
+    o_Ap = p.malloc<double>(N);
+    o_AqL = p.malloc<double>((size_t)m.Nelements * m.Np);
+  }
+  double* s = o_s.ptr();
+  LIBP_CALL(libp_set_f64(N, 0.0, o_X.ptr(), st));
+  LIBP_CALL(libp_set_f64(LIBP_PCG_NSCALARS, 0.0, s, st));
+  LIBP_CALL(libp_norm2sq_f64(N, o_r.ptr(), s + LIBP_PCG_RDOTR, ws, st));
+  const double rdotr0 = p.readScalars(s + LIBP_PCG_RDOTR, 1)[0];
+  if (rdotr0 == 0.0) return 0;
+  const double TOL = tol * tol * rdotr0;
+  LIBP_CALL(libp_set_f64(1, TOL > 0 ? TOL : 1e-300, s + LIBP_PCG_TOL, st));
+  LIBP_CALL(libp_pcg_jacobi_rdotz_f64(N, jacobi->o_invDiagA.ptr(), o_r.ptr(), o_z.ptr(), o_Ap.ptr(), 0, s, ws, st));
+  int it = 0;
+  const int chunk = std::max(1, solver->checkEvery);
+  while (it < maxit) {
+    const int n = std::min(chunk, maxit - it);
+    for (int c = 0; c < n; ++c)
+      LIBP_CALL(libp_pcg_jacobi_hex1_iteration_f64(
+          m.Nelements, elliptic->o_GlobalToLocal.ptr(), m.o_wJ.ptr(), m.o_ggeo.ptr(), m.o_D.ptr(), elliptic->lambda,
+          N, elliptic->ogsMasked.o_rowStartsT.ptr(), elliptic->ogsMasked.o_colIdsT.ptr(), jacobi->o_invDiagA.ptr(),
+          it + c == 0, o_X.ptr(), o_r.ptr(), o_z.ptr(), o_p.ptr(), o_Ap.ptr(), o_AqL.ptr(), s, ws, st));
+    const double* h = p.readScalars(s, LIBP_PCG_NSCALARS);
+    it = (int)h[LIBP_PCG_ITERS];
+    if (h[LIBP_PCG_STOP] != 0.0) break;
+  }
+  return it;
+}
 
 template <>
 void coarseJacobiPCG_t::solve<double>(deviceMemory<double>& o_rhs, deviceMemory<double>& o_X) {
   platform_t& p = elliptic->platform;
   const dlong N = elliptic->Ndofs;
   LIBP_HIP(hipMemcpyAsync(o_r.ptr(), o_rhs.ptr(), N * sizeof(double), hipMemcpyDeviceToDevice, p.stream));
+  if (fusedHex1) {
+    lastIterations = solveFusedHex1(o_X);
+    return;
+  }
   LIBP_CALL(libp_set_f64(N, 0.0, o_X.ptr(), p.s()));
   lastIterations = solver->Solve(*elliptic, *jacobi, o_X, o_r, tol, maxit, 0);
 }
@@ -462,8 +514,12 @@ void coarseJacobiPCG_t::solve<float>(deviceMemory<float>& o_rhs, deviceMemory<fl
   platform_t& p = elliptic->platform;
   const dlong N = elliptic->Ndofs;
   LIBP_CALL(libp_convert_f32_to_f64(N, o_rhs.ptr(), o_r.ptr(), p.s()));
-  LIBP_CALL(libp_set_f64(N, 0.0, o_x.ptr(), p.s()));
-  lastIterations = solver->Solve(*elliptic, *jacobi, o_x, o_r, tol, maxit, 0);
+  if (fusedHex1) {
+    lastIterations = solveFusedHex1(o_x);
+  } else {
+    LIBP_CALL(libp_set_f64(N, 0.0, o_x.ptr(), p.s()));
+    lastIterations = solver->Solve(*elliptic, *jacobi, o_x, o_r, tol, maxit, 0);
+  }
   LIBP_CALL(libp_convert_f64_to_f32(N, o_x.ptr(), o_X.ptr(), p.s()));
 }
 

@@ -116,5 +116,28 @@ class pcg : public linearSolverBase_t {
   void setFlexible(bool f) { flexible = f; }
 };
 
+/* Non-blocking (pipelined) PCG (libs/linearSolver/linearSolverNBPCG.cpp):
+ * two fused update/multi-dot kernels per iteration, alpha/beta/gamma on the
+ * device; the preconditioner must be linear (z is updated by recurrence). */
+class nbpcg : public linearSolverBase_t {
+  deviceMemory<dfloat> o_p, o_s, o_S, o_z, o_Z, o_Ax, o_sc;
+
+ public:
+  nbpcg(dlong N, dlong Nhalo, platform_t& platform, settings_t& settings, comm_t comm);
+  int Solve(operator_t& linearOperator, operator_t& precon, deviceMemory<dfloat>& o_x, deviceMemory<dfloat>& o_r,
+            const dfloat tol, const int MAXIT, const int verbose) override;
+};
+
+/* Non-blocking flexible PCG, Sanan's pipelined variant
+ * (libs/linearSolver/linearSolverNBFPCG.cpp), scalars on the device. */
+class nbfpcg : public linearSolverBase_t {
+  deviceMemory<dfloat> o_u, o_p, o_w, o_n, o_m, o_s, o_z, o_q, o_Ax, o_sc;
+
+ public:
+  nbfpcg(dlong N, dlong Nhalo, platform_t& platform, settings_t& settings, comm_t comm);
+  int Solve(operator_t& linearOperator, operator_t& precon, deviceMemory<dfloat>& o_x, deviceMemory<dfloat>& o_r,
+            const dfloat tol, const int MAXIT, const int verbose) override;
+};
+
 }  // namespace LinearSolver
 }  // namespace libp

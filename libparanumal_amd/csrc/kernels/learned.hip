@@ -70,8 +70,22 @@ __global__ void __launch_bounds__(256) learned_assemble_kernel(const int Nrows, 
   const float om = omega ? *omega : 0.f;
   for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < Nrows; g += gridDim.x * blockDim.x) {
     const int a = rowStarts[g], b = rowStarts[g + 1];
+    const int len = b - a;
     float acc = 0.f;
-    for (int j = a; j < b; ++j) {
+    int col[8];
+    float y[8], sc[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) col[k] = (k < len) ? colIds[a + k] : 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int e = col[k] / Np, n = col[k] - e * Np;
+      y[k] = (k < len) ? Y[(long)e * ldy + n] : 0.f;
+      sc[k] = (k < len) ? scale[e] : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (k < len) acc += sc[k] * y[k];
+    for (int j = a + 8; j < b; ++j) {
       const int id = colIds[j];
       const int e = id / Np, n = id - e * Np;
       acc += scale[e] * Y[(long)e * ldy + n];

@@ -2,24 +2,44 @@
 //
 // The reference stages blocks of <= 512 nonzeros through shared memory
 // (libs/ogs/okl/ogsKernels.okl:86-164). On the C0 hex path every row holds
-// 1-8 entries whose column ids are consecutive in local order, so a thread per
-// row already reads colIds/v in near-contiguous runs; the LDS staging buys
-// nothing and costs a barrier. Sums run from zero in ascending column order,
+// 1-8 entries (the copies of one node), so a thread per row needs no LDS
+// staging and no barrier. A row of up to 8 entries issues all of its column
+// loads, then all of its value loads, before summing: the copies of a node
+// lie in different elements, and a loop of dependent loads would leave the
+// memory system idle. Sums still run from zero in ascending column order,
 // which is the reference's order, so Gather results are bit-identical.
 #include "kcommon.hpp"
 
 namespace libp_amd {
 
+// sum_{j in [start,end)} v[colIds[j]] in ascending j; loads issued up front for rows of <= 8 entries
+template <typename T>
+__device__ inline T gather_row(const int start, const int end, const int* __restrict__ colIds,
+                               const T* __restrict__ v) {
+  const int len = end - start;
+  T val = 0;
+  if (len <= 8) {
+    int c[8];
+    T x[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) c[k] = (k < len) ? colIds[start + k] : 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = (k < len) ? v[c[k]] : T(0);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (k < len) val += x[k];
+  } else {
+    for (int g = start; g < end; ++g) val += v[colIds[g]];
+  }
+  return val;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) ogs_gather_kernel(const int Nrows, const int* __restrict__ rowStarts,
                                                          const int* __restrict__ colIds, const T* __restrict__ v,
                                                          T* __restrict__ gv) {
-  for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < Nrows; row += gridDim.x * blockDim.x) {
-    const int start = rowStarts[row], end = rowStarts[row + 1];
-    T val = 0;
-    for (int g = start; g < end; ++g) val += v[colIds[g]];
-    gv[row] = val;
-  }
+  for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < Nrows; row += gridDim.x * blockDim.x)
+    gv[row] = gather_row<T>(rowStarts[row], rowStarts[row + 1], colIds, v);
 }
 
 template <typename T>

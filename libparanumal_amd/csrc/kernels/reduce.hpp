@@ -15,7 +15,7 @@ namespace libp_amd {
 
 constexpr int RBS = 256;       // reduction block size
 constexpr int RMAXB = 1024;    // max reduction blocks
-constexpr int RNPART = 3;      // partial arrays available (fused multi-dot)
+constexpr int RNPART = 4;      // partial arrays available (fused multi-dot)
 
 struct ReduceWS {
   double part[RNPART][RMAXB];

@@ -143,6 +143,144 @@ PCGResult pcg_solve(dlong N, OpFn& A, OpFn& M, double* x, double* r, double tol,
   return res;
 }
 
+/* Fused multi-dot of the non-blocking solvers: blocksize 512, Nblocks =
+ * min(ceil(N/512), 512), thread t of block b sums n = t + 512 b step
+ * 512 Nblocks, tree to two values, partials summed on the host in block
+ * order (linearSolverUpdateNBPCG.okl / linearSolverUpdateNBFPCG.okl,
+ * linearSolverNBPCG.cpp:170-224, linearSolverNBFPCG.cpp:183-239). f(n, out)
+ * performs the pointwise update of entry n and adds its K products to out. */
+template <int K, typename F>
+static void nbDots(dlong N, F f, double* dots) {
+  const int bs = 512;
+  int Nblocks = (N + bs - 1) / bs;
+  Nblocks = std::min(Nblocks, bs);
+  for (int k = 0; k < K; ++k) dots[k] = 0.0;
+  std::vector<double> s((size_t)K * bs);
+  for (int b = 0; b < Nblocks; ++b) {
+    for (int t = 0; t < bs; ++t) {
+      double acc[K];
+      for (int k = 0; k < K; ++k) acc[k] = 0.0;
+      for (dlong id = t + (dlong)b * bs; id < N; id += (dlong)bs * Nblocks) f(id, acc);
+      for (int k = 0; k < K; ++k) s[(size_t)k * bs + t] = acc[k];
+    }
+    for (int st = 256; st >= 2; st /= 2)
+      for (int t = 0; t < st; ++t)
+        for (int k = 0; k < K; ++k) s[(size_t)k * bs + t] += s[(size_t)k * bs + t + st];
+    for (int k = 0; k < K; ++k) dots[k] += s[(size_t)k * bs] + s[(size_t)k * bs + 1];
+  }
+}
+
+/* LinearSolver::nbpcg::Solve (libs/linearSolver/linearSolverNBPCG.cpp:71-160). */
+PCGResult nbpcg_solve(dlong N, OpFn& A, OpFn& M, double* x, double* r, double tol, int maxit, bool pureRelative) {
+  PCGResult res;
+  std::vector<double> p(N, 0.0), s(N, 0.0), z(N, 0.0), Z(N, 0.0), S(N, 0.0), Ax(N, 0.0);
+  double dots[3];
+  A.apply(x, Ax.data());
+  axpy(N, -1.0, Ax.data(), 1.0, r);
+  M.apply(r, z.data());
+  double alpha = 0;
+  auto update2 = [&](double a) {
+    nbDots<3>(N, [&](dlong n, double* o) {
+      double rn = r[n] - a * s[n], zn = z[n] - a * S[n];
+      o[0] += rn * zn;
+      o[1] += zn * zn;
+      o[2] += rn * rn;
+      r[n] = rn;
+      z[n] = zn;
+    }, dots);
+  };
+  update2(alpha);
+  A.apply(z.data(), Z.data());
+  double gamma = dots[0], rdotr = dots[2], gamma1 = 0, beta = 0, delta = 0;
+  const double TOL = pureRelative ? tol * tol * rdotr : std::max(tol * tol * rdotr, tol * tol);
+  res.rdotr0 = rdotr;
+  res.resHistory.push_back(std::sqrt(rdotr));
+  int iter;
+  for (iter = 0; iter < maxit; ++iter) {
+    if (rdotr <= TOL) break;
+    nbDots<1>(N, [&](dlong n, double* o) {
+      const double pn = z[n] + beta * p[n], sn = Z[n] + beta * s[n];
+      o[0] += pn * sn;
+      p[n] = pn;
+      s[n] = sn;
+    }, dots);
+    M.apply(s.data(), S.data());
+    delta = dots[0];
+    alpha = gamma / delta;
+    update2(alpha);
+    axpy(N, alpha, p.data(), 1.0, x);
+    A.apply(z.data(), Z.data());
+    gamma1 = gamma;
+    gamma = dots[0];
+    rdotr = dots[2];
+    beta = gamma / gamma1;
+    res.resHistory.push_back(std::sqrt(std::max(rdotr, 0.0)));
+  }
+  res.iters = iter;
+  res.rdotrFinal = rdotr;
+  return res;
+}
+
+/* LinearSolver::nbfpcg::Solve (libs/linearSolver/linearSolverNBFPCG.cpp:71-181), Sanan's pipelined flexible CG. */
+PCGResult nbfpcg_solve(dlong N, OpFn& A, OpFn& M, double* x, double* r, double tol, int maxit, bool pureRelative) {
+  PCGResult res;
+  std::vector<double> u(N), p(N), w(N), m(N), n_(N), s(N), q(N), z(N), Ax(N);
+  double dots[4];
+  A.apply(x, Ax.data());
+  axpy(N, -1.0, Ax.data(), 1.0, r);
+  M.apply(r, u.data());
+  p = u;
+  A.apply(p.data(), w.data());
+  nbDots<3>(N, [&](dlong i, double* o) {
+    o[0] += u[i] * r[i];
+    o[1] += u[i] * w[i];
+    o[2] += r[i] * r[i];
+  }, dots);
+  M.apply(w.data(), m.data());
+  A.apply(m.data(), n_.data());
+  s = w;
+  q = m;
+  z = n_;
+  double gamma = dots[0], delta = dots[1], rdotr = dots[2], eta = delta, alpha = gamma / eta, beta = 0;
+  const double TOL = pureRelative ? tol * tol * rdotr : std::max(tol * tol * rdotr, tol * tol);
+  res.rdotr0 = rdotr;
+  res.resHistory.push_back(std::sqrt(rdotr));
+  int iter;
+  for (iter = 0; iter < maxit; ++iter) {
+    if (rdotr <= TOL) break;
+    nbDots<4>(N, [&](dlong i, double* o) {
+      const double xn = x[i] + alpha * p[i], rn = r[i] - alpha * s[i];
+      const double un = u[i] - alpha * q[i], wn = w[i] - alpha * z[i];
+      o[0] += un * rn;
+      o[1] += un * s[i];
+      o[2] += un * wn;
+      o[3] += rn * rn;
+      x[i] = xn;
+      r[i] = rn;
+      u[i] = un;
+      w[i] = wn;
+    }, dots);
+    for (dlong i = 0; i < N; ++i) n_[i] = 1.0 * w[i] - 1.0 * r[i];  // zaxpy
+    M.apply(n_.data(), m.data());
+    axpy(N, 1.0, u.data(), 1.0, m.data());
+    A.apply(m.data(), n_.data());
+    gamma = dots[0];
+    beta = -dots[1] / eta;
+    delta = dots[2];
+    rdotr = dots[3];
+    axpy(N, 1.0, u.data(), beta, p.data());
+    axpy(N, 1.0, w.data(), beta, s.data());
+    axpy(N, 1.0, m.data(), beta, q.data());
+    axpy(N, 1.0, n_.data(), beta, z.data());
+    eta = delta - beta * beta * eta;
+    alpha = gamma / eta;
+    res.resHistory.push_back(std::sqrt(std::max(rdotr, 0.0)));
+  }
+  res.iters = iter;
+  res.rdotrFinal = rdotr;
+  return res;
+}
+
 /* ----------------------------- p-multigrid ----------------------------- */
 namespace {
 struct EllOp : OpFn {
@@ -509,7 +647,13 @@ RunResult elliptic_run(const Settings& s, int verbose) {
                        !s.compare("LINEAR SOLVER STOPPING CRITERION", "ABS/REL");
   const int maxit = s.has("MAXIMUM ITERATIONS") ? s.geti("MAXIMUM ITERATIONS") : 5000;
   const double tol = s.has("LINEAR SOLVER TOLERANCE") ? s.getd("LINEAR SOLVER TOLERANCE") : 1.0e-8;
-  PCGResult pr = pcg_solve(N, A, *Mp, x.data(), r.data(), tol, maxit, flexible, rhsNorm, pureRel);
+  PCGResult pr;
+  if (s.compare("LINEAR SOLVER", "NBFPCG"))
+    pr = nbfpcg_solve(N, A, *Mp, x.data(), r.data(), tol, maxit, pureRel);
+  else if (s.compare("LINEAR SOLVER", "NBPCG"))
+    pr = nbpcg_solve(N, A, *Mp, x.data(), r.data(), tol, maxit, pureRel);
+  else
+    pr = pcg_solve(N, A, *Mp, x.data(), r.data(), tol, maxit, flexible, rhsNorm, pureRel);
   out.iters = pr.iters;
   if (verbose) printf("oracle: %d iterations, final |r| %.6e\n", pr.iters, std::sqrt(std::max(pr.rdotrFinal, 0.0)));
   el.ogsMasked.gatherLocal.Scatter(xL.data(), x.data(), false);

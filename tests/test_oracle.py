@@ -287,3 +287,27 @@ def test_pcg_matches_dense_solve():
     b = rng.standard_normal(A.shape[0])
     it, x, r, rd = o.pcg(np.zeros_like(b), b, 1e-12, 500, jacobi=True, pure_relative=True)
     assert np.allclose(x, np.linalg.solve(A, b), rtol=1e-9, atol=1e-11)
+
+
+# --------------------------------------------------------------------------
+# non-blocking solvers: in exact arithmetic NBPCG and NBFPCG produce the PCG
+# iterates (Gropp's and Sanan's recurrences are rearrangements of CG), so the
+# restatements must match PCG's iteration count and solution
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("precon", ["NONE", "JACOBI"])
+@pytest.mark.parametrize("solver", ["NBPCG", "NBFPCG"])
+def test_nonblocking_solvers_match_pcg(solver, precon):
+    base = ref_settings(PRECONDITIONER=precon, **{"BOX NX": 5, "BOX NY": 6, "BOX NZ": 4, "POLYNOMIAL DEGREE": 3})
+    n0, it0, _ = oracle_lib.run(dict(base, **{"LINEAR SOLVER": "PCG"}))
+    n1, it1, _ = oracle_lib.run(dict(base, **{"LINEAR SOLVER": solver}))
+    assert abs(it1 - it0) <= 1
+    assert abs(n1 - n0) < 1e-9 * n0
+
+
+def test_nbfpcg_with_multigrid_matches_fpcg():
+    base = ref_settings(PRECONDITIONER="MULTIGRID", **{"BOX NX": 4, "BOX NY": 4, "BOX NZ": 4, "POLYNOMIAL DEGREE": 4,
+                                                        "MULTIGRID COARSENING": "CUSTOM", "MULTIGRID DEGREES": "4,2,1"})
+    n0, it0, _ = oracle_lib.run(dict(base, **{"LINEAR SOLVER": "FPCG"}))
+    n1, it1, _ = oracle_lib.run(dict(base, **{"LINEAR SOLVER": "NBFPCG"}))
+    assert abs(it1 - it0) <= 1
+    assert abs(n1 - n0) < 1e-7 * n0

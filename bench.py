@@ -245,6 +245,17 @@ def main():
         "gather": {"ms": ms_gather, "GBs": bytes_gather / (ms_gather * 1e-3) / 1e9},
         "setup_seconds": setup_s,
     }
+    # achievable HBM: z = a x + b y over three 1 GiB FP64 arrays with the library's own streaming kernel
+    n = (1 << 30) // 8
+    xs = torch.rand(n, dtype=torch.float64, device="cuda")
+    ys = torch.rand(n, dtype=torch.float64, device="cuda")
+    zs = torch.empty_like(xs)
+    ms_tri = timed(lambda: lp.check(L.libp_zaxpy_f64(n, 1.5, xs.data_ptr(), 0.5, ys.data_ptr(), zs.data_ptr(),
+                                                    el.stream)), 20, 3)
+    result["hbm_stream_probe"] = {"kernel": "libp_zaxpy_f64, 3 x 1 GiB (2 reads + 1 write)", "ms": ms_tri,
+                                  "GBs": 3 * n * 8 / (ms_tri * 1e-3) / 1e9}
+    result["roofline"]["frac_of_measured_stream"] = gbs / result["hbm_stream_probe"]["GBs"]
+    del xs, ys, zs
     if args.copy_probe:
         nbytes = 1 << 30
         src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").uniform_()

@@ -14,7 +14,7 @@
 //
 //   phase T : waves take j-slices, qt = D q along k          (MFMA, LDS in/out)
 //   phase K : waves take k-slices: qr = Q D^T, qs = D Q       (MFMA)
-//             G factors (streamed, non-temporal, one slice ahead), Gqs D^T-part, Gqr via LDS
+//             G factors (streamed, non-temporal), Gqs D^T-part, Gqr via LDS
 //             transpose, partial Aq = r-part + s-part + lambda wJ q -> LDS
 //   phase T2: waves take j-slices, add D^T Gqt along k, write Aq
 //
@@ -29,19 +29,6 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 __device__ inline d4 mfma16(double a, double b, d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
-
-// the six geometric factors and wJ of the lane's four nodes (i=li, j=4r+lg) of slice k
-__device__ inline void load_slice(const double* __restrict__ gbase, const double* __restrict__ wbase, const int k,
-                                  double (&G)[6][4], double (&W)[4]) {
-  const int l = threadIdx.x & 63, li = l & 15, lg = l >> 4;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int n = k * 256 + (4 * r + lg) * 16 + li;
-#pragma unroll
-    for (int g = 0; g < 6; ++g) G[g][r] = __builtin_nontemporal_load(gbase + g * 4096 + n);
-    W[r] = __builtin_nontemporal_load(wbase + n);
-  }
 }
 
 __global__ void __launch_bounds__(256, 1) ax_hex_mfma_n15_kernel(const int* __restrict__ elementList,
@@ -68,12 +55,6 @@ __global__ void __launch_bounds__(256, 1) ax_hex_mfma_n15_kernel(const int* __re
     dT[c] = D[(4 * c + lg) * Nq + li];
   }
 
-  // geometric factors of this wave's first k-slice, issued before the gather
-  const double* gbase = ggeo + (size_t)element * 6 * Np;
-  const double* wbase = wJ + eb;
-  double Gn[6][4], Wn[4];
-  load_slice(gbase, wbase, 4 * w, Gn, Wn);
-
   // gather q into LDS
 #pragma unroll 4
   for (int n = t; n < Np; n += 256) {
@@ -94,23 +75,20 @@ __global__ void __launch_bounds__(256, 1) ax_hex_mfma_n15_kernel(const int* __re
   }
   __syncthreads();
 
-  // phase K: k = 4w..4w+3. The geometric factors of slice kk+1 are loaded
-  // while slice kk computes (those of slice 0 were issued before phase T).
-  double G[6][4], W[4];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    G[0][r] = Gn[0][r];
-    G[1][r] = Gn[1][r];
-    G[2][r] = Gn[2][r];
-    G[3][r] = Gn[3][r];
-    G[4][r] = Gn[4][r];
-    G[5][r] = Gn[5][r];
-    W[r] = Wn[r];
-  }
+  // phase K: k = 4w..4w+3
+  const double* gbase = ggeo + (size_t)element * 6 * Np;
+  const double* wbase = wJ + eb;
 #pragma unroll 1
   for (int kk = 0; kk < 4; ++kk) {
     const int k = 4 * w + kk;
-    if (kk + 1 < 4) load_slice(gbase, wbase, k + 1, Gn, Wn);
+    double G[6][4], W[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int n = k * NqNq + (4 * r + lg) * Nq + li;
+#pragma unroll
+      for (int g = 0; g < 6; ++g) G[g][r] = __builtin_nontemporal_load(gbase + g * Np + n);
+      W[r] = __builtin_nontemporal_load(wbase + n);
+    }
     d4 qr = {0, 0, 0, 0}, qs = {0, 0, 0, 0};
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
@@ -147,12 +125,6 @@ __global__ void __launch_bounds__(256, 1) ax_hex_mfma_n15_kernel(const int* __re
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-#pragma unroll
-      for (int g = 0; g < 6; ++g) G[g][r] = Gn[g][r];
-      W[r] = Wn[r];
-    }
   }
   __syncthreads();
 

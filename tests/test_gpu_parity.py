@@ -306,3 +306,38 @@ def test_nbfpcg_pmg_fp32_reaches_tolerance():
     assert r["r_final"] <= 1e-8 * r["r0"] * (1 + 1e-12)
     assert abs(r["iterations"] - f["iterations"]) <= 2
     assert abs(r["norm"] - f["norm"]) < 1e-7 * f["norm"]
+
+
+# ---------------------------------------------------------------------------
+# the reference harness's own setup.rc, through the ellipticMain driver
+# (test/testElliptic.py:34-72 writes it; test/test.py:91-99 formats it)
+# ---------------------------------------------------------------------------
+def harness_rc(precon, flag=1, linear_solver="PCG"):
+    keys = [("FORMAT", "2.0"), ("DATA FILE", "/path/to/solvers/elliptic/data/ellipticSine3D.h"), ("MESH FILE", "BOX"),
+            ("MESH DIMENSION", 3), ("ELEMENT TYPE", 12), ("BOX NX", 10), ("BOX NY", 10), ("BOX NZ", 10),
+            ("BOX DIMX", 1), ("BOX DIMY", 1), ("BOX DIMZ", 1), ("BOX BOUNDARY FLAG", flag), ("POLYNOMIAL DEGREE", 4),
+            ("THREAD MODEL", "HIP"), ("PLATFORM NUMBER", 0), ("DEVICE NUMBER", 0), ("DISCRETIZATION", "CONTINUOUS"),
+            ("LINEAR SOLVER", linear_solver), ("PRECONDITIONER", precon), ("MULTIGRID SMOOTHER", "CHEBYSHEV"),
+            ("PARALMOND CYCLE", "VCYCLE"), ("PARALMOND STRENGTH", "SYMMETRIC"), ("PARALMOND AGGREGATION", "UNSMOOTHED"),
+            ("PARALMOND SMOOTHER", "CHEBYSHEV"), ("OUTPUT TO FILE", "FALSE"), ("VERBOSE", "FALSE")]
+    return "".join(f"[{k}]\n{v}\n\n" for k, v in keys)
+
+
+@pytest.mark.parametrize("precon,flag,solver,refnorm", [
+    ("NONE", 1, "PCG", 0.353553390458384),       # testEllipticHex_C0 (testElliptic.py:95-98)
+    ("JACOBI", 1, "PCG", 0.353553400508458),     # testEllipticHex_C0_Jacobi (:199-203)
+    ("MULTIGRID", 1, "FPCG", 0.353553400508458), # testEllipticHex_C0_Multigrid (:209-213), coarse level differs
+    ("NONE", -1, "PCG", 0.059540839002614),      # testEllipticHex_C0_AllNeumann (:244-248)
+])
+def test_elliptic_main_with_harness_settings(tmp_path, precon, flag, solver, refnorm):
+    import subprocess
+    import libparanumal_amd as lp
+    rc = tmp_path / "setup.rc"
+    rc.write_text(harness_rc(precon, flag, solver))
+    run = subprocess.run([lp.MAIN_PATH, str(rc)], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert run.returncode == 0, run.stderr[-2000:]
+    last = run.stdout.strip().splitlines()[-1]
+    assert last.startswith("Solution norm = "), last
+    norm = float(last.split("=")[1])
+    print(precon, flag, solver, norm, flush=True)
+    assert abs(norm - refnorm) < 1e-5  # the harness tolerance (test/test.py)

@@ -142,6 +142,8 @@ settings_t ellipticDefaultSettings() {
   s.newSetting("MULTIGRID COARSE MAXIT", "200");
   s.newSetting("MULTIGRID COARSE FUSED", "TRUE", "Coarse Jacobi-PCG at N=1 in three fused kernels per iteration",
                {"TRUE", "FALSE"});
+  s.newSetting("MULTIGRID COARSE GRAPH", "TRUE", "Replay the fused coarse iterations as captured HIP graphs",
+               {"TRUE", "FALSE"});
   s.newSetting("MULTIGRID COARSE CHECK EVERY", "8",
                "Coarse Jacobi-PCG: iterations queued between host convergence reads (same iterates as 1)");
   s.newSetting("MULTIGRID LEARNED WEIGHTS", "", "Weight file of the learned fine smoother");

@@ -448,9 +448,24 @@ class coarseJacobiPCG_t {
     std::string fused = "TRUE";
     elliptic->settings.getSetting("MULTIGRID COARSE FUSED", fused);
     fusedHex1 = elliptic->mesh.N == 1 && !elliptic->allNeumann && fused == "TRUE";
+    std::string graph = "TRUE";
+    elliptic->settings.getSetting("MULTIGRID COARSE GRAPH", graph);
+    useGraph = graph == "TRUE";
+  }
+  ~coarseJacobiPCG_t() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
   }
   bool fusedHex1 = false;
+  bool useGraph = true;
   deviceMemory<double> o_s, o_z, o_p, o_Ap, o_AqL;
+  // captured chunks of fused iterations, per solution vector and first-iteration flag
+  struct chunkGraph {
+    double* x;
+    int first, n;
+    hipGraphExec_t exec;
+  };
+  std::vector<chunkGraph> graphs;
+  void enqueueChunk(deviceMemory<double>& o_X, int firstIt, int n);
   int solveFusedHex1(deviceMemory<double>& o_X);
   template <typename T>
   void solve(deviceMemory<T>& o_rhs, deviceMemory<T>& o_X);
@@ -485,16 +500,51 @@ int coarseJacobiPCG_t::solveFusedHex1(deviceMemory<double>& o_X) {
   const int chunk = std::max(1, solver->checkEvery);
   while (it < maxit) {
     const int n = std::min(chunk, maxit - it);
-    for (int c = 0; c < n; ++c)
-      LIBP_CALL(libp_pcg_jacobi_hex1_iteration_f64(
-          m.Nelements, elliptic->o_GlobalToLocal.ptr(), m.o_wJ.ptr(), m.o_ggeo.ptr(), m.o_D.ptr(), elliptic->lambda,
-          N, elliptic->ogsMasked.o_rowStartsT.ptr(), elliptic->ogsMasked.o_colIdsT.ptr(), jacobi->o_invDiagA.ptr(),
-          it + c == 0, o_X.ptr(), o_r.ptr(), o_z.ptr(), o_p.ptr(), o_Ap.ptr(), o_AqL.ptr(), s, ws, st));
+    enqueueChunk(o_X, it == 0, n);
     const double* h = p.readScalars(s, LIBP_PCG_NSCALARS);
     it = (int)h[LIBP_PCG_ITERS];
     if (h[LIBP_PCG_STOP] != 0.0) break;
   }
   return it;
+}
+
+/* n fused iterations (the first with first = 1 when firstIt). With
+ * [MULTIGRID COARSE GRAPH] TRUE the chunk is captured once as a HIP graph and
+ * replayed: the kernels take only fixed device pointers, read their scalars
+ * from device memory and stop themselves, so a replay is the same work as the
+ * launches it records (no memset nodes: the reduction counters reset
+ * themselves). */
+void coarseJacobiPCG_t::enqueueChunk(deviceMemory<double>& o_X, int firstIt, int n) {
+  platform_t& p = elliptic->platform;
+  void* st = p.s();
+  void* ws = p.o_reduceWS.ptr();
+  const mesh_t& m = elliptic->mesh;
+  const dlong N = elliptic->Ndofs;
+  auto launch = [&]() {
+    for (int c = 0; c < n; ++c)
+      LIBP_CALL(libp_pcg_jacobi_hex1_iteration_f64(
+          m.Nelements, elliptic->o_GlobalToLocal.ptr(), m.o_wJ.ptr(), m.o_ggeo.ptr(), m.o_D.ptr(), elliptic->lambda,
+          N, elliptic->ogsMasked.o_rowStartsT.ptr(), elliptic->ogsMasked.o_colIdsT.ptr(), jacobi->o_invDiagA.ptr(),
+          firstIt && c == 0, o_X.ptr(), o_r.ptr(), o_z.ptr(), o_p.ptr(), o_Ap.ptr(), o_AqL.ptr(), o_s.ptr(), ws, st));
+  };
+  if (!useGraph) {
+    launch();
+    return;
+  }
+  for (auto& g : graphs)
+    if (g.x == o_X.ptr() && g.first == firstIt && g.n == n) {
+      LIBP_HIP(hipGraphLaunch(g.exec, p.stream));
+      return;
+    }
+  hipGraph_t graph;
+  LIBP_HIP(hipStreamBeginCapture(p.stream, hipStreamCaptureModeThreadLocal));
+  launch();
+  LIBP_HIP(hipStreamEndCapture(p.stream, &graph));
+  chunkGraph g{o_X.ptr(), firstIt, n, nullptr};
+  LIBP_HIP(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
+  LIBP_HIP(hipGraphDestroy(graph));
+  graphs.push_back(g);
+  LIBP_HIP(hipGraphLaunch(g.exec, p.stream));
 }
 
 template <>

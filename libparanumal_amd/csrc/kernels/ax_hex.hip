@@ -22,17 +22,20 @@
 
 namespace libp_amd {
 
-template <int Nq>
+template <int Nq, typename T = double>
 struct AxShape {
+  static constexpr bool f32 = sizeof(T) == 4;
   static constexpr int NqNq = Nq * Nq;
   static constexpr int Np = Nq * Nq * Nq;
-  // elements per workgroup: fill a wavefront for small N; at N=7 four
-  // one-wavefront elements per workgroup measured fastest
-  static constexpr int NE = (Nq == 8) ? 4 : ((NqNq >= 64) ? 1 : (64 / NqNq));
+  // elements per workgroup: fill a wavefront for small N. At N=7 the FP64
+  // kernel measured fastest with four one-wavefront elements per workgroup,
+  // the FP32 kernel with one element and a higher occupancy target (32^3 box:
+  // 114 us against 128 us with the FP64 shape).
+  static constexpr int NE = (Nq == 8) ? (f32 ? 1 : 4) : ((NqNq >= 64) ? 1 : (64 / NqNq));
   static constexpr int threads = NE * NqNq;
   // occupancy target: bounds the register budget so that the compiler
   // re-reads loop-invariant LDS operands instead of pinning them in VGPRs
-  static constexpr int waves = (Nq <= 8) ? 4 : 2;
+  static constexpr int waves = (Nq == 8 && f32) ? 8 : ((Nq <= 8) ? 4 : 2);
 };
 
 // MODE 0: q is a gathered vector indexed through GlobalToLocal (-1 -> 0).
@@ -42,11 +45,11 @@ struct AxShape {
 // clamped/predicated tail inside this kernel defeats the compiler's uniform
 // address analysis and roughly quadruples register use.
 template <int Nq, int NE, typename T, int MODE>
-__global__ void __launch_bounds__(NE * Nq * Nq) __attribute__((amdgpu_waves_per_eu(AxShape<Nq>::waves, 8)))
+__global__ void __launch_bounds__(NE * Nq * Nq) __attribute__((amdgpu_waves_per_eu(AxShape<Nq, T>::waves, 8)))
     ax_hex_kernel(const int e0, const int* __restrict__ elementList, const int* __restrict__ G2L,
                   const T* __restrict__ wJ, const T* __restrict__ ggeo, const T* __restrict__ D, const T lambda,
                   const T* __restrict__ q, T* __restrict__ Aq) {
-  using S = AxShape<Nq>;
+  using S = AxShape<Nq, T>;
   constexpr int NqNq = S::NqNq, Np = S::Np, threads = NE * NqNq;
   __shared__ T s_D[Nq][Nq + 1];
   __shared__ T s_q[NE][Nq][Nq + 1];
@@ -124,7 +127,7 @@ __global__ void __launch_bounds__(NE * Nq * Nq) __attribute__((amdgpu_waves_per_
 template <int Nq, typename T, int MODE>
 static int launch_ax(int Nelements, const int* elist, const int* g2l, const T* wJ, const T* ggeo, const T* D,
                      T lambda, const T* q, T* Aq, hipStream_t st) {
-  using S = AxShape<Nq>;
+  using S = AxShape<Nq, T>;
   if (Nelements <= 0) return LIBP_OK;
   const int full = Nelements / S::NE;
   if (full > 0)

@@ -18,7 +18,8 @@
 
 namespace libp_amd {
 
-// thread per element; node n = i + 2 j + 4 k
+// eight lanes per element, lane n = i + 2 j + 4 k holds node n; the tensor
+// contractions exchange values with the lanes that differ in one index bit
 __global__ void __launch_bounds__(256) coarse_ax_hex1_kernel(const int E, const int* __restrict__ G2L,
                                                              const double* __restrict__ wJ,
                                                              const double* __restrict__ ggeo,
@@ -27,56 +28,53 @@ __global__ void __launch_bounds__(256) coarse_ax_hex1_kernel(const int E, const 
                                                              const double* __restrict__ p,
                                                              const double* __restrict__ s, double* __restrict__ AqL) {
   if (s[8] != 0.0) return;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= E) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = t >> 3, n = t & 7;
+  const bool live = e < E;  // every lane takes part in the shuffles
+  const int i = n & 1, j = (n >> 1) & 1, k = n >> 2;
+  const int lane = threadIdx.x & 63, base = lane & ~7;
   const double beta = first ? 0.0 : s[3];
-  double d[2][2];
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b) d[a][b] = D[a * 2 + b];
-  double q[8];
-#pragma unroll
-  for (int n = 0; n < 8; ++n) {
+  double q = 0.0, G00 = 0, G01 = 0, G02 = 0, G11 = 0, G12 = 0, G22 = 0, W = 0;
+  if (live) {
     const int g = G2L[(size_t)e * 8 + n];
-    q[n] = (g >= 0) ? z[g] + beta * p[g] : 0.0;
+    q = (g >= 0) ? z[g] + beta * p[g] : 0.0;
+    const double* G = ggeo + (size_t)e * 48 + n;
+    G00 = G[0];
+    G01 = G[8];
+    G02 = G[16];
+    G11 = G[24];
+    G12 = G[32];
+    G22 = G[40];
+    W = wJ[(size_t)e * 8 + n];
   }
-  const double* G = ggeo + (size_t)e * 48;
-  double Gr[8], Gs[8], Gt[8];
-#pragma unroll
-  for (int k = 0; k < 2; ++k)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        const int n = i + 2 * j + 4 * k;
-        double qr = 0, qs = 0, qt = 0;
-#pragma unroll
-        for (int m = 0; m < 2; ++m) {
-          qr += d[i][m] * q[m + 2 * j + 4 * k];
-          qs += d[j][m] * q[i + 2 * m + 4 * k];
-          qt += d[k][m] * q[i + 2 * j + 4 * m];
-        }
-        const double G00 = G[0 * 8 + n], G01 = G[1 * 8 + n], G02 = G[2 * 8 + n];
-        const double G11 = G[3 * 8 + n], G12 = G[4 * 8 + n], G22 = G[5 * 8 + n];
-        Gr[n] = G00 * qr + G01 * qs + G02 * qt;
-        Gs[n] = G01 * qr + G11 * qs + G12 * qt;
-        Gt[n] = G02 * qr + G12 * qs + G22 * qt;
-      }
-#pragma unroll
-  for (int k = 0; k < 2; ++k)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        const int n = i + 2 * j + 4 * k;
-        double a = lambda * wJ[(size_t)e * 8 + n] * q[n];
-#pragma unroll
-        for (int m = 0; m < 2; ++m) {
-          a += d[m][i] * Gr[m + 2 * j + 4 * k];
-          a += d[m][j] * Gs[i + 2 * m + 4 * k];
-          a += d[m][k] * Gt[i + 2 * j + 4 * m];
-        }
-        AqL[(size_t)e * 8 + n] = a;
-      }
+  const double D00 = D[0], D01 = D[1], D10 = D[2], D11 = D[3];
+  auto Dm = [&](int row, int col) { return row ? (col ? D11 : D10) : (col ? D01 : D00); };
+  // partner values: q at index bit b = 0 and 1
+  auto pair = [&](double v, int bit, double& v0, double& v1) {
+    const double a0 = __shfl(v, base + (n & ~bit), 64);
+    const double a1 = __shfl(v, base + (n | bit), 64);
+    v0 = a0;
+    v1 = a1;
+  };
+  double q0, q1;
+  pair(q, 1, q0, q1);
+  const double qr = Dm(i, 0) * q0 + Dm(i, 1) * q1;
+  pair(q, 2, q0, q1);
+  const double qs = Dm(j, 0) * q0 + Dm(j, 1) * q1;
+  pair(q, 4, q0, q1);
+  const double qt = Dm(k, 0) * q0 + Dm(k, 1) * q1;
+  const double Gr = G00 * qr + G01 * qs + G02 * qt;
+  const double Gs = G01 * qr + G11 * qs + G12 * qt;
+  const double Gt = G02 * qr + G12 * qs + G22 * qt;
+  double a = lambda * W * q;
+  double v0, v1;
+  pair(Gr, 1, v0, v1);
+  a += Dm(0, i) * v0 + Dm(1, i) * v1;
+  pair(Gs, 2, v0, v1);
+  a += Dm(0, j) * v0 + Dm(1, j) * v1;
+  pair(Gt, 4, v0, v1);
+  a += Dm(0, k) * v0 + Dm(1, k) * v1;
+  if (live) AqL[(size_t)e * 8 + n] = a;
 }
 
 __global__ void __launch_bounds__(RBS) coarse_gather_pap_kernel(const int Nrows, const int* __restrict__ rowStarts,
@@ -172,8 +170,8 @@ extern "C" int libp_pcg_jacobi_hex1_iteration_f64(int Nelements, const int* Glob
       !AqL || !scalars || !ws)
     return arg_error("pcg_jacobi_hex1_iteration", "null pointer");
   hipStream_t st = as_stream(stream);
-  hipLaunchKernelGGL(coarse_ax_hex1_kernel, dim3((Nelements + 255) / 256), dim3(256), 0, st, Nelements, GlobalToLocal,
-                     wJ, ggeo, D, lambda, first, z, p, scalars, AqL);
+  hipLaunchKernelGGL(coarse_ax_hex1_kernel, dim3((8 * (long)Nelements + 255) / 256), dim3(256), 0, st, Nelements,
+                     GlobalToLocal, wJ, ggeo, D, lambda, first, z, p, scalars, AqL);
   // one row per thread: these vectors are small and the kernels latency-bound
   const unsigned blocks = (unsigned)std::min<long>(((long)Nrows + RBS - 1) / RBS, RMAXB);
   hipLaunchKernelGGL(coarse_gather_pap_kernel, dim3(blocks), dim3(RBS), 0, st, Nrows, rowStarts, colIds, AqL, first, z,

@@ -341,3 +341,21 @@ def test_elliptic_main_with_harness_settings(tmp_path, precon, flag, solver, ref
     norm = float(last.split("=")[1])
     print(precon, flag, solver, norm, flush=True)
     assert abs(norm - refnorm) < 1e-5  # the harness tolerance (test/test.py)
+
+
+def test_coarse_graph_replays_match_eager():
+    """The fused coarse Jacobi-PCG replays captured HIP graphs; every replay (several
+    solves with eager work and host reads in between) must give the eager result."""
+    import libparanumal_amd as lp
+    base = mg_settings("FP32", **{"LINEAR SOLVER STOPPING CRITERION": "RELATIVE", "MULTIGRID COARSE TOLERANCE": 1e-2,
+                                  "MULTIGRID COARSE MAXIT": 100, "BOX NX": 4})
+    eager = lp.Elliptic(dict(base, **{"MULTIGRID COARSE GRAPH": "FALSE"}))
+    graph = lp.Elliptic(dict(base, **{"MULTIGRID COARSE GRAPH": "TRUE"}))
+    for rep in range(3):
+        a, b = eager.run(), graph.run()
+        print("replay", rep, a["iterations"], b["iterations"], a["norm"], b["norm"], flush=True)
+        assert a["iterations"] == b["iterations"]
+        assert a["norm"] == b["norm"]
+        assert eager.geti("coarseIterations") == graph.geti("coarseIterations")
+        assert np.array_equal(eager.solution(), graph.solution())
+

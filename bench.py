@@ -59,7 +59,7 @@ def pmg_solve(lp, settings, reps=2):
             "coarse_iterations_last": el.geti("coarseIterations")}
     if settings.get("MULTIGRID SMOOTHER") == "LEARNED":
         li = el.learned_info()
-        info["learned"] = {"active": bool(li["active"]), "fallback": bool(li["fallback"]),
+        info["learned"] = {"arch": li["arch"], "active": bool(li["active"]), "fallback": bool(li["fallback"]),
                            "rejected_steps": int(li["rejected_steps"]), "status": el.learned_status()}
     el.close()
     return info, x
@@ -116,6 +116,7 @@ def cpu_baseline(reps=20):
                       f"serial (1 thread) on {cpu}; setup {setup:.1f}s"}
 
 
+LEARNED_WEIGHTS_LINEAR = os.path.join(ROOT, "libparanumal_amd", "weights", "learned_hex_n7_linear.bin")
 LEARNED_WEIGHTS = os.path.join(ROOT, "libparanumal_amd", "weights", "learned_hex_n7.bin")
 LEARNED_WEIGHTS_AUGMENT = os.path.join(ROOT, "libparanumal_amd", "weights", "learned_hex_n7_augment.bin")
 
@@ -123,31 +124,39 @@ LEARNED_WEIGHTS_AUGMENT = os.path.join(ROOT, "libparanumal_amd", "weights", "lea
 def learned_ab(lp, nx=32, seed=1):
     """Configuration 5: time to 1e-8 (pure relative) with the learned fine smoother
     against the plain FP32 Chebyshev smoother, same problem (curved nx^3, N=7,
-    lambda=0, Fourier forcing of a test seed the network was not trained on),
-    and each solution's distance to an all-FP64 solve to 1e-12."""
+    lambda=0, Fourier forcing of a test seed none of the fits used), and each
+    solution's distance to an all-FP64 solve to 1e-12. `learned` is the linear
+    architecture (the default learned smoother of this comparison); the MLP
+    (REPLACE and AUGMENT) results are reported beside it."""
     import numpy as np
-    if not os.path.exists(LEARNED_WEIGHTS):
-        return {"error": "no weight file " + os.path.relpath(LEARNED_WEIGHTS, ROOT)}
     base = pmg_settings(nx=nx, lam=0, data="ellipticFourier3D", seed=seed)
     plain, xp = pmg_solve(lp, base)
-    learned, xl = pmg_solve(lp, dict(base, **{"MULTIGRID SMOOTHER": "LEARNED",
-                                              "MULTIGRID LEARNED WEIGHTS": LEARNED_WEIGHTS}))
-    out = {"workload": f"curved {nx}^3 box, N=7, lambda=0, Fourier forcing seed {seed} (training seeds 101-104), "
-                       f"FPCG + p-MG 7,3,1 FP32, stop |r| <= 1e-8 |r0|",
-           "plain_chebyshev": plain, "learned": learned,
-           "speedup_learned_vs_plain": plain["seconds"] / learned["seconds"]}
-    xa = None
-    if os.path.exists(LEARNED_WEIGHTS_AUGMENT):
-        aug, xa = pmg_solve(lp, dict(base, **{"MULTIGRID SMOOTHER": "LEARNED", "MULTIGRID LEARNED MODE": "AUGMENT",
-                                              "MULTIGRID LEARNED WEIGHTS": LEARNED_WEIGHTS_AUGMENT}))
-        out["learned_augment"] = aug
-        out["speedup_augment_vs_plain"] = plain["seconds"] / aug["seconds"]
+    out = {"workload": f"curved {nx}^3 box, N=7, lambda=0, Fourier forcing seed {seed}, FPCG + p-MG 7,3,1 FP32, "
+                       f"stop |r| <= 1e-8 |r0|",
+           "plain_chebyshev": plain}
+    sols = {"plain": xp}
+    runs = [("learned", LEARNED_WEIGHTS_LINEAR, {"MULTIGRID LEARNED ARCH": "LINEAR"},
+             "linear: one bias-free BF16 MFMA layer 896->512 per element, fixed step, least-squares fit"),
+            ("learned_mlp", LEARNED_WEIGHTS, {},
+             "MLP 896-256-256-512 ReLU, energy-optimal step, REPLACE mode"),
+            ("learned_mlp_augment", LEARNED_WEIGHTS_AUGMENT, {"MULTIGRID LEARNED MODE": "AUGMENT"},
+             "MLP, AUGMENT mode (Chebyshev plus one network step)")]
+    for key, path, extra, what in runs:
+        if not os.path.exists(path):
+            out[key] = {"error": "no weight file " + os.path.relpath(path, ROOT)}
+            continue
+        info, x = pmg_solve(lp, dict(base, **{"MULTIGRID SMOOTHER": "LEARNED", "MULTIGRID LEARNED WEIGHTS": path},
+                                     **extra))
+        info["smoother"] = what
+        info["speedup_vs_plain"] = plain["seconds"] / info["seconds"]
+        out[key] = info
+        sols[key] = x
+    if "seconds" in out["learned"]:
+        out["speedup_learned_vs_plain"] = out["learned"]["speedup_vs_plain"]
     ref, xr = pmg_solve(lp, dict(base, **{"MULTIGRID PRECISION": "FP64", "LINEAR SOLVER TOLERANCE": 1e-12}), reps=1)
     scale = float(np.max(np.abs(xr)))
-    out["max_rel_error_vs_fp64_plain"] = float(np.max(np.abs(xp - xr))) / scale
-    out["max_rel_error_vs_fp64_learned"] = float(np.max(np.abs(xl - xr))) / scale
-    if xa is not None:
-        out["max_rel_error_vs_fp64_augment"] = float(np.max(np.abs(xa - xr))) / scale
+    for key, x in sols.items():
+        out["max_rel_error_vs_fp64_" + key] = float(np.max(np.abs(x - xr))) / scale
     out["fp64_reference"] = {"iterations": ref["iterations"], "seconds": ref["seconds"], "stop": "1e-12 relative"}
     return out
 

@@ -308,6 +308,12 @@ int libp_learned_loss_f32(int N, const float* d, const float* Ad, const float* r
 int libp_learned_scatter_grad_f32(int Nelements, int Np, const int* GlobalToLocal, const float* gc,
                                   const float* invLen, const float* scale, unsigned short* dY, long lddy,
                                   void* stream);
+/* Linear smoother training target T[e][n] = bf16(eh[g] / scale[e]), g = GlobalToLocal[e*Np+n]
+ * (0 where g < 0). */
+int libp_learned_element_target_f32(int Nelements, int Np, const int* GlobalToLocal, const float* eh,
+                                    const float* scale, unsigned short* T, long ldt, void* stream);
+/* x[i] = uniform(-1, 1), a pure function of (seed, i) (splitmix64). */
+int libp_random_uniform_f32(long N, unsigned long long seed, float* x, void* stream);
 /* invLen[g] = 1 / (rowStarts[g+1] - rowStarts[g]). */
 int libp_inv_row_length_f32(int Nrows, const int* rowStarts, float* invLen, void* stream);
 
@@ -346,8 +352,10 @@ int libp_elliptic_solution(void* h, double* xL_host);
  * MULTIGRID PRECISION FP32 (no reference counterpart).
  * Training: options "key=value" separated by blanks (seeds=101,102
  * daggerSeeds=201 steps=3000 dagger=1 daggerSteps=1500 lr=1e-3 targetTol=1e-9
- * initSeed=1 logEvery=250 maxRecord=24 verbose=1). Returns the number of loss
- * values (mean loss per logEvery steps, up to nloss copied), or -status. */
+ * initSeed=1 logEvery=250 maxRecord=24 verbose=1; for MULTIGRID LEARNED ARCH
+ * LINEAR: samples=16 ridge=1e-4 coarseTol=1e-6 initSeed=1 verbose=1). Returns
+ * the number of loss values (MLP: mean loss per logEvery steps; LINEAR: the
+ * validation loss of the fit; up to nloss copied), or -status. */
 int libp_elliptic_learned_train(void* h, const char* options, double* loss, int nloss);
 /* Weight files: header + FP32 parameters + FNV-1a hash; a rejected file leaves
  * the Chebyshev smoother in use and returns LIBP_ERR_SETUP (reason in
@@ -355,7 +363,8 @@ int libp_elliptic_learned_train(void* h, const char* options, double* loss, int 
 int libp_elliptic_learned_load(void* h, const char* path);
 int libp_elliptic_learned_save(void* h, const char* path);
 /* out: [0] active, [1] fallen back to Chebyshev, [2] rejected steps, [3] parameter
- * count, [4] last step length alpha, [5] last training loss, [6] input width, [7] hidden width */
+ * count, [4] last step length alpha, [5] last training loss, [6] input width, [7] hidden width,
+ * [8] architecture (0 MLP, 1 LINEAR), [9] sweeps */
 int libp_elliptic_learned_info(void* h, double* out, int nout);
 const char* libp_elliptic_learned_status(void* h);
 /* FP32 parameter vector (host buffers of the parameter count). */
@@ -366,6 +375,10 @@ int libp_elliptic_learned_correction(void* h, const float* r, float* c);
 /* Training loss of one sample (device FP32 r, target eh; nrm = eh.A eh) and its
  * gradient, left in the handle's "learnedGrad" device array; no update. */
 int libp_elliptic_learned_loss_grad(void* h, const float* r, const float* eh, double nrm, double* loss);
+/* LINEAR architecture: Gram matrices of one least-squares sample (device FP32 r
+ * and eh; host G [nIn][nIn] = X^T X, B [Np][nIn] = T^T X with X the BF16 network
+ * input of r and T[e][n] = bf16(eh / s_e) at the element's nodes). */
+int libp_elliptic_learned_linear_gram(void* h, const float* r, const float* eh, double* G, double* B);
 
 #ifdef __cplusplus
 }

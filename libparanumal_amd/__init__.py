@@ -110,6 +110,9 @@ SIGNATURES = {
                                       c_void_p, c_void_p, c_void_p]),
     "libp_learned_scatter_grad_f32": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_long,
                                               c_void_p]),
+    "libp_learned_element_target_f32": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_long,
+                                                c_void_p]),
+    "libp_random_uniform_f32": (c_int, [c_long, ctypes.c_ulonglong, c_void_p, c_void_p]),
     "libp_inv_row_length_f32": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
     "libp_elliptic_create": (c_void_p, [c_char_p]),
     "libp_elliptic_free": (None, [c_void_p]),
@@ -130,6 +133,7 @@ SIGNATURES = {
     "libp_elliptic_learned_set_params": (c_int, [c_void_p, c_void_p, c_long, c_int]),
     "libp_elliptic_learned_correction": (c_int, [c_void_p, c_void_p, c_void_p]),
     "libp_elliptic_learned_loss_grad": (c_int, [c_void_p, c_void_p, c_void_p, c_double, ctypes.POINTER(c_double)]),
+    "libp_elliptic_learned_linear_gram": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "libp_elliptic_run": (c_int, [c_void_p, ctypes.POINTER(c_double), c_int]),
     "libp_elliptic_solution": (c_int, [c_void_p, c_void_p]),
 }
@@ -258,10 +262,24 @@ class Elliptic:
         check(lib().libp_elliptic_learned_save(self._h, str(path).encode()), "learned_save")
 
     def learned_info(self):
-        out = (c_double * 8)()
-        check(lib().libp_elliptic_learned_info(self._h, out, 8), "learned_info")
-        keys = ("active", "fallback", "rejected_steps", "parameters", "alpha", "loss", "inputs", "hidden")
-        return {k: out[i] for i, k in enumerate(keys)}
+        out = (c_double * 10)()
+        check(lib().libp_elliptic_learned_info(self._h, out, 10), "learned_info")
+        keys = ("active", "fallback", "rejected_steps", "parameters", "alpha", "loss", "inputs", "hidden", "arch",
+                "sweeps")
+        d = {k: out[i] for i, k in enumerate(keys)}
+        d["arch"] = "LINEAR" if d["arch"] == 1 else "MLP"
+        return d
+
+    def learned_linear_gram(self, r, eh):
+        """Gram matrices (G = X^T X, B = T^T X; numpy FP64) of one least-squares sample of the
+        linear smoother, from device FP32 gathered r and eh."""
+        import numpy as np
+        info = self.learned_info()
+        nIn, Np = int(info["inputs"]), self.geti("Np")
+        G = np.empty((nIn, nIn)); B = np.empty((Np, nIn))
+        check(lib().libp_elliptic_learned_linear_gram(self._h, r.data_ptr(), eh.data_ptr(), G.ctypes.data,
+                                                      B.ctypes.data), "learned_linear_gram")
+        return G, B
 
     def learned_status(self):
         return lib().libp_elliptic_learned_status(self._h).decode()

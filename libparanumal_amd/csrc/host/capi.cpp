@@ -254,6 +254,9 @@ extern "C" int libp_elliptic_learned_train(void* p, const char* options, double*
       else if (k == "initSeed") opt.initSeed = std::stoul(v);
       else if (k == "logEvery") opt.logEvery = std::stoi(v);
       else if (k == "maxRecord") opt.maxRecordPerSolve = std::stoi(v);
+      else if (k == "samples") opt.linearSamples = std::stoi(v);
+      else if (k == "ridge") opt.ridge = std::stod(v);
+      else if (k == "coarseTol") opt.linearCoarseTol = std::stod(v);
       else if (k == "verbose") verbose = std::stoi(v);
       else LIBP_FORCE_ABORT("unknown training option " << k);
     }
@@ -279,15 +282,16 @@ extern "C" int libp_elliptic_learned_save(void* p, const char* path) {
 }
 
 /* out: [0] active, [1] fallback, [2] rejected steps, [3] parameters, [4] last
- * step alpha, [5] last training loss, [6] input width, [7] hidden width */
+ * step alpha, [5] last training loss, [6] input width, [7] hidden width,
+ * [8] architecture (0 MLP, 1 linear), [9] sweeps */
 extern "C" int libp_elliptic_learned_info(void* p, double* out, int nout) {
   Handle* h = (Handle*)p;
   return guarded("libp_elliptic_learned_info", [&]() {
     learnedSmoother_t* L = learnedOf(h);
     std::vector<double> s = L->stats();
-    const double v[8] = {(double)L->active, (double)L->fallback, s[3], (double)L->nParams, s[2], s[4],
-                         (double)L->nIn, (double)L->H};
-    for (int i = 0; i < nout && i < 8; ++i) out[i] = v[i];
+    const double v[10] = {(double)L->active, (double)L->fallback, s[3], (double)L->nParams, s[2], s[4],
+                          (double)L->nIn, (double)L->H, (double)L->arch, (double)L->sweeps};
+    for (int i = 0; i < nout && i < 10; ++i) out[i] = v[i];
   });
 }
 
@@ -332,6 +336,22 @@ extern "C" int libp_elliptic_learned_correction(void* p, const float* r, float* 
     L->correction(r, iD.ptr());
     LIBP_HIP(hipMemcpyAsync(c, L->o_c.ptr(), L->Ndofs * sizeof(float), hipMemcpyDeviceToDevice, h->platform.stream));
     h->platform.finish();
+  });
+}
+
+/* Gram matrices of one least-squares sample of the linear smoother (device
+ * FP32 r and eh; host G [nIn][nIn] = X^T X and B [Np][nIn] = T^T X). */
+extern "C" int libp_elliptic_learned_linear_gram(void* p, const float* r, const float* eh, double* G, double* B) {
+  Handle* h = (Handle*)p;
+  return guarded("libp_elliptic_learned_linear_gram", [&]() {
+    learnedSmoother_t* L = learnedOf(h);
+    LIBP_ABORT("null pointer", !r || !eh || !G || !B);
+    auto* mg = dynamic_cast<MultiGridPrecon*>(h->elliptic->precon.get());
+    deviceMemory<float> iD = multigridFineInvDiag32(*mg);
+    std::vector<double> g, b;
+    L->linearGramAccumulate(r, eh, iD.ptr(), g, b);
+    std::memcpy(G, g.data(), g.size() * sizeof(double));
+    std::memcpy(B, b.data(), b.size() * sizeof(double));
   });
 }
 

@@ -151,6 +151,11 @@ settings_t ellipticDefaultSettings() {
                "REPLACE: network pre- and post-smoothing; PRE: network pre-smoothing, Chebyshev post-smoothing; "
                "AUGMENT: Chebyshev smoothing plus one network step after the pre-smoothing",
                {"REPLACE", "PRE", "AUGMENT"});
+  s.newSetting("MULTIGRID LEARNED ARCH", "MLP",
+               "MLP: 896-256-256-512 ReLU network with the energy-optimal step; LINEAR: one bias-free linear layer "
+               "(a learned element-overlapping local solve) with the fixed step, a linear smoother",
+               {"MLP", "LINEAR"});
+  s.newSetting("MULTIGRID LEARNED SWEEPS", "1", "Applications of the linear learned smoother per smoothing");
   s.newSetting("LINEAR SOLVER TOLERANCE", "1e-8");
   s.newSetting("MAXIMUM ITERATIONS", "5000");
   s.newSetting("DATA SEED", "1", "Seed of the Fourier forcing data set");

@@ -131,6 +131,10 @@ struct learnedTrainOptions {
   unsigned long initSeed = 1;
   int logEvery = 250;
   int maxRecordPerSolve = 24;
+  // linear architecture (least squares on white-noise error samples)
+  int linearSamples = 16;
+  double ridge = 1e-4;
+  double linearCoarseTol = 1e-6;
 };
 std::vector<double> multigridTrainLearned(MultiGridPrecon& mg, const learnedTrainOptions& opt, int verbose);
 

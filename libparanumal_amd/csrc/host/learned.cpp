@@ -1,6 +1,7 @@
 // Learned per-element smoother (no reference counterpart).
 #include "learned.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -13,7 +14,8 @@ namespace libp {
 namespace {
 
 const char kMagic[8] = {'L', 'P', 'A', 'M', 'D', 'M', 'L', 'P'};
-const uint32_t kVersion = 1;
+// version 1: MLP only; version 2 adds the architecture after the hidden width
+const uint32_t kVersion = 2;
 
 uint64_t fnv1a(const void* data, size_t n) {
   const unsigned char* p = (const unsigned char*)data;
@@ -39,21 +41,29 @@ struct splitmix {
 
 }  // namespace
 
-learnedSmoother_t::learnedSmoother_t(elliptic_t& el) : elliptic(el), platform(el.platform) {
+learnedSmoother_t::learnedSmoother_t(elliptic_t& el, int _arch) : elliptic(el), platform(el.platform), arch(_arch) {
   const mesh_t& mesh = el.mesh;
+  LIBP_ABORT("unknown learned smoother architecture " << arch, arch != ARCH_MLP && arch != ARCH_LINEAR);
   Np = mesh.Np;
   Nfp = mesh.Nfp;
   nIn = Np + 6 * Nfp;
   nOut = Np;
   E = mesh.Nelements;
   Ndofs = el.Ndofs;
-  oW1 = 0;
-  ob1 = oW1 + (size_t)H * nIn;
-  oW2 = ob1 + H;
-  ob2 = oW2 + (size_t)H * H;
-  oW3 = ob2 + H;
-  ob3 = oW3 + (size_t)nOut * H;
-  oOm = ob3 + nOut;
+  if (arch == ARCH_LINEAR) {
+    H = 0;
+    oW1 = ob1 = oW2 = ob2 = ob3 = 0;
+    oW3 = 0;
+    oOm = (size_t)nOut * nIn;
+  } else {
+    oW1 = 0;
+    ob1 = oW1 + (size_t)H * nIn;
+    oW2 = ob1 + H;
+    ob2 = oW2 + (size_t)H * H;
+    oW3 = ob2 + H;
+    ob3 = oW3 + (size_t)nOut * H;
+    oOm = ob3 + nOut;
+  }
   nParams = oOm + 1;
 
   o_theta = platform.malloc<float>(nParams);
@@ -62,8 +72,10 @@ learnedSmoother_t::learnedSmoother_t(elliptic_t& el) : elliptic(el), platform(el
   o_v = platform.malloc<float>(nParams);
   o_thetaBf = platform.malloc<unsigned short>(nParams);
   o_X = platform.malloc<unsigned short>((size_t)E * nIn);
-  o_H1 = platform.malloc<unsigned short>((size_t)E * H);
-  o_H2 = platform.malloc<unsigned short>((size_t)E * H);
+  if (H > 0) {
+    o_H1 = platform.malloc<unsigned short>((size_t)E * H);
+    o_H2 = platform.malloc<unsigned short>((size_t)E * H);
+  }
   o_scale = platform.malloc<float>(E);
   o_Y = platform.malloc<float>((size_t)E * nOut);
   o_c = platform.malloc<float>(Ndofs);
@@ -137,8 +149,10 @@ void learnedSmoother_t::initRandom(unsigned long seed) {
   auto fill = [&](size_t o, int fanOut, int fanIn, double bound) {
     for (size_t i = 0; i < (size_t)fanOut * fanIn; ++i) th[o + i] = (float)((2 * rng.uniform() - 1) * bound);
   };
-  fill(oW1, H, nIn, std::sqrt(6.0 / nIn));
-  fill(oW2, H, H, std::sqrt(6.0 / H));
+  if (arch == ARCH_MLP) {
+    fill(oW1, H, nIn, std::sqrt(6.0 / nIn));
+    fill(oW2, H, H, std::sqrt(6.0 / H));
+  }
   // last layer starts at zero: the initial correction is omega D^{-1} r (damped Jacobi)
   th[oOm] = 0.5f;
   setParameters(th);
@@ -182,8 +196,14 @@ bool learnedSmoother_t::load(const std::string& path) {
   f.read((char*)&h, 4);
   f.read((char*)&n, 8);
   f.read((char*)&hash, 8);
-  if (!f || std::memcmp(magic, kMagic, 8) != 0 || ver != kVersion) {
+  uint32_t fileArch = ARCH_MLP;
+  if (ver == 2) f.read((char*)&fileArch, 4);
+  if (!f || std::memcmp(magic, kMagic, 8) != 0 || (ver != 1 && ver != 2)) {
     status = "rejected: not a learned-smoother weight file";
+    return false;
+  }
+  if ((int)fileArch != arch) {
+    status = "rejected: weights are for another architecture";
     return false;
   }
   if ((int)np != Np || (int)nfp != Nfp || (int)h != H || n != nParams) {
@@ -215,7 +235,7 @@ void learnedSmoother_t::save(const std::string& path) const {
   std::vector<float> th = parameters();
   std::ofstream f(path, std::ios::binary);
   LIBP_ABORT("learned smoother: cannot write " << path, !f);
-  const uint32_t ver = kVersion, np = Np, nfp = Nfp, h = H;
+  const uint32_t ver = kVersion, np = Np, nfp = Nfp, h = H, a = arch;
   const uint64_t n = nParams, hash = fnv1a(th.data(), nParams * sizeof(float));
   f.write(kMagic, 8);
   f.write((const char*)&ver, 4);
@@ -224,6 +244,7 @@ void learnedSmoother_t::save(const std::string& path) const {
   f.write((const char*)&h, 4);
   f.write((const char*)&n, 8);
   f.write((const char*)&hash, 8);
+  f.write((const char*)&a, 4);
   f.write((const char*)th.data(), nParams * sizeof(float));
   LIBP_ABORT("learned smoother: write failed for " << path, !f);
 }
@@ -234,9 +255,13 @@ void learnedSmoother_t::correction(const float* r, const float* invDiag) {
   const float* th = o_theta.ptr();
   LIBP_CALL(libp_learned_features_f32(E, Np, 6 * Nfp, elliptic.o_GlobalToLocal.ptr(), o_trace.ptr(), r, invDiag,
                                       o_X.ptr(), nIn, o_scale.ptr(), s));
-  LIBP_CALL(libp_mlp_linear_fwd(E, nIn, H, o_X.ptr(), nIn, Wb + oW1, nIn, th + ob1, 1, o_H1.ptr(), H, s));
-  LIBP_CALL(libp_mlp_linear_fwd(E, H, H, o_H1.ptr(), H, Wb + oW2, H, th + ob2, 1, o_H2.ptr(), H, s));
-  LIBP_CALL(libp_mlp_linear_fwd(E, H, nOut, o_H2.ptr(), H, Wb + oW3, H, th + ob3, 0, o_Y.ptr(), nOut, s));
+  if (arch == ARCH_LINEAR) {
+    LIBP_CALL(libp_mlp_linear_fwd(E, nIn, nOut, o_X.ptr(), nIn, Wb + oW3, nIn, nullptr, 0, o_Y.ptr(), nOut, s));
+  } else {
+    LIBP_CALL(libp_mlp_linear_fwd(E, nIn, H, o_X.ptr(), nIn, Wb + oW1, nIn, th + ob1, 1, o_H1.ptr(), H, s));
+    LIBP_CALL(libp_mlp_linear_fwd(E, H, H, o_H1.ptr(), H, Wb + oW2, H, th + ob2, 1, o_H2.ptr(), H, s));
+    LIBP_CALL(libp_mlp_linear_fwd(E, H, nOut, o_H2.ptr(), H, Wb + oW3, H, th + ob3, 0, o_Y.ptr(), nOut, s));
+  }
   LIBP_CALL(libp_learned_assemble_f32(Ndofs, elliptic.ogsMasked.o_rowStartsT.ptr(), elliptic.ogsMasked.o_colIdsT.ptr(),
                                       Np, o_Y.ptr(), nOut, o_scale.ptr(), th + oOm, r, invDiag, o_c.ptr(), s));
 }
@@ -244,6 +269,22 @@ void learnedSmoother_t::correction(const float* r, const float* invDiag) {
 void learnedSmoother_t::smoothPre(deviceMemory<float>& o_r, deviceMemory<float>& o_x, deviceMemory<float>* o_resOut,
                                   const float* invDiag, const op32_t& A) {
   void* s = platform.s();
+  if (arch == ARCH_LINEAR) {  // x = c(r), then x += c(r - A x) for further sweeps
+    const size_t bytes = Ndofs * sizeof(float);
+    correction(o_r.ptr(), invDiag);
+    LIBP_HIP(hipMemcpyAsync(o_x.ptr(), o_c.ptr(), bytes, hipMemcpyDeviceToDevice, platform.stream));
+    for (int k = 1; k < sweeps; ++k) {
+      A(o_x, o_res);
+      LIBP_CALL(libp_axpy_f32(Ndofs, 1.f, o_r.ptr(), -1.f, o_res.ptr(), s));
+      correction(o_res.ptr(), invDiag);
+      LIBP_CALL(libp_axpy_f32(Ndofs, 1.f, o_c.ptr(), 1.f, o_x.ptr(), s));
+    }
+    if (o_resOut) {
+      A(o_x, *o_resOut);
+      LIBP_CALL(libp_axpy_f32(Ndofs, 1.f, o_r.ptr(), -1.f, o_resOut->ptr(), s));
+    }
+    return;
+  }
   correction(o_r.ptr(), invDiag);
   A(o_c, o_Ac);
   LIBP_CALL(libp_learned_alpha_f32(Ndofs, o_c.ptr(), o_r.ptr(), o_Ac.ptr(), o_sc.ptr(), o_rws.ptr(), s));
@@ -261,6 +302,11 @@ void learnedSmoother_t::augment(deviceMemory<float>& o_x, deviceMemory<float>& o
   void* s = platform.s();
   correction(o_resIO.ptr(), invDiag);
   A(o_c, o_Ac);
+  if (arch == ARCH_LINEAR) {  // fixed step: x += c, res -= A c
+    LIBP_CALL(libp_axpy_f32(Ndofs, 1.f, o_c.ptr(), 1.f, o_x.ptr(), s));
+    LIBP_CALL(libp_axpy_f32(Ndofs, -1.f, o_Ac.ptr(), 1.f, o_resIO.ptr(), s));
+    return;
+  }
   LIBP_CALL(libp_learned_alpha_f32(Ndofs, o_c.ptr(), o_resIO.ptr(), o_Ac.ptr(), o_sc.ptr(), o_rws.ptr(), s));
   LIBP_CALL(libp_learned_update_f32(Ndofs, 0, o_sc.ptr(), o_c.ptr(), o_Ac.ptr(), o_x.ptr(), o_resIO.ptr(), s));
 }
@@ -268,6 +314,15 @@ void learnedSmoother_t::augment(deviceMemory<float>& o_x, deviceMemory<float>& o
 void learnedSmoother_t::smoothPost(deviceMemory<float>& o_r, deviceMemory<float>& o_x, const float* invDiag,
                                    const op32_t& A) {
   void* s = platform.s();
+  if (arch == ARCH_LINEAR) {  // sweeps of x += c(r - A x)
+    for (int k = 0; k < sweeps; ++k) {
+      A(o_x, o_res);
+      LIBP_CALL(libp_axpy_f32(Ndofs, 1.f, o_r.ptr(), -1.f, o_res.ptr(), s));
+      correction(o_res.ptr(), invDiag);
+      LIBP_CALL(libp_axpy_f32(Ndofs, 1.f, o_c.ptr(), 1.f, o_x.ptr(), s));
+    }
+    return;
+  }
   A(o_x, o_res);
   LIBP_CALL(libp_axpy_f32(Ndofs, 1.f, o_r.ptr(), -1.f, o_res.ptr(), s));
   correction(o_res.ptr(), invDiag);
@@ -278,6 +333,8 @@ void learnedSmoother_t::smoothPost(deviceMemory<float>& o_r, deviceMemory<float>
 
 void learnedSmoother_t::trainStep(const float* r, const float* eh, const double* nrm, const float* invDiag,
                                   const op32_t& A, float lr, bool update) {
+  LIBP_ABORT("the linear smoother is fitted by least squares (linearGramAccumulate, linearSolve)",
+             arch == ARCH_LINEAR);
   void* s = platform.s();
   if (!o_dY.ptr()) {
     o_dY = platform.malloc<unsigned short>((size_t)E * nOut);
@@ -308,6 +365,80 @@ void learnedSmoother_t::trainStep(const float* r, const float* eh, const double*
   if (update)
     LIBP_CALL(libp_adam_step((int)nParams, o_theta.ptr(), o_thetaBf.ptr(), g, o_m.ptr(), o_v.ptr(), lr, 0.9f, 0.999f,
                              1e-8f, ++adamStep, s));
+}
+
+void learnedSmoother_t::linearGramAccumulate(const float* r, const float* eh, const float* invDiag,
+                                             std::vector<double>& G, std::vector<double>& B) {
+  LIBP_ABORT("least squares fits the linear smoother only", arch != ARCH_LINEAR);
+  void* s = platform.s();
+  if (!o_T.ptr()) {
+    o_T = platform.malloc<unsigned short>((size_t)E * nOut);
+    o_grad = platform.malloc<float>(std::max<size_t>(nParams, (size_t)nIn * nIn));
+    o_d = platform.malloc<float>(nIn);
+    o_ws = platform.malloc<char>(std::max(libp_mlp_ws_bytes(E, nIn, nIn), libp_mlp_ws_bytes(E, nIn, nOut)));
+  }
+  G.resize((size_t)nIn * nIn, 0.0);
+  B.resize((size_t)nOut * nIn, 0.0);
+  LIBP_CALL(libp_learned_features_f32(E, Np, 6 * Nfp, elliptic.o_GlobalToLocal.ptr(), o_trace.ptr(), r, invDiag,
+                                      o_X.ptr(), nIn, o_scale.ptr(), s));
+  LIBP_CALL(libp_learned_element_target_f32(E, Np, elliptic.o_GlobalToLocal.ptr(), eh, o_scale.ptr(), o_T.ptr(), nOut,
+                                            s));
+  std::vector<float> h;
+  auto addTo = [&](std::vector<double>& acc, size_t n) {
+    h.resize(n);
+    o_grad.copyTo(h.data(), n, 0, platform.stream);
+    platform.finish();
+    for (size_t i = 0; i < n; ++i) acc[i] += h[i];
+  };
+  // G[j][k] = sum_e X[e][j] X[e][k] ; B[n][k] = sum_e T[e][n] X[e][k]
+  LIBP_CALL(libp_mlp_linear_bwd_weight(E, nIn, nIn, o_X.ptr(), nIn, o_X.ptr(), nIn, o_grad.ptr(), o_d.ptr(),
+                                       o_ws.ptr(), s));
+  addTo(G, (size_t)nIn * nIn);
+  LIBP_CALL(libp_mlp_linear_bwd_weight(E, nIn, nOut, o_T.ptr(), nOut, o_X.ptr(), nIn, o_grad.ptr(), o_d.ptr(),
+                                       o_ws.ptr(), s));
+  addTo(B, (size_t)nOut * nIn);
+}
+
+void learnedSmoother_t::linearSolve(const std::vector<double>& G, const std::vector<double>& B, double ridge) {
+  LIBP_ABORT("least squares fits the linear smoother only", arch != ARCH_LINEAR);
+  const int n = nIn;
+  LIBP_ABORT("Gram matrices of the wrong size", G.size() != (size_t)n * n || B.size() != (size_t)nOut * n);
+  double tr = 0;
+  for (int i = 0; i < n; ++i) tr += G[(size_t)i * n + i];
+  LIBP_ABORT("empty Gram matrix (no samples)", !(tr > 0));
+  // Cholesky L L^T of the ridge-regularised (symmetrised) Gram matrix
+  std::vector<double> L((size_t)n * n, 0.0);
+  const double shift = ridge * tr / n;
+  for (int j = 0; j < n; ++j) {
+    double d = 0.5 * (G[(size_t)j * n + j] + G[(size_t)j * n + j]) + shift;
+    for (int k = 0; k < j; ++k) d -= L[(size_t)j * n + k] * L[(size_t)j * n + k];
+    LIBP_ABORT("Gram matrix not positive definite", !(d > 0));
+    const double ljj = std::sqrt(d);
+    L[(size_t)j * n + j] = ljj;
+    for (int i = j + 1; i < n; ++i) {
+      double v = 0.5 * (G[(size_t)i * n + j] + G[(size_t)j * n + i]);
+      for (int k = 0; k < j; ++k) v -= L[(size_t)i * n + k] * L[(size_t)j * n + k];
+      L[(size_t)i * n + j] = v / ljj;
+    }
+  }
+  // row o of W solves (G + shift I) w = B[o]
+  std::vector<float> th(nParams, 0.f);
+  std::vector<double> y(n);
+  for (int o = 0; o < nOut; ++o) {
+    for (int i = 0; i < n; ++i) {
+      double v = B[(size_t)o * n + i];
+      for (int k = 0; k < i; ++k) v -= L[(size_t)i * n + k] * y[k];
+      y[i] = v / L[(size_t)i * n + i];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      double v = y[i];
+      for (int k = i + 1; k < n; ++k) v -= L[(size_t)k * n + i] * y[k];
+      y[i] = v / L[(size_t)i * n + i];
+    }
+    for (int i = 0; i < n; ++i) th[oW3 + (size_t)o * n + i] = (float)y[i];
+  }
+  th[oOm] = 0.f;
+  setParameters(th);
 }
 
 void learnedSmoother_t::monitorStart() {

@@ -27,13 +27,26 @@ class elliptic_t;
  * over residual / target pairs, where eh is the part of the error A^{-1} r that
  * the coarse-grid correction leaves (computed by the multigrid trainer).
  *
- * theta: W1 [H][nIn], b1 [H], W2 [H][H], b2 [H], W3 [Np][H], b3 [Np], omega. */
+ * theta: W1 [H][nIn], b1 [H], W2 [H][H], b2 [H], W3 [Np][H], b3 [Np], omega.
+ *
+ * ARCH_LINEAR replaces the three layers by one bias-free linear layer
+ * y_e = W x_e (theta: W [Np][nIn], omega) and the energy-optimal step by the
+ * fixed step x += c: the smoother is then a fixed linear map of r, an
+ * element-overlapping Schwarz-type local solve whose local operator is
+ * learned, and the V-cycle a fixed linear preconditioner. It is fitted by
+ * least squares: for white-noise errors v (r = A v, eh = v - P A_1^{-1} R r) it
+ * minimises sum_e |W x_e - eh_e / s_e|^2 + ridge, from Gram matrices formed by
+ * the backward-weight GEMM. `sweeps` applications make one smoothing. */
 class learnedSmoother_t {
  public:
   typedef std::function<void(deviceMemory<float>&, deviceMemory<float>&)> op32_t;
 
+  enum { ARCH_MLP = 0, ARCH_LINEAR = 1 };
+
   elliptic_t& elliptic;
   platform_t platform;
+  int arch = ARCH_MLP;
+  int sweeps = 1;  // ARCH_LINEAR: applications per smoothing
   int Np = 0, Nfp = 0, nIn = 0, H = 256, nOut = 0;
   dlong E = 0, Ndofs = 0;
   size_t oW1 = 0, ob1 = 0, oW2 = 0, ob2 = 0, oW3 = 0, ob3 = 0, oOm = 0, nParams = 0;
@@ -47,7 +60,7 @@ class learnedSmoother_t {
   deviceMemory<float> o_invLen;  // [Ndofs] 1 / multiplicity
   deviceMemory<float> o_theta, o_grad, o_m, o_v;
   deviceMemory<unsigned short> o_thetaBf;
-  deviceMemory<unsigned short> o_X, o_H1, o_H2, o_dY, o_dH1, o_dH2;
+  deviceMemory<unsigned short> o_X, o_H1, o_H2, o_dY, o_dH1, o_dH2, o_T;
   deviceMemory<float> o_scale, o_Y, o_c, o_Ac, o_res, o_gc, o_d, o_Ad;
   deviceMemory<double> o_sc;  // [0..3] step statistics, [4] last loss, [5] running loss sum
   deviceMemory<char> o_ws, o_rws;
@@ -55,7 +68,7 @@ class learnedSmoother_t {
   double monitorRejected = 0;
   std::vector<double> monitorHist;
 
-  explicit learnedSmoother_t(elliptic_t& el);
+  explicit learnedSmoother_t(elliptic_t& el, int arch = ARCH_MLP);
 
   // trace map from the box structure; checked against the global node ids
   void setupMaps();
@@ -88,6 +101,14 @@ class learnedSmoother_t {
   // Chebyshev takes over for the rest of that solve
   void monitorStart();
   void monitorIteration(int it, double rdotr);
+  // ARCH_LINEAR least squares: add the Gram matrices of one sample (features
+  // X of r, element targets T of eh) to G [nIn][nIn] += X^T X and
+  // B [Np][nIn] += T^T X (FP32 GEMM on the device, FP64 sums on the host)
+  void linearGramAccumulate(const float* r, const float* eh, const float* invDiag, std::vector<double>& G,
+                            std::vector<double>& B);
+  // W = B (G + ridge tr(G)/nIn I)^{-1} (FP64 Cholesky), omega = 0
+  void linearSolve(const std::vector<double>& G, const std::vector<double>& B, double ridge);
+
   // statistics: [0] c.res, [1] c.Ac, [2] alpha, [3] rejected steps, [4] last loss, [5] loss sum
   std::vector<double> stats() const;
   void resetLossSum();

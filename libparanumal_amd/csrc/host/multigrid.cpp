@@ -632,6 +632,7 @@ class multigrid_t : public MultiGridPreconBase {
 
   void attachLearned();
   std::vector<double> trainLearned(const learnedTrainOptions& opt, int verbose);
+  std::vector<double> trainLinear(const learnedTrainOptions& opt, int verbose);
 
   // parAlmondVcycle.cpp:34-60
   void vcycle(size_t k, deviceMemory<T>& o_RHS, deviceMemory<T>& o_X) {
@@ -668,7 +669,11 @@ void multigrid_t<float>::Operator(deviceMemory<dfloat>& o_r, deviceMemory<dfloat
 template <>
 void multigrid_t<float>::attachLearned() {
   std::string path = fine.settings.getSetting("MULTIGRID LEARNED WEIGHTS");
-  auto ls = std::make_shared<learnedSmoother_t>(*ellipticLevels[0]);
+  const int arch = fine.settings.compareSetting("MULTIGRID LEARNED ARCH", "LINEAR") ? learnedSmoother_t::ARCH_LINEAR
+                                                                                     : learnedSmoother_t::ARCH_MLP;
+  auto ls = std::make_shared<learnedSmoother_t>(*ellipticLevels[0], arch);
+  fine.settings.getSetting("MULTIGRID LEARNED SWEEPS", ls->sweeps);
+  LIBP_ABORT("MULTIGRID LEARNED SWEEPS must be at least 1", ls->sweeps < 1);
   if (!path.empty() && !ls->load(path) && fine.settings.compareSetting("VERBOSE", "TRUE"))
     printf("learned smoother %s; smoothing with Chebyshev\n", ls->status.c_str());
   levels[0]->learned = ls;
@@ -739,6 +744,9 @@ int fpcgSolve(elliptic_t& el, operator_t& M, deviceMemory<dfloat>& o_x, deviceMe
 }  // namespace
 
 template <>
+std::vector<double> multigrid_t<float>::trainLinear(const learnedTrainOptions& opt, int verbose);
+
+template <>
 std::vector<double> multigrid_t<float>::trainLearned(const learnedTrainOptions& opt, int verbose) {
   LIBP_ABORT("training needs MULTIGRID SMOOTHER LEARNED", levels.empty() || !levels[0]->learned);
   LIBP_ABORT("training needs at least two p-MG levels above the coarse solve", ellipticLevels.size() < 3);
@@ -750,6 +758,7 @@ std::vector<double> multigrid_t<float>::trainLearned(const learnedTrainOptions& 
   void* st = p.s();
   const dlong N0 = el0.Ndofs, N1 = el1.Ndofs;
   levelVcycle_t M0(*this, 0), M1(*this, 1);
+  if (L.arch == learnedSmoother_t::ARCH_LINEAR) return trainLinear(opt, verbose);
   deviceMemory<double> b = p.malloc<double>(N0), x = p.malloc<double>(N0), w0 = p.malloc<double>(N0);
   deviceMemory<double> r64 = p.malloc<double>(N0), e64 = p.malloc<double>(N0), t64 = p.malloc<double>(N0);
   deviceMemory<double> rc64 = p.malloc<double>(N1), ec64 = p.malloc<double>(N1), w1 = p.malloc<double>(N1);
@@ -847,6 +856,75 @@ std::vector<double> multigrid_t<float>::trainLearned(const learnedTrainOptions& 
   L.fallback = false;
   L.status = "trained";
   return hist;
+}
+
+/* Least-squares fit of the linear smoother. Samples are white-noise errors v
+ * (uniform in [-1,1] at every unmasked node), r = A v, and the error left
+ * after an exact coarse correction eh = v - P A_1^{-1} R r (level-1 FPCG with
+ * the V-cycle below it to opt.linearCoarseTol). Returns {L2 fit loss
+ * |eh - c(r)|^2 / |eh|^2 on a fresh sample}. */
+template <>
+std::vector<double> multigrid_t<float>::trainLinear(const learnedTrainOptions& opt, int verbose) {
+  learnedSmoother_t& L = *levels[0]->learned;
+  MGLevel<float>& lev0 = *levels[0];
+  elliptic_t& el1 = *ellipticLevels[1];
+  platform_t& p = fine.platform;
+  void* st = p.s();
+  const dlong N0 = ellipticLevels[0]->Ndofs, N1 = el1.Ndofs;
+  levelVcycle_t M1(*this, 1);
+  deviceMemory<float> v = p.malloc<float>(N0), r = p.malloc<float>(N0), eh = p.malloc<float>(N0);
+  deviceMemory<float> ec32 = p.malloc<float>(N1);
+  deviceMemory<double> rc64 = p.malloc<double>(N1), ec64 = p.malloc<double>(N1), w1 = p.malloc<double>(N1);
+  double* tmp = p.o_scalars.ptr() + 20;
+  void* ws = p.o_reduceWS.ptr();
+  const bool wasActive = L.active;
+  L.active = false;
+  // one sample: v from the seed, r = A v, eh = v - P A_1^{-1} R r
+  auto sample = [&](unsigned long long seed) {
+    LIBP_CALL(libp_random_uniform_f32(N0, seed, v.ptr(), st));
+    lev0.Operator(v, r);
+    lev0.coarsen(r, o_rhs[1]);
+    LIBP_CALL(libp_convert_f32_to_f64(N1, o_rhs[1].ptr(), rc64.ptr(), st));
+    const int its = fpcgSolve(el1, M1, ec64, rc64, w1, opt.linearCoarseTol, 200);
+    LIBP_CALL(libp_convert_f64_to_f32(N1, ec64.ptr(), ec32.ptr(), st));
+    LIBP_CALL(libp_axpy_f32(N1, 0.f, ec32.ptr(), -1.f, ec32.ptr(), st));
+    LIBP_HIP(hipMemcpyAsync(eh.ptr(), v.ptr(), N0 * sizeof(float), hipMemcpyDeviceToDevice, p.stream));
+    lev0.prolongate(ec32, eh);
+    return its;
+  };
+  std::vector<double> G, B;
+  const unsigned long long base = 0x6a09e667f3bcc909ull * (unsigned long long)opt.initSeed;
+  for (int k = 0; k < opt.linearSamples; ++k) {
+    const int its = sample(base + (unsigned long long)k);
+    L.linearGramAccumulate(r.ptr(), eh.ptr(), lev0.o_invDiagA.ptr(), G, B);
+    if (verbose) printf("learned smoother (linear): sample %d, level-1 FPCG %d iterations\n", k + 1, its);
+  }
+  L.linearSolve(G, B, opt.ridge);
+  // fit loss on a sample that was not in the fit
+  sample(base + (unsigned long long)opt.linearSamples + 7919ull);
+  L.correction(r.ptr(), lev0.o_invDiagA.ptr());
+  deviceMemory<double> a = p.malloc<double>(N0), b = p.malloc<double>(N0);
+  LIBP_CALL(libp_convert_f32_to_f64(N0, eh.ptr(), a.ptr(), st));
+  LIBP_CALL(libp_norm2sq_f64(N0, a.ptr(), tmp, ws, st));
+  const double ee = p.readScalars(tmp, 1)[0];
+  LIBP_CALL(libp_convert_f32_to_f64(N0, L.o_c.ptr(), b.ptr(), st));
+  LIBP_CALL(libp_axpy_f64(N0, -1.0, b.ptr(), 1.0, a.ptr(), st));
+  LIBP_CALL(libp_norm2sq_f64(N0, a.ptr(), tmp, ws, st));
+  const double dd = p.readScalars(tmp, 1)[0];
+  const double loss = dd / ee;
+  if (verbose) printf("learned smoother (linear): %d samples, ridge %g, validation loss %.4f\n", opt.linearSamples,
+                      opt.ridge, loss);
+  (void)wasActive;
+  L.active = true;
+  L.fallback = false;
+  L.status = "trained";
+  return {loss};
+}
+
+template <>
+std::vector<double> multigrid_t<double>::trainLinear(const learnedTrainOptions&, int) {
+  LIBP_FORCE_ABORT("MULTIGRID SMOOTHER LEARNED needs MULTIGRID PRECISION FP32");
+  return {};
 }
 
 template <>

@@ -194,6 +194,37 @@ __global__ void __launch_bounds__(256) learned_scatter_grad_kernel(const int E, 
   }
 }
 
+// Least-squares target of the linear smoother: T[e][n] = eh[g] / s_e at the
+// element's nodes (zero at masked nodes), the element-local error that the
+// output y_e = W x_e should reproduce before it is multiplied by s_e.
+__global__ void __launch_bounds__(256) learned_element_target_kernel(const int E, const int Np,
+                                                                     const int* __restrict__ G2L,
+                                                                     const float* __restrict__ eh,
+                                                                     const float* __restrict__ scale,
+                                                                     u16* __restrict__ T, const long ldt) {
+  const long NL = (long)E * Np;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < NL; i += (long)gridDim.x * blockDim.x) {
+    const int e = (int)(i / Np), n = (int)(i - (long)e * Np);
+    const int g = G2L[i];
+    const float s = scale[e];
+    const float v = (g >= 0 && s > 0.f) ? eh[g] / s : 0.f;
+    T[(long)e * ldt + n] = f2bf_l(v);
+  }
+}
+
+// x[i] = uniform(-1, 1) from splitmix64 of (seed, i): white-noise error
+// samples for the linear smoother's training (a pure function of seed and i)
+__global__ void __launch_bounds__(256) random_uniform_kernel(const long N, const unsigned long long seed,
+                                                             float* __restrict__ x) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    unsigned long long z = seed + 0x9e3779b97f4a7c15ull * (unsigned long long)(i + 1);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    x[i] = (float)((double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0);
+  }
+}
+
 // invLen[g] = 1 / (rowStarts[g+1] - rowStarts[g])
 __global__ void __launch_bounds__(256) inv_row_length_kernel(const int Nrows, const int* __restrict__ rowStarts,
                                                              float* __restrict__ invLen) {
@@ -286,4 +317,21 @@ extern "C" int libp_inv_row_length_f32(int Nrows, const int* rowStarts, float* i
   hipLaunchKernelGGL(inv_row_length_kernel, dim3(stream_blocks(Nrows, 256)), dim3(256), 0, as_stream(stream), Nrows,
                      rowStarts, invLen);
   return launch_status("inv_row_length");
+}
+
+extern "C" int libp_learned_element_target_f32(int Nelements, int Np, const int* GlobalToLocal, const float* eh,
+                                               const float* scale, unsigned short* T, long ldt, void* stream) {
+  if (Nelements <= 0) return LIBP_OK;
+  if (!GlobalToLocal || !eh || !scale || !T) return arg_error("learned_element_target", "null pointer");
+  if (Np <= 0 || ldt < Np) return arg_error("learned_element_target", "bad size");
+  hipLaunchKernelGGL(learned_element_target_kernel, dim3(stream_blocks((long)Nelements * Np, 256)), dim3(256), 0,
+                     as_stream(stream), Nelements, Np, GlobalToLocal, eh, scale, T, ldt);
+  return launch_status("learned_element_target");
+}
+
+extern "C" int libp_random_uniform_f32(long N, unsigned long long seed, float* x, void* stream) {
+  if (N <= 0) return LIBP_OK;
+  if (!x) return arg_error("random_uniform", "null pointer");
+  hipLaunchKernelGGL(random_uniform_kernel, dim3(stream_blocks(N, 256)), dim3(256), 0, as_stream(stream), N, seed, x);
+  return launch_status("random_uniform");
 }

@@ -2,6 +2,7 @@
 write its weight file and a manifest next to it.
 
     python -m libparanumal_amd.train_learned [--nx 32] [--out libparanumal_amd/weights/learned_hex_n7.bin]
+    python -m libparanumal_amd.train_learned --arch LINEAR [--samples 16] [--ridge 1e-4]
 
 Workload: curved hex box nx^3 on [-1,1]^3 (x += 0.05 sin(2 pi y) sin(2 pi z)),
 N = 7, lambda = 0, Dirichlet, seeded 64-mode Fourier forcing; FPCG with p-MG
@@ -10,6 +11,11 @@ from solves with the training seeds (default 101-104, then 201-202 with the
 network in the cycle); the benchmark's test seed (1) is not among them.
 Everything runs through the library's own kernels (libp_mlp_*, libp_adam_step,
 libp_learned_*).
+
+--arch LINEAR fits the linear smoother instead (one bias-free 896 -> 512 layer,
+fixed step): least squares over white-noise error samples v (r = A v, target
+v - P A_1^{-1} R r), with the Gram matrices formed on the GPU by the
+backward-weight GEMM and the 896 x 896 system solved in FP64 on the host.
 """
 import argparse
 import hashlib
@@ -23,6 +29,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 DEFAULT_OUT = os.path.join(ROOT, "libparanumal_amd", "weights", "learned_hex_n7.bin")
+DEFAULT_OUT_LINEAR = os.path.join(ROOT, "libparanumal_amd", "weights", "learned_hex_n7_linear.bin")
 
 
 def settings(nx=32, smoother="LEARNED", seed=1, precision="FP32", weights=None):
@@ -42,7 +49,10 @@ def settings(nx=32, smoother="LEARNED", seed=1, precision="FP32", weights=None):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--nx", type=int, default=32)
-    ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--arch", default="MLP", choices=["MLP", "LINEAR"])
+    ap.add_argument("--samples", type=int, default=16, help="LINEAR: white-noise error samples")
+    ap.add_argument("--ridge", type=float, default=1e-4, help="LINEAR: ridge, relative to tr(G)/nIn")
     ap.add_argument("--seeds", default="101,102,103,104")
     ap.add_argument("--dagger-seeds", default="201,202")
     ap.add_argument("--steps", type=int, default=3000)
@@ -54,13 +64,19 @@ def main(argv=None):
     a = ap.parse_args(argv)
 
     import libparanumal_amd as lp
+    if a.out is None:
+        a.out = DEFAULT_OUT_LINEAR if a.arch == "LINEAR" else DEFAULT_OUT
     s = settings(a.nx)
     s["MULTIGRID LEARNED MODE"] = a.mode
+    s["MULTIGRID LEARNED ARCH"] = a.arch
     t0 = time.time()
     el = lp.Elliptic(s)
     setup = time.time() - t0
-    opts = dict(seeds=a.seeds, daggerSeeds=a.dagger_seeds, steps=a.steps, dagger=a.dagger,
-                daggerSteps=a.dagger_steps, lr=a.lr, initSeed=a.init_seed, logEvery=250, verbose=1)
+    if a.arch == "LINEAR":
+        opts = dict(samples=a.samples, ridge=a.ridge, initSeed=a.init_seed, verbose=1)
+    else:
+        opts = dict(seeds=a.seeds, daggerSeeds=a.dagger_seeds, steps=a.steps, dagger=a.dagger,
+                    daggerSteps=a.dagger_steps, lr=a.lr, initSeed=a.init_seed, logEvery=250, verbose=1)
     t0 = time.time()
     hist = el.learned_train(**opts)
     train_s = time.time() - t0
@@ -72,11 +88,16 @@ def main(argv=None):
         sha = hashlib.sha256(f.read()).hexdigest()
     manifest = {
         "weights": os.path.basename(a.out), "sha256": sha, "parameters": int(info["parameters"]),
-        "network": {"inputs": int(info["inputs"]), "hidden": [int(info["hidden"])] * 2, "outputs": 512,
-                    "activation": "relu", "weights": "bf16 (fp32 masters)", "accumulate": "fp32"},
+        "network": ({"architecture": "linear", "inputs": int(info["inputs"]), "outputs": 512, "bias": False,
+                     "step": "fixed (x += c)", "weights": "bf16 (fp32 masters)", "accumulate": "fp32"}
+                    if a.arch == "LINEAR" else
+                    {"architecture": "mlp", "inputs": int(info["inputs"]), "hidden": [int(info["hidden"])] * 2,
+                     "outputs": 512, "activation": "relu", "step": "energy-optimal",
+                     "weights": "bf16 (fp32 masters)", "accumulate": "fp32"}),
         "workload": {k: s[k] for k in ("BOX NX", "POLYNOMIAL DEGREE", "BOX CURVED MAP AMPLITUDE", "LAMBDA",
                                        "DATA FILE", "MULTIGRID DEGREES", "MULTIGRID PRECISION")},
-        "mode": a.mode, "training": opts, "loss_per_250_steps": hist,
+        "mode": a.mode, "training": opts,
+        ("validation_loss" if a.arch == "LINEAR" else "loss_per_250_steps"): hist,
         "setup_seconds": setup, "training_seconds": train_s,
     }
     with open(os.path.splitext(a.out)[0] + ".json", "w") as f:

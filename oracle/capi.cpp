@@ -331,6 +331,14 @@ void orc_learned_forward(void* p, int H, const double* theta, const double* r, c
   LearnedForward(h->el, t, H, theta, r, invD, bf != 0, X, scale, Y, c);
 }
 
+void orc_learned_linear_gram(void* p, const double* r, const double* eh, const double* invD, int bf, double* G,
+                             double* B) {
+  Handle* h = (Handle*)p;
+  std::vector<int> t;
+  LearnedTraceIds(h->el, t);
+  LearnedLinearGram(h->el, t, r, eh, invD, bf != 0, G, B);
+}
+
 double orc_bf16r(double x) { return bf16r(x); }
 
 }  // extern "C"

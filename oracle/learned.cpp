@@ -12,6 +12,8 @@
  *
  * Parameter vector theta (FP32), for nIn = Np + 6 Nfp, H hidden, nOut = Np:
  *   W1 [H][nIn], b1 [H], W2 [H][H], b2 [H], W3 [nOut][H], b3 [nOut], omega.
+ * H = 0 is the linear architecture: W [nOut][nIn] (no bias), omega; its
+ * least-squares fit is restated by LearnedLinearGram.
  */
 #include <cmath>
 #include <cstring>
@@ -30,6 +32,12 @@ LearnedDims learnedDims(int Np, int Nfp, int H) {
   d.nIn = Np + 6 * Nfp;
   d.H = H;
   d.nOut = Np;
+  if (H == 0) {
+    d.oW1 = d.ob1 = d.oW2 = d.ob2 = d.oW3 = d.ob3 = 0;
+    d.oOm = (size_t)d.nOut * d.nIn;
+    d.n = d.oOm + 1;
+    return d;
+  }
   d.oW1 = 0;
   d.ob1 = d.oW1 + (size_t)H * d.nIn;
   d.oW2 = d.ob1 + H;
@@ -102,30 +110,46 @@ static void forward(const Elliptic& el, const std::vector<int>& trace, const Lea
     for (int f = 0; f < nIn; ++f) F.X[(size_t)e * nIn + f] = R(z[f] * inv);
   }
   auto W = [&](size_t o) { return R(th[o]); };
-  F.H1.assign((size_t)E * H, 0.0);
-  F.H2.assign((size_t)E * H, 0.0);
-  F.Y.assign((size_t)E * nOut, 0.0);
-  std::vector<double> W1((size_t)H * nIn), W2((size_t)H * H), W3((size_t)nOut * H);
-  for (size_t i = 0; i < W1.size(); ++i) W1[i] = W(d.oW1 + i);
-  for (size_t i = 0; i < W2.size(); ++i) W2[i] = W(d.oW2 + i);
-  for (size_t i = 0; i < W3.size(); ++i) W3[i] = W(d.oW3 + i);
+  if (H == 0) {  // linear: y_e = W x_e
+    F.Y.assign((size_t)E * nOut, 0.0);
+    std::vector<double> Wl((size_t)nOut * nIn);
+    for (size_t i = 0; i < Wl.size(); ++i) Wl[i] = W(d.oW3 + i);
 #pragma omp parallel for schedule(static)
-  for (dlong e = 0; e < E; ++e) {
-    const double* x = &F.X[(size_t)e * nIn];
-    for (int h = 0; h < H; ++h) {
-      double a = th[d.ob1 + h];
-      for (int f = 0; f < nIn; ++f) a += W1[(size_t)h * nIn + f] * x[f];
-      F.H1[(size_t)e * H + h] = R(a > 0 ? a : 0.0);
+    for (dlong e = 0; e < E; ++e) {
+      const double* x = &F.X[(size_t)e * nIn];
+      for (int o = 0; o < nOut; ++o) {
+        double a = 0;
+        for (int f = 0; f < nIn; ++f) a += Wl[(size_t)o * nIn + f] * x[f];
+        F.Y[(size_t)e * nOut + o] = bf ? (double)(float)a : a;
+      }
     }
-    for (int h = 0; h < H; ++h) {
-      double a = th[d.ob2 + h];
-      for (int k = 0; k < H; ++k) a += W2[(size_t)h * H + k] * F.H1[(size_t)e * H + k];
-      F.H2[(size_t)e * H + h] = R(a > 0 ? a : 0.0);
-    }
-    for (int o = 0; o < nOut; ++o) {
-      double a = th[d.ob3 + o];
-      for (int k = 0; k < H; ++k) a += W3[(size_t)o * H + k] * F.H2[(size_t)e * H + k];
-      F.Y[(size_t)e * nOut + o] = bf ? (double)(float)a : a;
+  }
+  if (H > 0) {
+    F.H1.assign((size_t)E * H, 0.0);
+    F.H2.assign((size_t)E * H, 0.0);
+    F.Y.assign((size_t)E * nOut, 0.0);
+    std::vector<double> W1((size_t)H * nIn), W2((size_t)H * H), W3((size_t)nOut * H);
+    for (size_t i = 0; i < W1.size(); ++i) W1[i] = W(d.oW1 + i);
+    for (size_t i = 0; i < W2.size(); ++i) W2[i] = W(d.oW2 + i);
+    for (size_t i = 0; i < W3.size(); ++i) W3[i] = W(d.oW3 + i);
+#pragma omp parallel for schedule(static)
+    for (dlong e = 0; e < E; ++e) {
+      const double* x = &F.X[(size_t)e * nIn];
+      for (int h = 0; h < H; ++h) {
+        double a = th[d.ob1 + h];
+        for (int f = 0; f < nIn; ++f) a += W1[(size_t)h * nIn + f] * x[f];
+        F.H1[(size_t)e * H + h] = R(a > 0 ? a : 0.0);
+      }
+      for (int h = 0; h < H; ++h) {
+        double a = th[d.ob2 + h];
+        for (int k = 0; k < H; ++k) a += W2[(size_t)h * H + k] * F.H1[(size_t)e * H + k];
+        F.H2[(size_t)e * H + h] = R(a > 0 ? a : 0.0);
+      }
+      for (int o = 0; o < nOut; ++o) {
+        double a = th[d.ob3 + o];
+        for (int k = 0; k < H; ++k) a += W3[(size_t)o * H + k] * F.H2[(size_t)e * H + k];
+        F.Y[(size_t)e * nOut + o] = bf ? (double)(float)a : a;
+      }
     }
   }
   // assembly over the rows of the masked gather
@@ -232,6 +256,47 @@ double LearnedLossGrad(const Elliptic& el, const std::vector<int>& trace, int H,
     for (int f = 0; f < nIn; ++f) grad[d.oW1 + (size_t)h * nIn + f] = gw1[f];
   }
   return J;
+}
+
+/* Gram matrices of the linear smoother's least-squares fit for one sample:
+ * X = network input of r (BF16-rounded when bf), T[e][n] = eh[g] / s_e at the
+ * element's nodes (0 at masked nodes, BF16-rounded when bf);
+ * G [nIn][nIn] = X^T X, B [Np][nIn] = T^T X (sums over elements). */
+void LearnedLinearGram(const Elliptic& el, const std::vector<int>& trace, const double* r, const double* eh,
+                       const double* invD, bool bf, double* Gout, double* Bout) {
+  const Mesh& m = el.mesh;
+  const dlong E = m.Nelements;
+  const int Np = m.Np;
+  const LearnedDims d = learnedDims(Np, m.Nfp, 0);
+  std::vector<double> th(d.n, 0.0);
+  Fwd F;
+  forward(el, trace, d, th.data(), r, invD, bf, F);
+  const int nIn = d.nIn;
+  std::vector<double> T((size_t)E * Np, 0.0);
+  for (dlong e = 0; e < E; ++e)
+    for (int n = 0; n < Np; ++n) {
+      const long g = el.GlobalToLocal[(size_t)e * Np + n];
+      const double s = F.scale[e];
+      if (g < 0 || !(s > 0)) continue;
+      // the GPU divides in FP32 (s is already FP32 there) and stores BF16
+      T[(size_t)e * Np + n] = bf ? bf16r((double)((float)eh[g] / (float)s)) : eh[g] / s;
+    }
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < nIn; ++j) {
+    for (int k = 0; k < nIn; ++k) {
+      double a = 0;
+      for (dlong e = 0; e < E; ++e) a += F.X[(size_t)e * nIn + j] * F.X[(size_t)e * nIn + k];
+      Gout[(size_t)j * nIn + k] = a;
+    }
+  }
+#pragma omp parallel for schedule(static)
+  for (int o = 0; o < Np; ++o) {
+    for (int k = 0; k < nIn; ++k) {
+      double a = 0;
+      for (dlong e = 0; e < E; ++e) a += T[(size_t)e * Np + o] * F.X[(size_t)e * nIn + k];
+      Bout[(size_t)o * nIn + k] = a;
+    }
+  }
 }
 
 /* Network input and correction only (no loss): X [E][nIn], scale [E], Y [E][Np], c [Ndofs]. */

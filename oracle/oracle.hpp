@@ -228,6 +228,8 @@ RunResult elliptic_run(const Settings& s, int verbose = 0);
 /* learned smoother oracle (learned.cpp); no reference counterpart */
 double bf16r(double x);
 void LearnedTraceIds(const Elliptic& el, std::vector<int>& trace);
+void LearnedLinearGram(const Elliptic& el, const std::vector<int>& trace, const double* r, const double* eh,
+                       const double* invD, bool bf, double* G, double* B);
 double LearnedLossGrad(const Elliptic& el, const std::vector<int>& trace, int H, const double* th, const double* r,
                        const double* eh, double nrm, const double* invD, bool bf, double* grad, double* cOut);
 void LearnedForward(const Elliptic& el, const std::vector<int>& trace, int H, const double* th, const double* r,

@@ -61,6 +61,7 @@ def load():
                                                  c_int, c_void_p, c_void_p]),
             "orc_learned_forward": (None, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
                                            c_void_p, c_void_p]),
+            "orc_learned_linear_gram": (None, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
             "orc_bf16r": (c_double, [c_double]),
         }
         for n, (r, a) in sig.items():
@@ -159,8 +160,14 @@ class OracleElliptic:
 
 
 def learned_dims(Np, Nfp, H):
-    """Offsets of the learned smoother's parameter vector (W1, b1, W2, b2, W3, b3, omega)."""
+    """Offsets of the learned smoother's parameter vector (W1, b1, W2, b2, W3, b3, omega);
+    H = 0 is the linear architecture (W3 [Np][nIn], omega)."""
     nIn, nOut = Np + 6 * Nfp, Np
+    if H == 0:
+        o = {"W1": 0, "b1": 0, "W2": 0, "b2": 0, "W3": 0, "b3": 0, "omega": nOut * nIn}
+        o["n"] = o["omega"] + 1
+        o["nIn"], o["nOut"], o["H"] = nIn, nOut, H
+        return o
     o = {"W1": 0}
     o["b1"] = o["W1"] + H * nIn
     o["W2"] = o["b1"] + H
@@ -197,6 +204,17 @@ class OracleLearned:
         self.L.orc_learned_forward(self.el.h, self.H, th.ctypes.data, r.ctypes.data, invD.ctypes.data, int(bf16),
                                    X.ctypes.data, s.ctypes.data, Y.ctypes.data, c.ctypes.data)
         return X.reshape(self.E, -1), s, Y.reshape(self.E, -1), c
+
+    def linear_gram(self, r, eh, invD, bf16=True):
+        """G = X^T X [nIn][nIn] and B = T^T X [Np][nIn] of one least-squares sample."""
+        d = self.dims
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        eh = np.ascontiguousarray(eh, dtype=np.float64)
+        invD = np.ascontiguousarray(invD, dtype=np.float64)
+        G = np.empty((d["nIn"], d["nIn"])); B = np.empty((d["nOut"], d["nIn"]))
+        self.L.orc_learned_linear_gram(self.el.h, r.ctypes.data, eh.ctypes.data, invD.ctypes.data, int(bf16),
+                                       G.ctypes.data, B.ctypes.data)
+        return G, B
 
     def loss_grad(self, theta, r, eh, nrm, invD, bf16=False, grad=True):
         th = np.ascontiguousarray(theta, dtype=np.float64)

@@ -1,0 +1,109 @@
+"""GPU tests of the linear learned smoother (MULTIGRID LEARNED ARCH LINEAR): its
+least-squares Gram matrices and its correction against the CPU oracle, the fit
+on the GPU, use inside FPCG + p-MG against Chebyshev, linearity of the
+smoother, and the weight file's architecture check."""
+import numpy as np
+import pytest
+
+from tests.oracle_lib import OracleElliptic, OracleLearned, learned_dims
+from tests.test_gpu_learned import box, dev, learned_settings, lp, relnorm, solve_settings
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+LIN = {"MULTIGRID LEARNED ARCH": "LINEAR"}
+
+
+@pytest.fixture(scope="module")
+def pipeline():
+    p = lp().Elliptic(learned_settings(4, 3, "3,1", **LIN))
+    o = OracleElliptic(box(4, 3))
+    ol = OracleLearned(o, 0)
+    invD = p.host_array("learnedInvDiag")
+    return p, o, ol, invD
+
+
+def test_linear_layout(pipeline):
+    p, o, ol, invD = pipeline
+    info = p.learned_info()
+    assert info["arch"] == "LINEAR" and info["hidden"] == 0
+    assert int(info["parameters"]) == ol.dims["n"] == 64 * (64 + 6 * 16) + 1
+
+
+def test_linear_gram_vs_oracle(pipeline):
+    p, o, ol, invD = pipeline
+    rng = np.random.default_rng(31)
+    r = rng.standard_normal(ol.Ng).astype(np.float32)
+    eh = rng.standard_normal(ol.Ng).astype(np.float32)
+    G, B = p.learned_linear_gram(dev(r), dev(eh))
+    Gr, Br = ol.linear_gram(r, eh, invD, bf16=True)
+    print("gram errors", relnorm(G, Gr), relnorm(B, Br), flush=True)
+    assert relnorm(G, Gr) < 1e-5
+    assert relnorm(B, Br) < 1e-5
+    assert np.allclose(G, G.T, rtol=0, atol=1e-5 * np.abs(G).max())
+
+
+def test_linear_correction_vs_oracle_and_linearity(pipeline):
+    p, o, ol, invD = pipeline
+    d = ol.dims
+    rng = np.random.default_rng(32)
+    th = np.zeros(d["n"], dtype=np.float32)
+    th[:d["omega"]] = rng.uniform(-1, 1, d["omega"]) / np.sqrt(d["nIn"])
+    th[d["omega"]] = 0.3
+    p.learned_set_params(th, activate=False)
+    r1 = rng.standard_normal(ol.Ng).astype(np.float32)
+    r2 = rng.standard_normal(ol.Ng).astype(np.float32)
+
+    def corr(r):
+        rd = dev(r)
+        c = torch.empty_like(rd)
+        p.learned_correction(rd, c)
+        return c.cpu().numpy()
+
+    c1 = corr(r1)
+    _, _, _, cref = ol.forward(th, r1, invD, bf16=True)
+    print("linear correction error", relnorm(c1, cref), flush=True)
+    assert relnorm(c1, cref) < 1e-4
+    # positively homogeneous bit for bit, additive up to the BF16 rounding of the input
+    assert np.array_equal(corr(2 * r1), 2 * c1)
+    c12 = corr(r1 + r2)
+    print("additivity defect", relnorm(c12, c1 + corr(r2)), flush=True)
+    assert relnorm(c12, c1 + corr(r2)) < 2e-2
+
+
+@pytest.fixture(scope="module")
+def fitted(tmp_path_factory):
+    p = lp().Elliptic(solve_settings(**LIN))
+    hist = p.learned_train(samples=48, ridge=1e-4, verbose=0)
+    path = tmp_path_factory.mktemp("w") / "linear.bin"
+    p.learned_save(path)
+    return p, hist, path
+
+
+def test_linear_fit_and_pmg_solve(fitted):
+    p, hist, path = fitted
+    print("validation loss", hist, flush=True)
+    # |eh - c|^2 / |eh|^2 on a fresh white-noise sample: below 1 means the fit
+    # predicts part of the high-frequency error (zero correction scores 1)
+    assert len(hist) == 1 and 0 < hist[0] < 0.9
+    r = p.run()
+    info = p.learned_info()
+    print("linear learned", r, info, p.learned_status(), flush=True)
+    assert info["active"] == 1 and info["fallback"] == 0
+    assert r["r_final"] <= 1e-8 * r["r0"] * (1 + 1e-12)
+    c = lp().Elliptic(solve_settings(**{"MULTIGRID SMOOTHER": "CHEBYSHEV"})).run()
+    print("chebyshev", c, flush=True)
+    assert r["iterations"] <= c["iterations"] + 1
+    assert abs(r["norm"] - c["norm"]) < 1e-7 * c["norm"]
+
+
+def test_linear_weight_file_and_architecture_check(fitted):
+    p, hist, path = fitted
+    q = lp().Elliptic(solve_settings(**LIN))
+    assert q.learned_load(path), q.learned_status()
+    assert np.array_equal(q.learned_params(), p.learned_params())
+    m = lp().Elliptic(solve_settings())  # MLP handle
+    assert not m.learned_load(path)
+    assert "architecture" in m.learned_status()
+    assert m.learned_info()["active"] == 0

@@ -145,3 +145,47 @@ def test_bf16_rounding_known_values():
     assert L.orc_bf16r(1.0 + 3 * 2.0 ** -8) == 1.0 + 2.0 ** -6  # tie rounds up to even
     assert L.orc_bf16r(1.0 + 2.0 ** -7) == 1.0 + 2.0 ** -7
     assert L.orc_bf16r(-3.14159) == -3.140625
+
+
+def test_linear_identity_layer_is_jacobi():
+    """Linear architecture (H = 0) with W = [I | 0] (the element's own inputs) and
+    omega: each copy returns s_e z/s_e = z, so c = (1 + omega) D^{-1} r."""
+    o = OracleElliptic(box(3, 3, flag=1, curve=0.05))
+    ol = OracleLearned(o, 0)
+    d = ol.dims
+    th = np.zeros(d["n"])
+    W = th[:d["omega"]].reshape(d["nOut"], d["nIn"])
+    W[:, :d["nOut"]] = np.eye(d["nOut"])
+    th[d["omega"]] = 0.25
+    invD = 1.0 / o.diagonal()
+    r = np.random.default_rng(3).standard_normal(ol.Ng)
+    _, _, _, c = ol.forward(th, r, invD)
+    assert np.allclose(c, 1.25 * invD * r, rtol=1e-13, atol=0)
+
+
+def test_linear_gram_recovers_the_identity_map():
+    """Least squares with targets eh = D^{-1} r: the element target eh_e / s_e is
+    exactly the element's own block of its input, so the normal equations
+    G W^T = B^T built from the oracle's Gram matrices return W = [I | 0]. A
+    transposed or mis-indexed Gram matrix fails this."""
+    o = OracleElliptic(box(3, 3, flag=1, curve=0.05))
+    ol = OracleLearned(o, 0)
+    d = ol.dims
+    invD = 1.0 / o.diagonal()
+    rng = np.random.default_rng(4)
+    G = np.zeros((d["nIn"], d["nIn"]))
+    B = np.zeros((d["nOut"], d["nIn"]))
+    for _ in range(12):
+        r = rng.standard_normal(ol.Ng)
+        g, b = ol.linear_gram(r, invD * r, invD, bf16=False)
+        G += g
+        B += b
+    assert np.allclose(G, G.T, rtol=0, atol=1e-12 * np.abs(G).max())
+    W = np.linalg.solve(G + 1e-12 * np.trace(G) / d["nIn"] * np.eye(d["nIn"]), B.T).T
+    ref = np.zeros_like(W)
+    ref[:, :d["nOut"]] = np.eye(d["nOut"])
+    # masked nodes contribute zero rows and columns; compare on the live inputs
+    live = np.diag(G) > 0
+    err = np.abs(W[:, live] - ref[:, live]).max()
+    print("identity recovery error", err)
+    assert err < 1e-6

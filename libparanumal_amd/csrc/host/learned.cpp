@@ -171,6 +171,7 @@ void learnedSmoother_t::setParameters(const std::vector<float>& th) {
   LIBP_ABORT("learned smoother: parameter vector has " << th.size() << " entries, expected " << nParams,
              th.size() != nParams);
   o_theta.copyFrom(th.data(), nParams, 0, platform.stream);
+  omegaHost = th[oOm];
   LIBP_CALL(libp_f32_to_bf16(nParams, o_theta.ptr(), o_thetaBf.ptr(), platform.s()));
   LIBP_HIP(hipMemsetAsync(o_m.ptr(), 0, nParams * sizeof(float), platform.stream));
   LIBP_HIP(hipMemsetAsync(o_v.ptr(), 0, nParams * sizeof(float), platform.stream));
@@ -262,8 +263,10 @@ void learnedSmoother_t::correction(const float* r, const float* invDiag) {
     LIBP_CALL(libp_mlp_linear_fwd(E, H, H, o_H1.ptr(), H, Wb + oW2, H, th + ob2, 1, o_H2.ptr(), H, s));
     LIBP_CALL(libp_mlp_linear_fwd(E, H, nOut, o_H2.ptr(), H, Wb + oW3, H, th + ob3, 0, o_Y.ptr(), nOut, s));
   }
+  // the linear smoother's omega only changes through setParameters: skip the term when it is zero
+  const float* om = (arch == ARCH_LINEAR && omegaHost == 0.f) ? nullptr : th + oOm;
   LIBP_CALL(libp_learned_assemble_f32(Ndofs, elliptic.ogsMasked.o_rowStartsT.ptr(), elliptic.ogsMasked.o_colIdsT.ptr(),
-                                      Np, o_Y.ptr(), nOut, o_scale.ptr(), th + oOm, r, invDiag, o_c.ptr(), s));
+                                      Np, o_Y.ptr(), nOut, o_scale.ptr(), om, r, invDiag, o_c.ptr(), s));
 }
 
 void learnedSmoother_t::smoothPre(deviceMemory<float>& o_r, deviceMemory<float>& o_x, deviceMemory<float>* o_resOut,

@@ -65,6 +65,7 @@ class learnedSmoother_t {
   deviceMemory<double> o_sc;  // [0..3] step statistics, [4] last loss, [5] running loss sum
   deviceMemory<char> o_ws, o_rws;
   int adamStep = 0;
+  float omegaHost = 0.f;  // omega as last set from the host (the linear fit never changes it)
   double monitorRejected = 0;
   std::vector<double> monitorHist;
 

@@ -59,7 +59,10 @@ __global__ void __launch_bounds__(256) learned_features_kernel(const int E, cons
   if (l == 0) scale[e] = s;
 }
 
-// c[g] = invLen * sum_j s_e Y[j] + omega * invD[g] r[g]; omega read from device memory
+// c[g] = invLen * sum_j s_e Y[j] + omega * invD[g] r[g]; omega read from device
+// memory, the term skipped when omega is null. SHIFT = log2(Np) when Np is a
+// power of two (the element of a local id is then a shift), else -1.
+template <int SHIFT>
 __global__ void __launch_bounds__(256) learned_assemble_kernel(const int Nrows, const int* __restrict__ rowStarts,
                                                                const int* __restrict__ colIds, const int Np,
                                                                const float* __restrict__ Y, const long ldy,
@@ -68,6 +71,7 @@ __global__ void __launch_bounds__(256) learned_assemble_kernel(const int Nrows, 
                                                                const float* __restrict__ r,
                                                                const float* __restrict__ invD, float* __restrict__ c) {
   const float om = omega ? *omega : 0.f;
+  auto elem = [&](int id) { return SHIFT >= 0 ? (id >> SHIFT) : id / Np; };
   for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < Nrows; g += gridDim.x * blockDim.x) {
     const int a = rowStarts[g], b = rowStarts[g + 1];
     const int len = b - a;
@@ -78,7 +82,7 @@ __global__ void __launch_bounds__(256) learned_assemble_kernel(const int Nrows, 
     for (int k = 0; k < 8; ++k) col[k] = (k < len) ? colIds[a + k] : 0;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      const int e = col[k] / Np, n = col[k] - e * Np;
+      const int e = elem(col[k]), n = col[k] - e * Np;
       y[k] = (k < len) ? Y[(long)e * ldy + n] : 0.f;
       sc[k] = (k < len) ? scale[e] : 0.f;
     }
@@ -87,10 +91,12 @@ __global__ void __launch_bounds__(256) learned_assemble_kernel(const int Nrows, 
       if (k < len) acc += sc[k] * y[k];
     for (int j = a + 8; j < b; ++j) {
       const int id = colIds[j];
-      const int e = id / Np, n = id - e * Np;
+      const int e = elem(id), n = id - e * Np;
       acc += scale[e] * Y[(long)e * ldy + n];
     }
-    c[g] = (b > a ? acc / (float)(b - a) : 0.f) + om * invD[g] * r[g];
+    float v = (b > a ? acc / (float)(b - a) : 0.f);
+    if (omega) v += om * invD[g] * r[g];
+    c[g] = v;
   }
 }
 
@@ -269,8 +275,17 @@ extern "C" int libp_learned_assemble_f32(int Nrows, const int* rowStarts, const 
   if (Nrows <= 0) return LIBP_OK;
   if (!rowStarts || !colIds || !Y || !scale || !r || !invDiag || !c) return arg_error("learned_assemble", "null pointer");
   if (Np <= 0 || ldy < Np) return arg_error("learned_assemble", "bad size");
-  hipLaunchKernelGGL(learned_assemble_kernel, dim3(stream_blocks(Nrows, 256)), dim3(256), 0, as_stream(stream), Nrows,
-                     rowStarts, colIds, Np, Y, ldy, scale, omega, r, invDiag, c);
+  const dim3 grid(stream_blocks(Nrows, 256)), block(256);
+  hipStream_t st = as_stream(stream);
+  if (Np == 512)
+    hipLaunchKernelGGL(learned_assemble_kernel<9>, grid, block, 0, st, Nrows, rowStarts, colIds, Np, Y, ldy, scale,
+                       omega, r, invDiag, c);
+  else if (Np == 64)
+    hipLaunchKernelGGL(learned_assemble_kernel<6>, grid, block, 0, st, Nrows, rowStarts, colIds, Np, Y, ldy, scale,
+                       omega, r, invDiag, c);
+  else
+    hipLaunchKernelGGL(learned_assemble_kernel<-1>, grid, block, 0, st, Nrows, rowStarts, colIds, Np, Y, ldy, scale,
+                       omega, r, invDiag, c);
   return launch_status("learned_assemble");
 }
 

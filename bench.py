@@ -93,14 +93,18 @@ def ax_rate(lp, settings, steps=20, warmup=5):
 
 
 def cpu_baseline(reps=20):
-    """Oracle partial Ax (serial, 1 thread) on a 16^3 x N=7 sample of the workload."""
+    """Oracle partial Ax on a 16^3 x N=7 sample of the workload: OpenMP over
+    elements on every thread OpenMP is given (the reference's OpenMP mode runs
+    its @outer element loop that way), and serial on 1 thread (its Serial mode)."""
     from tests import oracle_lib
     s = workload_settings(nx=16)
     t0 = time.time()
     o = oracle_lib.OracleElliptic(s)
     setup = time.time() - t0
     L = oracle_lib.load()
-    t = L.orc_time_partial_ax(o.h, reps)
+    t = L.orc_time_partial_ax(o.h, reps, 0)
+    t_omp = L.orc_time_partial_ax(o.h, reps, 1)
+    threads = int(L.orc_omp_threads())
     ng = o.geti("Ngather")
     cpu = "unknown"
     try:
@@ -111,9 +115,10 @@ def cpu_baseline(reps=20):
                     break
     except OSError:
         pass
-    return {"value": ng / t / 1e9, "unit": "GDOF/s", "cores": 1, "kind": "port",
+    return {"value": ng / t_omp / 1e9, "unit": "GDOF/s", "cores": threads, "kind": "port",
             "sample": f"oracle partial Ax, 16^3 elements N=7 ({ng} gathered dofs), median of {reps} applications, "
-                      f"serial (1 thread) on {cpu}; setup {setup:.1f}s"}
+                      f"OpenMP over elements on {threads} threads of {cpu}; setup {setup:.1f}s",
+            "serial": {"value": ng / t / 1e9, "unit": "GDOF/s", "cores": 1}}
 
 
 LEARNED_WEIGHTS_LINEAR = os.path.join(ROOT, "libparanumal_amd", "weights", "learned_hex_n7_linear.bin")

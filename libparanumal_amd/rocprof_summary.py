@@ -3,6 +3,7 @@ benchmarks.
 
     python -m libparanumal_amd.rocprof_summary stats  <dir> [--top 12] [--cmd "..."]
     python -m libparanumal_amd.rocprof_summary kernel <dir> --kernel <substring> [--grid G] [--cmd "..."]
+    python -m libparanumal_amd.rocprof_summary kernels <dir> --match <sub1,sub2,...> [--cmd "..."]
     python -m libparanumal_amd.rocprof_summary traffic --fetch <dir> --write <dir>
            --kernel <substring> --copy-bytes <B> [--cmd-fetch ...] [--cmd-write ...] [--json out.json]
 
@@ -79,6 +80,26 @@ def kernel_text(d, kernel, grid=None, cmd=None):
     return "\n".join(out)
 
 
+def kernels_text(d, subs, cmd=None):
+    """Dispatch count, median and min duration per (kernel, grid) for every
+    kernel whose name contains one of the substrings."""
+    groups = {}
+    for r in csv.DictReader(open(_one(os.path.join(d, "*kernel_trace.csv")))):
+        name = r["Kernel_Name"]
+        if not any(s in name for s in subs):
+            continue
+        key = (name.split("(")[0].replace("void ", "").replace("libp_amd::", ""), int(r["Grid_Size_X"]))
+        groups.setdefault(key, []).append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    out = []
+    if cmd:
+        out.append(f"command: {cmd}")
+    out.append(f"{'dispatches':>10} {'median us':>10} {'min us':>8}  kernel (grid size x)")
+    for (name, grid), v in sorted(groups.items()):
+        v.sort()
+        out.append(f"{len(v):10d} {v[len(v) // 2] / 1e3:10.1f} {v[0] / 1e3:8.1f}  {name} ({grid})")
+    return "\n".join(out)
+
+
 def counters(d, counter):
     """{kernel name: [value per dispatch]} for one counter."""
     f = _one(os.path.join(d, "*counter_collection.csv"))
@@ -134,6 +155,10 @@ def main(argv=None):
     k.add_argument("--kernel", required=True)
     k.add_argument("--grid", type=int)
     k.add_argument("--cmd")
+    ks = sub.add_parser("kernels")
+    ks.add_argument("dir")
+    ks.add_argument("--match", required=True)
+    ks.add_argument("--cmd")
     t = sub.add_parser("traffic")
     t.add_argument("--fetch", required=True)
     t.add_argument("--write", required=True)
@@ -148,6 +173,9 @@ def main(argv=None):
         return 0
     if a.mode == "kernel":
         print(kernel_text(a.dir, a.kernel, a.grid, a.cmd))
+        return 0
+    if a.mode == "kernels":
+        print(kernels_text(a.dir, a.match.split(","), a.cmd))
         return 0
     r = traffic(a.fetch, a.write, a.kernel, a.copy_bytes)
     r["commands"] = {"fetch": a.cmd_fetch, "write": a.cmd_write}

@@ -9,6 +9,8 @@
 #include <stdexcept>
 #include <string>
 
+#include <omp.h>
+
 #include "oracle.hpp"
 
 namespace oracle {
@@ -125,6 +127,18 @@ long orc_array(void* p, const char* name, void* out, int* dtype) {
 }
 
 void orc_partial_ax(void* p, const double* q, double* AqL) { ((Handle*)p)->el.PartialAx(q, AqL); }
+
+void orc_partial_ax_omp(void* p, const double* q, double* AqL) { ((Handle*)p)->el.PartialAx(q, AqL, true); }
+
+int orc_omp_threads() {
+  int n = 1;
+#pragma omp parallel
+  {
+#pragma omp single
+    n = omp_get_num_threads();
+  }
+  return n;
+}
 
 void orc_operator(void* p, const double* q, double* Aq) {
   Handle* h = (Handle*)p;
@@ -290,7 +304,7 @@ int orc_hess_eigs(int n, const double* Hcm, double* re, double* im) {
 
 // CPU baseline: time `reps` applications of the partial Ax on the handle's
 // mesh. Returns the median seconds per application.
-double orc_time_partial_ax(void* p, int reps) {
+double orc_time_partial_ax(void* p, int reps, int parallel) {
   Handle* h = (Handle*)p;
   const size_t NL = (size_t)h->mesh.Nelements * h->mesh.Np;
   std::vector<double> q(h->el.Ndofs), AqL(NL);
@@ -298,7 +312,7 @@ double orc_time_partial_ax(void* p, int reps) {
   std::vector<double> t;
   for (int r = 0; r < reps; ++r) {
     auto a = std::chrono::steady_clock::now();
-    h->el.PartialAx(q.data(), AqL.data());
+    h->el.PartialAx(q.data(), AqL.data(), parallel != 0);
     auto b = std::chrono::steady_clock::now();
     t.push_back(std::chrono::duration<double>(b - a).count());
   }

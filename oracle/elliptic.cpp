@@ -106,13 +106,18 @@ void Elliptic::BoundarySetup(GlibcRand& rng) {
 /* ellipticPartialAxHex3D_v0 (okl/ellipticAxHex3D.okl:156-295) in OCCA Serial
  * form: each @inner block runs to completion before the next. Covers every
  * element (one rank: localGatherElementList is 0..E-1, split in two halves by
- * ellipticOperator.cpp:42-104; the split does not change any value). */
-void Elliptic::PartialAx(const double* q, double* Aq) const {
+ * ellipticOperator.cpp:42-104; the split does not change any value).
+ * parallel: elements are shared among OpenMP threads, as the reference's
+ * OpenMP mode runs the @outer loop; every element's values are the same. */
+void Elliptic::PartialAx(const double* q, double* Aq, bool parallel) const {
   const int Nq = mesh.Nq, Np = mesh.Np, NqNq = Nq * Nq;
   const double* DT = mesh.D.data();
+#pragma omp parallel if (parallel)
+  {
   std::vector<double> s_q(NqNq), s_Gqr(NqNq), s_Gqs(NqNq);
   std::vector<double> r_q(NqNq * Nq), r_Aq(NqNq * Nq);
   std::vector<double> r_qt(NqNq), r_Gqt(NqNq), r_Auk(NqNq), G(7 * NqNq);
+#pragma omp for schedule(static)
   for (dlong e = 0; e < mesh.Nelements; ++e) {
     const dlong element = mesh.localGatherElementList[e];
     for (int j = 0; j < Nq; ++j)
@@ -171,6 +176,7 @@ void Elliptic::PartialAx(const double* q, double* Aq) const {
       for (int i = 0; i < Nq; ++i)
         for (int k = 0; k < Nq; k++)
           Aq[(size_t)element * Np + k * NqNq + j * Nq + i] = r_Aq[(j * Nq + i) * Nq + k];
+  }
   }
 }
 

@@ -172,7 +172,7 @@ struct Elliptic {
   void Setup(const Mesh& mesh, const Settings& s, double lambda, GlibcRand& rng);
   void BoundarySetup(GlibcRand& rng);            // ellipticBoundarySetup.cpp:30-140
   // partial Ax over element list (ellipticAxHex3D.okl:156-295)
-  void PartialAx(const double* q, double* AqL) const;
+  void PartialAx(const double* q, double* AqL, bool parallel = false) const;
   void Operator(const double* q, double* Aq, std::vector<double>& scratchL) const; // ellipticOperator.cpp:29-107
   void BuildOperatorDiagonal(std::vector<double>& diagA) const; // ellipticBuildOperatorDiagonal.cpp:29-80,998-1057
   void ZeroMean(double* q) const;                // ellipticZeroMean.cpp

@@ -55,7 +55,9 @@ def load():
             "orc_rand_seq": (None, [c_int, c_void_p]),
             "orc_drand48_seq": (None, [c_long, c_int, c_void_p]),
             "orc_hess_eigs": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
-            "orc_time_partial_ax": (c_double, [c_void_p, c_int]),
+            "orc_time_partial_ax": (c_double, [c_void_p, c_int, c_int]),
+            "orc_partial_ax_omp": (None, [c_void_p, c_void_p, c_void_p]),
+            "orc_omp_threads": (c_int, []),
             "orc_learned_trace": (c_long, [c_void_p, c_void_p]),
             "orc_learned_loss_grad": (c_double, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p,
                                                  c_int, c_void_p, c_void_p]),
@@ -106,6 +108,13 @@ class OracleElliptic:
         out = np.empty(NL)
         q = np.ascontiguousarray(q, dtype=np.float64)
         self.L.orc_partial_ax(self.h, q.ctypes.data, out.ctypes.data)
+        return out
+
+    def partial_ax_omp(self, q):
+        NL = self.geti("Nelements") * self.geti("Np")
+        out = np.empty(NL)
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        self.L.orc_partial_ax_omp(self.h, q.ctypes.data, out.ctypes.data)
         return out
 
     def operator(self, q):

@@ -311,3 +311,15 @@ def test_nbfpcg_with_multigrid_matches_fpcg():
     n1, it1, _ = oracle_lib.run(dict(base, **{"LINEAR SOLVER": "NBFPCG"}))
     assert abs(it1 - it0) <= 1
     assert abs(n1 - n0) < 1e-7 * n0
+
+
+def test_openmp_partial_ax_matches_serial():
+    """The OpenMP element loop (the CPU baseline's parallel mode) gives the serial
+    values bit for bit: elements are independent."""
+    from tests.oracle_lib import OracleElliptic
+    s = {"MESH FILE": "BOX", "MESH DIMENSION": 3, "ELEMENT TYPE": 12, "BOX NX": 3, "BOX NY": 4, "BOX NZ": 2,
+         "BOX DIMX": 2, "BOX DIMY": 2, "BOX DIMZ": 2, "BOX BOUNDARY FLAG": 1, "POLYNOMIAL DEGREE": 5,
+         "LAMBDA": 0.5, "DATA FILE": "data/ellipticSine3D.h", "BOX CURVED MAP AMPLITUDE": 0.05}
+    o = OracleElliptic(s)
+    q = np.random.default_rng(9).standard_normal(o.geti("Ngather"))
+    assert np.array_equal(o.partial_ax(q), o.partial_ax_omp(q))

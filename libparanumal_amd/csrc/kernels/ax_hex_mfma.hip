@@ -55,13 +55,35 @@ __global__ void __launch_bounds__(256, 1) ax_hex_mfma_n15_kernel(const int* __re
     dT[c] = D[(4 * c + lg) * Nq + li];
   }
 
-  // gather q into LDS
-#pragma unroll 4
-  for (int n = t; n < Np; n += 256) {
-    const int id = __builtin_nontemporal_load(G2L + eb + n);
-    s_q[n] = (id >= 0) ? q[id] : 0.0;
+  // gather q into LDS: all 16 index loads in flight, then all 16 value loads
+  {
+    int id[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) id[u] = __builtin_nontemporal_load(G2L + eb + t + 256 * u);
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = (id[u] >= 0) ? q[id[u]] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) s_q[t + 256 * u] = v[u];
   }
   __syncthreads();
+
+  // phase K's geometric factors for its first k-slice are loaded here, so
+  // that their HBM latency overlaps phase T; each later slice's are loaded
+  // one slice ahead
+  const double* gbase = ggeo + (size_t)element * 6 * Np;
+  const double* wbase = wJ + eb;
+  double Gn[6][4], Wn[4];
+  auto loadG = [&](int k) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int n = k * NqNq + (4 * r + lg) * Nq + li;
+#pragma unroll
+      for (int g = 0; g < 6; ++g) Gn[g][r] = __builtin_nontemporal_load(gbase + g * Np + n);
+      Wn[r] = __builtin_nontemporal_load(wbase + n);
+    }
+  };
+  loadG(4 * w);
 
   // phase T: qt(i,j,k) = sum_m D[k][m] q(i,j,m) for j = 4w..4w+3
 #pragma unroll
@@ -76,19 +98,17 @@ __global__ void __launch_bounds__(256, 1) ax_hex_mfma_n15_kernel(const int* __re
   __syncthreads();
 
   // phase K: k = 4w..4w+3
-  const double* gbase = ggeo + (size_t)element * 6 * Np;
-  const double* wbase = wJ + eb;
 #pragma unroll 1
   for (int kk = 0; kk < 4; ++kk) {
     const int k = 4 * w + kk;
     double G[6][4], W[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int n = k * NqNq + (4 * r + lg) * Nq + li;
 #pragma unroll
-      for (int g = 0; g < 6; ++g) G[g][r] = __builtin_nontemporal_load(gbase + g * Np + n);
-      W[r] = __builtin_nontemporal_load(wbase + n);
+      for (int g = 0; g < 6; ++g) G[g][r] = Gn[g][r];
+      W[r] = Wn[r];
     }
+    if (kk < 3) loadG(k + 1);
     d4 qr = {0, 0, 0, 0}, qs = {0, 0, 0, 0};
 #pragma unroll
     for (int c = 0; c < 4; ++c) {

@@ -179,6 +179,33 @@ def measured_traffic(nx):
         return json.load(f)["traffic_bytes"]
 
 
+# configuration 4: 16^3 elements, N=15, curved, lambda=1
+N15_SETTINGS = {"MESH FILE": "BOX", "MESH DIMENSION": 3, "ELEMENT TYPE": 12, "BOX NX": 16, "BOX NY": 16,
+                "BOX NZ": 16, "BOX DIMX": 2, "BOX DIMY": 2, "BOX DIMZ": 2, "BOX BOUNDARY FLAG": 1,
+                "POLYNOMIAL DEGREE": 15, "BOX CURVED MAP AMPLITUDE": 0.05, "LAMBDA": 1,
+                "DATA FILE": "data/ellipticSine3D.h", "PRECONDITIONER": "JACOBI"}
+
+
+def copy_probe(torch):
+    """1 GiB device-to-device copy: achievable copy bandwidth, and the kernel of known
+    size that calibrates the FETCH_SIZE / WRITE_SIZE counters."""
+    nbytes = 1 << 30
+    src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").uniform_()
+    dst = torch.empty_like(src)
+    cs = torch.cuda.current_stream()
+    for _ in range(3):
+        dst.copy_(src)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(cs)
+    for _ in range(10):
+        dst.copy_(src)
+    b.record(cs)
+    b.synchronize()
+    ms = a.elapsed_time(b) / 10
+    del src, dst
+    return {"bytes_moved": 2 * nbytes, "ms": ms, "GBs": 2 * nbytes / (ms * 1e-3) / 1e9}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -191,7 +218,19 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="also time a 1 GiB device-to-device copy (achievable HBM bandwidth; "
                          "the reference kernel for scaling FETCH_SIZE/WRITE_SIZE counters)")
+    ap.add_argument("--n15-only", action="store_true",
+                    help="only the N=15 MFMA Ax of configuration 4 (for counter runs; with --copy-probe for the "
+                         "traffic calibration)")
     args = ap.parse_args()
+    if args.n15_only:
+        import torch
+
+        import libparanumal_amd as lp
+        out = {"ax_n15": ax_rate(lp, N15_SETTINGS, steps=args.steps, warmup=args.warmup)}
+        if args.copy_probe:
+            out["hbm_copy_probe"] = copy_probe(torch)
+        print(json.dumps(out))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(the path would shard by element slabs with a halo exchange, see DESIGN.md)", file=sys.stderr)
@@ -271,21 +310,7 @@ def main():
     result["roofline"]["frac_of_measured_stream"] = gbs / result["hbm_stream_probe"]["GBs"]
     del xs, ys, zs
     if args.copy_probe:
-        nbytes = 1 << 30
-        src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").uniform_()
-        dst = torch.empty_like(src)
-        cs = torch.cuda.current_stream()
-        for _ in range(3):
-            dst.copy_(src)
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record(cs)
-        for _ in range(10):
-            dst.copy_(src)
-        b.record(cs)
-        b.synchronize()
-        ms = a.elapsed_time(b) / 10
-        result["hbm_copy_probe"] = {"bytes_moved": 2 * nbytes, "ms": ms, "GBs": 2 * nbytes / (ms * 1e-3) / 1e9}
-        del src, dst
+        result["hbm_copy_probe"] = copy_probe(torch)
     if not args.no_solve:
         r = el.run()
         result["solve"] = {"solver": "PCG+Jacobi, stop |r|<=max(1e-8|r0|,1e-8)", "iterations": r["iterations"],
@@ -294,11 +319,7 @@ def main():
     el.close()
     if not args.no_extra:
         # configuration 4: 16^3 elements, N=15, curved, lambda=1
-        n15 = {"MESH FILE": "BOX", "MESH DIMENSION": 3, "ELEMENT TYPE": 12, "BOX NX": 16, "BOX NY": 16,
-               "BOX NZ": 16, "BOX DIMX": 2, "BOX DIMY": 2, "BOX DIMZ": 2, "BOX BOUNDARY FLAG": 1,
-               "POLYNOMIAL DEGREE": 15, "BOX CURVED MAP AMPLITUDE": 0.05, "LAMBDA": 1,
-               "DATA FILE": "data/ellipticSine3D.h", "PRECONDITIONER": "JACOBI"}
-        result["ax_n15"] = ax_rate(lp, n15)
+        result["ax_n15"] = ax_rate(lp, N15_SETTINGS)
         # configuration 3: FPCG + p-MG 7->3->1 in FP32 and FP64, same problem
         import numpy as np
         info32, x32 = pmg_solve(lp, pmg_settings(precision="FP32"))

@@ -56,7 +56,8 @@ def pmg_solve(lp, settings, reps=2):
     x = el.solution()
     info = {"iterations": best["iterations"], "seconds": best["seconds"], "r0": best["r0"],
             "r_final": best["r_final"], "solution_norm": best["norm"], "gathered_dofs": el.geti("Ngather"),
-            "coarse_iterations_last": el.geti("coarseIterations")}
+            "coarse_iterations_last": el.geti("coarseIterations"),
+            "coarse_solver": settings.get("MULTIGRID COARSE SOLVER", "JACOBI-PCG")}
     if settings.get("MULTIGRID SMOOTHER") == "LEARNED":
         li = el.learned_info()
         info["learned"] = {"arch": li["arch"], "active": bool(li["active"]), "fallback": bool(li["fallback"]),
@@ -158,6 +159,17 @@ def learned_ab(lp, nx=32, seed=1):
         sols[key] = x
     if "seconds" in out["learned"]:
         out["speedup_learned_vs_plain"] = out["learned"]["speedup_vs_plain"]
+    # the reference's own base level (parAlmond AMG, one V-cycle) in place of the Jacobi-PCG
+    amg = {"MULTIGRID COARSE SOLVER": "AMG"}
+    plain_amg, xpa = pmg_solve(lp, dict(base, **amg))
+    out["plain_chebyshev_amg_base"] = plain_amg
+    sols["plain_amg_base"] = xpa
+    if os.path.exists(LEARNED_WEIGHTS_LINEAR):
+        la, xla = pmg_solve(lp, dict(base, **amg, **{"MULTIGRID SMOOTHER": "LEARNED", "MULTIGRID LEARNED ARCH": "LINEAR",
+                                                     "MULTIGRID LEARNED WEIGHTS": LEARNED_WEIGHTS_LINEAR}))
+        la["speedup_vs_plain_amg_base"] = plain_amg["seconds"] / la["seconds"]
+        out["learned_amg_base"] = la
+        sols["learned_amg_base"] = xla
     ref, xr = pmg_solve(lp, dict(base, **{"MULTIGRID PRECISION": "FP64", "LINEAR SOLVER TOLERANCE": 1e-12}), reps=1)
     scale = float(np.max(np.abs(xr)))
     for key, x in sols.items():
@@ -324,8 +336,9 @@ def main():
         import numpy as np
         info32, x32 = pmg_solve(lp, pmg_settings(precision="FP32"))
         info64, x64 = pmg_solve(lp, pmg_settings(precision="FP64"))
+        info32a, _ = pmg_solve(lp, dict(pmg_settings(precision="FP32"), **{"MULTIGRID COARSE SOLVER": "AMG"}))
         diff = float(np.max(np.abs(x32 - x64)) / np.max(np.abs(x64)))
-        result["pmg_c3"] = {"fp32_smoother": info32, "fp64_smoother": info64,
+        result["pmg_c3"] = {"fp32_smoother": info32, "fp64_smoother": info64, "fp32_smoother_amg_base": info32a,
                             "max_rel_diff_fp32_vs_fp64_solution": diff,
                             "stop": "|r| <= 1e-8 |r0| (pure relative)"}
         # configuration 5: learned fine smoother against plain FP32 Chebyshev

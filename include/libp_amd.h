@@ -219,6 +219,17 @@ int libp_cheb_start_f64(int N, const double* invDiag, const double* r, const dou
 int libp_cheb_step_f64(int N, const double* invDiag, const double* Ad, double c1, double c2, double* RES,
                        double* d, double* X, void* stream);
 
+/* Algebraic multigrid of the p-MG base level (libs/parAlmond):
+ * csr_spmv: y = alpha A x + beta z for a CSR matrix (z may equal y, or be NULL with
+ *           beta = 0); lanesPerRow 1, 8 or 64 lanes reduce each row in a fixed order
+ *           (replaces SpMVcsr, okl/SpMVcsr.okl, and the Chebyshev CSR kernels of
+ *           parAlmondAMGSmoother.cpp:68-160);
+ * dense_gemv: y = M x with M dense row-major n x n, the coarsest level's inverse
+ *           (parAlmondCoarseExact.cpp:35-78, okl/dGEMV.okl). */
+int libp_csr_spmv_f64(int Nrows, const int* rowStarts, const int* cols, const double* vals, double alpha,
+                      const double* x, double beta, const double* z, double* y, int lanesPerRow, void* stream);
+int libp_dense_gemv_f64(int n, const double* M, const double* x, double* y, void* stream);
+
 /* ------------------------------------------------------------------ */
 /* Setup kernels (mesh geometry, operator diagonal, right-hand side)    */
 /* ------------------------------------------------------------------ */
@@ -375,6 +386,18 @@ int libp_elliptic_learned_correction(void* h, const float* r, float* c);
 /* Training loss of one sample (device FP32 r, target eh; nrm = eh.A eh) and its
  * gradient, left in the handle's "learnedGrad" device array; no update. */
 int libp_elliptic_learned_loss_grad(void* h, const float* r, const float* eh, double nrm, double* loss);
+/* Assembled operator of the handle's fine level (N <= 3) on the gathered dofs, CSR:
+ * libp_elliptic_csr_nnz returns the nonzero count (or -status); libp_elliptic_csr fills
+ * host arrays rowStarts [Ngather+1], cols [nnz], vals [nnz]
+ * (ellipticBuildOperatorMatrixContinuous restated through the element operator). */
+long libp_elliptic_csr_nnz(void* h);
+int libp_elliptic_csr(void* h, int* rowStarts, int* cols, double* vals);
+/* Smoothed-aggregation AMG on the handle's fine operator (N <= 3, not all-Neumann):
+ * setup, then x = V(b) for device FP64 gathered vectors (one V-cycle from zero);
+ * info: [0] levels, then rows and nonzeros of each level (up to nout values). */
+int libp_elliptic_amg_setup(void* h);
+int libp_elliptic_amg_vcycle(void* h, const double* b, double* x);
+int libp_elliptic_amg_info(void* h, double* out, int nout);
 /* LINEAR architecture: Gram matrices of one least-squares sample (device FP32 r
  * and eh; host G [nIn][nIn] = X^T X, B [Np][nIn] = T^T X with X the BF16 network
  * input of r and T[e][n] = bf16(eh / s_e) at the element's nodes). */

@@ -134,6 +134,14 @@ SIGNATURES = {
     "libp_elliptic_learned_correction": (c_int, [c_void_p, c_void_p, c_void_p]),
     "libp_elliptic_learned_loss_grad": (c_int, [c_void_p, c_void_p, c_void_p, c_double, ctypes.POINTER(c_double)]),
     "libp_elliptic_learned_linear_gram": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "libp_csr_spmv_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_double, c_void_p,
+                                  c_void_p, c_int, c_void_p]),
+    "libp_dense_gemv_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "libp_elliptic_csr_nnz": (c_long, [c_void_p]),
+    "libp_elliptic_csr": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "libp_elliptic_amg_setup": (c_int, [c_void_p]),
+    "libp_elliptic_amg_vcycle": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "libp_elliptic_amg_info": (c_int, [c_void_p, c_void_p, c_int]),
     "libp_elliptic_run": (c_int, [c_void_p, ctypes.POINTER(c_double), c_int]),
     "libp_elliptic_solution": (c_int, [c_void_p, c_void_p]),
 }
@@ -269,6 +277,30 @@ class Elliptic:
         d = {k: out[i] for i, k in enumerate(keys)}
         d["arch"] = "LINEAR" if d["arch"] == 1 else "MLP"
         return d
+
+    def assembled_csr(self):
+        """The fine operator assembled as CSR (numpy rowStarts, cols, vals); N <= 3."""
+        import numpy as np
+        nnz = lib().libp_elliptic_csr_nnz(self._h)
+        if nnz < 0:
+            check(-nnz, "csr_nnz")
+        n = self.geti("Ngather")
+        rs = np.empty(n + 1, dtype=np.int32); ci = np.empty(nnz, dtype=np.int32); v = np.empty(nnz)
+        check(lib().libp_elliptic_csr(self._h, rs.ctypes.data, ci.ctypes.data, v.ctypes.data), "csr")
+        return rs, ci, v
+
+    def amg_setup(self):
+        check(lib().libp_elliptic_amg_setup(self._h), "amg_setup")
+
+    def amg_vcycle(self, b, x):
+        check(lib().libp_elliptic_amg_vcycle(self._h, b.data_ptr(), x.data_ptr()), "amg_vcycle")
+
+    def amg_info(self):
+        out = (c_double * 64)()
+        check(lib().libp_elliptic_amg_info(self._h, out, 64), "amg_info")
+        L = int(out[0])
+        return {"levels": L, "rows": [int(out[1 + 2 * l]) for l in range(L)],
+                "nnz": [int(out[2 + 2 * l]) for l in range(L)]}
 
     def learned_linear_gram(self, r, eh):
         """Gram matrices (G = X^T X, B = T^T X; numpy FP64) of one least-squares sample of the

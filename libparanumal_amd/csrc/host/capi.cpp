@@ -6,6 +6,7 @@
 
 #include <sstream>
 
+#include "amg.hpp"
 #include "elliptic.hpp"
 #include "learned.hpp"
 
@@ -22,6 +23,8 @@ struct Handle {
   mesh_t mesh;
   std::shared_ptr<elliptic_t> elliptic;
   ellipticRunResult_t last;
+  std::shared_ptr<csrHost_t> csr;    // assembled fine operator (test hook)
+  std::shared_ptr<amgSolver_t> amg;  // AMG on the fine operator (test hook)
 };
 
 template <typename F>
@@ -336,6 +339,59 @@ extern "C" int libp_elliptic_learned_correction(void* p, const float* r, float* 
     L->correction(r, iD.ptr());
     LIBP_HIP(hipMemcpyAsync(c, L->o_c.ptr(), L->Ndofs * sizeof(float), hipMemcpyDeviceToDevice, h->platform.stream));
     h->platform.finish();
+  });
+}
+
+/* ------------------------------ assembled operator / AMG ------------------------------ */
+extern "C" long libp_elliptic_csr_nnz(void* p) {
+  Handle* h = (Handle*)p;
+  long n = 0;
+  int st = guarded("libp_elliptic_csr_nnz", [&]() {
+    if (!h->csr) h->csr = std::make_shared<csrHost_t>(ellipticAssembleCSR(*h->elliptic));
+    n = h->csr->nnz();
+  });
+  return st == LIBP_OK ? n : -st;
+}
+
+extern "C" int libp_elliptic_csr(void* p, int* rowStarts, int* cols, double* vals) {
+  Handle* h = (Handle*)p;
+  return guarded("libp_elliptic_csr", [&]() {
+    LIBP_ABORT("null pointer", !rowStarts || !cols || !vals);
+    if (!h->csr) h->csr = std::make_shared<csrHost_t>(ellipticAssembleCSR(*h->elliptic));
+    std::memcpy(rowStarts, h->csr->rs.data(), h->csr->rs.size() * sizeof(int));
+    std::memcpy(cols, h->csr->ci.data(), h->csr->ci.size() * sizeof(int));
+    std::memcpy(vals, h->csr->v.data(), h->csr->v.size() * sizeof(double));
+  });
+}
+
+extern "C" int libp_elliptic_amg_setup(void* p) {
+  Handle* h = (Handle*)p;
+  return guarded("libp_elliptic_amg_setup", [&]() {
+    unsigned short xs[3] = {0x330E, 0, 0};
+    h->amg = std::make_shared<amgSolver_t>(*h->elliptic, xs);
+  });
+}
+
+extern "C" int libp_elliptic_amg_vcycle(void* p, const double* b, double* x) {
+  Handle* h = (Handle*)p;
+  return guarded("libp_elliptic_amg_vcycle", [&]() {
+    LIBP_ABORT("AMG not set up", !h->amg);
+    LIBP_ABORT("null pointer", !b || !x);
+    h->amg->vcycle(b, x);
+    h->platform.finish();
+  });
+}
+
+extern "C" int libp_elliptic_amg_info(void* p, double* out, int nout) {
+  Handle* h = (Handle*)p;
+  return guarded("libp_elliptic_amg_info", [&]() {
+    LIBP_ABORT("AMG not set up", !h->amg);
+    std::vector<double> v{(double)h->amg->levels.size()};
+    for (auto& L : h->amg->levels) {
+      v.push_back((double)L.A.n);
+      v.push_back((double)L.A.nnz());
+    }
+    for (int i = 0; i < nout && i < (int)v.size(); ++i) out[i] = v[i];
   });
 }
 

@@ -144,6 +144,10 @@ settings_t ellipticDefaultSettings() {
                {"TRUE", "FALSE"});
   s.newSetting("MULTIGRID COARSE GRAPH", "TRUE", "Replay the fused coarse iterations as captured HIP graphs",
                {"TRUE", "FALSE"});
+  s.newSetting("MULTIGRID COARSE SOLVER", "JACOBI-PCG",
+               "Base level of the p-MG: FP64 Jacobi-PCG to MULTIGRID COARSE TOLERANCE, or one V-cycle of "
+               "smoothed-aggregation AMG on the assembled operator (the reference's parAlmond)",
+               {"JACOBI-PCG", "AMG"});
   s.newSetting("MULTIGRID COARSE CHECK EVERY", "8",
                "Coarse Jacobi-PCG: iterations queued between host convergence reads (same iterates as 1)");
   s.newSetting("MULTIGRID LEARNED WEIGHTS", "", "Weight file of the learned fine smoother");

@@ -12,6 +12,7 @@
 #include <random>
 #include <type_traits>
 
+#include "amg.hpp"
 #include "elliptic.hpp"
 #include "learned.hpp"
 
@@ -426,13 +427,14 @@ void MGLevel<double>::augmentLearned(deviceMemory<double>&, deviceMemory<double>
 class coarseJacobiPCG_t {
  public:
   std::shared_ptr<elliptic_t> elliptic;
+  std::shared_ptr<amgSolver_t> amg;  // [MULTIGRID COARSE SOLVER] AMG: one V-cycle replaces the PCG
   std::shared_ptr<LinearSolver::pcg> solver;
   std::shared_ptr<JacobiPrecon> jacobi;
   deviceMemory<double> o_x, o_r;
   double tol = 1e-2;
   int maxit = 200;
   int lastIterations = 0;
-  explicit coarseJacobiPCG_t(std::shared_ptr<elliptic_t> el) : elliptic(el) {
+  coarseJacobiPCG_t(std::shared_ptr<elliptic_t> el, unsigned short xs[3]) : elliptic(el) {
     elliptic->settings.getSetting("MULTIGRID COARSE TOLERANCE", tol);
     elliptic->settings.getSetting("MULTIGRID COARSE MAXIT", maxit);
     jacobi = std::make_shared<JacobiPrecon>(*elliptic);
@@ -451,6 +453,13 @@ class coarseJacobiPCG_t {
     std::string graph = "TRUE";
     elliptic->settings.getSetting("MULTIGRID COARSE GRAPH", graph);
     useGraph = graph == "TRUE";
+    // the reference's base level: parAlmond AMG on the assembled operator
+    // (ellipticPreconMultiGrid.cpp:104-131), one V-cycle per coarse solve
+    if (elliptic->settings.compareSetting("MULTIGRID COARSE SOLVER", "AMG")) {
+      LIBP_ABORT("MULTIGRID COARSE SOLVER AMG needs a Dirichlet or screened problem (not all-Neumann)",
+                 elliptic->allNeumann);
+      amg = std::make_shared<amgSolver_t>(*elliptic, xs);
+    }
   }
   ~coarseJacobiPCG_t() {
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
@@ -551,6 +560,11 @@ template <>
 void coarseJacobiPCG_t::solve<double>(deviceMemory<double>& o_rhs, deviceMemory<double>& o_X) {
   platform_t& p = elliptic->platform;
   const dlong N = elliptic->Ndofs;
+  if (amg) {
+    amg->vcycle(o_rhs.ptr(), o_X.ptr());
+    lastIterations = 1;
+    return;
+  }
   LIBP_HIP(hipMemcpyAsync(o_r.ptr(), o_rhs.ptr(), N * sizeof(double), hipMemcpyDeviceToDevice, p.stream));
   if (fusedHex1) {
     lastIterations = solveFusedHex1(o_X);
@@ -564,7 +578,10 @@ void coarseJacobiPCG_t::solve<float>(deviceMemory<float>& o_rhs, deviceMemory<fl
   platform_t& p = elliptic->platform;
   const dlong N = elliptic->Ndofs;
   LIBP_CALL(libp_convert_f32_to_f64(N, o_rhs.ptr(), o_r.ptr(), p.s()));
-  if (fusedHex1) {
+  if (amg) {
+    amg->vcycle(o_r.ptr(), o_x.ptr());
+    lastIterations = 1;
+  } else if (fusedHex1) {
     lastIterations = solveFusedHex1(o_x);
   } else {
     LIBP_CALL(libp_set_f64(N, 0.0, o_x.ptr(), p.s()));
@@ -604,7 +621,7 @@ class multigrid_t : public MultiGridPreconBase {
       ellipticLevels.push_back(elC);
       lev->ellipticC = elC;
     }
-    coarse = std::make_shared<coarseJacobiPCG_t>(ellipticLevels.back());
+    coarse = std::make_shared<coarseJacobiPCG_t>(ellipticLevels.back(), xs);
     o_rhs.resize(ellipticLevels.size());
     o_x.resize(ellipticLevels.size());
     dlong maxN = 0;

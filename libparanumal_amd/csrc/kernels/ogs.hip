@@ -1,4 +1,4 @@
-// Single-rank gather-scatter kernels (CSR rows, one thread per gathered row).
+// Single-rank gather-scatter kernels (CSR rows, a thread per gathered row).
 //
 // The reference stages blocks of <= 512 nonzeros through shared memory
 // (libs/ogs/okl/ogsKernels.okl:86-164). On the C0 hex path every row holds
@@ -12,34 +12,46 @@
 
 namespace libp_amd {
 
-// sum_{j in [start,end)} v[colIds[j]] in ascending j; loads issued up front for rows of <= 8 entries
-template <typename T>
-__device__ inline T gather_row(const int start, const int end, const int* __restrict__ colIds,
-                               const T* __restrict__ v) {
-  const int len = end - start;
-  T val = 0;
-  if (len <= 8) {
-    int c[8];
-    T x[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) c[k] = (k < len) ? colIds[start + k] : 0;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) x[k] = (k < len) ? v[c[k]] : T(0);
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-      if (k < len) val += x[k];
-  } else {
-    for (int g = start; g < end; ++g) val += v[colIds[g]];
-  }
-  return val;
-}
-
+// Each thread takes two rows (row and row + S, S = threads in the grid) per
+// step and issues the loads of both before summing either: a row is a chain
+// of three dependent loads (row start, column ids, values), and two chains in
+// flight per thread keep more bytes in flight. The grid gives each thread
+// about four rows. Each sum runs from zero over the row in ascending order.
 template <typename T>
 __global__ void __launch_bounds__(256) ogs_gather_kernel(const int Nrows, const int* __restrict__ rowStarts,
                                                          const int* __restrict__ colIds, const T* __restrict__ v,
                                                          T* __restrict__ gv) {
-  for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < Nrows; row += gridDim.x * blockDim.x)
-    gv[row] = gather_row<T>(rowStarts[row], rowStarts[row + 1], colIds, v);
+  const long S = (long)gridDim.x * blockDim.x;
+  for (long row0 = blockIdx.x * (long)blockDim.x + threadIdx.x; row0 < Nrows; row0 += 2 * S) {
+    int a[2], len[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const long row = row0 + r * S;
+      a[r] = (row < Nrows) ? rowStarts[row] : 0;
+      len[r] = (row < Nrows) ? rowStarts[row + 1] - a[r] : 0;
+    }
+    int c[2][8];
+    T x[2][8];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) c[r][k] = (k < len[r]) ? colIds[a[r] + k] : 0;
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) x[r][k] = (k < len[r]) ? v[c[r][k]] : T(0);
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const long row = row0 + r * S;
+      if (row >= Nrows) continue;
+      T val = 0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (k < len[r]) val += x[r][k];
+      for (int g = a[r] + 8; g < a[r] + len[r]; ++g) val += v[colIds[g]];
+      gv[row] = val;
+    }
+  }
 }
 
 template <typename T>
@@ -57,7 +69,7 @@ template <typename T>
 static int gather(int Nrows, const int* rs, const int* ci, const T* v, T* gv, void* stream) {
   if (Nrows <= 0) return LIBP_OK;
   if (!rs || !ci || !v || !gv) return arg_error("ogs_gather", "null pointer");
-  hipLaunchKernelGGL(ogs_gather_kernel<T>, dim3(stream_blocks(Nrows, 256)), dim3(256), 0, as_stream(stream), Nrows,
+  hipLaunchKernelGGL(ogs_gather_kernel<T>, dim3(stream_blocks(Nrows, 1024)), dim3(256), 0, as_stream(stream), Nrows,
                      rs, ci, v, gv);
   return launch_status("ogs_gather");
 }

@@ -264,6 +264,13 @@ ellipticRunResult_t elliptic_t::Run(int verboseOverride) {
                                            platform.o_reduceWS.ptr(), platform.s()));
     res.maxError = std::sqrt(platform.readScalars(tmp, 1)[0]);
   }
+  // field file (ellipticRun.cpp:223-235)
+  if (settings.compareSetting("OUTPUT TO FILE", "TRUE")) {
+    std::string name = settings.getSetting("OUTPUT FILE NAME");
+    char fname[4096];
+    std::snprintf(fname, sizeof(fname), "%s_%04d.vtu", name.c_str(), comm.rank());
+    PlotFields(o_xL, fname);
+  }
   if (comm.rank() == 0 && verbose) printf("Solution norm = %17.15lg\n", res.norm);
   res.iterations = iter;
   res.solveSeconds = elapsed;
@@ -271,6 +278,114 @@ ellipticRunResult_t elliptic_t::Run(int verboseOverride) {
   res.rFinal = std::sqrt(std::max(linearSolver.get()->stats.rdotrFinal, 0.0));
   o_xL_last = o_xL;
   return res;
+}
+
+/* Plot output in the reference's VTU layout (ellipticPlotFields.cpp,
+ * meshReferenceNodesHex3D.cpp:54-67, meshBasisHex3D.cpp EquispacedEToVHex3D):
+ * each element's nodal values are interpolated (Lagrange, tensor product) to
+ * plotNq = N+4 equispaced nodes per direction; every sub-cube is split into
+ * the same six tetrahedra (VTK type 10) with corner c = i + 2j + 4k:
+ * (0,3,2,7) (0,1,3,7) (0,2,6,7) (0,6,4,7) (0,5,1,7) (0,4,5,7). */
+void elliptic_t::PlotFields(const deviceMemory<dfloat>& o_qL, const std::string& fileName) {
+  const int N = mesh.N, Nq = mesh.Nq, Np = mesh.Np;
+  const int pN = N + 3, pNq = pN + 1, pNp = pNq * pNq * pNq;
+  const dlong E = mesh.Nelements;
+  const size_t NL = (size_t)E * Np;
+  std::vector<double> x(NL), y(NL), z(NL), q(NL);
+  mesh.o_x.copyTo(x.data(), NL, 0, platform.stream);
+  mesh.o_y.copyTo(y.data(), NL, 0, platform.stream);
+  mesh.o_z.copyTo(z.data(), NL, 0, platform.stream);
+  o_qL.copyTo(q.data(), NL, 0, platform.stream);
+  platform.finish();
+  // I[i][m] = l_m(r_i): Lagrange basis of the GLL nodes at the equispaced nodes
+  std::vector<double> I((size_t)pNq * Nq);
+  for (int i = 0; i < pNq; ++i) {
+    const double r = -1.0 + 2.0 * i / pN;
+    for (int m = 0; m < Nq; ++m) {
+      double l = 1.0;
+      for (int k = 0; k < Nq; ++k)
+        if (k != m) l *= (r - mesh.gllz[k]) / (mesh.gllz[m] - mesh.gllz[k]);
+      I[(size_t)i * Nq + m] = l;
+    }
+  }
+  std::vector<double> t1((size_t)pNq * Nq * Nq), t2((size_t)pNq * pNq * Nq);
+  auto interp = [&](const double* u, double* out) {
+    for (int k = 0; k < Nq; ++k)
+      for (int j = 0; j < Nq; ++j)
+        for (int i = 0; i < pNq; ++i) {
+          double s = 0;
+          for (int m = 0; m < Nq; ++m) s += I[(size_t)i * Nq + m] * u[m + Nq * (j + Nq * k)];
+          t1[i + pNq * (j + Nq * k)] = s;
+        }
+    for (int k = 0; k < Nq; ++k)
+      for (int j = 0; j < pNq; ++j)
+        for (int i = 0; i < pNq; ++i) {
+          double s = 0;
+          for (int m = 0; m < Nq; ++m) s += I[(size_t)j * Nq + m] * t1[i + pNq * (m + Nq * k)];
+          t2[i + pNq * (j + pNq * k)] = s;
+        }
+    for (int k = 0; k < pNq; ++k)
+      for (int j = 0; j < pNq; ++j)
+        for (int i = 0; i < pNq; ++i) {
+          double s = 0;
+          for (int m = 0; m < Nq; ++m) s += I[(size_t)k * Nq + m] * t2[i + pNq * (j + pNq * m)];
+          out[i + pNq * (j + pNq * k)] = s;
+        }
+  };
+  FILE* fp = std::fopen(fileName.c_str(), "w");
+  LIBP_ABORT("cannot open " << fileName, !fp);
+  const long Ncells = (long)E * 6 * pN * pN * pN;
+  std::fprintf(fp, "<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" byte_order=\"BigEndian\">\n");
+  std::fprintf(fp, "  <UnstructuredGrid>\n");
+  std::fprintf(fp, "    <Piece NumberOfPoints=\"%ld\" NumberOfCells=\"%ld\">\n", (long)E * pNp, Ncells);
+  std::fprintf(fp, "      <Points>\n");
+  std::fprintf(fp, "        <DataArray type=\"Float32\" NumberOfComponents=\"3\" Format=\"ascii\">\n");
+  std::vector<double> px(pNp), py(pNp), pz(pNp), pq(pNp);
+  for (dlong e = 0; e < E; ++e) {
+    interp(&x[(size_t)e * Np], px.data());
+    interp(&y[(size_t)e * Np], py.data());
+    interp(&z[(size_t)e * Np], pz.data());
+    for (int n = 0; n < pNp; ++n) std::fprintf(fp, "       %g %g %g\n", px[n], py[n], pz[n]);
+  }
+  std::fprintf(fp, "        </DataArray>\n");
+  std::fprintf(fp, "      </Points>\n");
+  std::fprintf(fp, "      <PointData Scalars=\"scalars\">\n");
+  std::fprintf(fp, "        <DataArray type=\"Float32\" Name=\"Fields\" NumberOfComponents=\"%d\" Format=\"ascii\">\n",
+               Nfields);
+  for (dlong e = 0; e < E; ++e) {
+    interp(&q[(size_t)e * Np], pq.data());
+    for (int n = 0; n < pNp; ++n) std::fprintf(fp, "       %g \n", pq[n]);
+  }
+  std::fprintf(fp, "       </DataArray>\n");
+  std::fprintf(fp, "     </PointData>\n");
+  std::fprintf(fp, "    <Cells>\n");
+  std::fprintf(fp, "      <DataArray type=\"Int32\" Name=\"connectivity\" Format=\"ascii\">\n");
+  static const int tets[6][4] = {{0, 3, 2, 7}, {0, 1, 3, 7}, {0, 2, 6, 7}, {0, 6, 4, 7}, {0, 5, 1, 7}, {0, 4, 5, 7}};
+  for (dlong e = 0; e < E; ++e)
+    for (int k = 0; k < pN; ++k)
+      for (int j = 0; j < pN; ++j)
+        for (int i = 0; i < pN; ++i)
+          for (int t = 0; t < 6; ++t) {
+            std::fprintf(fp, "       ");
+            for (int v = 0; v < 4; ++v) {
+              const int c = tets[t][v];
+              const int n = (i + (c & 1)) + pNq * ((j + ((c >> 1) & 1)) + pNq * (k + ((c >> 2) & 1)));
+              std::fprintf(fp, "%ld ", (long)e * pNp + n);
+            }
+            std::fprintf(fp, "\n");
+          }
+  std::fprintf(fp, "        </DataArray>\n");
+  std::fprintf(fp, "        <DataArray type=\"Int32\" Name=\"offsets\" Format=\"ascii\">\n");
+  for (long c = 1; c <= Ncells; ++c) std::fprintf(fp, "       %ld\n", 4 * c);
+  std::fprintf(fp, "       </DataArray>\n");
+  std::fprintf(fp, "       <DataArray type=\"Int32\" Name=\"types\" Format=\"ascii\">\n");
+  for (long c = 0; c < Ncells; ++c) std::fprintf(fp, "10\n");
+  std::fprintf(fp, "        </DataArray>\n");
+  std::fprintf(fp, "      </Cells>\n");
+  std::fprintf(fp, "    </Piece>\n");
+  std::fprintf(fp, "  </UnstructuredGrid>\n");
+  std::fprintf(fp, "</VTKFile>\n");
+  std::fclose(fp);
 }
 
 /* JacobiPrecon (ellipticPreconJacobi.cpp:30-51). */

@@ -81,6 +81,9 @@ class elliptic_t : public operator_t {
   void BuildRhs(long seed, deviceMemory<dfloat>& o_r);
 
   deviceMemory<dfloat> o_xL_last;  // solution E-vector of the last Run
+  // ellipticPlotFields.cpp: the E-vector interpolated to (N+4)^3 equispaced nodes per
+  // element, six tetrahedra per sub-cube, ASCII VTU (one file per rank)
+  void PlotFields(const deviceMemory<dfloat>& o_qL, const std::string& fileName);
 };
 
 class JacobiPrecon : public diagonalPrecon_t {

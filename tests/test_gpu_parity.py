@@ -359,3 +359,34 @@ def test_coarse_graph_replays_match_eager():
         assert eager.geti("coarseIterations") == graph.geti("coarseIterations")
         assert np.array_equal(eager.solution(), graph.solution())
 
+
+
+def test_vtu_output_matches_manufactured_solution(tmp_path):
+    """[OUTPUT TO FILE] TRUE writes <OUTPUT FILE NAME>_0000.vtu in the reference's
+    layout (ellipticPlotFields.cpp): (N+4)^3 equispaced plot nodes per element,
+    6 (N+3)^3 tetrahedra; the plotted field is the solution, which for the
+    sine data is close to sin(pi x) sin(pi y) sin(pi z) at every plot node."""
+    import xml.etree.ElementTree as ET
+
+    import libparanumal_amd as lp
+    s = dict(REF, **{"BOX NX": 4, "BOX NY": 4, "BOX NZ": 4, "POLYNOMIAL DEGREE": 5, "PRECONDITIONER": "JACOBI",
+                     "BOX DIMX": 2, "BOX DIMY": 2, "BOX DIMZ": 2, "OUTPUT TO FILE": "TRUE",
+                     "OUTPUT FILE NAME": str(tmp_path / "sol"), "LINEAR SOLVER STOPPING CRITERION": "RELATIVE",
+                     "LINEAR SOLVER TOLERANCE": 1e-10})
+    el = lp.Elliptic(s)
+    r = el.run()
+    f = tmp_path / "sol_0000.vtu"
+    assert f.exists()
+    piece = ET.parse(f).getroot().find("UnstructuredGrid").find("Piece")
+    E, pNq = 64, 5 + 4
+    assert int(piece.get("NumberOfPoints")) == E * pNq ** 3
+    assert int(piece.get("NumberOfCells")) == E * 6 * (pNq - 1) ** 3
+    pts = np.array(piece.find("Points").find("DataArray").text.split(), dtype=float).reshape(-1, 3)
+    fld = np.array(piece.find("PointData").find("DataArray").text.split(), dtype=float)
+    conn = np.array(piece.find("Cells").findall("DataArray")[0].text.split(), dtype=np.int64)
+    assert conn.size == 4 * E * 6 * (pNq - 1) ** 3 and conn.min() == 0 and conn.max() == E * pNq ** 3 - 1
+    assert np.abs(pts).max() <= 1.0 + 1e-6
+    exact = np.sin(np.pi * pts[:, 0]) * np.sin(np.pi * pts[:, 1]) * np.sin(np.pi * pts[:, 2])
+    err = np.abs(fld - exact).max()
+    print("vtu points", len(pts), "max |u_h - u| at plot nodes", err, r["iterations"], flush=True)
+    assert err < 1e-3

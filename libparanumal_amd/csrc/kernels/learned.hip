@@ -72,31 +72,48 @@ __global__ void __launch_bounds__(256) learned_assemble_kernel(const int Nrows, 
                                                                const float* __restrict__ invD, float* __restrict__ c) {
   const float om = omega ? *omega : 0.f;
   auto elem = [&](int id) { return SHIFT >= 0 ? (id >> SHIFT) : id / Np; };
-  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < Nrows; g += gridDim.x * blockDim.x) {
-    const int a = rowStarts[g], b = rowStarts[g + 1];
-    const int len = b - a;
-    float acc = 0.f;
-    int col[8];
-    float y[8], sc[8];
+  // two rows per step (row and row + S), every load of both issued before the
+  // sums, as in the gather; each row's arithmetic is unchanged
+  const long S = (long)gridDim.x * blockDim.x;
+  for (long g0 = blockIdx.x * (long)blockDim.x + threadIdx.x; g0 < Nrows; g0 += 2 * S) {
+    int a[2], len[2];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) col[k] = (k < len) ? colIds[a + k] : 0;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int e = elem(col[k]), n = col[k] - e * Np;
-      y[k] = (k < len) ? Y[(long)e * ldy + n] : 0.f;
-      sc[k] = (k < len) ? scale[e] : 0.f;
+    for (int r2 = 0; r2 < 2; ++r2) {
+      const long g = g0 + r2 * S;
+      a[r2] = (g < Nrows) ? rowStarts[g] : 0;
+      len[r2] = (g < Nrows) ? rowStarts[g + 1] - a[r2] : 0;
     }
+    int col[2][8];
+    float y[2][8], sc[2][8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-      if (k < len) acc += sc[k] * y[k];
-    for (int j = a + 8; j < b; ++j) {
-      const int id = colIds[j];
-      const int e = elem(id), n = id - e * Np;
-      acc += scale[e] * Y[(long)e * ldy + n];
+    for (int r2 = 0; r2 < 2; ++r2)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) col[r2][k] = (k < len[r2]) ? colIds[a[r2] + k] : 0;
+#pragma unroll
+    for (int r2 = 0; r2 < 2; ++r2)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int e = elem(col[r2][k]), n = col[r2][k] - e * Np;
+        y[r2][k] = (k < len[r2]) ? Y[(long)e * ldy + n] : 0.f;
+        sc[r2][k] = (k < len[r2]) ? scale[e] : 0.f;
+      }
+#pragma unroll
+    for (int r2 = 0; r2 < 2; ++r2) {
+      const long g = g0 + r2 * S;
+      if (g >= Nrows) continue;
+      float acc = 0.f;
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (k < len[r2]) acc += sc[r2][k] * y[r2][k];
+      for (int j = a[r2] + 8; j < a[r2] + len[r2]; ++j) {
+        const int id = colIds[j];
+        const int e = elem(id), n = id - e * Np;
+        acc += scale[e] * Y[(long)e * ldy + n];
+      }
+      float v = (len[r2] > 0 ? acc / (float)len[r2] : 0.f);
+      if (omega) v += om * invD[g] * r[g];
+      c[g] = v;
     }
-    float v = (b > a ? acc / (float)(b - a) : 0.f);
-    if (omega) v += om * invD[g] * r[g];
-    c[g] = v;
   }
 }
 
@@ -275,7 +292,7 @@ extern "C" int libp_learned_assemble_f32(int Nrows, const int* rowStarts, const 
   if (Nrows <= 0) return LIBP_OK;
   if (!rowStarts || !colIds || !Y || !scale || !r || !invDiag || !c) return arg_error("learned_assemble", "null pointer");
   if (Np <= 0 || ldy < Np) return arg_error("learned_assemble", "bad size");
-  const dim3 grid(stream_blocks(Nrows, 256)), block(256);
+  const dim3 grid(stream_blocks(Nrows, 1024)), block(256);
   hipStream_t st = as_stream(stream);
   if (Np == 512)
     hipLaunchKernelGGL(learned_assemble_kernel<9>, grid, block, 0, st, Nrows, rowStarts, colIds, Np, Y, ldy, scale,

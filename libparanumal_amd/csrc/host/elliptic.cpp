@@ -41,9 +41,17 @@ void elliptic_t::BoundarySetup() {
   allNeumannPenalty = 1.;
   // node-wise flags (ellipticBoundarySetup.cpp:53-65); box faces all carry one flag,
   // so the face-wise EToB translation only matters through allNeumann
-  if (!mesh.periodic && mesh.boundaryFlag > 0) {
+  if (mesh.boxMesh && !mesh.periodic && mesh.boundaryFlag > 0) {
     LIBP_ABORT("BOX BOUNDARY FLAG outside BCType table", mesh.boundaryFlag >= NBCTypes);
     if (BCType[mesh.boundaryFlag] != 2) allNeumann = 0;
+  }
+  if (!mesh.boxMesh) {  // any boundary node that is not Neumann (ellipticBoundarySetup.cpp:40-65)
+    for (size_t n = 0; n < (size_t)NL; ++n) {
+      const int bc = mesh.mapB[n];
+      if (bc <= 0) continue;
+      LIBP_ABORT("boundary flag " << bc << " outside BCType table", bc >= NBCTypes);
+      if (BCType[bc] != 2) allNeumann = 0;
+    }
   }
   Nmasked = 0;
   mapB.malloc(NL, 0);

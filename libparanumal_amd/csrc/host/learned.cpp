@@ -98,6 +98,7 @@ learnedSmoother_t::learnedSmoother_t(elliptic_t& el, int _arch) : elliptic(el), 
  * same global id. */
 void learnedSmoother_t::setupMaps() {
   const mesh_t& mesh = elliptic.mesh;
+  LIBP_ABORT("the learned smoother's neighbour traces need a BOX mesh", !mesh.boxMesh);
   const int Nq = mesh.Nq, N = mesh.N;
   const dlong nx = mesh.nx, ny = mesh.ny, nz = mesh.nz;
   std::vector<dlong> g2l((size_t)E * Np);

@@ -1,7 +1,14 @@
-// Hex box mesh: structured connectivity in closed form, geometry on the GPU.
+// Hex meshes: a box (structured connectivity in closed form) or a Gmsh file
+// (general numbering); geometry on the GPU.
 #include "mesh.hpp"
 
+#include <algorithm>
+#include <array>
 #include <cmath>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <unordered_map>
 
 namespace libp {
 
@@ -89,17 +96,199 @@ void mesh_t::Setup(platform_t& _platform, settings_t& _settings, comm_t _comm) {
   platform = _platform;
   settings = _settings;
   comm = _comm;
-  LIBP_ABORT("Only BOX meshes are supported", !settings.compareSetting("MESH FILE", "BOX"));
   LIBP_ABORT("Only hexahedral meshes (ELEMENT TYPE 12) are supported",
              !settings.compareSetting("ELEMENT TYPE", "12"));
   LIBP_ABORT("Only 3-D meshes are supported", !settings.compareSetting("MESH DIMENSION", "3"));
-  SetupBox();
+  const std::string file = settings.getSetting("MESH FILE");
+  boxMesh = file == "BOX";
+  if (boxMesh)
+    SetupBox();
+  else
+    ReadGmshHex3D(file);
   int NN;
   settings.getSetting("POLYNOMIAL DEGREE", NN);
   ReferenceNodes(NN);
-  ConnectNodes();
-  GeometricFactors();
+  if (boxMesh) {
+    ConnectNodes();
+    GeometricFactors();
+  } else {
+    GeometricFactors();  // the general numbering matches nodes by their coordinates
+    ConnectNodesGeneral();
+  }
   GatherScatterSetup();
+}
+
+/* Gmsh 2.2 ASCII hexahedral mesh (meshReadGmshHex3D.cpp): $Nodes "id x y z";
+ * $Elements "id type ntags tag... nodes", type 5 = hexahedron (8 vertices, in
+ * the order of the reference hex), type 3 = quadrilateral boundary face
+ * whose first tag is the boundary flag. One rank reads every element. */
+void mesh_t::ReadGmshHex3D(const std::string& fileName) {
+  std::ifstream f(fileName);
+  LIBP_ABORT("Cannot open file: " << fileName, !f);
+  std::string line;
+  auto seek = [&](const char* tag) {
+    while (std::getline(f, line))
+      if (line.find(tag) != std::string::npos) return;
+    LIBP_FORCE_ABORT("Error reading mesh file: " << fileName << " (no " << tag << ")");
+  };
+  seek("$Nodes");
+  hlong Nnodes = 0;
+  LIBP_ABORT("Error reading mesh file: " << fileName, !std::getline(f, line));
+  Nnodes = std::stoll(line);
+  std::vector<double> VX(Nnodes), VY(Nnodes), VZ(Nnodes);
+  for (hlong n = 0; n < Nnodes; ++n) {
+    LIBP_ABORT("Error reading mesh file: " << fileName, !std::getline(f, line));
+    std::istringstream ss(line);
+    hlong id;
+    ss >> id >> VX[n] >> VY[n] >> VZ[n];
+    LIBP_ABORT("Error reading node " << n + 1 << " of " << fileName, !ss);
+  }
+  seek("$Elements");
+  LIBP_ABORT("Error reading mesh file: " << fileName, !std::getline(f, line));
+  const hlong gN = std::stoll(line);
+  std::vector<hlong> hexes;
+  boundaryInfo.clear();
+  for (hlong n = 0; n < gN; ++n) {
+    LIBP_ABORT("Error reading mesh file: " << fileName, !std::getline(f, line));
+    std::istringstream ss(line);
+    hlong id;
+    int type, ntags;
+    ss >> id >> type >> ntags;
+    std::vector<hlong> tags(ntags);
+    for (auto& t : tags) ss >> t;
+    if (type == 5) {
+      for (int v = 0; v < 8; ++v) {
+        hlong vid;
+        ss >> vid;
+        LIBP_ABORT("vertex id " << vid << " out of range in " << fileName, vid < 1 || vid > Nnodes);
+        hexes.push_back(vid - 1);
+      }
+    } else if (type == 3) {
+      boundaryInfo.push_back(ntags > 0 ? tags[0] : 0);
+      for (int v = 0; v < 4; ++v) {
+        hlong vid;
+        ss >> vid;
+        LIBP_ABORT("vertex id " << vid << " out of range in " << fileName, vid < 1 || vid > Nnodes);
+        boundaryInfo.push_back(vid - 1);
+      }
+    }
+    LIBP_ABORT("Error reading element " << id << " of " << fileName, !ss);
+  }
+  Nelements = (dlong)(hexes.size() / 8);
+  LIBP_ABORT("no hexahedra in " << fileName, Nelements == 0);
+  nx = ny = nz = 0;
+  periodic = false;
+  boundaryFlag = 0;
+  curveAmp = 0.0;
+  EToV.malloc((size_t)Nelements * Nverts);
+  EX.malloc((size_t)Nelements * Nverts);
+  EY.malloc((size_t)Nelements * Nverts);
+  EZ.malloc((size_t)Nelements * Nverts);
+  for (size_t k = 0; k < hexes.size(); ++k) {
+    EToV[k] = hexes[k];
+    EX[k] = VX[hexes[k]];
+    EY[k] = VY[hexes[k]];
+    EZ[k] = VZ[hexes[k]];
+  }
+  o_EX = platform.malloc(EX);
+  o_EY = platform.malloc(EY);
+  o_EZ = platform.malloc(EZ);
+}
+
+/* Node numbering of a mesh file. On one rank the reference's fixed point
+ * (meshConnectNodes.cpp:32-117) gives every node 1 + the smallest local
+ * index among its copies; here the copies are found by their coordinates
+ * (hashed on a grid of 1e-6 of the smallest element extent, the 27
+ * neighbouring cells searched, copies within 1e-9 of it). mapB: the smallest
+ * positive flag of the boundary faces (the mesh file's tagged quadrilaterals,
+ * matched by vertex set as meshConnectBoundary.cpp:54-156 does) touching any
+ * copy of the node, else -1. */
+void mesh_t::ConnectNodesGeneral() {
+  const size_t NL = (size_t)Nelements * Np;
+  std::vector<double> x(NL), y(NL), z(NL);
+  o_x.copyTo(x.data(), NL, 0, platform.stream);
+  o_y.copyTo(y.data(), NL, 0, platform.stream);
+  o_z.copyTo(z.data(), NL, 0, platform.stream);
+  platform.finish();
+  double h = 1e300;  // smallest element extent along any axis
+  for (dlong e = 0; e < Nelements; ++e)
+    for (const memory<double>* V : {&EX, &EY, &EZ}) {
+      double lo = 1e300, hi = -1e300;
+      for (int v = 0; v < 8; ++v) {
+        lo = std::min(lo, (*V)[(size_t)e * 8 + v]);
+        hi = std::max(hi, (*V)[(size_t)e * 8 + v]);
+      }
+      h = std::min(h, hi - lo);
+    }
+  LIBP_ABORT("degenerate element extent", !(h > 0));
+  const double cell = 1e-6 * h, tol = 1e-9 * h;
+  struct key_t {
+    long long a, b, c;
+    bool operator==(const key_t& o) const { return a == o.a && b == o.b && c == o.c; }
+  };
+  struct hash_t {
+    size_t operator()(const key_t& k) const {
+      return std::hash<long long>()(k.a * 73856093LL ^ k.b * 19349663LL ^ k.c * 83492791LL);
+    }
+  };
+  std::unordered_map<key_t, std::vector<size_t>, hash_t> grid;
+  std::vector<size_t> rep(NL);
+  for (size_t n = 0; n < NL; ++n) {
+    const key_t k{(long long)std::floor(x[n] / cell), (long long)std::floor(y[n] / cell),
+                  (long long)std::floor(z[n] / cell)};
+    size_t found = n;
+    for (int da = -1; da <= 1 && found == n; ++da)
+      for (int db = -1; db <= 1 && found == n; ++db)
+        for (int dc = -1; dc <= 1 && found == n; ++dc) {
+          auto it = grid.find(key_t{k.a + da, k.b + db, k.c + dc});
+          if (it == grid.end()) continue;
+          for (size_t m : it->second)
+            if (std::fabs(x[m] - x[n]) <= tol && std::fabs(y[m] - y[n]) <= tol && std::fabs(z[m] - z[n]) <= tol) {
+              found = m;
+              break;
+            }
+        }
+    rep[n] = found;  // the first copy met (only first copies enter the grid): the smallest local index
+    if (found == n) grid[k].push_back(n);
+  }
+  globalIds.malloc(NL);
+  for (size_t n = 0; n < NL; ++n) globalIds[n] = 1 + (hlong)rep[n];
+  // boundary faces by vertex set; face f of the reference hex (faces t=-1, s=-1, r=+1, s=+1, r=-1, t=+1)
+  static const int fv[6][4] = {{0, 1, 2, 3}, {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {0, 3, 7, 4}, {4, 5, 6, 7}};
+  std::map<std::array<hlong, 4>, int> tagged;
+  for (size_t b = 0; b + 4 < boundaryInfo.size(); b += 5) {
+    std::array<hlong, 4> key{boundaryInfo[b + 1], boundaryInfo[b + 2], boundaryInfo[b + 3], boundaryInfo[b + 4]};
+    std::sort(key.begin(), key.end());
+    tagged[key] = (int)boundaryInfo[b];
+  }
+  std::vector<int> flag(NL, -1);
+  auto onFace = [&](int f, int i, int j, int k) {
+    switch (f) {
+      case 0: return k == 0;
+      case 1: return j == 0;
+      case 2: return i == N;
+      case 3: return j == N;
+      case 4: return i == 0;
+      default: return k == N;
+    }
+  };
+  for (dlong e = 0; e < Nelements; ++e)
+    for (int f = 0; f < 6; ++f) {
+      std::array<hlong, 4> key;
+      for (int v = 0; v < 4; ++v) key[v] = EToV[(size_t)e * 8 + fv[f][v]];
+      std::sort(key.begin(), key.end());
+      auto it = tagged.find(key);
+      if (it == tagged.end() || it->second <= 0) continue;
+      for (int k = 0; k < Nq; ++k)
+        for (int j = 0; j < Nq; ++j)
+          for (int i = 0; i < Nq; ++i)
+            if (onFace(f, i, j, k)) {
+              const size_t r = rep[(size_t)e * Np + i + Nq * (j + Nq * k)];
+              if (flag[r] < 0 || it->second < flag[r]) flag[r] = it->second;
+            }
+    }
+  mapB.malloc(NL);
+  for (size_t n = 0; n < NL; ++n) mapB[n] = flag[rep[n]];
 }
 
 void mesh_t::SetupBox() {
@@ -264,8 +453,13 @@ mesh_t mesh_t::SetupNewDegree(int Nf) {
   m.o_wJ32 = deviceMemory<float>();
   m.o_D32 = deviceMemory<float>();
   m.ReferenceNodes(Nf);
-  m.ConnectNodes();
-  m.GeometricFactors();
+  if (boxMesh) {
+    m.ConnectNodes();
+    m.GeometricFactors();
+  } else {
+    m.GeometricFactors();
+    m.ConnectNodesGeneral();
+  }
   m.GatherScatterSetup();
   return m;
 }

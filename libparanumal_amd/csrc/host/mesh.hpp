@@ -1,6 +1,9 @@
 // Hex box mesh for the MI355X elliptic path (mesh_t subset).
 #pragma once
 
+#include <string>
+#include <vector>
+
 #include "core.hpp"
 
 namespace libp {
@@ -26,6 +29,9 @@ class mesh_t {
   dlong nx = 0, ny = 0, nz = 0;
   int boundaryFlag = 1;
   bool periodic = false;
+  bool boxMesh = true;  // BOX (closed-form numbering) or a mesh file (general numbering)
+  // mesh files: boundary faces as (physical tag, 4 vertex ids) (meshReadGmshHex3D.cpp)
+  std::vector<hlong> boundaryInfo;
   double curveAmp = 0.0;
 
   memory<double> EX, EY, EZ;  // [e][8]
@@ -45,12 +51,14 @@ class mesh_t {
   mesh_t() {}
   mesh_t(platform_t& platform, settings_t& settings, comm_t comm) { Setup(platform, settings, comm); }
 
-  // meshSetup.cpp:31-102 (BOX meshes only)
+  // meshSetup.cpp:31-102 (BOX meshes and Gmsh hexahedral mesh files)
   void Setup(platform_t& platform, settings_t& settings, comm_t comm);
   // meshSetupNewDegree.cpp:32-64
   mesh_t SetupNewDegree(int Nf);
 
   void SetupBox();           // meshSetupBoxHex3D.cpp:31-232
+  void ReadGmshHex3D(const std::string& fileName);  // meshReadGmshHex3D.cpp
+  void ConnectNodesGeneral();  // numbering and boundary flags of a mesh file (see mesh.cpp)
   void ReferenceNodes(int N);
   void ConnectNodes();       // globalIds + mapB in closed form (see mesh.cpp)
   void GeometricFactors();   // physical nodes + ggeo + wJ on the device

@@ -390,3 +390,40 @@ def test_vtu_output_matches_manufactured_solution(tmp_path):
     err = np.abs(fld - exact).max()
     print("vtu points", len(pts), "max |u_h - u| at plot nodes", err, r["iterations"], flush=True)
     assert err < 1e-3
+
+
+# ---------------------------------------------------------------------------
+# Gmsh mesh files
+# ---------------------------------------------------------------------------
+def msh_settings(N, **kw):
+    import os
+    root = os.path.dirname(os.path.abspath(__file__))
+    s = {"MESH FILE": os.path.join(root, "golden", "cubeHex.msh"), "MESH DIMENSION": 3, "ELEMENT TYPE": 12,
+         "POLYNOMIAL DEGREE": N, "LAMBDA": 1.0, "DATA FILE": "data/ellipticSine3D.h",
+         "DISCRETIZATION": "CONTINUOUS", "LINEAR SOLVER": "PCG", "PRECONDITIONER": "JACOBI",
+         "LINEAR SOLVER STOPPING CRITERION": "RELATIVE", "LINEAR SOLVER TOLERANCE": 1e-10}
+    s.update(kw)
+    return s
+
+
+@pytest.mark.parametrize("N,precon", [(3, "JACOBI"), (4, "MULTIGRID")])
+def test_gmsh_cube_matches_box(N, precon):
+    """The reference's test mesh test/cubeHex.msh (read by the Gmsh reader, nodes
+    numbered by the general path) is the uniform 6^3 grid of [-1,1]^3 with every
+    boundary face tagged 1: it must give the BOX solve's dof count and solution
+    norm (element and node orders differ, so the sums differ in rounding only)."""
+    import libparanumal_amd as lp
+    extra = {}
+    if precon == "MULTIGRID":
+        extra = {"LINEAR SOLVER": "FPCG", "MULTIGRID COARSENING": "CUSTOM", "MULTIGRID DEGREES": f"{N},1",
+                 "MULTIGRID PRECISION": "FP32"}
+    m = lp.Elliptic(msh_settings(N, PRECONDITIONER=precon, **extra))
+    b = lp.Elliptic(dict(msh_settings(N, PRECONDITIONER=precon, **extra), **{
+        "MESH FILE": "BOX", "BOX NX": 6, "BOX NY": 6, "BOX NZ": 6, "BOX DIMX": 2, "BOX DIMY": 2, "BOX DIMZ": 2,
+        "BOX BOUNDARY FLAG": 1}))
+    assert m.geti("Nelements") == 216 and m.geti("Ngather") == b.geti("Ngather") == (6 * N - 1) ** 3
+    rm, rb = m.run(), b.run()
+    print("gmsh", N, precon, rm["iterations"], rb["iterations"], rm["norm"], rb["norm"], flush=True)
+    assert rm["r_final"] <= 1e-10 * rm["r0"] * (1 + 1e-12)
+    assert abs(rm["norm"] - rb["norm"]) <= 1e-9 * rb["norm"]
+    assert abs(rm["iterations"] - rb["iterations"]) <= 1

@@ -453,6 +453,11 @@ class coarseJacobiPCG_t {
     std::string graph = "TRUE";
     elliptic->settings.getSetting("MULTIGRID COARSE GRAPH", graph);
     useGraph = graph == "TRUE";
+    // the whole fused solve in one persistent launch (no host reads)
+    std::string persist = "FALSE";
+    elliptic->settings.getSetting("MULTIGRID COARSE PERSISTENT", persist);
+    persistent = fusedHex1 && persist == "TRUE";
+    elliptic->settings.getSetting("MULTIGRID COARSE PERSISTENT BLOCKS", persistentBlocks);
     // the reference's base level: parAlmond AMG on the assembled operator
     // (ellipticPreconMultiGrid.cpp:104-131), one V-cycle per coarse solve
     if (elliptic->settings.compareSetting("MULTIGRID COARSE SOLVER", "AMG")) {
@@ -466,7 +471,11 @@ class coarseJacobiPCG_t {
   }
   bool fusedHex1 = false;
   bool useGraph = true;
+  bool persistent = false;
+  int persistentBlocks = 64;    // 0: one block per CU
+  bool iterationsOnDevice = false;  // lastIterations is still in o_s (persistent solve)
   deviceMemory<double> o_s, o_z, o_p, o_Ap, o_AqL;
+  deviceMemory<unsigned> o_bar;
   // captured chunks of fused iterations, per solution vector and first-iteration flag
   struct chunkGraph {
     double* x;
@@ -476,6 +485,10 @@ class coarseJacobiPCG_t {
   std::vector<chunkGraph> graphs;
   void enqueueChunk(deviceMemory<double>& o_X, int firstIt, int n);
   int solveFusedHex1(deviceMemory<double>& o_X);
+  void solvePersistentHex1(deviceMemory<double>& o_X);
+  void allocFused();
+  // iterations of the last coarse solve (reads them back after a persistent solve)
+  int iterations();
   template <typename T>
   void solve(deviceMemory<T>& o_rhs, deviceMemory<T>& o_X);
 };
@@ -483,19 +496,53 @@ class coarseJacobiPCG_t {
 /* Jacobi-PCG from x = 0 with the N=1 operator fused into three kernels per
  * iteration (libp_pcg_jacobi_hex1_iteration_f64): the same iterates and
  * stopping rule as LinearSolver::pcg with JacobiPrecon on this level. */
+void coarseJacobiPCG_t::allocFused() {
+  if (o_s.ptr()) return;
+  platform_t& p = elliptic->platform;
+  const dlong N = elliptic->Ndofs;
+  const mesh_t& m = elliptic->mesh;
+  o_s = p.malloc<double>(LIBP_PCG_NSCALARS);
+  o_z = p.malloc<double>(N);
+  o_p = p.malloc<double>(N);
+  o_Ap = p.malloc<double>(N);
+  o_AqL = p.malloc<double>((size_t)m.Nelements * m.Np);
+  o_bar = p.malloc<unsigned>(4);
+  LIBP_HIP(hipMemsetAsync(o_bar.ptr(), 0, 4 * sizeof(unsigned), p.stream));
+}
+
+/* The fused iterations of solveFusedHex1 as one persistent kernel
+ * (libp_pcg_jacobi_hex1_solve_f64): the same stopping rule, with every
+ * reduction summed in a fixed order on the device; nothing is read back. */
+void coarseJacobiPCG_t::solvePersistentHex1(deviceMemory<double>& o_X) {
+  platform_t& p = elliptic->platform;
+  allocFused();
+  const mesh_t& m = elliptic->mesh;
+  LIBP_CALL(libp_pcg_jacobi_hex1_solve_f64(
+      m.Nelements, elliptic->o_GlobalToLocal.ptr(), m.o_wJ.ptr(), m.o_ggeo.ptr(), m.o_D.ptr(), elliptic->lambda,
+      elliptic->Ndofs, elliptic->ogsMasked.o_rowStartsT.ptr(), elliptic->ogsMasked.o_colIdsT.ptr(),
+      jacobi->o_invDiagA.ptr(), tol, maxit, persistentBlocks, o_X.ptr(), o_r.ptr(), o_z.ptr(), o_p.ptr(),
+      o_Ap.ptr(), o_AqL.ptr(), o_s.ptr(), o_bar.ptr(), p.o_reduceWS.ptr(), p.s()));
+  iterationsOnDevice = true;
+}
+
+int coarseJacobiPCG_t::iterations() {
+  if (iterationsOnDevice) {
+    platform_t& p = elliptic->platform;
+    unsigned err = 0;
+    LIBP_HIP(hipMemcpyAsync(&err, o_bar.ptr() + 2, sizeof(unsigned), hipMemcpyDeviceToHost, p.stream));
+    lastIterations = (int)p.readScalars(o_s.ptr() + LIBP_PCG_ITERS, 1)[0];
+    LIBP_ABORT("coarse Jacobi-PCG: a grid barrier of the persistent solve timed out", err != 0);
+    iterationsOnDevice = false;
+  }
+  return lastIterations;
+}
+
 int coarseJacobiPCG_t::solveFusedHex1(deviceMemory<double>& o_X) {
   platform_t& p = elliptic->platform;
   void* st = p.s();
   void* ws = p.o_reduceWS.ptr();
   const dlong N = elliptic->Ndofs;
-  const mesh_t& m = elliptic->mesh;
-  if (!o_s.ptr()) {
-    o_s = p.malloc<double>(LIBP_PCG_NSCALARS);
-    o_z = p.malloc<double>(N);
-    o_p = p.malloc<double>(N);
-    o_Ap = p.malloc<double>(N);
-    o_AqL = p.malloc<double>((size_t)m.Nelements * m.Np);
-  }
+  allocFused();
   double* s = o_s.ptr();
   LIBP_CALL(libp_set_f64(N, 0.0, o_X.ptr(), st));
   LIBP_CALL(libp_set_f64(LIBP_PCG_NSCALARS, 0.0, s, st));
@@ -566,6 +613,11 @@ void coarseJacobiPCG_t::solve<double>(deviceMemory<double>& o_rhs, deviceMemory<
     return;
   }
   LIBP_HIP(hipMemcpyAsync(o_r.ptr(), o_rhs.ptr(), N * sizeof(double), hipMemcpyDeviceToDevice, p.stream));
+  iterationsOnDevice = false;
+  if (persistent) {
+    solvePersistentHex1(o_X);
+    return;
+  }
   if (fusedHex1) {
     lastIterations = solveFusedHex1(o_X);
     return;
@@ -578,9 +630,12 @@ void coarseJacobiPCG_t::solve<float>(deviceMemory<float>& o_rhs, deviceMemory<fl
   platform_t& p = elliptic->platform;
   const dlong N = elliptic->Ndofs;
   LIBP_CALL(libp_convert_f32_to_f64(N, o_rhs.ptr(), o_r.ptr(), p.s()));
+  iterationsOnDevice = false;
   if (amg) {
     amg->vcycle(o_r.ptr(), o_x.ptr());
     lastIterations = 1;
+  } else if (persistent) {
+    solvePersistentHex1(o_x);
   } else if (fusedHex1) {
     lastIterations = solveFusedHex1(o_x);
   } else {
@@ -975,8 +1030,8 @@ void multigridApplyFineOperator32(MultiGridPrecon& mg, deviceMemory<float>& x, d
 }
 
 int multigridCoarseIterations(MultiGridPrecon& mg) {
-  if (auto* m = dynamic_cast<multigrid_t<float>*>(mg.impl.get())) return m->coarse->lastIterations;
-  if (auto* m = dynamic_cast<multigrid_t<double>*>(mg.impl.get())) return m->coarse->lastIterations;
+  if (auto* m = dynamic_cast<multigrid_t<float>*>(mg.impl.get())) return m->coarse->iterations();
+  if (auto* m = dynamic_cast<multigrid_t<double>*>(mg.impl.get())) return m->coarse->iterations();
   return -1;
 }
 

@@ -153,12 +153,20 @@ int libp_pcg_jacobi_rdotz_f64(int N, const double* invDiag, const double* r, dou
  * ellipticOperator.cpp:29-107). Used for the p-MG coarse level. Before the
  * first call: scalars zeroed, scalars[TOL] set, z = invDiag.*r and
  * scalars[RDOTZ1] = r.z (libp_pcg_jacobi_rdotz_f64). first = 1 on the first
- * iteration. AqL holds Nelements*8 doubles. Stops itself via scalars[STOP]. */
+ * iteration. AqL holds Nelements*8 doubles. Each reduction is finished by
+ * the next launch, so the stopping test of an iteration runs at the start of
+ * the next one (then every launch returns at once via scalars[STOP]); call
+ * libp_pcg_jacobi_hex1_check_f64 after the last queued iteration before
+ * reading scalars[STOP], scalars[RDOTR] or scalars[ITERS]. ws must not be
+ * used by other reductions between the calls. */
 int libp_pcg_jacobi_hex1_iteration_f64(int Nelements, const int* GlobalToLocal, const double* wJ, const double* ggeo,
                                        const double* D, double lambda, int Nrows, const int* rowStarts,
                                        const int* colIds, const double* invDiag, int first, double* x, double* r,
                                        double* z, double* p, double* Ap, double* AqL, double* scalars, void* ws,
                                        void* stream);
+/* Stopping test of the last libp_pcg_jacobi_hex1_iteration_f64: sets
+ * scalars[RDOTR] = r.r and, when r.r <= scalars[TOL], scalars[STOP] = 1. */
+int libp_pcg_jacobi_hex1_check_f64(int Nrows, double* scalars, void* ws, void* stream);
 
 /* The whole fused Jacobi-PCG of libp_pcg_jacobi_hex1_iteration_f64 in one
  * launch (the same counterpart: linearSolverPCG.cpp:107-171 with

@@ -582,6 +582,8 @@ void coarseJacobiPCG_t::enqueueChunk(deviceMemory<double>& o_X, int firstIt, int
           m.Nelements, elliptic->o_GlobalToLocal.ptr(), m.o_wJ.ptr(), m.o_ggeo.ptr(), m.o_D.ptr(), elliptic->lambda,
           N, elliptic->ogsMasked.o_rowStartsT.ptr(), elliptic->ogsMasked.o_colIdsT.ptr(), jacobi->o_invDiagA.ptr(),
           firstIt && c == 0, o_X.ptr(), o_r.ptr(), o_z.ptr(), o_p.ptr(), o_Ap.ptr(), o_AqL.ptr(), o_s.ptr(), ws, st));
+    // the stopping test of the last iteration, before the host reads the flag
+    LIBP_CALL(libp_pcg_jacobi_hex1_check_f64(N, o_s.ptr(), ws, st));
   };
   if (!useGraph) {
     launch();

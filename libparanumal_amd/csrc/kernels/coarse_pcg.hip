@@ -4,14 +4,23 @@
 // The coarse level (8 nodes per element, ~3e4 rows on a 32^3 box) is
 // latency-bound: the six launches of a plain PCG iteration (z and r.z, p
 // update, Ax, gather, p.Ap, x/r update) cost more than their traffic. Here:
-//   A: AqL = A_e (z + beta p) per element, the p update folded into the read
-//   B: Ap = gather(AqL); p = z + beta p; p.Ap -> alpha        (one reduction)
-//   C: x += alpha p; r -= alpha Ap; z = invD r; r.r and r.z -> beta, STOP
-// The arithmetic of each step is that of linearSolverPCG.cpp:107-171 with
-// the Jacobi preconditioner (ellipticPreconJacobi.cpp:42-51) and of the
-// partial Ax (ellipticAxHex3D.okl:156-295) at N=1. Every kernel returns at
-// once when the STOP flag of the scalar block is set, so iterations can be
-// queued in chunks (see libp_pcg_update_f64).
+//   A: r.r, r.z <- partials of C; stop test; beta;
+//      AqL = A_e (z + beta p) per element, the p update folded into the read
+//   B: Ap = gather(AqL); p = z + beta p; partials of p.Ap
+//   C: p.Ap <- partials of B; alpha; x += alpha p; r -= alpha Ap; z = invD r;
+//      partials of r.r and r.z
+// A reduction is finished by the kernel that needs it: every block of that
+// kernel sums all partials of the previous kernel in the same fixed order, so
+// all blocks hold bit-identical scalars, and the kernel boundary makes the
+// partials visible. No block fences, no tickets: on this GPU an agent-scope
+// fence writes back the L2, which in these small kernels cost more than the
+// work. Block 0 publishes the scalars for the host. The arithmetic of each
+// step is that of linearSolverPCG.cpp:107-171 with the Jacobi preconditioner
+// (ellipticPreconJacobi.cpp:42-51) and of the partial Ax
+// (ellipticAxHex3D.okl:156-295) at N=1. Every kernel returns at once when
+// the STOP flag of the scalar block is set, so iterations can be queued in
+// chunks; coarse_check_kernel applies the stopping test to the last queued
+// iteration before the host reads the flag.
 #include <algorithm>
 
 #include "reduce.hpp"
@@ -76,19 +85,86 @@ __device__ inline double hex1_apply(const Hex1Node& v, const int n, const double
   return a;
 }
 
+// block count of kernels B and C (and of their partials)
+inline int coarse_row_blocks(const int Nrows) { return std::min((Nrows + RBS - 1) / RBS, RMAXB); }
+
+// sum over the nb partials of one slot, in a fixed order, by the first wavefront
+// (valid in lane 0 of wave 0)
+__device__ inline double coarse_sum_slot(const ReduceWS* __restrict__ ws, const int slot, const int nb) {
+  double a = 0;
+  for (int b = threadIdx.x; b < nb; b += 64) a += ws->part[slot][b];
+  return wave_sum(a);
+}
+
 __global__ void __launch_bounds__(256) coarse_ax_hex1_kernel(const int E, const int* __restrict__ G2L,
                                                              const double* __restrict__ wJ,
                                                              const double* __restrict__ ggeo,
                                                              const double* __restrict__ D, const double lambda,
-                                                             const int first, const double* __restrict__ z,
-                                                             const double* __restrict__ p,
-                                                             const double* __restrict__ s, double* __restrict__ AqL) {
+                                                             const int first, const int nb,
+                                                             const double* __restrict__ z,
+                                                             const double* __restrict__ p, double* s,
+                                                             const ReduceWS* __restrict__ ws,
+                                                             double* __restrict__ AqL) {
+  __shared__ double s_beta;
+  __shared__ int s_stop;
   if (s[8] != 0.0) return;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int e = t >> 3, n = t & 7;
   const bool live = e < E;  // every lane takes part in the shuffles
+  // loads that do not depend on beta go first
   Hex1Node v{};
-  if (live) v = hex1_load(e, n, G2L, wJ, ggeo, z, p, first ? 0.0 : s[3], !first);
+  double zg = 0, pg = 0;
+  if (live) {
+    const int g = G2L[(size_t)e * 8 + n];
+    if (g >= 0) {
+      zg = z[g];
+      if (!first) pg = p[g];
+    }
+    const double* G = ggeo + (size_t)e * 48 + n;
+    v.G00 = G[0];
+    v.G01 = G[8];
+    v.G02 = G[16];
+    v.G11 = G[24];
+    v.G12 = G[32];
+    v.G22 = G[40];
+    v.W = wJ[(size_t)e * 8 + n];
+  }
+  if (threadIdx.x < 64) {
+    double rr = 0, rz = 0;
+    if (!first) {
+      rr = coarse_sum_slot(ws, 1, nb);
+      rz = coarse_sum_slot(ws, 2, nb);
+    }
+    if (threadIdx.x == 0) {
+      int stop = 0;
+      double beta = 0.0;
+      if (!first) {
+        if (s[7] > 0.0 && rr <= s[7])
+          stop = 1;
+        else
+          beta = rz / s[0];  // linearSolverPCG.cpp:116-124
+      }
+      s_stop = stop;
+      s_beta = beta;
+      if (blockIdx.x == 0) {
+        if (first) {
+          s[3] = 0.0;
+          s[1] = s[0];
+        } else {
+          s[5] = rr;
+          if (stop) {
+            s[8] = 1.0;
+          } else {
+            s[3] = beta;
+            s[1] = rz;
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (s_stop) return;  // the same decision in every block
+  v.q = first ? zg : zg + s_beta * pg;
   const double a = hex1_apply(v, n, D, lambda);
   if (live) AqL[(size_t)e * 8 + n] = a;
 }
@@ -98,7 +174,7 @@ __global__ void __launch_bounds__(RBS) coarse_gather_pap_kernel(const int Nrows,
                                                                 const double* __restrict__ AqL, const int first,
                                                                 const double* __restrict__ z,
                                                                 double* __restrict__ p, double* __restrict__ Ap,
-                                                                double* __restrict__ s, ReduceWS* __restrict__ ws) {
+                                                                double* s, ReduceWS* __restrict__ ws) {
   __shared__ double s_w[16];
   if (s[8] != 0.0) return;
   const double beta = first ? 0.0 : s[3];
@@ -124,29 +200,56 @@ __global__ void __launch_bounds__(RBS) coarse_gather_pap_kernel(const int Nrows,
     Ap[g] = ap;
     acc += pn * ap;
   }
-  double v[1] = {acc};
-  bool last;
-  finish_partials<1>(ws, v, s_w, last);
-  if (last) {
-    double out[1];
-    sum_partials<1>(ws, out);
-    if (threadIdx.x == 0) {
-      s[4] = out[0];
-      s[2] = s[0] / out[0];
-    }
+  const double bs = block_sum(acc, s_w);
+  if (threadIdx.x == 0) {
+    ws->part[0][blockIdx.x] = bs;
+    if (blockIdx.x == 0) s[0] = s[1];  // r.z of this iteration, for alpha
   }
 }
 
-__global__ void __launch_bounds__(RBS) coarse_update_kernel(const int N, const double* __restrict__ invD,
+__global__ void __launch_bounds__(RBS) coarse_update_kernel(const int N, const int nb, const double* __restrict__ invD,
                                                             const double* __restrict__ p,
                                                             const double* __restrict__ Ap, double* __restrict__ x,
-                                                            double* __restrict__ r, double* __restrict__ z,
-                                                            double* __restrict__ s, ReduceWS* __restrict__ ws) {
+                                                            double* __restrict__ r, double* __restrict__ z, double* s,
+                                                            ReduceWS* __restrict__ ws) {
   __shared__ double s_w[16];
+  __shared__ double s_alpha;
   if (s[8] != 0.0) return;
-  const double alpha = s[2];
+  const int stride = gridDim.x * blockDim.x;
+  const int n0 = blockIdx.x * blockDim.x + threadIdx.x;
+  // the first row's loads go before the p.Ap sum
+  double x0 = 0, p0 = 0, r0 = 0, a0v = 0, d0 = 0;
+  if (n0 < N) {
+    x0 = x[n0];
+    p0 = p[n0];
+    r0 = r[n0];
+    a0v = Ap[n0];
+    d0 = invD[n0];
+  }
+  if (threadIdx.x < 64) {
+    const double pap = coarse_sum_slot(ws, 0, nb);
+    if (threadIdx.x == 0) {
+      const double alpha = s[0] / pap;
+      s_alpha = alpha;
+      if (blockIdx.x == 0) {
+        s[4] = pap;
+        s[2] = alpha;
+      }
+    }
+  }
+  __syncthreads();
+  const double alpha = s_alpha;
   double a0 = 0, a1 = 0;
-  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
+  if (n0 < N) {
+    x[n0] = x0 + alpha * p0;
+    const double rn = r0 - alpha * a0v;
+    r[n0] = rn;
+    const double zn = 1.0 * d0 * rn;
+    z[n0] = zn;
+    a0 += rn * rn;
+    a1 += rn * zn;
+  }
+  for (int n = n0 + stride; n < N; n += stride) {
     x[n] += alpha * p[n];
     const double rn = r[n] - alpha * Ap[n];
     r[n] = rn;
@@ -155,20 +258,22 @@ __global__ void __launch_bounds__(RBS) coarse_update_kernel(const int N, const d
     a0 += rn * rn;
     a1 += rn * zn;
   }
-  double v[2] = {a0, a1};
-  bool last;
-  finish_partials<2>(ws, v, s_w, last);
-  if (last) {
-    double out[2];
-    sum_partials<2>(ws, out);
-    if (threadIdx.x == 0) {
-      s[5] = out[0];
-      s[9] += 1.0;
-      if (s[7] > 0.0 && out[0] <= s[7]) s[8] = 1.0;
-      s[1] = s[0];
-      s[0] = out[1];
-      s[3] = out[1] / s[1];
-    }
+  const double b0 = block_sum(a0, s_w);
+  const double b1 = block_sum(a1, s_w);
+  if (threadIdx.x == 0) {
+    ws->part[1][blockIdx.x] = b0;
+    ws->part[2][blockIdx.x] = b1;
+    if (blockIdx.x == 0) s[9] += 1.0;
+  }
+}
+
+// stopping test on the partials of the last queued iteration (one wavefront)
+__global__ void __launch_bounds__(64) coarse_check_kernel(const int nb, double* s, const ReduceWS* __restrict__ ws) {
+  if (s[8] != 0.0) return;
+  const double rr = coarse_sum_slot(ws, 1, nb);
+  if (threadIdx.x == 0) {
+    s[5] = rr;
+    if (s[7] > 0.0 && rr <= s[7]) s[8] = 1.0;
   }
 }
 
@@ -380,15 +485,23 @@ extern "C" int libp_pcg_jacobi_hex1_iteration_f64(int Nelements, const int* Glob
       !AqL || !scalars || !ws)
     return arg_error("pcg_jacobi_hex1_iteration", "null pointer");
   hipStream_t st = as_stream(stream);
-  hipLaunchKernelGGL(coarse_ax_hex1_kernel, dim3((8 * (long)Nelements + 255) / 256), dim3(256), 0, st, Nelements,
-                     GlobalToLocal, wJ, ggeo, D, lambda, first, z, p, scalars, AqL);
   // one row per thread: these vectors are small and the kernels latency-bound
-  const unsigned blocks = (unsigned)std::min<long>(((long)Nrows + RBS - 1) / RBS, RMAXB);
-  hipLaunchKernelGGL(coarse_gather_pap_kernel, dim3(blocks), dim3(RBS), 0, st, Nrows, rowStarts, colIds, AqL, first, z,
+  const int nb = coarse_row_blocks(Nrows);
+  hipLaunchKernelGGL(coarse_ax_hex1_kernel, dim3((8 * (long)Nelements + 255) / 256), dim3(256), 0, st, Nelements,
+                     GlobalToLocal, wJ, ggeo, D, lambda, first, nb, z, p, scalars, (const ReduceWS*)ws, AqL);
+  hipLaunchKernelGGL(coarse_gather_pap_kernel, dim3(nb), dim3(RBS), 0, st, Nrows, rowStarts, colIds, AqL, first, z,
                      p, Ap, scalars, (ReduceWS*)ws);
-  hipLaunchKernelGGL(coarse_update_kernel, dim3(blocks), dim3(RBS), 0, st, Nrows, invDiag, p, Ap, x, r, z, scalars,
+  hipLaunchKernelGGL(coarse_update_kernel, dim3(nb), dim3(RBS), 0, st, Nrows, nb, invDiag, p, Ap, x, r, z, scalars,
                      (ReduceWS*)ws);
   return launch_status("pcg_jacobi_hex1_iteration");
+}
+
+extern "C" int libp_pcg_jacobi_hex1_check_f64(int Nrows, double* scalars, void* ws, void* stream) {
+  if (Nrows <= 0) return LIBP_OK;
+  if (!scalars || !ws) return arg_error("pcg_jacobi_hex1_check", "null pointer");
+  hipLaunchKernelGGL(coarse_check_kernel, dim3(1), dim3(64), 0, as_stream(stream), coarse_row_blocks(Nrows), scalars,
+                     (const ReduceWS*)ws);
+  return launch_status("pcg_jacobi_hex1_check");
 }
 
 extern "C" int libp_pcg_jacobi_hex1_solve_f64(int Nelements, const int* GlobalToLocal, const double* wJ,

@@ -141,6 +141,19 @@ int libp_pcg_rdotz_f64(int N, const double* r, const double* z, const double* Ap
 int libp_pcg_update_p_f64(int N, int first_iter, const double* z, double* p, const double* scalars, void* stream);
 /* p.Ap into scalars[PAP]. */
 int libp_pcg_pap_f64(int N, const double* p, const double* Ap, double* scalars, void* ws, void* stream);
+/* The gather of the operator fused with p.Ap (ellipticOperator.cpp:29-107's
+ * gather followed by linearSolverPCG.cpp:134-140): gv = gather(v) over the
+ * CSR rows, bit-identical to libp_ogs_gather_f64, and scalars[PAP] = p.gv,
+ * scalars[ALPHA] = scalars[RDOTZ1] / p.gv. */
+int libp_pcg_gather_pap_f64(int Nrows, const int* rowStarts, const int* colIds, const double* v, double* gv,
+                            const double* p, double* scalars, void* ws, void* stream);
+/* The update of libp_pcg_update_f64 fused with the Jacobi preconditioner and
+ * the r.z of the next iteration (linearSolverPCG.cpp:107-124,142-160 with
+ * ellipticPreconJacobi.cpp:42-51): x += alpha p, r -= alpha Ap,
+ * z = invDiag.*r, scalars[RDOTR] = r.r, ITERS and STOP as in
+ * libp_pcg_update_f64, then RDOTZ2 <- RDOTZ1 <- r.z and BETA = r.z / RDOTZ2. */
+int libp_pcg_jacobi_update_f64(int N, const double* invDiag, const double* p, const double* Ap, double* x, double* r,
+                               double* z, double* scalars, void* ws, void* stream);
 /* Jacobi preconditioner z = invDiag.*r fused with r.z (JacobiPrecon::Operator,
  * solvers/elliptic/src/ellipticPreconJacobi.cpp:42-51 followed by the r.z dot). */
 int libp_pcg_jacobi_rdotz_f64(int N, const double* invDiag, const double* r, double* z, const double* Ap,

@@ -144,6 +144,9 @@ settings_t ellipticDefaultSettings() {
                {"TRUE", "FALSE"});
   s.newSetting("MULTIGRID COARSE GRAPH", "TRUE", "Replay the fused coarse iterations as captured HIP graphs",
                {"TRUE", "FALSE"});
+  s.newSetting("PCG JACOBI FUSED", "TRUE",
+               "Jacobi-PCG in fused launches: p.Ap in the operator's gather, z = invDiag.*r and r.z in the update",
+               {"TRUE", "FALSE"});
   s.newSetting("MULTIGRID COARSE PERSISTENT", "FALSE",
                "Fused coarse Jacobi-PCG as one persistent kernel with grid barriers (no host reads); slower "
                "than the replayed graphs on a 32^3 box (31.5 against 30.0 ms per p-MG solve)",

@@ -108,6 +108,13 @@ void elliptic_t::Operator(deviceMemory<dfloat>& o_q, deviceMemory<dfloat>& o_Aq)
   ogsMasked.Gather(o_Aq, o_AqL, 1, ogs::Add, ogs::Trans);
 }
 
+bool elliptic_t::OperatorPAP(deviceMemory<dfloat>& o_q, deviceMemory<dfloat>& o_Aq, double* scalars, void* ws) {
+  PartialAx(o_q, o_AqL);
+  LIBP_CALL(libp_pcg_gather_pap_f64(ogsMasked.NlocalT, ogsMasked.o_rowStartsT.ptr(), ogsMasked.o_colIdsT.ptr(),
+                                    o_AqL.ptr(), o_Aq.ptr(), o_q.ptr(), scalars, ws, platform.s()));
+  return true;
+}
+
 void elliptic_t::OperatorF32(deviceMemory<float>& o_q, deviceMemory<float>& o_Aq) {
   mesh.MakeFP32();
   if (!o_AqL32.isInitialized()) o_AqL32 = platform.malloc<float>((size_t)mesh.Nelements * mesh.Np);

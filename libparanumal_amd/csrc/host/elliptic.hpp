@@ -61,6 +61,8 @@ class elliptic_t : public operator_t {
 
   // ellipticOperator.cpp:29-107 (single launch over all local elements)
   void Operator(deviceMemory<dfloat>& o_q, deviceMemory<dfloat>& o_Aq) override;
+  // the operator's gather fused with the PCG p.Ap (libp_pcg_gather_pap_f64)
+  bool OperatorPAP(deviceMemory<dfloat>& o_q, deviceMemory<dfloat>& o_Aq, double* scalars, void* ws) override;
   void PartialAx(deviceMemory<dfloat>& o_q, deviceMemory<dfloat>& o_AqL);
   // FP32 operator used by the p-MG smoother (no reference counterpart)
   void OperatorF32(deviceMemory<float>& o_q, deviceMemory<float>& o_Aq);

@@ -11,6 +11,7 @@ namespace LinearSolver {
 pcg::pcg(dlong _N, dlong _Nhalo, platform_t& _platform, settings_t& _settings, comm_t _comm)
     : linearSolverBase_t(_N, _Nhalo, _platform, _settings, _comm) {
   flexible = settings.compareSetting("LINEAR SOLVER", "FPCG");
+  fuseJacobi = !settings.compareSetting("PCG JACOBI FUSED", "FALSE");
   const dlong Ntotal = N + Nhalo;
   o_p = platform.malloc<dfloat>(Ntotal);
   o_z = platform.malloc<dfloat>(Ntotal);
@@ -22,6 +23,46 @@ pcg::pcg(dlong _N, dlong _Nhalo, platform_t& _platform, settings_t& _settings, c
   LIBP_CALL(libp_set_f64(Ntotal, 0.0, o_z.ptr(), platform.s()));
   LIBP_CALL(libp_set_f64(Ntotal, 0.0, o_Ax.ptr(), platform.s()));
   LIBP_CALL(libp_set_f64(Ntotal, 0.0, o_Ap.ptr(), platform.s()));
+}
+
+/* One PCG iteration (linearSolverPCG.cpp:107-171). With a Jacobi
+ * preconditioner and a plain (not flexible) iteration the launches fuse:
+ *   p = z + beta p; Ap = A p with p.Ap and alpha in the gather
+ *   (operator_t::OperatorPAP); x, r updated with z = invDiag.*r, r.r, r.z and
+ *   beta for the next iteration (libp_pcg_jacobi_update_f64).
+ * The first iteration forms z and r.z of the initial residual first. The
+ * arithmetic of every entry is that of the separate kernels; only the
+ * partitions of the reductions differ. */
+void pcg::iteration(operator_t& linearOperator, operator_t& precon, diagonalPrecon_t* jac, int it,
+                    deviceMemory<dfloat>& o_x, deviceMemory<dfloat>& o_r) {
+  void* st = platform.s();
+  void* ws = platform.o_reduceWS.ptr();
+  double* s = o_s.ptr();
+  const bool fused = jac && !flexible && fuseJacobi;
+  if (fused) {
+    if (it == 0)
+      LIBP_CALL(libp_pcg_jacobi_rdotz_f64(N, jac->invDiagonal().ptr(), o_r.ptr(), o_z.ptr(), o_Ap.ptr(), 0, s, ws,
+                                          st));
+    LIBP_CALL(libp_pcg_update_p_f64(N, it == 0, o_z.ptr(), o_p.ptr(), s, st));
+    if (!linearOperator.OperatorPAP(o_p, o_Ap, s, ws)) {
+      linearOperator.Operator(o_p, o_Ap);
+      LIBP_CALL(libp_pcg_pap_f64(N, o_p.ptr(), o_Ap.ptr(), s, ws, st));
+    }
+    LIBP_CALL(libp_pcg_jacobi_update_f64(N, jac->invDiagonal().ptr(), o_p.ptr(), o_Ap.ptr(), o_x.ptr(), o_r.ptr(),
+                                         o_z.ptr(), s, ws, st));
+    return;
+  }
+  if (jac) {
+    LIBP_CALL(libp_pcg_jacobi_rdotz_f64(N, jac->invDiagonal().ptr(), o_r.ptr(), o_z.ptr(), o_Ap.ptr(), flexible, s,
+                                        ws, st));
+  } else {
+    precon.Operator(o_r, o_z);
+    LIBP_CALL(libp_pcg_rdotz_f64(N, o_r.ptr(), o_z.ptr(), o_Ap.ptr(), flexible, s, ws, st));
+  }
+  LIBP_CALL(libp_pcg_update_p_f64(N, it == 0, o_z.ptr(), o_p.ptr(), s, st));
+  linearOperator.Operator(o_p, o_Ap);
+  LIBP_CALL(libp_pcg_pap_f64(N, o_p.ptr(), o_Ap.ptr(), s, ws, st));
+  LIBP_CALL(libp_pcg_update_f64(N, o_p.ptr(), o_Ap.ptr(), o_x.ptr(), o_r.ptr(), s, ws, st));
 }
 
 int pcg::Solve(operator_t& linearOperator, operator_t& precon, deviceMemory<dfloat>& o_x, deviceMemory<dfloat>& o_r,
@@ -51,7 +92,11 @@ int pcg::Solve(operator_t& linearOperator, operator_t& precon, deviceMemory<dflo
   if (verbose) printf("PCG: initial res norm %12.12f \n", std::sqrt(rdotr0));
 
   precon.solveStarting();
-  diagonalPrecon_t* jac = dynamic_cast<diagonalPrecon_t*>(&precon);
+  // a diagonal preconditioner, also behind the precon_t wrapper that elliptic_t holds
+  operator_t* inner = &precon;
+  if (auto* w = dynamic_cast<precon_t*>(&precon))
+    if (w->get()) inner = w->get();
+  diagonalPrecon_t* jac = dynamic_cast<diagonalPrecon_t*>(inner);
   if (jac && jac->needsZeroMean()) jac = nullptr;
 
   if (checkEvery > 1) {
@@ -64,17 +109,7 @@ int pcg::Solve(operator_t& linearOperator, operator_t& precon, deviceMemory<dflo
         const int chunk = std::min(checkEvery, MAXIT - iter);
         for (int c = 0; c < chunk; ++c) {
           const int it = iter + c;
-          if (jac) {
-            LIBP_CALL(libp_pcg_jacobi_rdotz_f64(N, jac->invDiagonal().ptr(), o_r.ptr(), o_z.ptr(), o_Ap.ptr(),
-                                                flexible, s, ws, st));
-          } else {
-            precon.Operator(o_r, o_z);
-            LIBP_CALL(libp_pcg_rdotz_f64(N, o_r.ptr(), o_z.ptr(), o_Ap.ptr(), flexible, s, ws, st));
-          }
-          LIBP_CALL(libp_pcg_update_p_f64(N, it == 0, o_z.ptr(), o_p.ptr(), s, st));
-          linearOperator.Operator(o_p, o_Ap);
-          LIBP_CALL(libp_pcg_pap_f64(N, o_p.ptr(), o_Ap.ptr(), s, ws, st));
-          LIBP_CALL(libp_pcg_update_f64(N, o_p.ptr(), o_Ap.ptr(), o_x.ptr(), o_r.ptr(), s, ws, st));
+          iteration(linearOperator, precon, jac, it, o_x, o_r);
         }
         const double* h = platform.readScalars(s, LIBP_PCG_NSCALARS);
         rdotr0 = h[LIBP_PCG_RDOTR];
@@ -90,17 +125,7 @@ int pcg::Solve(operator_t& linearOperator, operator_t& precon, deviceMemory<dflo
   int iter;
   for (iter = 0; iter < MAXIT; ++iter) {
     if (((iter == 0) && (rdotr0 == 0.0)) || ((iter > 0) && (rdotr0 <= TOL))) break;
-    if (jac) {
-      LIBP_CALL(libp_pcg_jacobi_rdotz_f64(N, jac->invDiagonal().ptr(), o_r.ptr(), o_z.ptr(), o_Ap.ptr(), flexible, s,
-                                          ws, st));
-    } else {
-      precon.Operator(o_r, o_z);
-      LIBP_CALL(libp_pcg_rdotz_f64(N, o_r.ptr(), o_z.ptr(), o_Ap.ptr(), flexible, s, ws, st));
-    }
-    LIBP_CALL(libp_pcg_update_p_f64(N, iter == 0, o_z.ptr(), o_p.ptr(), s, st));
-    linearOperator.Operator(o_p, o_Ap);
-    LIBP_CALL(libp_pcg_pap_f64(N, o_p.ptr(), o_Ap.ptr(), s, ws, st));
-    LIBP_CALL(libp_pcg_update_f64(N, o_p.ptr(), o_Ap.ptr(), o_x.ptr(), o_r.ptr(), s, ws, st));
+    iteration(linearOperator, precon, jac, iter, o_x, o_r);
     const double* h = platform.readScalars(s, LIBP_PCG_NSCALARS);
     rdotr0 = h[LIBP_PCG_RDOTR];
     precon.iterationDone(iter, rdotr0);

@@ -11,6 +11,17 @@ namespace libp {
 class operator_t {
  public:
   virtual void Operator(deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_Mr) = 0;
+  // Aq = A q together with the PCG scalars scalars[PAP] = q.Aq and
+  // scalars[ALPHA] = scalars[RDOTZ1] / q.Aq on the device (libp_pcg_pap_f64).
+  // Returns false when the operator has no fused form; the solver then runs
+  // Operator and the separate reduction.
+  virtual bool OperatorPAP(deviceMemory<dfloat>& o_q, deviceMemory<dfloat>& o_Aq, double* scalars, void* ws) {
+    (void)o_q;
+    (void)o_Aq;
+    (void)scalars;
+    (void)ws;
+    return false;
+  }
   // Solver notifications (no-ops by default): a solve starts; iteration `it`
   // ended with residual norm^2 rdotr (already on the host).
   virtual void solveStarting() {}
@@ -106,8 +117,11 @@ namespace LinearSolver {
  * LINEAR SOLVER setting contains "FPCG" (linearSolverPCG.cpp:43). */
 class pcg : public linearSolverBase_t {
   bool flexible = false;
+  bool fuseJacobi = true;  // [PCG JACOBI FUSED]: three fused launches per Jacobi-PCG iteration
   deviceMemory<dfloat> o_p, o_z, o_Ax, o_Ap;
   deviceMemory<dfloat> o_s;  // LIBP_PCG_NSCALARS device scalars
+  void iteration(operator_t& linearOperator, operator_t& precon, diagonalPrecon_t* jac, int it,
+                 deviceMemory<dfloat>& o_x, deviceMemory<dfloat>& o_r);
 
  public:
   pcg(dlong N, dlong Nhalo, platform_t& platform, settings_t& settings, comm_t comm);

@@ -182,6 +182,48 @@ __global__ void __launch_bounds__(RBS) pcg_update_kernel(const int N, const doub
   }
 }
 
+// x += alpha p; r -= alpha Ap; z = invDiag.*r; r.r and r.z in one pass: the
+// update of linearSolverPCG.cpp:142-160 fused with the Jacobi preconditioner
+// and the r.z of the next iteration (:107-124). The last block writes r.r,
+// counts the iteration, raises STOP once r.r <= TOL, shifts r.z (RDOTZ2 <-
+// RDOTZ1 <- r.z) and sets beta = r.z / old r.z.
+__global__ void __launch_bounds__(RBS) pcg_jacobi_update_kernel(const int N, const double* __restrict__ invDiag,
+                                                                const double* __restrict__ p,
+                                                                const double* __restrict__ Ap,
+                                                                double* __restrict__ x, double* __restrict__ r,
+                                                                double* __restrict__ z, double* __restrict__ s,
+                                                                ReduceWS* __restrict__ ws) {
+  __shared__ double s_w[16];
+  if (s[8] != 0.0) return;
+  const double alpha = s[2];
+  double a0 = 0, a1 = 0;
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
+    double rn = r[n];
+    x[n] += alpha * p[n];
+    rn -= alpha * Ap[n];
+    r[n] = rn;
+    const double zn = 1.0 * invDiag[n] * rn;
+    z[n] = zn;
+    a0 += rn * rn;
+    a1 += rn * zn;
+  }
+  double v[2] = {a0, a1};
+  bool last;
+  finish_partials<2>(ws, v, s_w, last);
+  if (last) {
+    double out[2];
+    sum_partials<2>(ws, out);
+    if (threadIdx.x == 0) {
+      s[5] = out[0];
+      s[9] += 1.0;
+      if (s[7] > 0.0 && out[0] <= s[7]) s[8] = 1.0;
+      s[1] = s[0];
+      s[0] = out[1];
+      s[3] = out[1] / s[1];
+    }
+  }
+}
+
 /* ------------------------------ Chebyshev ------------------------------ */
 // RES = invDiag.*(r - Ax) [Ax==NULL: invDiag.*r] ; d = invTheta*RES ; X = d (xIsZero) or X += d
 template <typename T>
@@ -281,6 +323,13 @@ extern "C" int libp_pcg_jacobi_rdotz_f64(int N, const double* invDiag, const dou
   hipLaunchKernelGGL(pcg_rdotz_kernel<true>, dim3(reduce_blocks(N)), dim3(RBS), 0, as_stream(stream), N, invDiag, r,
                      z, Ap, flexible, scalars, (ReduceWS*)ws);
   return launch_status("pcg_jacobi_rdotz");
+}
+extern "C" int libp_pcg_jacobi_update_f64(int N, const double* invDiag, const double* p, const double* Ap,
+                                          double* x, double* r, double* z, double* scalars, void* ws, void* stream) {
+  if (!invDiag || !p || !Ap || !x || !r || !z || !scalars || !ws) return arg_error("pcg_jacobi_update", "null pointer");
+  hipLaunchKernelGGL(pcg_jacobi_update_kernel, dim3(reduce_blocks(N)), dim3(RBS), 0, as_stream(stream), N, invDiag, p,
+                     Ap, x, r, z, scalars, (ReduceWS*)ws);
+  return launch_status("pcg_jacobi_update");
 }
 extern "C" int libp_pcg_update_p_f64(int N, int first_iter, const double* z, double* p, const double* scalars,
                                      void* stream) {

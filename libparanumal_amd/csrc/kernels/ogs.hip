@@ -8,7 +8,9 @@
 // lie in different elements, and a loop of dependent loads would leave the
 // memory system idle. Sums still run from zero in ascending column order,
 // which is the reference's order, so Gather results are bit-identical.
-#include "kcommon.hpp"
+#include <algorithm>
+
+#include "reduce.hpp"
 
 namespace libp_amd {
 
@@ -50,6 +52,78 @@ __global__ void __launch_bounds__(256) ogs_gather_kernel(const int Nrows, const 
         if (k < len[r]) val += x[r][k];
       for (int g = a[r] + 8; g < a[r] + len[r]; ++g) val += v[colIds[g]];
       gv[row] = val;
+    }
+  }
+}
+
+// Gather fused with the p.Ap reduction of a PCG iteration (FP64): gv = gather(v)
+// with the arithmetic of ogs_gather_kernel, and sum_row y[row] gv[row]. Each
+// block stores one partial in a flat array of up to RNPART*RMAXB entries (the
+// ReduceWS partials seen as one buffer); the last block to finish sums them in
+// block order and writes scalars[PAP] and scalars[ALPHA] = scalars[RDOTZ1] /
+// p.Ap (linearSolverPCG.cpp:134-140). Returns at once when scalars[STOP] is set.
+constexpr int GATHER_PAP_MAXB = RNPART * RMAXB;
+
+__global__ void __launch_bounds__(256) ogs_gather_pap_kernel(const int Nrows, const int* __restrict__ rowStarts,
+                                                             const int* __restrict__ colIds,
+                                                             const double* __restrict__ v, double* __restrict__ gv,
+                                                             const double* __restrict__ y, double* __restrict__ s,
+                                                             ReduceWS* __restrict__ ws) {
+  __shared__ double s_w[16];
+  __shared__ bool s_last;
+  if (s[8] != 0.0) return;
+  const long S = (long)gridDim.x * blockDim.x;
+  double acc = 0;
+  for (long row0 = blockIdx.x * (long)blockDim.x + threadIdx.x; row0 < Nrows; row0 += 2 * S) {
+    int a[2], len[2];
+    double yv[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const long row = row0 + r * S;
+      a[r] = (row < Nrows) ? rowStarts[row] : 0;
+      len[r] = (row < Nrows) ? rowStarts[row + 1] - a[r] : 0;
+      yv[r] = (row < Nrows) ? y[row] : 0.0;
+    }
+    int c[2][8];
+    double x[2][8];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) c[r][k] = (k < len[r]) ? colIds[a[r] + k] : 0;
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) x[r][k] = (k < len[r]) ? v[c[r][k]] : 0.0;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const long row = row0 + r * S;
+      if (row >= Nrows) continue;
+      double val = 0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (k < len[r]) val += x[r][k];
+      for (int g = a[r] + 8; g < a[r] + len[r]; ++g) val += v[colIds[g]];
+      gv[row] = val;
+      acc += yv[r] * val;
+    }
+  }
+  double* flat = &ws->part[0][0];
+  const double b = block_sum(acc, s_w);
+  if (threadIdx.x == 0) {
+    flat[blockIdx.x] = b;
+    __threadfence();
+    s_last = atomicAdd(&ws->counter, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (s_last) {
+    __threadfence();
+    double t = 0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) t += __builtin_nontemporal_load(&flat[i]);
+    const double pap = block_sum(t, s_w);
+    if (threadIdx.x == 0) {
+      s[4] = pap;
+      s[2] = s[0] / pap;
+      ws->counter = 0;
     }
   }
 }
@@ -102,4 +176,14 @@ extern "C" int libp_ogs_scatter_f64(int Nrows, const int* rowStarts, const int* 
 extern "C" int libp_ogs_scatter_i32(int Nrows, const int* rowStarts, const int* colIds, const int* gv, int* v,
                                     void* stream) {
   return scatter<int>(Nrows, rowStarts, colIds, gv, v, stream);
+}
+
+extern "C" int libp_pcg_gather_pap_f64(int Nrows, const int* rowStarts, const int* colIds, const double* v,
+                                       double* gv, const double* p, double* scalars, void* ws, void* stream) {
+  if (Nrows <= 0) return LIBP_OK;
+  if (!rowStarts || !colIds || !v || !gv || !p || !scalars || !ws) return arg_error("pcg_gather_pap", "null pointer");
+  const unsigned blocks = std::min<unsigned>(stream_blocks(Nrows, 1024), GATHER_PAP_MAXB);
+  hipLaunchKernelGGL(ogs_gather_pap_kernel, dim3(blocks), dim3(256), 0, as_stream(stream), Nrows, rowStarts, colIds,
+                     v, gv, p, scalars, (ReduceWS*)ws);
+  return launch_status("pcg_gather_pap");
 }

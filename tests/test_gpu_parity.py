@@ -97,6 +97,25 @@ def test_ax_kernel_element_list_and_tail():
     assert np.all(got[E - 3:] == 0)
 
 
+def test_zero_length_inputs():
+    """Empty inputs through the C ABI: zero elements and zero rows write nothing
+    and return OK, and a reduction over zero entries writes 0 (not a stale value)."""
+    args = [torch.zeros(8, dtype=torch.float64, device="cuda") for _ in range(4)]
+    g2l = torch.zeros(8, dtype=torch.int32, device="cuda")
+    out = torch.full((8,), 7.0, dtype=torch.float64, device="cuda")
+    assert L().libp_ax_hex_partial_f64(7, 0, None, g2l.data_ptr(), *[a.data_ptr() for a in args[:3]], 1.0,
+                                       args[3].data_ptr(), out.data_ptr(), stream()) == 0
+    rs = torch.zeros(1, dtype=torch.int32, device="cuda")
+    assert L().libp_ogs_gather_f64(0, rs.data_ptr(), g2l.data_ptr(), args[0].data_ptr(), out.data_ptr(),
+                                   stream()) == 0
+    ws = torch.zeros(L().libp_reduce_ws_bytes(), dtype=torch.uint8, device="cuda")
+    res = torch.full((1,), 5.0, dtype=torch.float64, device="cuda")
+    assert L().libp_inner_prod_f64(0, args[0].data_ptr(), args[1].data_ptr(), res.data_ptr(), ws.data_ptr(),
+                                   stream()) == 0
+    torch.cuda.synchronize()
+    assert torch.all(out == 7.0)
+    assert res.item() == 0.0
+
 def test_ax_kernel_f32():
     N = 7
     o = OracleElliptic(box(3, N, curve=0.05))
@@ -223,6 +242,33 @@ def test_pcg_jacobi_c1_iteration_parity():
     assert abs(r["iterations"] - oit) <= 1
     assert abs(r["norm"] - onorm) < 1e-9
     assert r["error_l2"] < 1e-3
+
+
+def test_fused_jacobi_pcg_matches_unfused():
+    """Jacobi-PCG in fused launches (p.Ap in the gather, z and r.z in the update)
+    against the six-launch iteration: the same iterates up to the partition of the
+    reductions. Also through the chunked solver (device STOP flag), which the
+    p-MG coarse level uses when its N=1 kernels are off."""
+    import libparanumal_amd as lp
+    s = dict(REF, PRECONDITIONER="JACOBI", LAMBDA=0, **{"BOX NX": 8, "BOX NY": 8, "BOX NZ": 8, "BOX DIMX": 2,
+                                                         "BOX DIMY": 2, "BOX DIMZ": 2})
+    a = lp.Elliptic(dict(s, **{"PCG JACOBI FUSED": "FALSE"})).run()
+    b = lp.Elliptic(dict(s, **{"PCG JACOBI FUSED": "TRUE"})).run()
+    print("fused pcg", a["iterations"], b["iterations"], a["norm"], b["norm"], a["r_final"], b["r_final"], flush=True)
+    assert abs(a["iterations"] - b["iterations"]) <= 1
+    assert abs(a["norm"] - b["norm"]) <= 1e-10 * a["norm"]
+    assert b["r_final"] <= 1e-8 * b["r0"] * (1 + 1e-12) or b["r_final"] <= 1e-8
+    m = mg_settings("FP64", **{"LINEAR SOLVER STOPPING CRITERION": "RELATIVE", "MULTIGRID COARSE TOLERANCE": 1e-2,
+                               "MULTIGRID COARSE MAXIT": 60, "MULTIGRID COARSE FUSED": "FALSE",
+                               "MULTIGRID COARSE CHECK EVERY": 4})
+    c = lp.Elliptic(dict(m, **{"PCG JACOBI FUSED": "FALSE"}))
+    d = lp.Elliptic(dict(m, **{"PCG JACOBI FUSED": "TRUE"}))
+    rc, rd = c.run(), d.run()
+    print("fused coarse", rc["iterations"], rd["iterations"], c.geti("coarseIterations"), d.geti("coarseIterations"),
+          flush=True)
+    assert abs(rc["iterations"] - rd["iterations"]) <= 1
+    assert abs(c.geti("coarseIterations") - d.geti("coarseIterations")) <= 1
+    assert abs(rc["norm"] - rd["norm"]) <= 1e-9 * abs(rc["norm"])
 
 
 # ---------------------------------------------------------------------------

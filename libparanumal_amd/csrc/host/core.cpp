@@ -145,7 +145,8 @@ settings_t ellipticDefaultSettings() {
   s.newSetting("MULTIGRID COARSE GRAPH", "TRUE", "Replay the fused coarse iterations as captured HIP graphs",
                {"TRUE", "FALSE"});
   s.newSetting("PCG JACOBI FUSED", "TRUE",
-               "Jacobi-PCG in fused launches: p.Ap in the operator's gather, z = invDiag.*r and r.z in the update",
+               "(F)PCG in fused launches: p.Ap in the operator's gather; with Jacobi, z = invDiag.*r and r.z in "
+               "the update",
                {"TRUE", "FALSE"});
   s.newSetting("MULTIGRID COARSE PERSISTENT", "FALSE",
                "Fused coarse Jacobi-PCG as one persistent kernel with grid barriers (no host reads); slower "

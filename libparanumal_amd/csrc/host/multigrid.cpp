@@ -554,12 +554,18 @@ int coarseJacobiPCG_t::solveFusedHex1(deviceMemory<double>& o_X) {
   LIBP_CALL(libp_pcg_jacobi_rdotz_f64(N, jacobi->o_invDiagA.ptr(), o_r.ptr(), o_z.ptr(), o_Ap.ptr(), 0, s, ws, st));
   int it = 0;
   const int chunk = std::max(1, solver->checkEvery);
+  // The first chunk is sized from the previous solve (the coarse problem is
+  // solved once per V-cycle, with similar counts), rounded down to a multiple
+  // of 4 so that few distinct chunks are captured; the rest go in chunks of
+  // 4. Iterations past convergence return at once (device STOP flag).
+  int n = (lastIterations >= 8) ? (lastIterations / 4) * 4 : chunk;
   while (it < maxit) {
-    const int n = std::min(chunk, maxit - it);
+    n = std::min(n, maxit - it);
     enqueueChunk(o_X, it == 0, n);
     const double* h = p.readScalars(s, LIBP_PCG_NSCALARS);
     it = (int)h[LIBP_PCG_ITERS];
     if (h[LIBP_PCG_STOP] != 0.0) break;
+    n = std::min(chunk, 4);
   }
   return it;
 }

@@ -25,8 +25,10 @@ pcg::pcg(dlong _N, dlong _Nhalo, platform_t& _platform, settings_t& _settings, c
   LIBP_CALL(libp_set_f64(Ntotal, 0.0, o_Ap.ptr(), platform.s()));
 }
 
-/* One PCG iteration (linearSolverPCG.cpp:107-171). With a Jacobi
- * preconditioner and a plain (not flexible) iteration the launches fuse:
+/* One PCG iteration (linearSolverPCG.cpp:107-171). With [PCG JACOBI FUSED]
+ * the operator's gather also forms p.Ap whatever the preconditioner, and with
+ * a Jacobi preconditioner and a plain (not flexible) iteration the launches
+ * fuse further:
  *   p = z + beta p; Ap = A p with p.Ap and alpha in the gather
  *   (operator_t::OperatorPAP); x, r updated with z = invDiag.*r, r.r, r.z and
  *   beta for the next iteration (libp_pcg_jacobi_update_f64).
@@ -60,8 +62,10 @@ void pcg::iteration(operator_t& linearOperator, operator_t& precon, diagonalPrec
     LIBP_CALL(libp_pcg_rdotz_f64(N, o_r.ptr(), o_z.ptr(), o_Ap.ptr(), flexible, s, ws, st));
   }
   LIBP_CALL(libp_pcg_update_p_f64(N, it == 0, o_z.ptr(), o_p.ptr(), s, st));
-  linearOperator.Operator(o_p, o_Ap);
-  LIBP_CALL(libp_pcg_pap_f64(N, o_p.ptr(), o_Ap.ptr(), s, ws, st));
+  if (!fuseJacobi || !linearOperator.OperatorPAP(o_p, o_Ap, s, ws)) {
+    linearOperator.Operator(o_p, o_Ap);
+    LIBP_CALL(libp_pcg_pap_f64(N, o_p.ptr(), o_Ap.ptr(), s, ws, st));
+  }
   LIBP_CALL(libp_pcg_update_f64(N, o_p.ptr(), o_Ap.ptr(), o_x.ptr(), o_r.ptr(), s, ws, st));
 }
 

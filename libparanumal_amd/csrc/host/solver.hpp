@@ -117,7 +117,7 @@ namespace LinearSolver {
  * LINEAR SOLVER setting contains "FPCG" (linearSolverPCG.cpp:43). */
 class pcg : public linearSolverBase_t {
   bool flexible = false;
-  bool fuseJacobi = true;  // [PCG JACOBI FUSED]: three fused launches per Jacobi-PCG iteration
+  bool fuseJacobi = true;  // [PCG JACOBI FUSED]: p.Ap in the gather; fused Jacobi-PCG iterations
   deviceMemory<dfloat> o_p, o_z, o_Ax, o_Ap;
   deviceMemory<dfloat> o_s;  // LIBP_PCG_NSCALARS device scalars
   void iteration(operator_t& linearOperator, operator_t& precon, diagonalPrecon_t* jac, int it,

@@ -155,7 +155,7 @@ __global__ void __launch_bounds__(RBS) pcg_pap_kernel(const int N, const double*
 // x += alpha p ; r -= alpha Ap ; r.r   (linearSolverUpdatePCG.okl:29-75)
 __global__ void __launch_bounds__(RBS) pcg_update_kernel(const int N, const double* __restrict__ p,
                                                          const double* __restrict__ Ap, double* __restrict__ x,
-                                                         double* __restrict__ r, double* __restrict__ s,
+                                                         double* __restrict__ r, const double* __restrict__ s,
                                                          ReduceWS* __restrict__ ws) {
   __shared__ double s_w[16];
   if (s[8] != 0.0) return;  // converged in an earlier iteration of this chunk
@@ -169,16 +169,21 @@ __global__ void __launch_bounds__(RBS) pcg_update_kernel(const int N, const doub
     r[n] = rn;
   }
   double v[1] = {a0};
-  bool last;
-  finish_partials<1>(ws, v, s_w, last);
-  if (last) {
-    double out[1];
-    sum_partials<1>(ws, out);
-    if (threadIdx.x == 0) {
-      s[5] = out[0];
-      s[9] += 1.0;
-      if (s[7] > 0.0 && out[0] <= s[7]) s[8] = 1.0;
-    }
+  store_block_partials<1>(ws, v, s_w);
+}
+
+// r.r of pcg_update_kernel's partials; counts the iteration and raises STOP
+// once r.r <= TOL (TOL > 0 only)
+__global__ void __launch_bounds__(RBS) pcg_update_finish_kernel(const int nb, double* __restrict__ s,
+                                                                const ReduceWS* __restrict__ ws) {
+  __shared__ double s_w[16];
+  if (s[8] != 0.0) return;
+  double out[1];
+  sum_block_partials<1>(ws, nb, out, s_w);
+  if (threadIdx.x == 0) {
+    s[5] = out[0];
+    s[9] += 1.0;
+    if (s[7] > 0.0 && out[0] <= s[7]) s[8] = 1.0;
   }
 }
 
@@ -191,7 +196,7 @@ __global__ void __launch_bounds__(RBS) pcg_jacobi_update_kernel(const int N, con
                                                                 const double* __restrict__ p,
                                                                 const double* __restrict__ Ap,
                                                                 double* __restrict__ x, double* __restrict__ r,
-                                                                double* __restrict__ z, double* __restrict__ s,
+                                                                double* __restrict__ z, const double* __restrict__ s,
                                                                 ReduceWS* __restrict__ ws) {
   __shared__ double s_w[16];
   if (s[8] != 0.0) return;
@@ -208,19 +213,22 @@ __global__ void __launch_bounds__(RBS) pcg_jacobi_update_kernel(const int N, con
     a1 += rn * zn;
   }
   double v[2] = {a0, a1};
-  bool last;
-  finish_partials<2>(ws, v, s_w, last);
-  if (last) {
-    double out[2];
-    sum_partials<2>(ws, out);
-    if (threadIdx.x == 0) {
-      s[5] = out[0];
-      s[9] += 1.0;
-      if (s[7] > 0.0 && out[0] <= s[7]) s[8] = 1.0;
-      s[1] = s[0];
-      s[0] = out[1];
-      s[3] = out[1] / s[1];
-    }
+  store_block_partials<2>(ws, v, s_w);
+}
+
+__global__ void __launch_bounds__(RBS) pcg_jacobi_update_finish_kernel(const int nb, double* __restrict__ s,
+                                                                       const ReduceWS* __restrict__ ws) {
+  __shared__ double s_w[16];
+  if (s[8] != 0.0) return;
+  double out[2];
+  sum_block_partials<2>(ws, nb, out, s_w);
+  if (threadIdx.x == 0) {
+    s[5] = out[0];
+    s[9] += 1.0;
+    if (s[7] > 0.0 && out[0] <= s[7]) s[8] = 1.0;
+    s[1] = s[0];
+    s[0] = out[1];
+    s[3] = out[1] / s[1];
   }
 }
 
@@ -327,8 +335,11 @@ extern "C" int libp_pcg_jacobi_rdotz_f64(int N, const double* invDiag, const dou
 extern "C" int libp_pcg_jacobi_update_f64(int N, const double* invDiag, const double* p, const double* Ap,
                                           double* x, double* r, double* z, double* scalars, void* ws, void* stream) {
   if (!invDiag || !p || !Ap || !x || !r || !z || !scalars || !ws) return arg_error("pcg_jacobi_update", "null pointer");
-  hipLaunchKernelGGL(pcg_jacobi_update_kernel, dim3(reduce_blocks(N)), dim3(RBS), 0, as_stream(stream), N, invDiag, p,
-                     Ap, x, r, z, scalars, (ReduceWS*)ws);
+  const unsigned nb = reduce_blocks(N);
+  hipLaunchKernelGGL(pcg_jacobi_update_kernel, dim3(nb), dim3(RBS), 0, as_stream(stream), N, invDiag, p, Ap, x, r, z,
+                     scalars, (ReduceWS*)ws);
+  hipLaunchKernelGGL(pcg_jacobi_update_finish_kernel, dim3(1), dim3(RBS), 0, as_stream(stream), (int)nb, scalars,
+                     (const ReduceWS*)ws);
   return launch_status("pcg_jacobi_update");
 }
 extern "C" int libp_pcg_update_p_f64(int N, int first_iter, const double* z, double* p, const double* scalars,
@@ -342,8 +353,11 @@ extern "C" int libp_pcg_pap_f64(int N, const double* p, const double* Ap, double
 }
 extern "C" int libp_pcg_update_f64(int N, const double* p, const double* Ap, double* x, double* r, double* scalars,
                                    void* ws, void* stream) {
-  hipLaunchKernelGGL(pcg_update_kernel, dim3(reduce_blocks(N)), dim3(RBS), 0, as_stream(stream), N, p, Ap, x, r,
-                     scalars, (ReduceWS*)ws);
+  const unsigned nb = reduce_blocks(N);
+  hipLaunchKernelGGL(pcg_update_kernel, dim3(nb), dim3(RBS), 0, as_stream(stream), N, p, Ap, x, r, scalars,
+                     (ReduceWS*)ws);
+  hipLaunchKernelGGL(pcg_update_finish_kernel, dim3(1), dim3(RBS), 0, as_stream(stream), (int)nb, scalars,
+                     (const ReduceWS*)ws);
   return launch_status("pcg_update");
 }
 

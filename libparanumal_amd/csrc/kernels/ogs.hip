@@ -59,18 +59,20 @@ __global__ void __launch_bounds__(256) ogs_gather_kernel(const int Nrows, const 
 // Gather fused with the p.Ap reduction of a PCG iteration (FP64): gv = gather(v)
 // with the arithmetic of ogs_gather_kernel, and sum_row y[row] gv[row]. Each
 // block stores one partial in a flat array of up to RNPART*RMAXB entries (the
-// ReduceWS partials seen as one buffer); the last block to finish sums them in
-// block order and writes scalars[PAP] and scalars[ALPHA] = scalars[RDOTZ1] /
-// p.Ap (linearSolverPCG.cpp:134-140). Returns at once when scalars[STOP] is set.
+// ReduceWS partials seen as one buffer), with no fence: the kernel writes gv,
+// and a per-block agent-scope fence would write back the dirty L2 of every
+// block. gather_pap_finish_kernel then sums the partials in block order and
+// writes scalars[PAP] and scalars[ALPHA] = scalars[RDOTZ1] / p.Ap
+// (linearSolverPCG.cpp:134-140). Both return at once when scalars[STOP] is set.
 constexpr int GATHER_PAP_MAXB = RNPART * RMAXB;
 
 __global__ void __launch_bounds__(256) ogs_gather_pap_kernel(const int Nrows, const int* __restrict__ rowStarts,
                                                              const int* __restrict__ colIds,
                                                              const double* __restrict__ v, double* __restrict__ gv,
-                                                             const double* __restrict__ y, double* __restrict__ s,
+                                                             const double* __restrict__ y,
+                                                             const double* __restrict__ s,
                                                              ReduceWS* __restrict__ ws) {
   __shared__ double s_w[16];
-  __shared__ bool s_last;
   if (s[8] != 0.0) return;
   const long S = (long)gridDim.x * blockDim.x;
   double acc = 0;
@@ -107,24 +109,21 @@ __global__ void __launch_bounds__(256) ogs_gather_pap_kernel(const int Nrows, co
       acc += yv[r] * val;
     }
   }
-  double* flat = &ws->part[0][0];
   const double b = block_sum(acc, s_w);
+  if (threadIdx.x == 0) (&ws->part[0][0])[blockIdx.x] = b;
+}
+
+__global__ void __launch_bounds__(256) gather_pap_finish_kernel(const int nb, double* __restrict__ s,
+                                                                const ReduceWS* __restrict__ ws) {
+  __shared__ double s_w[16];
+  if (s[8] != 0.0) return;
+  const double* flat = &ws->part[0][0];
+  double t = 0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) t += flat[i];
+  const double pap = block_sum(t, s_w);
   if (threadIdx.x == 0) {
-    flat[blockIdx.x] = b;
-    __threadfence();
-    s_last = atomicAdd(&ws->counter, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (s_last) {
-    __threadfence();
-    double t = 0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) t += __builtin_nontemporal_load(&flat[i]);
-    const double pap = block_sum(t, s_w);
-    if (threadIdx.x == 0) {
-      s[4] = pap;
-      s[2] = s[0] / pap;
-      ws->counter = 0;
-    }
+    s[4] = pap;
+    s[2] = s[0] / pap;
   }
 }
 
@@ -185,5 +184,7 @@ extern "C" int libp_pcg_gather_pap_f64(int Nrows, const int* rowStarts, const in
   const unsigned blocks = std::min<unsigned>(stream_blocks(Nrows, 1024), GATHER_PAP_MAXB);
   hipLaunchKernelGGL(ogs_gather_pap_kernel, dim3(blocks), dim3(256), 0, as_stream(stream), Nrows, rowStarts, colIds,
                      v, gv, p, scalars, (ReduceWS*)ws);
+  hipLaunchKernelGGL(gather_pap_finish_kernel, dim3(1), dim3(256), 0, as_stream(stream), (int)blocks, scalars,
+                     (const ReduceWS*)ws);
   return launch_status("pcg_gather_pap");
 }

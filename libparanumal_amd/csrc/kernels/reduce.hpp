@@ -78,4 +78,29 @@ __device__ inline void sum_partials(ReduceWS* ws, double (&out)[NV]) {
   if (threadIdx.x == 0) ws->counter = 0;
 }
 
+// Partials without fence or ticket, for kernels that write large vectors: an
+// agent-scope fence per block writes back the dirty L2, which in such kernels
+// costs more than the reduction. A separate one-block finisher kernel then
+// sums the partials (the kernel boundary makes them visible).
+template <int NV>
+__device__ inline void store_block_partials(ReduceWS* ws, const double (&v)[NV], double* s_w) {
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const double b = block_sum(v[k], s_w);
+    if (threadIdx.x == 0) ws->part[k][blockIdx.x] = b;
+  }
+}
+
+// Finisher side (one block): the nb partials of each slot summed in a fixed
+// order; valid in thread 0.
+template <int NV>
+__device__ inline void sum_block_partials(const ReduceWS* ws, const int nb, double (&out)[NV], double* s_w) {
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double acc = 0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += ws->part[k][b];
+    out[k] = block_sum(acc, s_w);
+  }
+}
+
 }  // namespace libp_amd

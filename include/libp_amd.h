@@ -154,6 +154,11 @@ int libp_pcg_gather_pap_f64(int Nrows, const int* rowStarts, const int* colIds, 
  * libp_pcg_update_f64, then RDOTZ2 <- RDOTZ1 <- r.z and BETA = r.z / RDOTZ2. */
 int libp_pcg_jacobi_update_f64(int N, const double* invDiag, const double* p, const double* Ap, double* x, double* r,
                                double* z, double* scalars, void* ws, void* stream);
+/* scalars[TOL] = tol^2 * scalars[RDOTR] (1e-300 when that is 0), and
+ * scalars[STOP] = 1 when scalars[RDOTR] is 0: the pure relative stopping rule
+ * of a solve from a zero guess, formed on the device instead of after a host
+ * read of r0.r0 (linearSolverPCG.cpp:93-97, 107-108). */
+int libp_pcg_relative_tol_f64(double tol, double* scalars, void* stream);
 /* Jacobi preconditioner z = invDiag.*r fused with r.z (JacobiPrecon::Operator,
  * solvers/elliptic/src/ellipticPreconJacobi.cpp:42-51 followed by the r.z dot). */
 int libp_pcg_jacobi_rdotz_f64(int N, const double* invDiag, const double* r, double* z, const double* Ap,

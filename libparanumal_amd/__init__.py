@@ -91,6 +91,7 @@ SIGNATURES = {
     "libp_pcg_jacobi_hex1_check_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
     "libp_pcg_gather_pap_f64": (c_int, [c_int] + [c_void_p] * 8),
     "libp_pcg_jacobi_update_f64": (c_int, [c_int] + [c_void_p] * 9),
+    "libp_pcg_relative_tol_f64": (c_int, [c_double, c_void_p, c_void_p]),
     "libp_nbpcg_update1_f64": (c_int, [c_int] + [c_void_p] * 7),
     "libp_nbpcg_update2_f64": (c_int, [c_int, c_int] + [c_void_p] * 7),
     "libp_axpy_dev_f64": (c_int, [c_int] + [c_void_p] * 4),

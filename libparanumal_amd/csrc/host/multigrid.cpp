@@ -547,10 +547,8 @@ int coarseJacobiPCG_t::solveFusedHex1(deviceMemory<double>& o_X) {
   LIBP_CALL(libp_set_f64(N, 0.0, o_X.ptr(), st));
   LIBP_CALL(libp_set_f64(LIBP_PCG_NSCALARS, 0.0, s, st));
   LIBP_CALL(libp_norm2sq_f64(N, o_r.ptr(), s + LIBP_PCG_RDOTR, ws, st));
-  const double rdotr0 = p.readScalars(s + LIBP_PCG_RDOTR, 1)[0];
-  if (rdotr0 == 0.0) return 0;
-  const double TOL = tol * tol * rdotr0;
-  LIBP_CALL(libp_set_f64(1, TOL > 0 ? TOL : 1e-300, s + LIBP_PCG_TOL, st));
+  // TOL = tol^2 r0.r0 on the device; a zero residual raises STOP at once
+  LIBP_CALL(libp_pcg_relative_tol_f64(tol, s, st));
   LIBP_CALL(libp_pcg_jacobi_rdotz_f64(N, jacobi->o_invDiagA.ptr(), o_r.ptr(), o_z.ptr(), o_Ap.ptr(), 0, s, ws, st));
   int it = 0;
   const int chunk = std::max(1, solver->checkEvery);

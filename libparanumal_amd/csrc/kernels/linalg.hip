@@ -332,6 +332,20 @@ extern "C" int libp_pcg_jacobi_rdotz_f64(int N, const double* invDiag, const dou
                      z, Ap, flexible, scalars, (ReduceWS*)ws);
   return launch_status("pcg_jacobi_rdotz");
 }
+// relative stopping threshold formed on the device (no host read of r0.r0)
+__global__ void pcg_relative_tol_kernel(const double tol2, double* __restrict__ s) {
+  if (threadIdx.x == 0) {
+    const double t = tol2 * s[5];
+    s[7] = (t > 0.0) ? t : 1e-300;
+    if (s[5] == 0.0) s[8] = 1.0;
+  }
+}
+extern "C" int libp_pcg_relative_tol_f64(double tol, double* scalars, void* stream) {
+  if (!scalars) return arg_error("pcg_relative_tol", "null pointer");
+  hipLaunchKernelGGL(pcg_relative_tol_kernel, dim3(1), dim3(64), 0, as_stream(stream), tol * tol, scalars);
+  return launch_status("pcg_relative_tol");
+}
+
 extern "C" int libp_pcg_jacobi_update_f64(int N, const double* invDiag, const double* p, const double* Ap,
                                           double* x, double* r, double* z, double* scalars, void* ws, void* stream) {
   if (!invDiag || !p || !Ap || !x || !r || !z || !scalars || !ws) return arg_error("pcg_jacobi_update", "null pointer");

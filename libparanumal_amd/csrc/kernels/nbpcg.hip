@@ -4,6 +4,10 @@
 //
 // NBPCG scalars:  [0] gamma=r.z  [1] previous gamma  [2] alpha  [3] beta  [4] delta=p.s  [5] r.r  [6] z.z
 // NBFPCG scalars: [0] gamma=u.r  [1] delta=u.w  [2] eta  [3] alpha  [4] beta  [5] r.r
+//
+// The kernels that write vectors store fence-free block partials; a one-block
+// finisher kernel sums them in block order and updates the scalars (a
+// per-block agent-scope fence would write back the dirty L2 of every block).
 #include "reduce.hpp"
 
 namespace libp_amd {
@@ -25,15 +29,17 @@ __global__ void __launch_bounds__(RBS) nbpcg_update1_kernel(const int N, const d
     s[n] = sn;
   }
   double v[1] = {a0};
-  bool last;
-  finish_partials<1>(ws, v, s_w, last);
-  if (last) {
-    double out[1];
-    sum_partials<1>(ws, out);
-    if (threadIdx.x == 0) {
-      sc[4] = out[0];
-      sc[2] = sc[0] / out[0];
-    }
+  store_block_partials<1>(ws, v, s_w);
+}
+
+__global__ void __launch_bounds__(RBS) nbpcg_update1_finish_kernel(const int nb, double* __restrict__ sc,
+                                                                   const ReduceWS* __restrict__ ws) {
+  __shared__ double s_w[16];
+  double out[1];
+  sum_block_partials<1>(ws, nb, out, s_w);
+  if (threadIdx.x == 0) {
+    sc[4] = out[0];
+    sc[2] = sc[0] / out[0];
   }
 }
 
@@ -57,18 +63,21 @@ __global__ void __launch_bounds__(RBS) nbpcg_update2_kernel(const int N, const i
     z[n] = zn;
   }
   double v[3] = {a0, a1, a2};
-  bool last;
-  finish_partials<3>(ws, v, s_w, last);
-  if (last) {
-    double out[3];
-    sum_partials<3>(ws, out);
-    if (threadIdx.x == 0) {
-      sc[1] = first ? 0.0 : sc[0];
-      sc[0] = out[0];
-      sc[6] = out[1];
-      sc[5] = out[2];
-      sc[3] = first ? 0.0 : out[0] / sc[1];
-    }
+  store_block_partials<3>(ws, v, s_w);
+}
+
+__global__ void __launch_bounds__(RBS) nbpcg_update2_finish_kernel(const int nb, const int first,
+                                                                   double* __restrict__ sc,
+                                                                   const ReduceWS* __restrict__ ws) {
+  __shared__ double s_w[16];
+  double out[3];
+  sum_block_partials<3>(ws, nb, out, s_w);
+  if (threadIdx.x == 0) {
+    sc[1] = first ? 0.0 : sc[0];
+    sc[0] = out[0];
+    sc[6] = out[1];
+    sc[5] = out[2];
+    sc[3] = first ? 0.0 : out[0] / sc[1];
   }
 }
 
@@ -140,23 +149,25 @@ __global__ void __launch_bounds__(RBS) nbfpcg_update1_kernel(const int N, const 
     w[n] = wn;
   }
   double v[4] = {a0, a1, a2, a3};
-  bool last;
-  finish_partials<4>(ws, v, s_w, last);
-  if (last) {
-    double out[4];
-    sum_partials<4>(ws, out);
-    if (threadIdx.x == 0) {
-      const double eta = sc[2];
-      const double beta = -out[1] / eta;
-      const double delta = out[2];
-      const double etaNew = delta - beta * beta * eta;
-      sc[0] = out[0];
-      sc[1] = delta;
-      sc[4] = beta;
-      sc[5] = out[3];
-      sc[2] = etaNew;
-      sc[3] = out[0] / etaNew;
-    }
+  store_block_partials<4>(ws, v, s_w);
+}
+
+__global__ void __launch_bounds__(RBS) nbfpcg_update1_finish_kernel(const int nb, double* __restrict__ sc,
+                                                                    const ReduceWS* __restrict__ ws) {
+  __shared__ double s_w[16];
+  double out[4];
+  sum_block_partials<4>(ws, nb, out, s_w);
+  if (threadIdx.x == 0) {
+    const double eta = sc[2];
+    const double beta = -out[1] / eta;
+    const double delta = out[2];
+    const double etaNew = delta - beta * beta * eta;
+    sc[0] = out[0];
+    sc[1] = delta;
+    sc[4] = beta;
+    sc[5] = out[3];
+    sc[2] = etaNew;
+    sc[3] = out[0] / etaNew;
   }
 }
 
@@ -186,15 +197,21 @@ using namespace libp_amd;
 extern "C" int libp_nbpcg_update1_f64(int N, const double* z, const double* Z, double* p, double* s, double* scalars,
                                       void* ws, void* stream) {
   NB_CHECK(!z || !Z || !p || !s || !scalars || !ws, "nbpcg_update1");
-  hipLaunchKernelGGL(nbpcg_update1_kernel, dim3(reduce_blocks(N)), dim3(RBS), 0, as_stream(stream), N, z, Z, p, s,
-                     scalars, (ReduceWS*)ws);
+  const unsigned nb = reduce_blocks(N);
+  hipLaunchKernelGGL(nbpcg_update1_kernel, dim3(nb), dim3(RBS), 0, as_stream(stream), N, z, Z, p, s, scalars,
+                     (ReduceWS*)ws);
+  hipLaunchKernelGGL(nbpcg_update1_finish_kernel, dim3(1), dim3(RBS), 0, as_stream(stream), (int)nb, scalars,
+                     (const ReduceWS*)ws);
   return launch_status("nbpcg_update1");
 }
 extern "C" int libp_nbpcg_update2_f64(int N, int first, const double* s, const double* S, double* r, double* z,
                                       double* scalars, void* ws, void* stream) {
   NB_CHECK(!s || !S || !r || !z || !scalars || !ws, "nbpcg_update2");
-  hipLaunchKernelGGL(nbpcg_update2_kernel, dim3(reduce_blocks(N)), dim3(RBS), 0, as_stream(stream), N, first, s, S, r,
-                     z, scalars, (ReduceWS*)ws);
+  const unsigned nb = reduce_blocks(N);
+  hipLaunchKernelGGL(nbpcg_update2_kernel, dim3(nb), dim3(RBS), 0, as_stream(stream), N, first, s, S, r, z, scalars,
+                     (ReduceWS*)ws);
+  hipLaunchKernelGGL(nbpcg_update2_finish_kernel, dim3(1), dim3(RBS), 0, as_stream(stream), (int)nb, first, scalars,
+                     (const ReduceWS*)ws);
   return launch_status("nbpcg_update2");
 }
 extern "C" int libp_axpy_dev_f64(int N, const double* alpha, const double* x, double* y, void* stream) {
@@ -214,8 +231,11 @@ extern "C" int libp_nbfpcg_update1_f64(int N, const double* p, const double* s, 
                                        double* x, double* r, double* u, double* w, double* scalars, void* ws,
                                        void* stream) {
   NB_CHECK(!p || !s || !q || !z || !x || !r || !u || !w || !scalars || !ws, "nbfpcg_update1");
-  hipLaunchKernelGGL(nbfpcg_update1_kernel, dim3(reduce_blocks(N)), dim3(RBS), 0, as_stream(stream), N, p, s, q, z, x,
-                     r, u, w, scalars, (ReduceWS*)ws);
+  const unsigned nb = reduce_blocks(N);
+  hipLaunchKernelGGL(nbfpcg_update1_kernel, dim3(nb), dim3(RBS), 0, as_stream(stream), N, p, s, q, z, x, r, u, w,
+                     scalars, (ReduceWS*)ws);
+  hipLaunchKernelGGL(nbfpcg_update1_finish_kernel, dim3(1), dim3(RBS), 0, as_stream(stream), (int)nb, scalars,
+                     (const ReduceWS*)ws);
   return launch_status("nbfpcg_update1");
 }
 extern "C" int libp_nbfpcg_update2_f64(int N, const double* u, const double* w, const double* m, const double* n,

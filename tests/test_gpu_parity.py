@@ -358,14 +358,14 @@ def test_nbfpcg_pmg_fp32_reaches_tolerance():
 # the reference harness's own setup.rc, through the ellipticMain driver
 # (test/testElliptic.py:34-72 writes it; test/test.py:91-99 formats it)
 # ---------------------------------------------------------------------------
-def harness_rc(precon, flag=1, linear_solver="PCG"):
+def harness_rc(precon, flag=1, linear_solver="PCG", extra=()):
     keys = [("FORMAT", "2.0"), ("DATA FILE", "/path/to/solvers/elliptic/data/ellipticSine3D.h"), ("MESH FILE", "BOX"),
             ("MESH DIMENSION", 3), ("ELEMENT TYPE", 12), ("BOX NX", 10), ("BOX NY", 10), ("BOX NZ", 10),
             ("BOX DIMX", 1), ("BOX DIMY", 1), ("BOX DIMZ", 1), ("BOX BOUNDARY FLAG", flag), ("POLYNOMIAL DEGREE", 4),
             ("THREAD MODEL", "HIP"), ("PLATFORM NUMBER", 0), ("DEVICE NUMBER", 0), ("DISCRETIZATION", "CONTINUOUS"),
             ("LINEAR SOLVER", linear_solver), ("PRECONDITIONER", precon), ("MULTIGRID SMOOTHER", "CHEBYSHEV"),
             ("PARALMOND CYCLE", "VCYCLE"), ("PARALMOND STRENGTH", "SYMMETRIC"), ("PARALMOND AGGREGATION", "UNSMOOTHED"),
-            ("PARALMOND SMOOTHER", "CHEBYSHEV"), ("OUTPUT TO FILE", "FALSE"), ("VERBOSE", "FALSE")]
+            ("PARALMOND SMOOTHER", "CHEBYSHEV"), ("OUTPUT TO FILE", "FALSE"), ("VERBOSE", "FALSE")] + list(extra)
     return "".join(f"[{k}]\n{v}\n\n" for k, v in keys)
 
 
@@ -374,12 +374,16 @@ def harness_rc(precon, flag=1, linear_solver="PCG"):
     ("JACOBI", 1, "PCG", 0.353553400508458),     # testEllipticHex_C0_Jacobi (:199-203)
     ("MULTIGRID", 1, "FPCG", 0.353553400508458), # testEllipticHex_C0_Multigrid (:209-213), coarse level differs
     ("NONE", -1, "PCG", 0.059540839002614),      # testEllipticHex_C0_AllNeumann (:244-248)
+    ("MULTIGRID-AMG", 1, "FPCG", 0.353553400508458),  # testEllipticHex_C0_Multigrid with the parAlmond base level
 ])
 def test_elliptic_main_with_harness_settings(tmp_path, precon, flag, solver, refnorm):
     import subprocess
     import libparanumal_amd as lp
     rc = tmp_path / "setup.rc"
-    rc.write_text(harness_rc(precon, flag, solver))
+    extra = ()
+    if precon == "MULTIGRID-AMG":
+        precon, extra = "MULTIGRID", (("MULTIGRID COARSE SOLVER", "AMG"),)
+    rc.write_text(harness_rc(precon, flag, solver, extra))
     run = subprocess.run([lp.MAIN_PATH, str(rc)], capture_output=True, text=True, timeout=120, cwd=tmp_path)
     assert run.returncode == 0, run.stderr[-2000:]
     last = run.stdout.strip().splitlines()[-1]

@@ -133,19 +133,21 @@ __global__ void __launch_bounds__(RBS) learned_alpha_kernel(const int N, const f
     a1 += cn * Ac[n];
   }
   double v[2] = {a0, a1};
-  bool last;
-  finish_partials<2>(ws, v, s_w, last);
-  if (last) {
-    double out[2];
-    sum_partials<2>(ws, out);
-    if (threadIdx.x == 0) {
-      sc[0] = out[0];
-      sc[1] = out[1];
-      const double alpha = out[0] / out[1];
-      const bool ok = (out[1] > 0.0) && __builtin_isfinite(alpha) && __builtin_isfinite(out[0]);
-      sc[2] = ok ? alpha : 0.0;
-      if (!ok) sc[3] += 1.0;
-    }
+  store_block_partials<2>(ws, v, s_w);
+}
+
+__global__ void __launch_bounds__(RBS) learned_alpha_finish_kernel(const int nb, double* __restrict__ sc,
+                                                                   const ReduceWS* __restrict__ ws) {
+  __shared__ double s_w[16];
+  double out[2];
+  sum_block_partials<2>(ws, nb, out, s_w);
+  if (threadIdx.x == 0) {
+    sc[0] = out[0];
+    sc[1] = out[1];
+    const double alpha = out[0] / out[1];
+    const bool ok = (out[1] > 0.0) && __builtin_isfinite(alpha) && __builtin_isfinite(out[0]);
+    sc[2] = ok ? alpha : 0.0;
+    if (!ok) sc[3] += 1.0;
   }
 }
 
@@ -187,17 +189,21 @@ __global__ void __launch_bounds__(RBS) learned_loss_kernel(const int N, const fl
     a1 += g * (double)invD[n] * (double)r[n];
   }
   double v[2] = {a0, a1};
-  bool last;
-  finish_partials<2>(ws, v, s_w, last);
-  if (last) {
-    double out[2];
-    sum_partials<2>(ws, out);
-    if (threadIdx.x == 0) {
-      const double J = out[0] / *nrm;
-      stat[0] = J;
-      stat[1] += J;
-      if (gradOmega) *gradOmega = (float)out[1];
-    }
+  store_block_partials<2>(ws, v, s_w);
+}
+
+__global__ void __launch_bounds__(RBS) learned_loss_finish_kernel(const int nb, const double* __restrict__ nrm,
+                                                                  double* __restrict__ stat,
+                                                                  float* __restrict__ gradOmega,
+                                                                  const ReduceWS* __restrict__ ws) {
+  __shared__ double s_w[16];
+  double out[2];
+  sum_block_partials<2>(ws, nb, out, s_w);
+  if (threadIdx.x == 0) {
+    const double J = out[0] / *nrm;
+    stat[0] = J;
+    stat[1] += J;
+    if (gradOmega) *gradOmega = (float)out[1];
   }
 }
 
@@ -311,6 +317,8 @@ extern "C" int libp_learned_alpha_f32(int N, const float* c, const float* res, c
   if (!c || !res || !Ac || !sc || !ws) return arg_error("learned_alpha", "null pointer");
   hipLaunchKernelGGL(learned_alpha_kernel, dim3(reduce_blocks(N)), dim3(RBS), 0, as_stream(stream), N, c, res, Ac, sc,
                      (ReduceWS*)ws);
+  hipLaunchKernelGGL(learned_alpha_finish_kernel, dim3(1), dim3(RBS), 0, as_stream(stream), (int)reduce_blocks(N), sc,
+                     (const ReduceWS*)ws);
   return launch_status("learned_alpha");
 }
 
@@ -329,6 +337,8 @@ extern "C" int libp_learned_loss_f32(int N, const float* d, const float* Ad, con
   if (!d || !Ad || !r || !invDiag || !nrm || !gc || !stat || !ws) return arg_error("learned_loss", "null pointer");
   hipLaunchKernelGGL(learned_loss_kernel, dim3(reduce_blocks(N)), dim3(RBS), 0, as_stream(stream), N, d, Ad, r, invDiag,
                      nrm, gc, stat, gradOmega, (ReduceWS*)ws);
+  hipLaunchKernelGGL(learned_loss_finish_kernel, dim3(1), dim3(RBS), 0, as_stream(stream), (int)reduce_blocks(N), nrm,
+                     stat, gradOmega, (const ReduceWS*)ws);
   return launch_status("learned_loss");
 }
 

@@ -10,7 +10,7 @@ namespace libp_amd {
 template <int KIND>  // 0 x.y, 1 x.x, 2 sum x, 3 w.x.y
 __global__ void __launch_bounds__(RBS) reduce_kernel(const int N, const double* __restrict__ w,
                                                      const double* __restrict__ x, const double* __restrict__ y,
-                                                     double* __restrict__ result, ReduceWS* __restrict__ ws) {
+                                                     ReduceWS* __restrict__ ws) {
   __shared__ double s_w[16];
   double acc = 0;
   for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
@@ -20,20 +20,26 @@ __global__ void __launch_bounds__(RBS) reduce_kernel(const int N, const double* 
     if (KIND == 3) acc += w[n] * x[n] * y[n];
   }
   double v[1] = {acc};
-  bool last;
-  if (finish_partials<1>(ws, v, s_w, last)) {}
-  if (last) {
-    double out[1];
-    sum_partials<1>(ws, out);
-    if (threadIdx.x == 0) *result = out[0];
+  store_block_partials<1>(ws, v, s_w);
+}
+
+__global__ void __launch_bounds__(RBS) reduce_finish_kernel(const int nb, double* __restrict__ result,
+                                                            const ReduceWS* __restrict__ ws) {
+  __shared__ double s_w[16];
+  double out[1];
+  sum_block_partials<1>(ws, nb, out, s_w);
+  if (threadIdx.x == 0) {
+    *result = out[0];
   }
 }
 
 template <int KIND>
 static int reduce(int N, const double* w, const double* x, const double* y, double* result, void* ws, void* stream) {
   if (!x || !result || !ws) return arg_error("reduce", "null pointer");
-  hipLaunchKernelGGL(reduce_kernel<KIND>, dim3(reduce_blocks(N)), dim3(RBS), 0, as_stream(stream), N, w, x, y,
-                     result, (ReduceWS*)ws);
+  const unsigned nb = reduce_blocks(N);
+  hipLaunchKernelGGL(reduce_kernel<KIND>, dim3(nb), dim3(RBS), 0, as_stream(stream), N, w, x, y, (ReduceWS*)ws);
+  hipLaunchKernelGGL(reduce_finish_kernel, dim3(1), dim3(RBS), 0, as_stream(stream), (int)nb, result,
+                     (const ReduceWS*)ws);
   return launch_status("reduce");
 }
 
@@ -107,19 +113,23 @@ __global__ void __launch_bounds__(RBS) pcg_rdotz_kernel(const int N, const doubl
     if (flexible) a1 += zn * Ap[n];
   }
   double v[2] = {a0, a1};
-  bool last;
-  finish_partials<2>(ws, v, s_w, last);
-  if (last) {
-    double out[2];
-    sum_partials<2>(ws, out);
-    if (threadIdx.x == 0) {
-      // linearSolverPCG.cpp:116-124 ; beta of the first iteration is handled by the p update
-      const double rdotz1_old = s[0];
-      s[1] = rdotz1_old;
-      s[0] = out[0];
-      s[6] = out[1];
-      s[3] = flexible ? (-s[2] * out[1] / rdotz1_old) : (out[0] / rdotz1_old);
-    }
+  store_block_partials<2>(ws, v, s_w);
+}
+
+__global__ void __launch_bounds__(RBS) pcg_rdotz_finish_kernel(const int nb, const int flexible,
+                                                               double* __restrict__ s,
+                                                               const ReduceWS* __restrict__ ws) {
+  __shared__ double s_w[16];
+  if (s[8] != 0.0) return;
+  double out[2];
+  sum_block_partials<2>(ws, nb, out, s_w);
+  if (threadIdx.x == 0) {
+    // linearSolverPCG.cpp:116-124 ; beta of the first iteration is handled by the p update
+    const double rdotz1_old = s[0];
+    s[1] = rdotz1_old;
+    s[0] = out[0];
+    s[6] = out[1];
+    s[3] = flexible ? (-s[2] * out[1] / rdotz1_old) : (out[0] / rdotz1_old);
   }
 }
 
@@ -140,15 +150,18 @@ __global__ void __launch_bounds__(RBS) pcg_pap_kernel(const int N, const double*
   double a0 = 0;
   for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) a0 += p[n] * Ap[n];
   double v[1] = {a0};
-  bool last;
-  finish_partials<1>(ws, v, s_w, last);
-  if (last) {
-    double out[1];
-    sum_partials<1>(ws, out);
-    if (threadIdx.x == 0) {
-      s[4] = out[0];
-      s[2] = s[0] / out[0];  // alpha = rdotz1/pAp
-    }
+  store_block_partials<1>(ws, v, s_w);
+}
+
+__global__ void __launch_bounds__(RBS) pcg_pap_finish_kernel(const int nb, double* __restrict__ s,
+                                                             const ReduceWS* __restrict__ ws) {
+  __shared__ double s_w[16];
+  if (s[8] != 0.0) return;
+  double out[1];
+  sum_block_partials<1>(ws, nb, out, s_w);
+  if (threadIdx.x == 0) {
+    s[4] = out[0];
+    s[2] = s[0] / out[0];  // alpha = rdotz1/pAp
   }
 }
 
@@ -189,7 +202,7 @@ __global__ void __launch_bounds__(RBS) pcg_update_finish_kernel(const int nb, do
 
 // x += alpha p; r -= alpha Ap; z = invDiag.*r; r.r and r.z in one pass: the
 // update of linearSolverPCG.cpp:142-160 fused with the Jacobi preconditioner
-// and the r.z of the next iteration (:107-124). The last block writes r.r,
+// and the r.z of the next iteration (:107-124). Its finisher writes r.r,
 // counts the iteration, raises STOP once r.r <= TOL, shifts r.z (RDOTZ2 <-
 // RDOTZ1 <- r.z) and sets beta = r.z / old r.z.
 __global__ void __launch_bounds__(RBS) pcg_jacobi_update_kernel(const int N, const double* __restrict__ invDiag,
@@ -324,12 +337,16 @@ extern "C" int libp_pcg_rdotz_f64(int N, const double* r, const double* z, const
                                   double* scalars, void* ws, void* stream) {
   hipLaunchKernelGGL(pcg_rdotz_kernel<false>, dim3(reduce_blocks(N)), dim3(RBS), 0, as_stream(stream), N, nullptr,
                      r, const_cast<double*>(z), Ap, flexible, scalars, (ReduceWS*)ws);
+  hipLaunchKernelGGL(pcg_rdotz_finish_kernel, dim3(1), dim3(RBS), 0, as_stream(stream), (int)reduce_blocks(N), flexible, scalars,
+                     (const ReduceWS*)ws);
   return launch_status("pcg_rdotz");
 }
 extern "C" int libp_pcg_jacobi_rdotz_f64(int N, const double* invDiag, const double* r, double* z, const double* Ap,
                                          int flexible, double* scalars, void* ws, void* stream) {
   hipLaunchKernelGGL(pcg_rdotz_kernel<true>, dim3(reduce_blocks(N)), dim3(RBS), 0, as_stream(stream), N, invDiag, r,
                      z, Ap, flexible, scalars, (ReduceWS*)ws);
+  hipLaunchKernelGGL(pcg_rdotz_finish_kernel, dim3(1), dim3(RBS), 0, as_stream(stream), (int)reduce_blocks(N), flexible, scalars,
+                     (const ReduceWS*)ws);
   return launch_status("pcg_jacobi_rdotz");
 }
 // relative stopping threshold formed on the device (no host read of r0.r0)
@@ -363,6 +380,8 @@ extern "C" int libp_pcg_update_p_f64(int N, int first_iter, const double* z, dou
 extern "C" int libp_pcg_pap_f64(int N, const double* p, const double* Ap, double* scalars, void* ws, void* stream) {
   hipLaunchKernelGGL(pcg_pap_kernel, dim3(reduce_blocks(N)), dim3(RBS), 0, as_stream(stream), N, p, Ap, scalars,
                      (ReduceWS*)ws);
+  hipLaunchKernelGGL(pcg_pap_finish_kernel, dim3(1), dim3(RBS), 0, as_stream(stream), (int)reduce_blocks(N), scalars,
+                     (const ReduceWS*)ws);
   return launch_status("pcg_pap");
 }
 extern "C" int libp_pcg_update_f64(int N, const double* p, const double* Ap, double* x, double* r, double* scalars,

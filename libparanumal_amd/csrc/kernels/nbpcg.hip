@@ -103,19 +103,21 @@ __global__ void __launch_bounds__(RBS) nbfpcg_update0_kernel(const int N, const 
     a2 += rn * rn;
   }
   double v[3] = {a0, a1, a2};
-  bool last;
-  finish_partials<3>(ws, v, s_w, last);
-  if (last) {
-    double out[3];
-    sum_partials<3>(ws, out);
-    if (threadIdx.x == 0) {
-      sc[0] = out[0];
-      sc[1] = out[1];
-      sc[5] = out[2];
-      sc[2] = out[1];
-      sc[3] = out[0] / out[1];
-      sc[4] = 0.0;
-    }
+  store_block_partials<3>(ws, v, s_w);
+}
+
+__global__ void __launch_bounds__(RBS) nbfpcg_update0_finish_kernel(const int nb, double* __restrict__ sc,
+                                                                    const ReduceWS* __restrict__ ws) {
+  __shared__ double s_w[16];
+  double out[3];
+  sum_block_partials<3>(ws, nb, out, s_w);
+  if (threadIdx.x == 0) {
+    sc[0] = out[0];
+    sc[1] = out[1];
+    sc[5] = out[2];
+    sc[2] = out[1];
+    sc[3] = out[0] / out[1];
+    sc[4] = 0.0;
   }
 }
 
@@ -225,6 +227,8 @@ extern "C" int libp_nbfpcg_update0_f64(int N, const double* u, const double* r, 
   NB_CHECK(!u || !r || !w || !scalars || !ws, "nbfpcg_update0");
   hipLaunchKernelGGL(nbfpcg_update0_kernel, dim3(reduce_blocks(N)), dim3(RBS), 0, as_stream(stream), N, u, r, w,
                      scalars, (ReduceWS*)ws);
+  hipLaunchKernelGGL(nbfpcg_update0_finish_kernel, dim3(1), dim3(RBS), 0, as_stream(stream), (int)reduce_blocks(N),
+                     scalars, (const ReduceWS*)ws);
   return launch_status("nbfpcg_update0");
 }
 extern "C" int libp_nbfpcg_update1_f64(int N, const double* p, const double* s, const double* q, const double* z,

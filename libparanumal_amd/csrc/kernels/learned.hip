@@ -25,9 +25,13 @@ __device__ inline u16 f2bf_l(float f) {
   return (u16)(u >> 16);
 }
 
-// One wavefront per element, four per block. NPER = ceil(nIn/64) values per lane.
-template <int NPER>
-__global__ void __launch_bounds__(256) learned_features_kernel(const int E, const int Np, const int Ntr,
+// One wavefront per element, four per block. NPER = ceil(nIn/64) values per
+// lane. NP, NTR > 0 fix Np and Ntr at compile time (the N=7 instance, 512 +
+// 384): which map a lane's feature comes from is then known statically, and
+// every index load of the lane is issued before the value loads that depend
+// on them. NP = 0 takes the sizes at run time.
+template <int NPER, int NP, int NTR>
+__global__ void __launch_bounds__(256) learned_features_kernel(const int E, const int Np_, const int Ntr_,
                                                                const int* __restrict__ G2L,
                                                                const int* __restrict__ traceIds,
                                                                const float* __restrict__ r,
@@ -36,18 +40,26 @@ __global__ void __launch_bounds__(256) learned_features_kernel(const int E, cons
   const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int l = threadIdx.x & 63;
   if (e >= E) return;
+  const int Np = NP > 0 ? NP : Np_, Ntr = NP > 0 ? NTR : Ntr_;
   const int nIn = Np + Ntr;
-  float z[NPER];
-  double ss = 0;
+  int gid[NPER];
 #pragma unroll
   for (int i = 0; i < NPER; ++i) {
     const int f = l + 64 * i;
     int g = -1;
     if (f < Np) g = G2L[(long)e * Np + f];
     else if (f < nIn) g = traceIds[(long)e * Ntr + (f - Np)];
-    z[i] = (g >= 0) ? invD[g] * r[g] : 0.f;
-    ss += (double)z[i] * z[i];
+    gid[i] = g;
   }
+  float z[NPER];
+  double ss = 0;
+#pragma unroll
+  for (int i = 0; i < NPER; ++i) {
+    const int g = gid[i];
+    z[i] = (g >= 0) ? invD[g] * r[g] : 0.f;
+  }
+#pragma unroll
+  for (int i = 0; i < NPER; ++i) ss += (double)z[i] * z[i];
   ss = wave_sum(ss);
   const float s = (float)sqrt(ss / nIn);
   const float inv = (s > 0.f) ? 1.f / s : 0.f;
@@ -278,10 +290,15 @@ extern "C" int libp_learned_features_f32(int Nelements, int Np, int Ntr, const i
   const int nper = (nIn + 63) / 64;
   dim3 grid((Nelements + 3) / 4), block(256);
   hipStream_t s = as_stream(stream);
-#define LF(NP)                                                                                                    \
-  case NP:                                                                                                        \
-    hipLaunchKernelGGL(learned_features_kernel<NP>, grid, block, 0, s, Nelements, Np, Ntr, GlobalToLocal,        \
-                       traceIds, r, invDiag, X, ldx, scale);                                                      \
+  if (Np == 512 && Ntr == 384) {
+    hipLaunchKernelGGL((learned_features_kernel<14, 512, 384>), grid, block, 0, s, Nelements, Np, Ntr, GlobalToLocal,
+                       traceIds, r, invDiag, X, ldx, scale);
+    return launch_status("learned_features");
+  }
+#define LF(NPER)                                                                                                  \
+  case NPER:                                                                                                      \
+    hipLaunchKernelGGL((learned_features_kernel<NPER, 0, 0>), grid, block, 0, s, Nelements, Np, Ntr,             \
+                       GlobalToLocal, traceIds, r, invDiag, X, ldx, scale);                                       \
     break;
   switch (nper) {
     LF(1) LF(2) LF(3) LF(4) LF(5) LF(6) LF(7) LF(8) LF(9) LF(10) LF(11) LF(12) LF(13) LF(14) LF(15) LF(16)

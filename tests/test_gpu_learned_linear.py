@@ -107,3 +107,45 @@ def test_linear_weight_file_and_architecture_check(fitted):
     assert not m.learned_load(path)
     assert "architecture" in m.learned_status()
     assert m.learned_info()["active"] == 0
+
+
+def bf16_bits(x):
+    """float32 -> bfloat16 bits, round to nearest even (NaN is not produced here)."""
+    u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = u + 0x7FFF + ((u >> 16) & 1)
+    return (u >> 16).astype(np.uint16)
+
+
+def test_features_n7_instance_against_numpy():
+    """The N=7 network input (512 element nodes + 6 x 64 neighbour traces, the
+    kernel instance with the sizes fixed at compile time) against a numpy
+    restatement: z = invD[g] r[g] in FP32, s = sqrt(sum z^2 / 896) with the sum
+    in FP64, X = bf16(z * (1/s)) at every feature."""
+    import ctypes
+    p = lp().Elliptic(learned_settings(2, 7, "7,3,1", **LIN))
+    E, Np = p.geti("Nelements"), p.geti("Np")
+    assert Np == 512
+    g2l = p.host_array("GlobalToLocal").reshape(E, 512)
+    tr = p.host_array("learnedTrace").reshape(E, 384)
+    Ng = p.geti("Ngather")
+    rng = np.random.default_rng(77)
+    r = rng.standard_normal(Ng).astype(np.float32)
+    invD = rng.uniform(0.5, 2.0, Ng).astype(np.float32)
+    X = torch.zeros(E * 896, dtype=torch.int16, device="cuda")
+    sc = torch.zeros(E, dtype=torch.float32, device="cuda")
+    args = [dev(np.ascontiguousarray(g2l)), dev(np.ascontiguousarray(tr)), dev(r), dev(invD)]
+    st = lp().lib().libp_learned_features_f32(E, 512, 384, *[a.data_ptr() for a in args], X.data_ptr(), 896,
+                                              sc.data_ptr(), ctypes.c_void_p(0))
+    assert st == 0
+    torch.cuda.synchronize()
+    gid = np.concatenate([g2l, tr], axis=1)
+    z = np.where(gid >= 0, invD[np.maximum(gid, 0)] * r[np.maximum(gid, 0)], np.float32(0)).astype(np.float32)
+    s = np.sqrt((z.astype(np.float64) ** 2).sum(axis=1) / 896).astype(np.float32)
+    s_gpu = sc.cpu().numpy()
+    assert np.max(np.abs(s_gpu - s) / s) < 1e-6
+    inv = (np.float32(1) / s_gpu).astype(np.float32)
+    ref = bf16_bits(z * inv[:, None])
+    got = X.cpu().numpy().view(np.uint16).reshape(E, 896)
+    diff = np.abs(got.astype(np.int32) - ref.astype(np.int32))
+    print("features n7: mismatched", int((diff > 0).sum()), "of", diff.size, flush=True)
+    assert diff.max() <= 1 and (diff > 0).mean() < 1e-3

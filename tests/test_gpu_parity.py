@@ -159,6 +159,67 @@ def test_gather_bitexact():
         assert np.array_equal(gv.cpu().numpy(), ref)
 
 
+def test_fused_gather_pap_and_jacobi_update():
+    """The fused PCG entry points through the C ABI. libp_pcg_gather_pap_f64:
+    gv bit-identical to the oracle's gather, p.gv and alpha = rdotz1/p.gv
+    against numpy. libp_pcg_jacobi_update_f64: x, r, z entry by entry as the
+    formulas (within 2 ulps: the compiler may fuse a*b+c; z = invD*r exactly
+    from the kernel's own r), r.r and r.z against numpy,
+    ITERS counted, RDOTZ shifted and beta = r.z/old r.z. A planted STOP flag
+    leaves every output untouched. Ragged sizes: 4x5x3 elements, N=4."""
+    o = OracleElliptic(box(4, 4, curve=0.03))
+    rng = np.random.default_rng(8)
+    vL = rng.standard_normal(o.geti("Nelements") * o.geti("Np"))
+    ref = o.gather(vL, True)
+    n = ref.size
+    p = rng.standard_normal(n)
+    rs, ci, v, pd = dev(o.array("rowStartsT")), dev(o.array("colIdsT")), dev(vL), dev(p)
+    ws = torch.zeros(L().libp_reduce_ws_bytes(), dtype=torch.uint8, device="cuda")
+    s = np.zeros(10)
+    s[0] = 2.5  # RDOTZ1
+    sd = dev(s)
+    gv = torch.zeros(n, dtype=torch.float64, device="cuda")
+    assert L().libp_pcg_gather_pap_f64(n, rs.data_ptr(), ci.data_ptr(), v.data_ptr(), gv.data_ptr(), pd.data_ptr(),
+                                       sd.data_ptr(), ws.data_ptr(), stream()) == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(gv.cpu().numpy(), ref)
+    sh = sd.cpu().numpy()
+    pap = float(p @ ref)
+    assert abs(sh[4] - pap) <= 1e-13 * abs(p) @ np.abs(ref)
+    assert abs(sh[2] - 2.5 / sh[4]) <= 1e-15 * abs(sh[2])
+    # fused Jacobi update with alpha = sh[2]
+    x, r, invD = rng.standard_normal(n), rng.standard_normal(n), rng.uniform(0.5, 2, n)
+    xd, rd, zd, Ad, dd = dev(x), dev(r), torch.zeros(n, dtype=torch.float64, device="cuda"), gv, dev(invD)
+    s2 = sh.copy()
+    s2[7] = 1e-300  # TOL
+    sd2 = dev(s2)
+    assert L().libp_pcg_jacobi_update_f64(n, dd.data_ptr(), pd.data_ptr(), Ad.data_ptr(), xd.data_ptr(), rd.data_ptr(),
+                                          zd.data_ptr(), sd2.data_ptr(), ws.data_ptr(), stream()) == 0
+    torch.cuda.synchronize()
+    alpha = s2[2]
+    eps = np.finfo(np.float64).eps
+    xg, rg, zg = xd.cpu().numpy(), rd.cpu().numpy(), zd.cpu().numpy()
+    assert np.all(np.abs(xg - (x + alpha * p)) <= 2 * eps * (np.abs(x) + np.abs(alpha * p)))
+    assert np.all(np.abs(rg - (r - alpha * ref)) <= 2 * eps * (np.abs(r) + np.abs(alpha * ref)))
+    assert np.array_equal(zg, invD * rg)
+    rr, zr = rg, zg
+    out = sd2.cpu().numpy()
+    assert abs(out[5] - rr @ rr) <= 1e-13 * (rr @ rr)
+    assert abs(out[0] - rr @ zr) <= 1e-13 * abs(rr) @ np.abs(zr)
+    assert out[1] == s2[0] and out[9] == 1.0 and out[8] == 0.0
+    assert abs(out[3] - out[0] / out[1]) <= 1e-15 * abs(out[3])
+    # a raised STOP flag: both entries return without touching anything
+    out[8] = 1.0
+    sd3 = dev(out)
+    before = [t.clone() for t in (xd, rd, zd, gv)]
+    L().libp_pcg_gather_pap_f64(n, rs.data_ptr(), ci.data_ptr(), v.data_ptr(), gv.data_ptr(), pd.data_ptr(),
+                                sd3.data_ptr(), ws.data_ptr(), stream())
+    L().libp_pcg_jacobi_update_f64(n, dd.data_ptr(), pd.data_ptr(), Ad.data_ptr(), xd.data_ptr(), rd.data_ptr(),
+                                   zd.data_ptr(), sd3.data_ptr(), ws.data_ptr(), stream())
+    torch.cuda.synchronize()
+    assert all(torch.equal(a, b) for a, b in zip(before, (xd, rd, zd, gv)))
+    assert np.array_equal(sd3.cpu().numpy(), out)
+
 def test_scatter_bitexact():
     o = OracleElliptic(box(3, 3))
     g = np.random.default_rng(6).standard_normal(o.geti("Ngather"))

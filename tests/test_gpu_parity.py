@@ -592,3 +592,35 @@ def test_gmsh_cube_matches_box(N, precon):
     assert rm["r_final"] <= 1e-10 * rm["r0"] * (1 + 1e-12)
     assert abs(rm["norm"] - rb["norm"]) <= 1e-9 * rb["norm"]
     assert abs(rm["iterations"] - rb["iterations"]) <= 1
+
+
+def test_gmsh_neumann_tags_match_box(tmp_path):
+    """cubeHex.msh with its boundary faces retagged 2 (Neumann, written to a
+    temporary copy) against the BOX mesh with BOX BOUNDARY FLAG 2, screened
+    (lambda = 1) so that the problem is definite: every node is a dof and the
+    solutions agree."""
+    import os
+    import libparanumal_amd as lp
+    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cubeHex.msh")
+    lines = open(src).read().split("\n")
+    out, inside = [], False
+    for ln in lines:
+        if ln.startswith("$Elements"):
+            inside = True
+        elif ln.startswith("$EndElements"):
+            inside = False
+        t = ln.split()
+        if inside and len(t) > 3 and t[1] == "3":  # quadrilateral: id type ntags tag...
+            t[3] = "2"
+            ln = " ".join(t)
+        out.append(ln)
+    msh = tmp_path / "cubeHexNeumann.msh"
+    msh.write_text("\n".join(out))
+    m = lp.Elliptic(msh_settings(3, **{"MESH FILE": str(msh), "LAMBDA": 1.0}))
+    b = lp.Elliptic(dict(msh_settings(3, LAMBDA=1.0), **{
+        "MESH FILE": "BOX", "BOX NX": 6, "BOX NY": 6, "BOX NZ": 6, "BOX DIMX": 2, "BOX DIMY": 2, "BOX DIMZ": 2,
+        "BOX BOUNDARY FLAG": 2}))
+    assert m.geti("Ngather") == b.geti("Ngather") == 19 ** 3
+    rm, rb = m.run(), b.run()
+    print("gmsh neumann", rm["iterations"], rb["iterations"], rm["norm"], rb["norm"], flush=True)
+    assert abs(rm["norm"] - rb["norm"]) <= 1e-9 * rb["norm"]

@@ -185,6 +185,25 @@ int libp_pcg_jacobi_hex1_iteration_f64(int Nelements, const int* GlobalToLocal, 
 /* Stopping test of the last libp_pcg_jacobi_hex1_iteration_f64: sets
  * scalars[RDOTR] = r.r and, when r.r <= scalars[TOL], scalars[STOP] = 1. */
 int libp_pcg_jacobi_hex1_check_f64(int Nrows, double* scalars, void* ws, void* stream);
+/* The same Jacobi-PCG on the N=1 hex operator in its single-reduction form
+ * (Chronopoulos and Gear 1989; the PCG iterates of linearSolverPCG.cpp:107-171
+ * in exact arithmetic, with ellipticPreconJacobi.cpp:42-51 and the N=1
+ * ellipticAxHex3D.okl:156-295): gamma = r.u and delta = w.u (u = invDiag.*r,
+ * w = A u) in one reduction, two launches per iteration. The vector updates
+ * run inside the next Ax launch; owner (one byte per local node, 1 at the
+ * first local copy of each gathered node) says which copy writes them. p2,
+ * s2, r2, u2: host arrays of two device vectors each, used by iteration
+ * parity. mode 0 starts from x = 0 with the right-hand side in r2[0] (parity
+ * 0); mode 1 is iteration 0 (parity 0); mode 2 iteration i >= 1 with parity
+ * i % 2; mode 3 applies the stopping test (r.r <= tol^2 r0.r0) of the last
+ * queued iteration before the host reads scalars[STOP], [ITERS], [RDOTR].
+ * scalars zeroed before mode 0; ws must not be used by other reductions
+ * between the calls. */
+int libp_pcg_jacobi_hex1_cg1_f64(int Nelements, const int* GlobalToLocal, const unsigned char* owner, const double* wJ,
+                                 const double* ggeo, const double* D, double lambda, int Nrows, const int* rowStarts,
+                                 const int* colIds, const double* invDiag, double tol, int mode, int parity, double* x,
+                                 double* w, double* const* p2, double* const* s2, double* const* r2,
+                                 double* const* u2, double* AqL, double* scalars, void* ws, void* stream);
 
 /* The whole fused Jacobi-PCG of libp_pcg_jacobi_hex1_iteration_f64 in one
  * launch (the same counterpart: linearSolverPCG.cpp:107-171 with

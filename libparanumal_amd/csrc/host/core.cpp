@@ -148,6 +148,10 @@ settings_t ellipticDefaultSettings() {
                "(F)PCG in fused launches: p.Ap in the operator's gather; with Jacobi, z = invDiag.*r and r.z in "
                "the update",
                {"TRUE", "FALSE"});
+  s.newSetting("MULTIGRID COARSE SINGLE REDUCTION", "TRUE",
+               "Fused coarse Jacobi-PCG in its single-reduction (Chronopoulos-Gear) form: two launches per "
+               "iteration instead of three",
+               {"TRUE", "FALSE"});
   s.newSetting("MULTIGRID COARSE PERSISTENT", "FALSE",
                "Fused coarse Jacobi-PCG as one persistent kernel with grid barriers (no host reads); slower "
                "than the replayed graphs on a 32^3 box (31.5 against 30.0 ms per p-MG solve)",

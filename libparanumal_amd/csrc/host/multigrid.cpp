@@ -458,6 +458,10 @@ class coarseJacobiPCG_t {
     elliptic->settings.getSetting("MULTIGRID COARSE PERSISTENT", persist);
     persistent = fusedHex1 && persist == "TRUE";
     elliptic->settings.getSetting("MULTIGRID COARSE PERSISTENT BLOCKS", persistentBlocks);
+    // one fused reduction per iteration (Chronopoulos-Gear), two launches
+    std::string cg1s = "TRUE";
+    elliptic->settings.getSetting("MULTIGRID COARSE SINGLE REDUCTION", cg1s);
+    cg1 = fusedHex1 && !persistent && cg1s == "TRUE";
     // the reference's base level: parAlmond AMG on the assembled operator
     // (ellipticPreconMultiGrid.cpp:104-131), one V-cycle per coarse solve
     if (elliptic->settings.compareSetting("MULTIGRID COARSE SOLVER", "AMG")) {
@@ -474,17 +478,24 @@ class coarseJacobiPCG_t {
   bool persistent = false;
   int persistentBlocks = 64;    // 0: one block per CU
   bool iterationsOnDevice = false;  // lastIterations is still in o_s (persistent solve)
+  bool cg1 = false;
   deviceMemory<double> o_s, o_z, o_p, o_Ap, o_AqL;
   deviceMemory<unsigned> o_bar;
+  // single-reduction form: w, the parity pairs of p, s, r (r[0] is o_r), u, owner flags
+  deviceMemory<double> o_w, o_p2[2], o_s2[2], o_r1, o_u2[2];
+  deviceMemory<unsigned char> o_owner;
   // captured chunks of fused iterations, per solution vector and first-iteration flag
   struct chunkGraph {
     double* x;
     int first, n;
     hipGraphExec_t exec;
+    int kind = 0, parity = 0;  // kind 1: single-reduction chunk starting at this parity
   };
   std::vector<chunkGraph> graphs;
   void enqueueChunk(deviceMemory<double>& o_X, int firstIt, int n);
   int solveFusedHex1(deviceMemory<double>& o_X);
+  void enqueueCG1Chunk(deviceMemory<double>& o_X, int itStart, int n);
+  int solveCG1Hex1(deviceMemory<double>& o_X);
   void solvePersistentHex1(deviceMemory<double>& o_X);
   void allocFused();
   // iterations of the last coarse solve (reads them back after a persistent solve)
@@ -594,7 +605,7 @@ void coarseJacobiPCG_t::enqueueChunk(deviceMemory<double>& o_X, int firstIt, int
     return;
   }
   for (auto& g : graphs)
-    if (g.x == o_X.ptr() && g.first == firstIt && g.n == n) {
+    if (g.kind == 0 && g.x == o_X.ptr() && g.first == firstIt && g.n == n) {
       LIBP_HIP(hipGraphLaunch(g.exec, p.stream));
       return;
     }
@@ -602,7 +613,94 @@ void coarseJacobiPCG_t::enqueueChunk(deviceMemory<double>& o_X, int firstIt, int
   LIBP_HIP(hipStreamBeginCapture(p.stream, hipStreamCaptureModeThreadLocal));
   launch();
   LIBP_HIP(hipStreamEndCapture(p.stream, &graph));
-  chunkGraph g{o_X.ptr(), firstIt, n, nullptr};
+  chunkGraph g{o_X.ptr(), firstIt, n, nullptr, 0, 0};
+  LIBP_HIP(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
+  LIBP_HIP(hipGraphDestroy(graph));
+  graphs.push_back(g);
+  LIBP_HIP(hipGraphLaunch(g.exec, p.stream));
+}
+
+/* The fused Jacobi-PCG in its single-reduction form
+ * (libp_pcg_jacobi_hex1_cg1_f64): two launches per iteration, the PCG iterates
+ * in exact arithmetic, the same stopping rule. On entry o_r holds the
+ * right-hand side; it is the parity-0 residual buffer. */
+int coarseJacobiPCG_t::solveCG1Hex1(deviceMemory<double>& o_X) {
+  platform_t& p = elliptic->platform;
+  const dlong N = elliptic->Ndofs;
+  const mesh_t& m = elliptic->mesh;
+  allocFused();
+  if (!o_w.ptr()) {
+    o_w = p.malloc<double>(N);
+    for (int k = 0; k < 2; ++k) {
+      o_p2[k] = p.malloc<double>(N);
+      o_s2[k] = p.malloc<double>(N);
+      o_u2[k] = p.malloc<double>(N);
+    }
+    o_r1 = p.malloc<double>(N);
+    // the first local copy of each gathered node writes its updates
+    const ogs::ogs_t& og = elliptic->ogsMasked;
+    std::vector<unsigned char> own((size_t)m.Nelements * m.Np, 0);
+    for (dlong g = 0; g < og.NlocalT; ++g)
+      if (og.rowStartsT[g + 1] > og.rowStartsT[g]) own[og.colIdsT[og.rowStartsT[g]]] = 1;
+    o_owner = p.malloc(own);
+  }
+  LIBP_CALL(libp_set_f64(LIBP_PCG_NSCALARS, 0.0, o_s.ptr(), p.s()));
+  int it = 0;
+  const int chunk = std::max(1, solver->checkEvery);
+  // first chunk sized from the previous solve, as in solveFusedHex1
+  int n = (lastIterations >= 8) ? (lastIterations / 4) * 4 : chunk;
+  while (it < maxit) {
+    n = std::min(n, maxit - it);
+    enqueueCG1Chunk(o_X, it, n);
+    const double* h = p.readScalars(o_s.ptr(), LIBP_PCG_NSCALARS);
+    it = (int)h[LIBP_PCG_ITERS];
+    if (h[LIBP_PCG_STOP] != 0.0) break;
+    n = std::min(chunk, 4);
+  }
+  return it;
+}
+
+void coarseJacobiPCG_t::enqueueCG1Chunk(deviceMemory<double>& o_X, int itStart, int n) {
+  platform_t& p = elliptic->platform;
+  void* st = p.s();
+  void* ws = p.o_reduceWS.ptr();
+  const mesh_t& m = elliptic->mesh;
+  const dlong N = elliptic->Ndofs;
+  double* P2[2] = {o_p2[0].ptr(), o_p2[1].ptr()};
+  double* S2[2] = {o_s2[0].ptr(), o_s2[1].ptr()};
+  double* R2[2] = {o_r.ptr(), o_r1.ptr()};
+  double* U2[2] = {o_u2[0].ptr(), o_u2[1].ptr()};
+  const int first = itStart == 0;
+  auto call = [&](int mode, int parity) {
+    LIBP_CALL(libp_pcg_jacobi_hex1_cg1_f64(
+        m.Nelements, elliptic->o_GlobalToLocal.ptr(), o_owner.ptr(), m.o_wJ.ptr(), m.o_ggeo.ptr(), m.o_D.ptr(),
+        elliptic->lambda, N, elliptic->ogsMasked.o_rowStartsT.ptr(), elliptic->ogsMasked.o_colIdsT.ptr(),
+        jacobi->o_invDiagA.ptr(), tol, mode, parity, o_X.ptr(), o_w.ptr(), P2, S2, R2, U2, o_AqL.ptr(), o_s.ptr(), ws,
+        st));
+  };
+  auto launch = [&]() {
+    if (first) call(0, 0);
+    for (int c = 0; c < n; ++c) {
+      const int itc = itStart + c;
+      call(itc == 0 ? 1 : 2, itc & 1);
+    }
+    call(3, 0);  // the stopping test of the last iteration
+  };
+  if (!useGraph) {
+    launch();
+    return;
+  }
+  const int parity = itStart & 1;
+  for (auto& g : graphs)
+    if (g.kind == 1 && g.x == o_X.ptr() && g.first == first && g.n == n && g.parity == parity) {
+      LIBP_HIP(hipGraphLaunch(g.exec, p.stream));
+      return;
+    }
+  hipGraph_t graph;
+  LIBP_HIP(hipStreamBeginCapture(p.stream, hipStreamCaptureModeThreadLocal));
+  launch();
+  LIBP_HIP(hipStreamEndCapture(p.stream, &graph));
+  chunkGraph g{o_X.ptr(), first, n, nullptr, 1, parity};
   LIBP_HIP(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
   LIBP_HIP(hipGraphDestroy(graph));
   graphs.push_back(g);
@@ -624,6 +722,10 @@ void coarseJacobiPCG_t::solve<double>(deviceMemory<double>& o_rhs, deviceMemory<
     solvePersistentHex1(o_X);
     return;
   }
+  if (cg1) {
+    lastIterations = solveCG1Hex1(o_X);
+    return;
+  }
   if (fusedHex1) {
     lastIterations = solveFusedHex1(o_X);
     return;
@@ -642,6 +744,8 @@ void coarseJacobiPCG_t::solve<float>(deviceMemory<float>& o_rhs, deviceMemory<fl
     lastIterations = 1;
   } else if (persistent) {
     solvePersistentHex1(o_x);
+  } else if (cg1) {
+    lastIterations = solveCG1Hex1(o_x);
   } else if (fusedHex1) {
     lastIterations = solveFusedHex1(o_x);
   } else {

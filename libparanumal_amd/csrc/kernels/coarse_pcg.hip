@@ -278,6 +278,196 @@ __global__ void __launch_bounds__(64) coarse_check_kernel(const int nb, double* 
 }
 
 // ---------------------------------------------------------------------------
+// One global reduction per iteration: the Chronopoulos-Gear form of PCG
+// (A. T. Chronopoulos and C. W. Gear, J. Comput. Appl. Math. 25 (1989)),
+// which gives the PCG iterates in exact arithmetic. With u = invD r and
+// w = A u:
+//   gamma = r.u, delta = w.u (one fused reduction), beta = gamma/gamma_old,
+//   alpha = gamma / (delta - beta gamma / alpha_old),
+//   p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s, u = invD r.
+// Every vector update is local to a node once alpha and beta are known, so it
+// is folded into the next Ax kernel: each element lane recomputes its nodes'
+// updated values from the old ones, the first local copy of each node (owner
+// flag) writes them, and the Ax runs on the new u. p, s, r, u are double
+// buffered by iteration parity (a block must not read a value another block
+// has already replaced); x is read and written by its owner only. Two
+// kernels per iteration:
+//   cg1_ax:     partials -> gamma, delta, r.r; stop test; alpha, beta;
+//               node updates; AqL = A_e u
+//   cg1_gather: w = gather(AqL); partials of r.u, w.u, r.r
+// Scalars: gamma and alpha of iteration i at [3P] and [3P+1] (P = i%2, so that
+// the blocks reading the previous pair never see block 0 replace it), [2]
+// beta, [5] r.r, [7] TOL, [8] STOP, [9] ITERS.
+struct CG1Vec {
+  double *p, *s, *r, *u;
+};
+
+__global__ void __launch_bounds__(256) coarse_cg1_ax_kernel(
+    const int E, const int* __restrict__ G2L, const unsigned char* __restrict__ owner, const double* __restrict__ wJ,
+    const double* __restrict__ ggeo, const double* __restrict__ D, const double lambda, const int mode,
+    const int P, const double tol2, const int nb, const double* __restrict__ invD, const double* __restrict__ w,
+    double* __restrict__ x, const CG1Vec cur, const CG1Vec nxt, double* s, const ReduceWS* __restrict__ ws,
+    double* __restrict__ AqL) {
+  // mode 0: start (u = invD r, x = 0, no update); 1: first update (beta = 0); 2: later updates
+  __shared__ double s_ab[2];
+  __shared__ int s_stop;
+  if (s[8] != 0.0) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = t >> 3, n = t & 7;
+  const bool live = e < E;
+  Hex1Node v{};
+  int g = -1;
+  bool own = false;
+  double uo = 0, wo = 0, po = 0, so = 0, xo = 0, ro = 0, dg = 0;
+  if (live) {
+    g = G2L[(size_t)e * 8 + n];
+    if (g >= 0) {
+      own = owner[(size_t)e * 8 + n] != 0;
+      ro = cur.r[g];
+      dg = invD[g];
+      if (mode > 0) {
+        uo = cur.u[g];
+        wo = w[g];
+        if (own) xo = x[g];
+        if (mode > 1) {
+          po = cur.p[g];
+          so = cur.s[g];
+        }
+      }
+    }
+    const double* G = ggeo + (size_t)e * 48 + n;
+    v.G00 = G[0];
+    v.G01 = G[8];
+    v.G02 = G[16];
+    v.G11 = G[24];
+    v.G12 = G[32];
+    v.G22 = G[40];
+    v.W = wJ[(size_t)e * 8 + n];
+  }
+  if (threadIdx.x < 64) {
+    double ga = 0, de = 0, rr = 0;
+    if (mode > 0) {
+      ga = coarse_sum_slot(ws, 0, nb);
+      de = coarse_sum_slot(ws, 1, nb);
+      rr = coarse_sum_slot(ws, 2, nb);
+    }
+    if (threadIdx.x == 0) {
+      int stop = 0;
+      double alpha = 0.0, beta = 0.0;
+      if (mode == 1) {
+        // r0.r0 gives the relative threshold; a zero residual stops at once
+        const double T = tol2 * rr;
+        if (rr == 0.0) stop = 1;
+        alpha = ga / de;
+        if (blockIdx.x == 0) s[7] = (T > 0.0) ? T : 1e-300;
+      } else if (mode == 2) {
+        if (s[7] > 0.0 && rr <= s[7])
+          stop = 1;
+        else {
+          const int q = 3 * (P ^ 1);  // the previous iteration's pair
+          beta = ga / s[q];
+          alpha = ga / (de - beta * ga / s[q + 1]);
+        }
+      }
+      s_stop = stop;
+      s_ab[0] = alpha;
+      s_ab[1] = beta;
+      if (blockIdx.x == 0 && mode > 0) {
+        s[5] = rr;
+        if (stop) {
+          s[8] = 1.0;
+        } else {
+          s[3 * P] = ga;
+          s[3 * P + 1] = alpha;
+          s[2] = beta;
+          s[9] += 1.0;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (s_stop) return;  // the same decision in every block
+  double un = 0.0;
+  if (live && g >= 0) {
+    if (mode == 0) {
+      un = 1.0 * dg * ro;
+      if (own) {
+        x[g] = 0.0;
+        cur.u[g] = un;  // mode 0 writes u0 in place: nothing reads u in this launch
+      }
+    } else {
+      const double alpha = s_ab[0], beta = s_ab[1];
+      const double pn = (mode == 1) ? uo : uo + beta * po;
+      const double sn = (mode == 1) ? wo : wo + beta * so;
+      const double rn = ro - alpha * sn;
+      un = 1.0 * dg * rn;
+      if (own) {
+        x[g] = xo + alpha * pn;
+        nxt.p[g] = pn;
+        nxt.s[g] = sn;
+        nxt.r[g] = rn;
+        nxt.u[g] = un;
+      }
+    }
+  }
+  v.q = un;
+  const double a = hex1_apply(v, n, D, lambda);
+  if (live) AqL[(size_t)e * 8 + n] = a;
+}
+
+// w = gather(AqL); partials of r.u, w.u and r.r for the next cg1_ax
+__global__ void __launch_bounds__(RBS) coarse_cg1_gather_kernel(const int Nrows, const int* __restrict__ rowStarts,
+                                                                const int* __restrict__ colIds,
+                                                                const double* __restrict__ AqL,
+                                                                const double* __restrict__ r,
+                                                                const double* __restrict__ u, double* __restrict__ w,
+                                                                const double* s, ReduceWS* __restrict__ ws) {
+  __shared__ double s_w[16];
+  if (s[8] != 0.0) return;
+  double ga = 0, de = 0, rr = 0;
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < Nrows; g += gridDim.x * blockDim.x) {
+    const int a = rowStarts[g], b = rowStarts[g + 1];
+    double wg = 0;
+    {
+      const int len = b - a;
+      int c[8];
+      double xv[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) c[k] = (k < len) ? colIds[a + k] : 0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) xv[k] = (k < len) ? AqL[c[k]] : 0.0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (k < len) wg += xv[k];
+      for (int j = a + 8; j < b; ++j) wg += AqL[colIds[j]];
+    }
+    w[g] = wg;
+    const double rg = r[g], ug = u[g];
+    ga += rg * ug;
+    de += wg * ug;
+    rr += rg * rg;
+  }
+  const double b0 = block_sum(ga, s_w);
+  const double b1 = block_sum(de, s_w);
+  const double b2 = block_sum(rr, s_w);
+  if (threadIdx.x == 0) {
+    ws->part[0][blockIdx.x] = b0;
+    ws->part[1][blockIdx.x] = b1;
+    ws->part[2][blockIdx.x] = b2;
+  }
+}
+
+// stopping test on the r.r partials of the last cg1_gather (one wavefront)
+__global__ void __launch_bounds__(64) coarse_cg1_check_kernel(const int nb, double* s, const ReduceWS* __restrict__ ws) {
+  if (s[8] != 0.0) return;
+  const double rr = coarse_sum_slot(ws, 2, nb);
+  if (threadIdx.x == 0) {
+    s[5] = rr;
+    if (s[7] > 0.0 && rr <= s[7]) s[8] = 1.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // The whole coarse solve in one launch.
 //
 // Phases A, B and C above become phases of one persistent kernel separated by
@@ -563,4 +753,33 @@ extern "C" int libp_pcg_jacobi_hex1_solve_blocks(int* maxBlocks) {
     return launch_status("pcg_jacobi_hex1_solve_blocks");
   *maxBlocks = std::min(cus * perCU, RMAXB);
   return LIBP_OK;
+}
+
+extern "C" int libp_pcg_jacobi_hex1_cg1_f64(int Nelements, const int* GlobalToLocal, const unsigned char* owner,
+                                            const double* wJ, const double* ggeo, const double* D, double lambda,
+                                            int Nrows, const int* rowStarts, const int* colIds, const double* invDiag,
+                                            double tol, int mode, int parity, double* x, double* w, double* const* p2,
+                                            double* const* s2, double* const* r2, double* const* u2, double* AqL,
+                                            double* scalars, void* ws, void* stream) {
+  if (Nelements <= 0 || Nrows <= 0) return LIBP_OK;
+  if (!GlobalToLocal || !owner || !wJ || !ggeo || !D || !rowStarts || !colIds || !invDiag || !x || !w || !p2 ||
+      !s2 || !r2 || !u2 || !AqL || !scalars || !ws)
+    return arg_error("pcg_jacobi_hex1_cg1", "null pointer");
+  if (mode < 0 || mode > 3 || (parity & ~1)) return arg_error("pcg_jacobi_hex1_cg1", "mode 0..3, parity 0 or 1");
+  hipStream_t st = as_stream(stream);
+  const int nb = coarse_row_blocks(Nrows);
+  if (mode == 3) {
+    hipLaunchKernelGGL(coarse_cg1_check_kernel, dim3(1), dim3(64), 0, st, nb, scalars, (const ReduceWS*)ws);
+    return launch_status("pcg_jacobi_hex1_cg1 (check)");
+  }
+  const int a = parity, b = parity ^ 1;
+  const CG1Vec cur{p2[a], s2[a], r2[a], u2[a]}, nxt{p2[b], s2[b], r2[b], u2[b]};
+  hipLaunchKernelGGL(coarse_cg1_ax_kernel, dim3((8 * (long)Nelements + 255) / 256), dim3(256), 0, st, Nelements,
+                     GlobalToLocal, owner, wJ, ggeo, D, lambda, mode, parity, tol * tol, nb, invDiag, w, x, cur, nxt,
+                     scalars, (const ReduceWS*)ws, AqL);
+  // the gather reads the vectors this launch wrote: the start wrote u in place
+  const CG1Vec& out = (mode == 0) ? cur : nxt;
+  hipLaunchKernelGGL(coarse_cg1_gather_kernel, dim3(nb), dim3(RBS), 0, st, Nrows, rowStarts, colIds, AqL, out.r,
+                     out.u, w, scalars, (ReduceWS*)ws);
+  return launch_status("pcg_jacobi_hex1_cg1");
 }

@@ -507,6 +507,35 @@ def test_coarse_persistent_matches_fused(blocks):
             assert 1 < pers.geti("coarseIterations") < 40
 
 
+def test_coarse_single_reduction_matches_three_kernel():
+    """The coarse Jacobi-PCG in its single-reduction form (Chronopoulos-Gear,
+    two launches per iteration) against the three-launch form: with a fixed
+    count of coarse iterations (tolerance 0) the p-MG solves agree to rounding
+    and take the same outer iterations; with a tolerance the coarse counts
+    agree to +-1. Repeated solves (graph replays, parity-alternating buffers)
+    are bit-identical."""
+    import libparanumal_amd as lp
+    for ctol in (0.0, 1e-2):
+        base = mg_settings("FP32", **{"LINEAR SOLVER STOPPING CRITERION": "RELATIVE", "BOX NX": 4,
+                                      "MULTIGRID COARSE TOLERANCE": ctol, "MULTIGRID COARSE MAXIT": 40})
+        three = lp.Elliptic(dict(base, **{"MULTIGRID COARSE SINGLE REDUCTION": "FALSE"}))
+        one = lp.Elliptic(dict(base, **{"MULTIGRID COARSE SINGLE REDUCTION": "TRUE"}))
+        a = three.run()
+        runs = [one.run() for _ in range(3)]
+        b = runs[0]
+        print("single reduction", ctol, a["iterations"], b["iterations"], a["norm"], b["norm"],
+              three.geti("coarseIterations"), one.geti("coarseIterations"), flush=True)
+        assert all(r["norm"] == b["norm"] and r["iterations"] == b["iterations"] for r in runs)
+        assert abs(a["iterations"] - b["iterations"]) <= (0 if ctol == 0 else 1)
+        assert abs(a["norm"] - b["norm"]) <= 1e-9 * abs(a["norm"])
+        assert b["r_final"] <= 1e-8 * b["r0"] * (1 + 1e-12)
+        if ctol == 0:
+            assert one.geti("coarseIterations") == three.geti("coarseIterations") == 40
+            x1, x3 = one.solution(), three.solution()
+            assert np.max(np.abs(x1 - x3)) <= 1e-9 * np.max(np.abs(x3))
+        else:
+            assert abs(one.geti("coarseIterations") - three.geti("coarseIterations")) <= 1
+
 def test_coarse_persistent_vcycle_vs_oracle():
     """The V-cycle tests above run the default chunked coarse solve; this one runs
     the persistent one-launch coarse solve (15 coarse iterations) against the

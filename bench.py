@@ -244,8 +244,8 @@ def main():
         print(json.dumps(out))
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(the path would shard by element slabs with a halo exchange, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the path would shard by "
+              "element slabs with a halo exchange of the shared nodes, which is not built", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -73,6 +73,18 @@ int libp_ogs_gather_f64(int Nrows, const int* rowStarts, const int* colIds, cons
                         void* stream);
 int libp_ogs_gather_f32(int Nrows, const int* rowStarts, const int* colIds, const float* v, float* gv,
                         void* stream);
+/* Gather fused with the step after it (no reference counterpart as one call:
+ * ogsOperator.cpp:186-201 followed by linAlg.axpy, linAlg.cpp:37-224):
+ * residual: gv = r - gather(v), the r - A x of an operator whose last step is
+ * the gather; add: gv += gather(v). Bit-identical to the two calls. */
+int libp_ogs_gather_residual_f64(int Nrows, const int* rowStarts, const int* colIds, const double* v, const double* r,
+                                 double* gv, void* stream);
+int libp_ogs_gather_residual_f32(int Nrows, const int* rowStarts, const int* colIds, const float* v, const float* r,
+                                 float* gv, void* stream);
+int libp_ogs_gather_add_f64(int Nrows, const int* rowStarts, const int* colIds, const double* v, double* gv,
+                            void* stream);
+int libp_ogs_gather_add_f32(int Nrows, const int* rowStarts, const int* colIds, const float* v, float* gv,
+                            void* stream);
 /* Replaces "scatter" (ogsKernels.okl:127-164, ogsOperator.cpp:295-321):
  * v[colIds[g]] = gv[row] for every g of the row. */
 int libp_ogs_scatter_f64(int Nrows, const int* rowStarts, const int* colIds, const double* gv, double* v,
@@ -266,6 +278,15 @@ int libp_mg_coarsen_hex_f64(int Nf, int Nc, int Nelements, const int* elementLis
                             const double* P, const double* qf, double* qc, void* stream);
 int libp_mg_coarsen_hex_f32(int Nf, int Nc, int Nelements, const int* elementList, const int* GlobalToLocal,
                             const float* P, const float* qf, float* qc, void* stream);
+/* The coarsening with the fine residual's inverse-multiplicity weight
+ * (ellipticPreconMultiGridLevel.cpp:51) applied as the kernel loads it:
+ * coarsen(weight .* qf), bit-identical to an amxpy pass followed by the
+ * coarsening. */
+int libp_mg_coarsen_weighted_hex_f64(int Nf, int Nc, int Nelements, const int* elementList, const int* GlobalToLocal,
+                                     const double* P, const double* weight, const double* qf, double* qc,
+                                     void* stream);
+int libp_mg_coarsen_weighted_hex_f32(int Nf, int Nc, int Nelements, const int* elementList, const int* GlobalToLocal,
+                                     const float* P, const float* weight, const float* qf, float* qc, void* stream);
 /* Replaces ellipticPartialPreconProlongateHex3D (okl/ellipticPreconProlongateHex3D.okl:216-301),
  * called by MGLevel::prolongate (ellipticPreconMultiGridLevel.cpp:84-125). */
 int libp_mg_prolong_hex_f64(int Nf, int Nc, int Nelements, const int* elementList, const int* GlobalToLocalC,
@@ -371,6 +392,10 @@ int libp_learned_features_f32(int Nelements, int Np, int Ntr, const int* GlobalT
 int libp_learned_assemble_f32(int Nrows, const int* rowStarts, const int* colIds, int Np, const float* Y, long ldy,
                               const float* scale, const float* omega, const float* r, const float* invDiag, float* c,
                               void* stream);
+/* The same assembly added to c (c += correction: the smoother's x += c). */
+int libp_learned_assemble_add_f32(int Nrows, const int* rowStarts, const int* colIds, int Np, const float* Y,
+                                  long ldy, const float* scale, const float* omega, const float* r,
+                                  const float* invDiag, float* c, void* stream);
 /* Energy-optimal step: sc[0] = c.res, sc[1] = c.Ac, sc[2] = alpha = sc[0]/sc[1]
  * (0, and sc[3] += 1, if not finite or c.Ac <= 0). ws: libp_reduce_ws_bytes(). */
 int libp_learned_alpha_f32(int N, const float* c, const float* res, const float* Ac, double* sc, void* ws,

@@ -61,6 +61,9 @@ class elliptic_t : public operator_t {
 
   // ellipticOperator.cpp:29-107 (single launch over all local elements)
   void Operator(deviceMemory<dfloat>& o_q, deviceMemory<dfloat>& o_Aq) override;
+  // res = r - A q, the subtraction fused into the operator's gather (FP64 and FP32)
+  void OperatorResidual(deviceMemory<dfloat>& o_q, deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_res);
+  void OperatorResidualF32(deviceMemory<float>& o_q, deviceMemory<float>& o_r, deviceMemory<float>& o_res);
   // the operator's gather fused with the PCG p.Ap (libp_pcg_gather_pap_f64)
   bool OperatorPAP(deviceMemory<dfloat>& o_q, deviceMemory<dfloat>& o_Aq, double* scalars, void* ws) override;
   void PartialAx(deviceMemory<dfloat>& o_q, deviceMemory<dfloat>& o_AqL);

@@ -251,7 +251,7 @@ void learnedSmoother_t::save(const std::string& path) const {
   LIBP_ABORT("learned smoother: write failed for " << path, !f);
 }
 
-void learnedSmoother_t::correction(const float* r, const float* invDiag) {
+void learnedSmoother_t::correction(const float* r, const float* invDiag, float* out, bool accumulate) {
   void* s = platform.s();
   const unsigned short* Wb = o_thetaBf.ptr();
   const float* th = o_theta.ptr();
@@ -266,27 +266,25 @@ void learnedSmoother_t::correction(const float* r, const float* invDiag) {
   }
   // the linear smoother's omega only changes through setParameters: skip the term when it is zero
   const float* om = (arch == ARCH_LINEAR && omegaHost == 0.f) ? nullptr : th + oOm;
-  LIBP_CALL(libp_learned_assemble_f32(Ndofs, elliptic.ogsMasked.o_rowStartsT.ptr(), elliptic.ogsMasked.o_colIdsT.ptr(),
-                                      Np, o_Y.ptr(), nOut, o_scale.ptr(), om, r, invDiag, o_c.ptr(), s));
+  const int* rs = elliptic.ogsMasked.o_rowStartsT.ptr();
+  const int* ci = elliptic.ogsMasked.o_colIdsT.ptr();
+  if (accumulate)
+    LIBP_CALL(libp_learned_assemble_add_f32(Ndofs, rs, ci, Np, o_Y.ptr(), nOut, o_scale.ptr(), om, r, invDiag, out, s));
+  else
+    LIBP_CALL(libp_learned_assemble_f32(Ndofs, rs, ci, Np, o_Y.ptr(), nOut, o_scale.ptr(), om, r, invDiag,
+                                        out ? out : o_c.ptr(), s));
 }
 
 void learnedSmoother_t::smoothPre(deviceMemory<float>& o_r, deviceMemory<float>& o_x, deviceMemory<float>* o_resOut,
-                                  const float* invDiag, const op32_t& A) {
+                                  const float* invDiag, const op32_t& A, const res32_t& R) {
   void* s = platform.s();
   if (arch == ARCH_LINEAR) {  // x = c(r), then x += c(r - A x) for further sweeps
-    const size_t bytes = Ndofs * sizeof(float);
-    correction(o_r.ptr(), invDiag);
-    LIBP_HIP(hipMemcpyAsync(o_x.ptr(), o_c.ptr(), bytes, hipMemcpyDeviceToDevice, platform.stream));
+    correction(o_r.ptr(), invDiag, o_x.ptr(), false);
     for (int k = 1; k < sweeps; ++k) {
-      A(o_x, o_res);
-      LIBP_CALL(libp_axpy_f32(Ndofs, 1.f, o_r.ptr(), -1.f, o_res.ptr(), s));
-      correction(o_res.ptr(), invDiag);
-      LIBP_CALL(libp_axpy_f32(Ndofs, 1.f, o_c.ptr(), 1.f, o_x.ptr(), s));
+      R(o_x, o_r, o_res);
+      correction(o_res.ptr(), invDiag, o_x.ptr(), true);
     }
-    if (o_resOut) {
-      A(o_x, *o_resOut);
-      LIBP_CALL(libp_axpy_f32(Ndofs, 1.f, o_r.ptr(), -1.f, o_resOut->ptr(), s));
-    }
+    if (o_resOut) R(o_x, o_r, *o_resOut);
     return;
   }
   correction(o_r.ptr(), invDiag);
@@ -316,19 +314,16 @@ void learnedSmoother_t::augment(deviceMemory<float>& o_x, deviceMemory<float>& o
 }
 
 void learnedSmoother_t::smoothPost(deviceMemory<float>& o_r, deviceMemory<float>& o_x, const float* invDiag,
-                                   const op32_t& A) {
+                                   const op32_t& A, const res32_t& R) {
   void* s = platform.s();
   if (arch == ARCH_LINEAR) {  // sweeps of x += c(r - A x)
     for (int k = 0; k < sweeps; ++k) {
-      A(o_x, o_res);
-      LIBP_CALL(libp_axpy_f32(Ndofs, 1.f, o_r.ptr(), -1.f, o_res.ptr(), s));
-      correction(o_res.ptr(), invDiag);
-      LIBP_CALL(libp_axpy_f32(Ndofs, 1.f, o_c.ptr(), 1.f, o_x.ptr(), s));
+      R(o_x, o_r, o_res);
+      correction(o_res.ptr(), invDiag, o_x.ptr(), true);
     }
     return;
   }
-  A(o_x, o_res);
-  LIBP_CALL(libp_axpy_f32(Ndofs, 1.f, o_r.ptr(), -1.f, o_res.ptr(), s));
+  R(o_x, o_r, o_res);
   correction(o_res.ptr(), invDiag);
   A(o_c, o_Ac);
   LIBP_CALL(libp_learned_alpha_f32(Ndofs, o_c.ptr(), o_res.ptr(), o_Ac.ptr(), o_sc.ptr(), o_rws.ptr(), s));

@@ -40,6 +40,8 @@ class elliptic_t;
 class learnedSmoother_t {
  public:
   typedef std::function<void(deviceMemory<float>&, deviceMemory<float>&)> op32_t;
+  // (x, r, res): res = r - A x
+  typedef std::function<void(deviceMemory<float>&, deviceMemory<float>&, deviceMemory<float>&)> res32_t;
 
   enum { ARCH_MLP = 0, ARCH_LINEAR = 1 };
 
@@ -82,15 +84,17 @@ class learnedSmoother_t {
   void setParameters(const std::vector<float>& th);
 
   // c = c(r) in o_c (features, three layers, assembly)
-  void correction(const float* r, const float* invDiag);
+  // the correction c(r) into o_c, or into out (added to it when accumulate)
+  void correction(const float* r, const float* invDiag, float* out = nullptr, bool accumulate = false);
   // pre-smoothing: x = alpha c(r); res = r - alpha A c when res != null (the
   // V-cycle residual comes free with the step)
   void smoothPre(deviceMemory<float>& o_r, deviceMemory<float>& o_x, deviceMemory<float>* o_resOut,
-                 const float* invDiag, const op32_t& A);
+                 const float* invDiag, const op32_t& A, const res32_t& R);
   // one network step on a given residual: x += alpha c(res), res -= alpha A c
   void augment(deviceMemory<float>& o_x, deviceMemory<float>& o_res, const float* invDiag, const op32_t& A);
   // post-smoothing: res = r - A x, x += alpha c(res)
-  void smoothPost(deviceMemory<float>& o_r, deviceMemory<float>& o_x, const float* invDiag, const op32_t& A);
+  void smoothPost(deviceMemory<float>& o_r, deviceMemory<float>& o_x, const float* invDiag, const op32_t& A,
+                  const res32_t& R);
 
   // loss and gradient of one sample (r, target eh: FP32, gathered; *nrm = eh.A eh on
   // the device) into o_grad, then one Adam step when update is set

@@ -104,6 +104,16 @@ struct prec<double> {
     return libp_cheb_step_f64(N, iD, Ad, c1, c2, R, d, X, s);
   }
   static void op(elliptic_t& e, deviceMemory<double>& x, deviceMemory<double>& y) { e.Operator(x, y); }
+  static void res(elliptic_t& e, deviceMemory<double>& x, deviceMemory<double>& r, deviceMemory<double>& y) {
+    e.OperatorResidual(x, r, y);
+  }
+  static int coarsenW(int Nf, int Nc, int E, const int* g2l, const double* P, const double* w, const double* in,
+                      double* out, void* s) {
+    return libp_mg_coarsen_weighted_hex_f64(Nf, Nc, E, nullptr, g2l, P, w, in, out, s);
+  }
+  static int gatherAdd(int N, const int* rs, const int* ci, const double* v, double* gv, void* s) {
+    return libp_ogs_gather_add_f64(N, rs, ci, v, gv, s);
+  }
   static deviceMemory<double> fromDouble(platform_t& p, deviceMemory<double>& v, dlong) { return v; }
 };
 template <>
@@ -127,6 +137,16 @@ struct prec<float> {
     return libp_cheb_step_f32(N, iD, Ad, c1, c2, R, d, X, s);
   }
   static void op(elliptic_t& e, deviceMemory<float>& x, deviceMemory<float>& y) { e.OperatorF32(x, y); }
+  static void res(elliptic_t& e, deviceMemory<float>& x, deviceMemory<float>& r, deviceMemory<float>& y) {
+    e.OperatorResidualF32(x, r, y);
+  }
+  static int coarsenW(int Nf, int Nc, int E, const int* g2l, const float* P, const float* w, const float* in,
+                      float* out, void* s) {
+    return libp_mg_coarsen_weighted_hex_f32(Nf, Nc, E, nullptr, g2l, P, w, in, out, s);
+  }
+  static int gatherAdd(int N, const int* rs, const int* ci, const float* v, float* gv, void* s) {
+    return libp_ogs_gather_add_f32(N, rs, ci, v, gv, s);
+  }
   static deviceMemory<float> fromDouble(platform_t& p, deviceMemory<double>& v, dlong n) {
     deviceMemory<float> f = p.malloc<float>(n);
     LIBP_CALL(libp_convert_f64_to_f32(n, v.ptr(), f.ptr(), p.s()));
@@ -270,29 +290,29 @@ class MGLevel : public parAlmond::multigridLevel<T> {
 
   void Operator(deviceMemory<T>& o_X, deviceMemory<T>& o_Ax) override { prec<T>::op(*elliptic, o_X, o_Ax); }
 
+  // RES = RHS - A X, the subtraction fused into the operator's gather
   void residual(deviceMemory<T>& o_RHS, deviceMemory<T>& o_X, deviceMemory<T>& o_RES) override {
-    Operator(o_X, o_RES);
-    LIBP_CALL(prec<T>::axpy(elliptic->Ndofs, T(1), o_RHS.ptr(), T(-1), o_RES.ptr(), platform.s()));
+    prec<T>::res(*elliptic, o_X, o_RHS, o_RES);
   }
 
-  // MGLevel::coarsen (ellipticPreconMultiGridLevel.cpp:41-82)
+  // MGLevel::coarsen (ellipticPreconMultiGridLevel.cpp:41-82); the inverse
+  // multiplicity weight is applied as the kernel loads the fine residual
   void coarsen(deviceMemory<T>& o_X, deviceMemory<T>& o_Rx) override {
-    deviceMemory<T>& o_wx = scratch->RES;
     deviceMemory<T>& o_RxL = scratch->transfer;
-    LIBP_CALL(prec<T>::amxpy(elliptic->Ndofs, T(1), o_weightG.ptr(), o_X.ptr(), T(0), o_wx.ptr(), platform.s()));
-    LIBP_CALL(prec<T>::coarsen(Nf, Nc, elliptic->mesh.Nelements, elliptic->o_GlobalToLocal.ptr(), o_P.ptr(),
-                               o_wx.ptr(), o_RxL.ptr(), platform.s()));
+    LIBP_CALL(prec<T>::coarsenW(Nf, Nc, elliptic->mesh.Nelements, elliptic->o_GlobalToLocal.ptr(), o_P.ptr(),
+                                o_weightG.ptr(), o_X.ptr(), o_RxL.ptr(), platform.s()));
     ellipticC->ogsMasked.Gather(o_Rx, o_RxL, 1, ogs::Add, ogs::Trans);
   }
 
-  // MGLevel::prolongate (ellipticPreconMultiGridLevel.cpp:84-125)
+  // MGLevel::prolongate (ellipticPreconMultiGridLevel.cpp:84-125); Px += the
+  // gathered prolongation in the gather itself
   void prolongate(deviceMemory<T>& o_X, deviceMemory<T>& o_Px) override {
-    deviceMemory<T>& o_PxG = scratch->RES;
     deviceMemory<T>& o_PxL = scratch->transfer;
     LIBP_CALL(prec<T>::prolong(Nf, Nc, ellipticC->mesh.Nelements, ellipticC->o_GlobalToLocal.ptr(), o_P.ptr(),
                                o_X.ptr(), o_PxL.ptr(), platform.s()));
-    elliptic->ogsMasked.Gather(o_PxG, o_PxL, 1, ogs::Add, ogs::NoTrans);
-    LIBP_CALL(prec<T>::axpy(elliptic->Ndofs, T(1), o_PxG.ptr(), T(1), o_Px.ptr(), platform.s()));
+    const ogs::ogs_t& og = elliptic->ogsMasked;
+    LIBP_CALL(prec<T>::gatherAdd(og.NlocalP, og.o_rowStartsN.ptr(), og.o_colIdsN.ptr(), o_PxL.ptr(), o_Px.ptr(),
+                                 platform.s()));
   }
 
   bool learnedUsable() const { return learned && learned->active && !learned->fallback; }
@@ -400,10 +420,11 @@ template <>
 void MGLevel<float>::smoothLearned(deviceMemory<float>& o_r, deviceMemory<float>& o_X, bool xIsZero,
                                    deviceMemory<float>* o_RES) {
   auto A = [this](deviceMemory<float>& x, deviceMemory<float>& y) { Operator(x, y); };
+  auto R = [this](deviceMemory<float>& x, deviceMemory<float>& r, deviceMemory<float>& res) { residual(r, x, res); };
   if (xIsZero) {
-    learned->smoothPre(o_r, o_X, o_RES, o_invDiagA.ptr(), A);
+    learned->smoothPre(o_r, o_X, o_RES, o_invDiagA.ptr(), A, R);
   } else {
-    learned->smoothPost(o_r, o_X, o_invDiagA.ptr(), A);
+    learned->smoothPost(o_r, o_X, o_invDiagA.ptr(), A, R);
   }
 }
 template <>

@@ -74,7 +74,9 @@ __global__ void __launch_bounds__(256) learned_features_kernel(const int E, cons
 // c[g] = invLen * sum_j s_e Y[j] + omega * invD[g] r[g]; omega read from device
 // memory, the term skipped when omega is null. SHIFT = log2(Np) when Np is a
 // power of two (the element of a local id is then a shift), else -1.
-template <int SHIFT>
+// ACC: c += the correction (the smoother's x += c in the same pass; the sum
+// rounds once, as the separate axpy it replaces does).
+template <int SHIFT, bool ACC = false>
 __global__ void __launch_bounds__(256) learned_assemble_kernel(const int Nrows, const int* __restrict__ rowStarts,
                                                                const int* __restrict__ colIds, const int Np,
                                                                const float* __restrict__ Y, const long ldy,
@@ -124,7 +126,10 @@ __global__ void __launch_bounds__(256) learned_assemble_kernel(const int Nrows, 
       }
       float v = (len[r2] > 0 ? acc / (float)len[r2] : 0.f);
       if (omega) v += om * invD[g] * r[g];
-      c[g] = v;
+      if (ACC)
+        c[g] = 1.f * v + 1.f * c[g];
+      else
+        c[g] = v;
     }
   }
 }
@@ -309,24 +314,36 @@ extern "C" int libp_learned_features_f32(int Nelements, int Np, int Ntr, const i
   return launch_status("learned_features");
 }
 
-extern "C" int libp_learned_assemble_f32(int Nrows, const int* rowStarts, const int* colIds, int Np, const float* Y,
-                                         long ldy, const float* scale, const float* omega, const float* r,
-                                         const float* invDiag, float* c, void* stream) {
+template <bool ACC>
+static int assemble(int Nrows, const int* rowStarts, const int* colIds, int Np, const float* Y, long ldy,
+                    const float* scale, const float* omega, const float* r, const float* invDiag, float* c,
+                    void* stream) {
   if (Nrows <= 0) return LIBP_OK;
   if (!rowStarts || !colIds || !Y || !scale || !r || !invDiag || !c) return arg_error("learned_assemble", "null pointer");
   if (Np <= 0 || ldy < Np) return arg_error("learned_assemble", "bad size");
   const dim3 grid(stream_blocks(Nrows, 1024)), block(256);
   hipStream_t st = as_stream(stream);
   if (Np == 512)
-    hipLaunchKernelGGL(learned_assemble_kernel<9>, grid, block, 0, st, Nrows, rowStarts, colIds, Np, Y, ldy, scale,
-                       omega, r, invDiag, c);
+    hipLaunchKernelGGL((learned_assemble_kernel<9, ACC>), grid, block, 0, st, Nrows, rowStarts, colIds, Np, Y, ldy,
+                       scale, omega, r, invDiag, c);
   else if (Np == 64)
-    hipLaunchKernelGGL(learned_assemble_kernel<6>, grid, block, 0, st, Nrows, rowStarts, colIds, Np, Y, ldy, scale,
-                       omega, r, invDiag, c);
+    hipLaunchKernelGGL((learned_assemble_kernel<6, ACC>), grid, block, 0, st, Nrows, rowStarts, colIds, Np, Y, ldy,
+                       scale, omega, r, invDiag, c);
   else
-    hipLaunchKernelGGL(learned_assemble_kernel<-1>, grid, block, 0, st, Nrows, rowStarts, colIds, Np, Y, ldy, scale,
-                       omega, r, invDiag, c);
+    hipLaunchKernelGGL((learned_assemble_kernel<-1, ACC>), grid, block, 0, st, Nrows, rowStarts, colIds, Np, Y, ldy,
+                       scale, omega, r, invDiag, c);
   return launch_status("learned_assemble");
+}
+
+extern "C" int libp_learned_assemble_f32(int Nrows, const int* rowStarts, const int* colIds, int Np, const float* Y,
+                                         long ldy, const float* scale, const float* omega, const float* r,
+                                         const float* invDiag, float* c, void* stream) {
+  return assemble<false>(Nrows, rowStarts, colIds, Np, Y, ldy, scale, omega, r, invDiag, c, stream);
+}
+extern "C" int libp_learned_assemble_add_f32(int Nrows, const int* rowStarts, const int* colIds, int Np,
+                                             const float* Y, long ldy, const float* scale, const float* omega,
+                                             const float* r, const float* invDiag, float* c, void* stream) {
+  return assemble<true>(Nrows, rowStarts, colIds, Np, Y, ldy, scale, omega, r, invDiag, c, stream);
 }
 
 extern "C" int libp_learned_alpha_f32(int N, const float* c, const float* res, const float* Ac, double* sc, void* ws,

@@ -13,7 +13,8 @@ namespace libp_amd {
 template <int NqF, typename T>
 __global__ void __launch_bounds__(NqF * NqF) mg_coarsen_kernel(const int NqC, const int* __restrict__ elementList,
                                                                const int* __restrict__ G2L, const T* __restrict__ P,
-                                                               const T* __restrict__ qf, T* __restrict__ qc) {
+                                                               const T* __restrict__ qf, const T* __restrict__ w,
+                                                               T* __restrict__ qc) {
   constexpr int NpF = NqF * NqF * NqF;
   // LDS: s_f is dead after the k sweep, so the j sweep writes into it (s_ccf).
   // Sizes keep NqF=16 in FP64 within 64 KiB.
@@ -27,7 +28,9 @@ __global__ void __launch_bounds__(NqF * NqF) mg_coarsen_kernel(const int NqC, co
   const int NpC = NqC * NqC * NqC;
   for (int k = 0; k < NqF; ++k) {
     const int id = G2L[(size_t)element * NpF + k * NqF * NqF + t];
-    s_f[k][j][i] = (id >= 0) ? qf[id] : T(0);
+    // w: the inverse-multiplicity weight of the fine residual (ellipticPreconMultiGridLevel.cpp:51),
+    // applied here instead of in a separate pass (the same single rounding)
+    s_f[k][j][i] = (id >= 0) ? (w ? T(1) * w[id] * qf[id] : qf[id]) : T(0);
   }
   if (t < NqF * NqC) s_P[t / NqC][t % NqC] = P[t];
   __syncthreads();
@@ -97,7 +100,7 @@ __global__ void __launch_bounds__(NqF * NqF) mg_prolong_kernel(const int NqC, co
 
 template <typename T, bool COARSEN>
 static int transfer(int Nf, int Nc, int Nelements, const int* elist, const int* g2l, const T* P, const T* in, T* out,
-                    void* stream) {
+                    void* stream, const T* w = nullptr) {
   if (Nelements <= 0) return LIBP_OK;
   if (Nc < 1 || Nc >= Nf || Nf > 15) return arg_error("mg_transfer", "need 1 <= Nc < Nf <= 15");
   if (!g2l || !P || !in || !out) return arg_error("mg_transfer", "null pointer");
@@ -108,7 +111,7 @@ static int transfer(int Nf, int Nc, int Nelements, const int* elist, const int* 
   case NQ:                                                                                                       \
     if (COARSEN)                                                                                                 \
       hipLaunchKernelGGL((mg_coarsen_kernel<NQ, T>), dim3(Nelements), dim3(NQ * NQ), 0, st, NqC, elist, g2l, P, \
-                         in, out);                                                                               \
+                         in, w, out);                                                                            \
     else                                                                                                         \
       hipLaunchKernelGGL((mg_prolong_kernel<NQ, T>), dim3(Nelements), dim3(NQ * NQ), 0, st, NqC, elist, g2l, P,  \
                          in, out);                                                                               \
@@ -157,4 +160,17 @@ extern "C" int libp_mg_prolong_hex_f32(int Nf, int Nc, int Nelements, const int*
                                        const int* GlobalToLocalC, const float* P, const float* qc, float* qf,
                                        void* stream) {
   return transfer<float, false>(Nf, Nc, Nelements, elementList, GlobalToLocalC, P, qc, qf, stream);
+}
+
+extern "C" int libp_mg_coarsen_weighted_hex_f64(int Nf, int Nc, int Nelements, const int* elementList,
+                                                const int* GlobalToLocal, const double* P, const double* weight,
+                                                const double* qf, double* qc, void* stream) {
+  if (!weight) return arg_error("mg_coarsen_weighted", "null pointer");
+  return transfer<double, true>(Nf, Nc, Nelements, elementList, GlobalToLocal, P, qf, qc, stream, weight);
+}
+extern "C" int libp_mg_coarsen_weighted_hex_f32(int Nf, int Nc, int Nelements, const int* elementList,
+                                                const int* GlobalToLocal, const float* P, const float* weight,
+                                                const float* qf, float* qc, void* stream) {
+  if (!weight) return arg_error("mg_coarsen_weighted", "null pointer");
+  return transfer<float, true>(Nf, Nc, Nelements, elementList, GlobalToLocal, P, qf, qc, stream, weight);
 }

@@ -19,18 +19,23 @@ namespace libp_amd {
 // of three dependent loads (row start, column ids, values), and two chains in
 // flight per thread keep more bytes in flight. The grid gives each thread
 // about four rows. Each sum runs from zero over the row in ascending order.
-template <typename T>
+// OP 0: gv = sum; OP 1: gv = r - sum (the residual r - A x of an operator
+// whose last step is this gather); OP 2: gv += sum. Each fused form rounds
+// once, as the separate axpy it replaces does, so results are bit-identical.
+template <typename T, int OP = 0>
 __global__ void __launch_bounds__(256) ogs_gather_kernel(const int Nrows, const int* __restrict__ rowStarts,
                                                          const int* __restrict__ colIds, const T* __restrict__ v,
-                                                         T* __restrict__ gv) {
+                                                         const T* __restrict__ r_vec, T* __restrict__ gv) {
   const long S = (long)gridDim.x * blockDim.x;
   for (long row0 = blockIdx.x * (long)blockDim.x + threadIdx.x; row0 < Nrows; row0 += 2 * S) {
     int a[2], len[2];
+    T rr[2] = {T(0), T(0)};
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
       const long row = row0 + r * S;
       a[r] = (row < Nrows) ? rowStarts[row] : 0;
       len[r] = (row < Nrows) ? rowStarts[row + 1] - a[r] : 0;
+      if (OP == 1 && row < Nrows) rr[r] = r_vec[row];
     }
     int c[2][8];
     T x[2][8];
@@ -51,7 +56,9 @@ __global__ void __launch_bounds__(256) ogs_gather_kernel(const int Nrows, const 
       for (int k = 0; k < 8; ++k)
         if (k < len[r]) val += x[r][k];
       for (int g = a[r] + 8; g < a[r] + len[r]; ++g) val += v[colIds[g]];
-      gv[row] = val;
+      if (OP == 0) gv[row] = val;
+      if (OP == 1) gv[row] = T(1) * rr[r] + T(-1) * val;
+      if (OP == 2) gv[row] = T(1) * val + T(1) * gv[row];
     }
   }
 }
@@ -138,12 +145,12 @@ __global__ void __launch_bounds__(256) ogs_scatter_kernel(const int Nrows, const
   }
 }
 
-template <typename T>
-static int gather(int Nrows, const int* rs, const int* ci, const T* v, T* gv, void* stream) {
+template <typename T, int OP = 0>
+static int gather(int Nrows, const int* rs, const int* ci, const T* v, T* gv, void* stream, const T* r = nullptr) {
   if (Nrows <= 0) return LIBP_OK;
-  if (!rs || !ci || !v || !gv) return arg_error("ogs_gather", "null pointer");
-  hipLaunchKernelGGL(ogs_gather_kernel<T>, dim3(stream_blocks(Nrows, 1024)), dim3(256), 0, as_stream(stream), Nrows,
-                     rs, ci, v, gv);
+  if (!rs || !ci || !v || !gv || (OP == 1 && !r)) return arg_error("ogs_gather", "null pointer");
+  hipLaunchKernelGGL((ogs_gather_kernel<T, OP>), dim3(stream_blocks(Nrows, 1024)), dim3(256), 0, as_stream(stream),
+                     Nrows, rs, ci, v, r, gv);
   return launch_status("ogs_gather");
 }
 
@@ -187,4 +194,21 @@ extern "C" int libp_pcg_gather_pap_f64(int Nrows, const int* rowStarts, const in
   hipLaunchKernelGGL(gather_pap_finish_kernel, dim3(1), dim3(256), 0, as_stream(stream), (int)blocks, scalars,
                      (const ReduceWS*)ws);
   return launch_status("pcg_gather_pap");
+}
+
+extern "C" int libp_ogs_gather_residual_f64(int Nrows, const int* rowStarts, const int* colIds, const double* v,
+                                            const double* r, double* gv, void* stream) {
+  return gather<double, 1>(Nrows, rowStarts, colIds, v, gv, stream, r);
+}
+extern "C" int libp_ogs_gather_residual_f32(int Nrows, const int* rowStarts, const int* colIds, const float* v,
+                                            const float* r, float* gv, void* stream) {
+  return gather<float, 1>(Nrows, rowStarts, colIds, v, gv, stream, r);
+}
+extern "C" int libp_ogs_gather_add_f64(int Nrows, const int* rowStarts, const int* colIds, const double* v, double* gv,
+                                       void* stream) {
+  return gather<double, 2>(Nrows, rowStarts, colIds, v, gv, stream);
+}
+extern "C" int libp_ogs_gather_add_f32(int Nrows, const int* rowStarts, const int* colIds, const float* v, float* gv,
+                                       void* stream) {
+  return gather<float, 2>(Nrows, rowStarts, colIds, v, gv, stream);
 }

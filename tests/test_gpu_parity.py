@@ -220,6 +220,72 @@ def test_fused_gather_pap_and_jacobi_update():
     assert all(torch.equal(a, b) for a, b in zip(before, (xd, rd, zd, gv)))
     assert np.array_equal(sd3.cpu().numpy(), out)
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_fused_gather_coarsen_assemble_match_two_calls(prec):
+    """The fused forms used inside the V-cycle are bit-identical to the two
+    calls they replace: gather-residual (r - gather) and gather-add (y +=
+    gather) against gather + axpy; the weighted coarsening against amxpy +
+    coarsening (a random transfer matrix: both paths use the same one); the
+    accumulating learned assembly against assembly + axpy (FP32 only)."""
+    dt = torch.float64 if prec == "f64" else torch.float32
+    npdt = np.float64 if prec == "f64" else np.float32
+    lib = L()
+    fn = lambda name: getattr(lib, f"libp_{name}_{prec}")
+    o = OracleElliptic(box(3, 4, curve=0.03))
+    rng = np.random.default_rng(31)
+    E, Np = o.geti("Nelements"), o.geti("Np")
+    n = o.geti("Ngather")
+    rs, ci = dev(o.array("rowStartsT")), dev(o.array("colIdsT"))
+    vL = dev(rng.standard_normal(E * Np).astype(npdt))
+    r = dev(rng.standard_normal(n).astype(npdt))
+    y0 = rng.standard_normal(n).astype(npdt)
+    st = stream()
+    # residual
+    a = torch.empty(n, dtype=dt, device="cuda")
+    b = torch.empty(n, dtype=dt, device="cuda")
+    assert fn("ogs_gather")(n, rs.data_ptr(), ci.data_ptr(), vL.data_ptr(), a.data_ptr(), st) == 0
+    assert fn("axpy")(n, 1.0, r.data_ptr(), -1.0, a.data_ptr(), st) == 0
+    assert fn("ogs_gather_residual")(n, rs.data_ptr(), ci.data_ptr(), vL.data_ptr(), r.data_ptr(), b.data_ptr(),
+                                      st) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    # add
+    a, b = dev(y0), dev(y0)
+    g = torch.empty(n, dtype=dt, device="cuda")
+    assert fn("ogs_gather")(n, rs.data_ptr(), ci.data_ptr(), vL.data_ptr(), g.data_ptr(), st) == 0
+    assert fn("axpy")(n, 1.0, g.data_ptr(), 1.0, a.data_ptr(), st) == 0
+    assert fn("ogs_gather_add")(n, rs.data_ptr(), ci.data_ptr(), vL.data_ptr(), b.data_ptr(), st) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    # weighted coarsening 4 -> 2
+    Nf, Nc = 4, 2
+    P = dev(rng.standard_normal((Nf + 1) * (Nc + 1)).astype(npdt))
+    w = dev(rng.uniform(0.1, 1.0, n).astype(npdt))
+    qf = dev(rng.standard_normal(n).astype(npdt))
+    g2l = dev(o.array("GlobalToLocal"))
+    wq = torch.empty(n, dtype=dt, device="cuda")
+    c1 = torch.empty(E * (Nc + 1) ** 3, dtype=dt, device="cuda")
+    c2 = torch.empty_like(c1)
+    assert fn("amxpy")(n, 1.0, w.data_ptr(), qf.data_ptr(), 0.0, wq.data_ptr(), st) == 0
+    assert fn("mg_coarsen_hex")(Nf, Nc, E, None, g2l.data_ptr(), P.data_ptr(), wq.data_ptr(), c1.data_ptr(), st) == 0
+    assert fn("mg_coarsen_weighted_hex")(Nf, Nc, E, None, g2l.data_ptr(), P.data_ptr(), w.data_ptr(), qf.data_ptr(),
+                                         c2.data_ptr(), st) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(c1, c2)
+    if prec == "f32":
+        Y = dev(rng.standard_normal(E * Np).astype(np.float32))
+        sc = dev(rng.uniform(0.5, 2, E).astype(np.float32))
+        invD = dev(rng.uniform(0.5, 2, n).astype(np.float32))
+        a, b = dev(y0), dev(y0)
+        cc = torch.empty(n, dtype=dt, device="cuda")
+        assert lib.libp_learned_assemble_f32(n, rs.data_ptr(), ci.data_ptr(), Np, Y.data_ptr(), Np, sc.data_ptr(),
+                                             None, r.data_ptr(), invD.data_ptr(), cc.data_ptr(), st) == 0
+        assert lib.libp_axpy_f32(n, 1.0, cc.data_ptr(), 1.0, a.data_ptr(), st) == 0
+        assert lib.libp_learned_assemble_add_f32(n, rs.data_ptr(), ci.data_ptr(), Np, Y.data_ptr(), Np, sc.data_ptr(),
+                                                 None, r.data_ptr(), invD.data_ptr(), b.data_ptr(), st) == 0
+        torch.cuda.synchronize()
+        assert torch.equal(a, b)
+
 def test_scatter_bitexact():
     o = OracleElliptic(box(3, 3))
     g = np.random.default_rng(6).standard_normal(o.geti("Ngather"))

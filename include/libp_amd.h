@@ -85,6 +85,23 @@ int libp_ogs_gather_add_f64(int Nrows, const int* rowStarts, const int* colIds, 
                             void* stream);
 int libp_ogs_gather_add_f32(int Nrows, const int* rowStarts, const int* colIds, const float* v, float* gv,
                             void* stream);
+/* Gather fused with a Chebyshev step of the p-MG smoother
+ * (ellipticPreconMultiGridLevel.cpp:154-206): v is the E-vector A d (start:
+ * A x) before its gather. start: RES = invDiag.*(r - Av), d = invTheta*RES,
+ * X += d; step: RES -= invDiag.*Av, d = c1*RES + c2*d, X += d. Bit-identical to
+ * the gather followed by libp_cheb_start / libp_cheb_step. */
+int libp_ogs_gather_cheb_start_f64(int Nrows, const int* rowStarts, const int* colIds, const double* v,
+                                   const double* invDiag, const double* r, double invTheta, double* RES, double* d,
+                                   double* X, void* stream);
+int libp_ogs_gather_cheb_start_f32(int Nrows, const int* rowStarts, const int* colIds, const float* v,
+                                   const float* invDiag, const float* r, float invTheta, float* RES, float* d, float* X,
+                                   void* stream);
+int libp_ogs_gather_cheb_step_f64(int Nrows, const int* rowStarts, const int* colIds, const double* v,
+                                  const double* invDiag, double c1, double c2, double* RES, double* d, double* X,
+                                  void* stream);
+int libp_ogs_gather_cheb_step_f32(int Nrows, const int* rowStarts, const int* colIds, const float* v,
+                                  const float* invDiag, float c1, float c2, float* RES, float* d, float* X,
+                                  void* stream);
 /* Replaces "scatter" (ogsKernels.okl:127-164, ogsOperator.cpp:295-321):
  * v[colIds[g]] = gv[row] for every g of the row. */
 int libp_ogs_scatter_f64(int Nrows, const int* rowStarts, const int* colIds, const double* gv, double* v,

@@ -99,6 +99,10 @@ SIGNATURES = {
     "libp_ogs_gather_residual_f32": (c_int, [c_int] + [c_void_p] * 6),
     "libp_ogs_gather_add_f64": (c_int, [c_int] + [c_void_p] * 5),
     "libp_ogs_gather_add_f32": (c_int, [c_int] + [c_void_p] * 5),
+    "libp_ogs_gather_cheb_start_f64": (c_int, [c_int] + [c_void_p] * 5 + [c_double] + [c_void_p] * 4),
+    "libp_ogs_gather_cheb_start_f32": (c_int, [c_int] + [c_void_p] * 5 + [c_float] + [c_void_p] * 4),
+    "libp_ogs_gather_cheb_step_f64": (c_int, [c_int] + [c_void_p] * 4 + [c_double, c_double] + [c_void_p] * 4),
+    "libp_ogs_gather_cheb_step_f32": (c_int, [c_int] + [c_void_p] * 4 + [c_float, c_float] + [c_void_p] * 4),
     "libp_mg_coarsen_weighted_hex_f64": (c_int, [c_int, c_int, c_int] + [c_void_p] * 7),
     "libp_mg_coarsen_weighted_hex_f32": (c_int, [c_int, c_int, c_int] + [c_void_p] * 7),
     "libp_learned_assemble_add_f32": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_long] + [c_void_p] * 6),

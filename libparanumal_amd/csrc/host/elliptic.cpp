@@ -115,6 +115,20 @@ bool elliptic_t::OperatorPAP(deviceMemory<dfloat>& o_q, deviceMemory<dfloat>& o_
   return true;
 }
 
+double* elliptic_t::PartialAxLocal(deviceMemory<dfloat>& o_q) {
+  PartialAx(o_q, o_AqL);
+  return o_AqL.ptr();
+}
+
+float* elliptic_t::PartialAxLocalF32(deviceMemory<float>& o_q) {
+  mesh.MakeFP32();
+  if (!o_AqL32.isInitialized()) o_AqL32 = platform.malloc<float>((size_t)mesh.Nelements * mesh.Np);
+  LIBP_CALL(libp_ax_hex_partial_f32(mesh.N, mesh.NlocalGatherElements, nullptr, o_GlobalToLocal.ptr(),
+                                    mesh.o_wJ32.ptr(), mesh.o_ggeo32.ptr(), mesh.o_D32.ptr(), (float)lambda,
+                                    o_q.ptr(), o_AqL32.ptr(), platform.s()));
+  return o_AqL32.ptr();
+}
+
 void elliptic_t::OperatorResidual(deviceMemory<dfloat>& o_q, deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_res) {
   PartialAx(o_q, o_AqL);
   LIBP_CALL(libp_ogs_gather_residual_f64(ogsMasked.NlocalT, ogsMasked.o_rowStartsT.ptr(), ogsMasked.o_colIdsT.ptr(),
@@ -122,13 +136,9 @@ void elliptic_t::OperatorResidual(deviceMemory<dfloat>& o_q, deviceMemory<dfloat
 }
 
 void elliptic_t::OperatorResidualF32(deviceMemory<float>& o_q, deviceMemory<float>& o_r, deviceMemory<float>& o_res) {
-  mesh.MakeFP32();
-  if (!o_AqL32.isInitialized()) o_AqL32 = platform.malloc<float>((size_t)mesh.Nelements * mesh.Np);
-  LIBP_CALL(libp_ax_hex_partial_f32(mesh.N, mesh.NlocalGatherElements, nullptr, o_GlobalToLocal.ptr(),
-                                    mesh.o_wJ32.ptr(), mesh.o_ggeo32.ptr(), mesh.o_D32.ptr(), (float)lambda,
-                                    o_q.ptr(), o_AqL32.ptr(), platform.s()));
+  const float* AqL = PartialAxLocalF32(o_q);
   LIBP_CALL(libp_ogs_gather_residual_f32(ogsMasked.NlocalT, ogsMasked.o_rowStartsT.ptr(), ogsMasked.o_colIdsT.ptr(),
-                                         o_AqL32.ptr(), o_r.ptr(), o_res.ptr(), platform.s()));
+                                         AqL, o_r.ptr(), o_res.ptr(), platform.s()));
 }
 
 void elliptic_t::OperatorF32(deviceMemory<float>& o_q, deviceMemory<float>& o_Aq) {

@@ -61,6 +61,10 @@ class elliptic_t : public operator_t {
 
   // ellipticOperator.cpp:29-107 (single launch over all local elements)
   void Operator(deviceMemory<dfloat>& o_q, deviceMemory<dfloat>& o_Aq) override;
+  // the partial Ax into the level's own E-vector buffer (returned), for callers
+  // that fuse the gather with what follows it
+  double* PartialAxLocal(deviceMemory<dfloat>& o_q);
+  float* PartialAxLocalF32(deviceMemory<float>& o_q);
   // res = r - A q, the subtraction fused into the operator's gather (FP64 and FP32)
   void OperatorResidual(deviceMemory<dfloat>& o_q, deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_res);
   void OperatorResidualF32(deviceMemory<float>& o_q, deviceMemory<float>& o_r, deviceMemory<float>& o_res);

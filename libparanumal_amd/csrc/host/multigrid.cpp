@@ -114,6 +114,15 @@ struct prec<double> {
   static int gatherAdd(int N, const int* rs, const int* ci, const double* v, double* gv, void* s) {
     return libp_ogs_gather_add_f64(N, rs, ci, v, gv, s);
   }
+  static const double* partialAx(elliptic_t& e, deviceMemory<double>& x) { return e.PartialAxLocal(x); }
+  static int gatherChebStart(int N, const int* rs, const int* ci, const double* v, const double* iD, const double* r,
+                             double it, double* R, double* d, double* X, void* s) {
+    return libp_ogs_gather_cheb_start_f64(N, rs, ci, v, iD, r, it, R, d, X, s);
+  }
+  static int gatherChebStep(int N, const int* rs, const int* ci, const double* v, const double* iD, double c1,
+                            double c2, double* R, double* d, double* X, void* s) {
+    return libp_ogs_gather_cheb_step_f64(N, rs, ci, v, iD, c1, c2, R, d, X, s);
+  }
   static deviceMemory<double> fromDouble(platform_t& p, deviceMemory<double>& v, dlong) { return v; }
 };
 template <>
@@ -146,6 +155,15 @@ struct prec<float> {
   }
   static int gatherAdd(int N, const int* rs, const int* ci, const float* v, float* gv, void* s) {
     return libp_ogs_gather_add_f32(N, rs, ci, v, gv, s);
+  }
+  static const float* partialAx(elliptic_t& e, deviceMemory<float>& x) { return e.PartialAxLocalF32(x); }
+  static int gatherChebStart(int N, const int* rs, const int* ci, const float* v, const float* iD, const float* r,
+                             float it, float* R, float* d, float* X, void* s) {
+    return libp_ogs_gather_cheb_start_f32(N, rs, ci, v, iD, r, it, R, d, X, s);
+  }
+  static int gatherChebStep(int N, const int* rs, const int* ci, const float* v, const float* iD, float c1, float c2,
+                            float* R, float* d, float* X, void* s) {
+    return libp_ogs_gather_cheb_step_f32(N, rs, ci, v, iD, c1, c2, R, d, X, s);
   }
   static deviceMemory<float> fromDouble(platform_t& p, deviceMemory<double>& v, dlong n) {
     deviceMemory<float> f = p.malloc<float>(n);
@@ -388,22 +406,26 @@ class MGLevel : public parAlmond::multigridLevel<T> {
     const double sigma = theta / delta;
     double rho_n = 1. / sigma, rho_np1;
     deviceMemory<T>& o_RES = scratch->RES;
-    deviceMemory<T>& o_Ad = scratch->Ad;
     deviceMemory<T>& o_d = scratch->d;
+    // each A d (and A x) is gathered and used at once: the Chebyshev update of a
+    // row runs in the gather that forms its value (ogs_gather_cheb_kernel)
+    const ogs::ogs_t& og = elliptic->ogsMasked;
+    const int* rs = og.o_rowStartsT.ptr();
+    const int* ci = og.o_colIdsT.ptr();
     if (xIsZero) {
       LIBP_CALL(prec<T>::chebStart(N, o_invDiagA.ptr(), o_r.ptr(), nullptr, (T)invTheta, o_RES.ptr(), o_d.ptr(),
                                    o_X.ptr(), 1, platform.s()));
     } else {
-      Operator(o_X, o_Ad);
-      LIBP_CALL(prec<T>::chebStart(N, o_invDiagA.ptr(), o_r.ptr(), o_Ad.ptr(), (T)invTheta, o_RES.ptr(), o_d.ptr(),
-                                   o_X.ptr(), 0, platform.s()));
+      const T* AxL = prec<T>::partialAx(*elliptic, o_X);
+      LIBP_CALL(prec<T>::gatherChebStart(og.NlocalT, rs, ci, AxL, o_invDiagA.ptr(), o_r.ptr(), (T)invTheta,
+                                         o_RES.ptr(), o_d.ptr(), o_X.ptr(), platform.s()));
     }
     for (int k = 0; k < ChebyshevIterations; k++) {
-      Operator(o_d, o_Ad);
+      const T* AdL = prec<T>::partialAx(*elliptic, o_d);
       rho_np1 = 1.0 / (2. * sigma - rho_n);
       const double rhoDivDelta = 2.0 * rho_np1 / delta;
-      LIBP_CALL(prec<T>::chebStep(N, o_invDiagA.ptr(), o_Ad.ptr(), (T)rhoDivDelta, (T)(rho_np1 * rho_n),
-                                  o_RES.ptr(), o_d.ptr(), o_X.ptr(), platform.s()));
+      LIBP_CALL(prec<T>::gatherChebStep(og.NlocalT, rs, ci, AdL, o_invDiagA.ptr(), (T)rhoDivDelta,
+                                        (T)(rho_np1 * rho_n), o_RES.ptr(), o_d.ptr(), o_X.ptr(), platform.s()));
       rho_n = rho_np1;
     }
   }

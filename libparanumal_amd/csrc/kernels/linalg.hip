@@ -252,6 +252,8 @@ __global__ void __launch_bounds__(256) cheb_start_kernel(const int N, const T* _
                                                          const T* __restrict__ r, const T* __restrict__ Ax,
                                                          const T invTheta, T* __restrict__ RES, T* __restrict__ d,
                                                          T* __restrict__ X, const int xIsZero) {
+  // no a*b+c contraction: the gather-fused form (ogs_gather_cheb_kernel) must round the same way
+#pragma clang fp contract(off)
   for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
     const T res = Ax ? T(1) * invDiag[n] * (T(1) * r[n] - T(1) * Ax[n]) : T(1) * invDiag[n] * r[n];
     const T dn = invTheta * res;
@@ -265,6 +267,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) cheb_step_kernel(const int N, const T* __restrict__ invDiag,
                                                         const T* __restrict__ Ad, const T c1, const T c2,
                                                         T* __restrict__ RES, T* __restrict__ d, T* __restrict__ X) {
+#pragma clang fp contract(off)
   for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
     const T res = T(-1) * invDiag[n] * Ad[n] + RES[n];
     const T dn = c1 * res + c2 * d[n];

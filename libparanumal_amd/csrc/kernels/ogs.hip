@@ -63,6 +63,75 @@ __global__ void __launch_bounds__(256) ogs_gather_kernel(const int Nrows, const 
   }
 }
 
+// Gather fused with a Chebyshev step of the smoother (linalg.hip
+// cheb_start_kernel / cheb_step_kernel, ellipticPreconMultiGridLevel.cpp:
+// 154-206): the gathered A d (START: A x) of a row is used at once instead of
+// being written out and read back. Per row, with a = the gathered value:
+//   START: res = invD (r - a); d = invTheta res; RES = res; X += d
+//   step:  res = RES - invD a; d = c1 res + c2 d; RES = res; X += d
+// The expressions are those of the separate kernels, so results are
+// bit-identical.
+template <typename T, bool START>
+__device__ inline void cheb_row_update(const long row, const T val, const T* __restrict__ invDiag,
+                                       const T* __restrict__ rvec, const T c1, const T c2, T* __restrict__ RES,
+                                       T* __restrict__ d, T* __restrict__ X) {
+  // no a*b+c contraction, as in cheb_start_kernel / cheb_step_kernel
+#pragma clang fp contract(off)
+  if (START) {
+    const T res = T(1) * invDiag[row] * (T(1) * rvec[row] - T(1) * val);
+    const T dn = c1 * res;
+    RES[row] = res;
+    d[row] = dn;
+    X[row] = X[row] + dn;
+  } else {
+    const T res = T(-1) * invDiag[row] * val + RES[row];
+    const T dn = c1 * res + c2 * d[row];
+    RES[row] = res;
+    d[row] = dn;
+    X[row] = X[row] + dn;
+  }
+}
+
+template <typename T, bool START>
+__global__ void __launch_bounds__(256) ogs_gather_cheb_kernel(const int Nrows, const int* __restrict__ rowStarts,
+                                                              const int* __restrict__ colIds,
+                                                              const T* __restrict__ v, const T* __restrict__ invDiag,
+                                                              const T* __restrict__ rvec, const T c1, const T c2,
+                                                              T* __restrict__ RES, T* __restrict__ d,
+                                                              T* __restrict__ X) {
+  const long S = (long)gridDim.x * blockDim.x;
+  for (long row0 = blockIdx.x * (long)blockDim.x + threadIdx.x; row0 < Nrows; row0 += 2 * S) {
+    int a[2], len[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const long row = row0 + r * S;
+      a[r] = (row < Nrows) ? rowStarts[row] : 0;
+      len[r] = (row < Nrows) ? rowStarts[row + 1] - a[r] : 0;
+    }
+    int c[2][8];
+    T x[2][8];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) c[r][k] = (k < len[r]) ? colIds[a[r] + k] : 0;
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) x[r][k] = (k < len[r]) ? v[c[r][k]] : T(0);
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const long row = row0 + r * S;
+      if (row >= Nrows) continue;
+      T val = 0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (k < len[r]) val += x[r][k];
+      for (int g = a[r] + 8; g < a[r] + len[r]; ++g) val += v[colIds[g]];
+      cheb_row_update<T, START>(row, val, invDiag, rvec, c1, c2, RES, d, X);
+    }
+  }
+}
+
 // Gather fused with the p.Ap reduction of a PCG iteration (FP64): gv = gather(v)
 // with the arithmetic of ogs_gather_kernel, and sum_row y[row] gv[row]. Each
 // block stores one partial in a flat array of up to RNPART*RMAXB entries (the
@@ -211,4 +280,35 @@ extern "C" int libp_ogs_gather_add_f64(int Nrows, const int* rowStarts, const in
 extern "C" int libp_ogs_gather_add_f32(int Nrows, const int* rowStarts, const int* colIds, const float* v, float* gv,
                                        void* stream) {
   return gather<float, 2>(Nrows, rowStarts, colIds, v, gv, stream);
+}
+
+template <typename T, bool START>
+static int gather_cheb(int Nrows, const int* rs, const int* ci, const T* v, const T* invDiag, const T* r, T c1, T c2,
+                       T* RES, T* d, T* X, void* stream) {
+  if (Nrows <= 0) return LIBP_OK;
+  if (!rs || !ci || !v || !invDiag || !RES || !d || !X || (START && !r))
+    return arg_error("ogs_gather_cheb", "null pointer");
+  hipLaunchKernelGGL((ogs_gather_cheb_kernel<T, START>), dim3(stream_blocks(Nrows, 1024)), dim3(256), 0,
+                     as_stream(stream), Nrows, rs, ci, v, invDiag, r, c1, c2, RES, d, X);
+  return launch_status("ogs_gather_cheb");
+}
+extern "C" int libp_ogs_gather_cheb_start_f64(int Nrows, const int* rowStarts, const int* colIds, const double* v,
+                                              const double* invDiag, const double* r, double invTheta, double* RES,
+                                              double* d, double* X, void* stream) {
+  return gather_cheb<double, true>(Nrows, rowStarts, colIds, v, invDiag, r, invTheta, 0.0, RES, d, X, stream);
+}
+extern "C" int libp_ogs_gather_cheb_start_f32(int Nrows, const int* rowStarts, const int* colIds, const float* v,
+                                              const float* invDiag, const float* r, float invTheta, float* RES,
+                                              float* d, float* X, void* stream) {
+  return gather_cheb<float, true>(Nrows, rowStarts, colIds, v, invDiag, r, invTheta, 0.f, RES, d, X, stream);
+}
+extern "C" int libp_ogs_gather_cheb_step_f64(int Nrows, const int* rowStarts, const int* colIds, const double* v,
+                                             const double* invDiag, double c1, double c2, double* RES, double* d,
+                                             double* X, void* stream) {
+  return gather_cheb<double, false>(Nrows, rowStarts, colIds, v, invDiag, nullptr, c1, c2, RES, d, X, stream);
+}
+extern "C" int libp_ogs_gather_cheb_step_f32(int Nrows, const int* rowStarts, const int* colIds, const float* v,
+                                             const float* invDiag, float c1, float c2, float* RES, float* d, float* X,
+                                             void* stream) {
+  return gather_cheb<float, false>(Nrows, rowStarts, colIds, v, invDiag, nullptr, c1, c2, RES, d, X, stream);
 }

@@ -226,6 +226,7 @@ def test_fused_gather_coarsen_assemble_match_two_calls(prec):
     calls they replace: gather-residual (r - gather) and gather-add (y +=
     gather) against gather + axpy; the weighted coarsening against amxpy +
     coarsening (a random transfer matrix: both paths use the same one); the
+    Chebyshev start and step in the gather against gather + cheb kernels; the
     accumulating learned assembly against assembly + axpy (FP32 only)."""
     dt = torch.float64 if prec == "f64" else torch.float32
     npdt = np.float64 if prec == "f64" else np.float32
@@ -272,6 +273,26 @@ def test_fused_gather_coarsen_assemble_match_two_calls(prec):
                                          c2.data_ptr(), st) == 0
     torch.cuda.synchronize()
     assert torch.equal(c1, c2)
+    # Chebyshev start (with A x) and step fused into the gather
+    iD = dev(rng.uniform(0.5, 2, n).astype(npdt))
+    c1s, c2s = 0.7, -0.3
+    for start in (True, False):
+        R0, d0, X0 = (rng.standard_normal(n).astype(npdt) for _ in range(3))
+        Ra, da, Xa = dev(R0), dev(d0), dev(X0)
+        Rb, db, Xb = dev(R0), dev(d0), dev(X0)
+        assert fn("ogs_gather")(n, rs.data_ptr(), ci.data_ptr(), vL.data_ptr(), g.data_ptr(), st) == 0
+        if start:
+            assert fn("cheb_start")(n, iD.data_ptr(), r.data_ptr(), g.data_ptr(), c1s, Ra.data_ptr(), da.data_ptr(),
+                                    Xa.data_ptr(), 0, st) == 0
+            assert fn("ogs_gather_cheb_start")(n, rs.data_ptr(), ci.data_ptr(), vL.data_ptr(), iD.data_ptr(),
+                                               r.data_ptr(), c1s, Rb.data_ptr(), db.data_ptr(), Xb.data_ptr(), st) == 0
+        else:
+            assert fn("cheb_step")(n, iD.data_ptr(), g.data_ptr(), c1s, c2s, Ra.data_ptr(), da.data_ptr(),
+                                   Xa.data_ptr(), st) == 0
+            assert fn("ogs_gather_cheb_step")(n, rs.data_ptr(), ci.data_ptr(), vL.data_ptr(), iD.data_ptr(), c1s, c2s,
+                                              Rb.data_ptr(), db.data_ptr(), Xb.data_ptr(), st) == 0
+        torch.cuda.synchronize()
+        assert torch.equal(Ra, Rb) and torch.equal(da, db) and torch.equal(Xa, Xb), start
     if prec == "f32":
         Y = dev(rng.standard_normal(E * Np).astype(np.float32))
         sc = dev(rng.uniform(0.5, 2, E).astype(np.float32))

@@ -112,7 +112,7 @@ __device__ inline void store_tile(const TileRegs& R, u16* __restrict__ s) {
 }
 
 template <bool TA, bool TB, int EPI>
-__global__ void __launch_bounds__(GTHREADS) gemm_bf16_kernel(const GemmArgs g) {
+__global__ void __launch_bounds__(GTHREADS, 4) gemm_bf16_kernel(const GemmArgs g) {
   __shared__ __attribute__((aligned(16))) u16 sA[2][GBM * GLD];
   __shared__ __attribute__((aligned(16))) u16 sB[2][GBN * GLD];
   const int t = threadIdx.x, w = t >> 6, l = t & 63;

@@ -167,6 +167,11 @@ int libp_pcg_update_f64(int N, const double* p, const double* Ap, double* x, dou
  * linearSolverPCG.cpp:113-127 (iter==0 gives beta=0). */
 int libp_pcg_rdotz_f64(int N, const double* r, const double* z, const double* Ap, int flexible, double* scalars,
                        void* ws, void* stream);
+/* The same with z given in FP32 (the output of an FP32 preconditioner): z is
+ * widened into the FP64 vector in the pass that forms the scalars; results
+ * are those of libp_convert_f32_to_f64 followed by libp_pcg_rdotz_f64. */
+int libp_pcg_rdotz_from_f32_f64(int N, const double* r, const float* z32, double* z, const double* Ap, int flexible,
+                                double* scalars, void* ws, void* stream);
 int libp_pcg_update_p_f64(int N, int first_iter, const double* z, double* p, const double* scalars, void* stream);
 /* p.Ap into scalars[PAP]. */
 int libp_pcg_pap_f64(int N, const double* p, const double* Ap, double* scalars, void* ws, void* stream);

@@ -112,6 +112,8 @@ class MultiGridPrecon : public operator_t {
   std::shared_ptr<MultiGridPreconBase> impl;
   explicit MultiGridPrecon(elliptic_t& elliptic);
   void Operator(deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_Mr) override;
+  bool OperatorRdotz(deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_z, deviceMemory<dfloat>& o_Ap, int flexible,
+                     double* scalars, void* ws) override;
   void solveStarting() override;
   void iterationDone(int it, double rdotr) override;
 };

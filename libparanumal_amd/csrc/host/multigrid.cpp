@@ -801,6 +801,10 @@ void coarseJacobiPCG_t::solve<float>(deviceMemory<float>& o_rhs, deviceMemory<fl
 class MultiGridPreconBase {
  public:
   virtual void Operator(deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_Mr) = 0;
+  virtual bool OperatorRdotz(deviceMemory<dfloat>&, deviceMemory<dfloat>&, deviceMemory<dfloat>&, int, double*,
+                             void*) {
+    return false;
+  }
   virtual ~MultiGridPreconBase() {}
 };
 
@@ -877,7 +881,26 @@ class multigrid_t : public MultiGridPreconBase {
   }
 
   void Operator(deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_Mr) override;
+  bool OperatorRdotz(deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_z, deviceMemory<dfloat>& o_Ap, int flexible,
+                     double* scalars, void* ws) override;
 };
+
+template <>
+bool multigrid_t<double>::OperatorRdotz(deviceMemory<dfloat>&, deviceMemory<dfloat>&, deviceMemory<dfloat>&, int,
+                                        double*, void*) {
+  return false;
+}
+// the V-cycle's FP32 output widened to z in the pass that forms r.z and z.Ap
+template <>
+bool multigrid_t<float>::OperatorRdotz(deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_z,
+                                       deviceMemory<dfloat>& o_Ap, int flexible, double* scalars, void* ws) {
+  const dlong N = fine.Ndofs;
+  LIBP_CALL(libp_convert_f64_to_f32(N, o_r.ptr(), o_r0.ptr(), fine.platform.s()));
+  vcycle(0, o_r0, o_z0);
+  LIBP_CALL(libp_pcg_rdotz_from_f32_f64(N, o_r.ptr(), o_z0.ptr(), o_z.ptr(), o_Ap.ptr(), flexible, scalars, ws,
+                                        fine.platform.s()));
+  return true;
+}
 
 template <>
 void multigrid_t<double>::Operator(deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_Mr) {
@@ -922,6 +945,12 @@ MultiGridPrecon::MultiGridPrecon(elliptic_t& _elliptic) : elliptic(_elliptic) {
 void MultiGridPrecon::Operator(deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_Mr) {
   impl->Operator(o_r, o_Mr);
   if (elliptic.allNeumann) elliptic.ZeroMean(o_Mr);
+}
+
+bool MultiGridPrecon::OperatorRdotz(deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_z, deviceMemory<dfloat>& o_Ap,
+                                    int flexible, double* scalars, void* ws) {
+  if (elliptic.allNeumann) return false;  // the mean is removed after the V-cycle
+  return impl->OperatorRdotz(o_r, o_z, o_Ap, flexible, scalars, ws);
 }
 
 void MultiGridPrecon::solveStarting() {

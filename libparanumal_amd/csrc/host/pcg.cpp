@@ -57,7 +57,7 @@ void pcg::iteration(operator_t& linearOperator, operator_t& precon, diagonalPrec
   if (jac) {
     LIBP_CALL(libp_pcg_jacobi_rdotz_f64(N, jac->invDiagonal().ptr(), o_r.ptr(), o_z.ptr(), o_Ap.ptr(), flexible, s,
                                         ws, st));
-  } else {
+  } else if (!fuseJacobi || !precon.OperatorRdotz(o_r, o_z, o_Ap, flexible, s, ws)) {
     precon.Operator(o_r, o_z);
     LIBP_CALL(libp_pcg_rdotz_f64(N, o_r.ptr(), o_z.ptr(), o_Ap.ptr(), flexible, s, ws, st));
   }

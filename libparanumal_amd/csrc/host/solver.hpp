@@ -22,6 +22,19 @@ class operator_t {
     (void)ws;
     return false;
   }
+  // For a preconditioner: z = M r together with the PCG scalars of
+  // libp_pcg_rdotz_f64 (r.z, and z.Ap when flexible) in one pass over z.
+  // Returns false when there is no fused form.
+  virtual bool OperatorRdotz(deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_z, deviceMemory<dfloat>& o_Ap,
+                             int flexible, double* scalars, void* ws) {
+    (void)o_r;
+    (void)o_z;
+    (void)o_Ap;
+    (void)flexible;
+    (void)scalars;
+    (void)ws;
+    return false;
+  }
   // Solver notifications (no-ops by default): a solve starts; iteration `it`
   // ended with residual norm^2 rdotr (already on the host).
   virtual void solveStarting() {}
@@ -49,6 +62,10 @@ class precon_t : public operator_t {
     precon = std::make_shared<Precon>(std::forward<Args>(args)...);
   }
   void Operator(deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_Mr) override { precon->Operator(o_r, o_Mr); }
+  bool OperatorRdotz(deviceMemory<dfloat>& o_r, deviceMemory<dfloat>& o_z, deviceMemory<dfloat>& o_Ap, int flexible,
+                     double* scalars, void* ws) override {
+    return precon && precon->OperatorRdotz(o_r, o_z, o_Ap, flexible, scalars, ws);
+  }
   void solveStarting() override {
     if (precon) precon->solveStarting();
   }

@@ -133,6 +133,27 @@ __global__ void __launch_bounds__(RBS) pcg_rdotz_finish_kernel(const int nb, con
   }
 }
 
+// The FP32 output of a preconditioner widened to z and, in the same pass, the
+// r.z and (flexible) z.Ap of pcg_rdotz_kernel<false>: the same grid and the
+// same per-thread order, so the partials are those of the two separate calls.
+__global__ void __launch_bounds__(RBS) pcg_rdotz_f32in_kernel(const int N, const double* __restrict__ r,
+                                                              const float* __restrict__ z32, double* __restrict__ z,
+                                                              const double* __restrict__ Ap, const int flexible,
+                                                              const double* __restrict__ s, ReduceWS* __restrict__ ws) {
+  __shared__ double s_w[16];
+  if (s[8] != 0.0) return;
+  double a0 = 0, a1 = 0;
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
+    const double rn = r[n];
+    const double zn = (double)z32[n];
+    z[n] = zn;
+    a0 += rn * zn;
+    if (flexible) a1 += zn * Ap[n];
+  }
+  double v[2] = {a0, a1};
+  store_block_partials<2>(ws, v, s_w);
+}
+
 // p = z + beta*p  (axpy(N, 1, z, beta, p); beta==0 on the first iteration: p = z)
 __global__ void __launch_bounds__(256) pcg_update_p_kernel(const int N, const int first, const double* __restrict__ z,
                                                            double* __restrict__ p, const double* __restrict__ s) {
@@ -343,6 +364,16 @@ extern "C" int libp_pcg_rdotz_f64(int N, const double* r, const double* z, const
   hipLaunchKernelGGL(pcg_rdotz_finish_kernel, dim3(1), dim3(RBS), 0, as_stream(stream), (int)reduce_blocks(N), flexible, scalars,
                      (const ReduceWS*)ws);
   return launch_status("pcg_rdotz");
+}
+extern "C" int libp_pcg_rdotz_from_f32_f64(int N, const double* r, const float* z32, double* z, const double* Ap,
+                                           int flexible, double* scalars, void* ws, void* stream) {
+  if (!r || !z32 || !z || !scalars || !ws || (flexible && !Ap)) return arg_error("pcg_rdotz_from_f32", "null pointer");
+  const unsigned nb = reduce_blocks(N);
+  hipLaunchKernelGGL(pcg_rdotz_f32in_kernel, dim3(nb), dim3(RBS), 0, as_stream(stream), N, r, z32, z, Ap, flexible,
+                     scalars, (ReduceWS*)ws);
+  hipLaunchKernelGGL(pcg_rdotz_finish_kernel, dim3(1), dim3(RBS), 0, as_stream(stream), (int)nb, flexible, scalars,
+                     (const ReduceWS*)ws);
+  return launch_status("pcg_rdotz_from_f32");
 }
 extern "C" int libp_pcg_jacobi_rdotz_f64(int N, const double* invDiag, const double* r, double* z, const double* Ap,
                                          int flexible, double* scalars, void* ws, void* stream) {

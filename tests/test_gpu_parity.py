@@ -307,6 +307,33 @@ def test_fused_gather_coarsen_assemble_match_two_calls(prec):
         torch.cuda.synchronize()
         assert torch.equal(a, b)
 
+def test_rdotz_from_f32_matches_convert_and_rdotz():
+    """The FP32 preconditioner output widened in the pass that forms r.z and
+    z.Ap (libp_pcg_rdotz_from_f32_f64) against libp_convert_f32_to_f64 followed
+    by libp_pcg_rdotz_f64: z and all four scalars they write are bit-identical,
+    flexible or not, at a ragged size."""
+    lib = L()
+    n = 123457
+    rng = np.random.default_rng(41)
+    r, Ap = dev(rng.standard_normal(n)), dev(rng.standard_normal(n))
+    z32 = dev(rng.standard_normal(n).astype(np.float32))
+    st = stream()
+    for flexible in (0, 1):
+        ws = torch.zeros(lib.libp_reduce_ws_bytes(), dtype=torch.uint8, device="cuda")
+        s0 = np.zeros(10)
+        s0[0], s0[2] = 1.7, 0.3
+        sa, sb = dev(s0), dev(s0)
+        za = torch.empty(n, dtype=torch.float64, device="cuda")
+        zb = torch.empty_like(za)
+        assert lib.libp_convert_f32_to_f64(n, z32.data_ptr(), za.data_ptr(), st) == 0
+        assert lib.libp_pcg_rdotz_f64(n, r.data_ptr(), za.data_ptr(), Ap.data_ptr(), flexible, sa.data_ptr(),
+                                      ws.data_ptr(), st) == 0
+        assert lib.libp_pcg_rdotz_from_f32_f64(n, r.data_ptr(), z32.data_ptr(), zb.data_ptr(), Ap.data_ptr(), flexible,
+                                               sb.data_ptr(), ws.data_ptr(), st) == 0
+        torch.cuda.synchronize()
+        assert torch.equal(za, zb)
+        assert np.array_equal(sa.cpu().numpy(), sb.cpu().numpy()), flexible
+
 def test_scatter_bitexact():
     o = OracleElliptic(box(3, 3))
     g = np.random.default_rng(6).standard_normal(o.geti("Ngather"))

@@ -239,6 +239,21 @@ int libp_pcg_jacobi_hex1_cg1_f64(int Nelements, const int* GlobalToLocal, const 
                                  double* w, double* const* p2, double* const* s2, double* const* r2,
                                  double* const* u2, double* AqL, double* scalars, void* ws, void* stream);
 
+/* The iteration of libp_pcg_jacobi_hex1_cg1_f64 on the assembled operator
+ * (CSR over the gathered dofs, columns ascending; the matrix of the N=1
+ * level as ellipticBuildOperatorMatrixContinuous assembles it for parAlmond,
+ * ellipticPreconMultiGrid.cpp:104-131): one launch per iteration, each row
+ * recomputing the updated u at its columns. Modes, parity, scalars and ws
+ * as there, except: w2 holds two device vectors used by parity, as p2; ws
+ * holds two reduction workspaces (2 * libp_reduce_ws_bytes()), alternated
+ * by parity; mode 3 takes the parity of the last queued iteration. Same
+ * iterates in exact arithmetic; in floating point A u is summed by rows of
+ * the assembled matrix instead of by elements. */
+int libp_pcg_jacobi_csr_cg1_f64(int Nrows, const int* rowStarts, const int* colIds, const double* vals,
+                                const double* invDiag, double tol, int mode, int parity, double* x, double* const* w2,
+                                double* const* p2, double* const* s2, double* const* r2, double* const* u2,
+                                double* scalars, void* ws, void* stream);
+
 /* The whole fused Jacobi-PCG of libp_pcg_jacobi_hex1_iteration_f64 in one
  * launch (the same counterpart: linearSolverPCG.cpp:107-171 with
  * ellipticPreconJacobi.cpp:42-51 and the N=1 ellipticAxHex3D.okl:156-295).

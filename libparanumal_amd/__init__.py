@@ -92,6 +92,8 @@ SIGNATURES = {
     "libp_pcg_jacobi_hex1_cg1_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int,
                                              c_void_p, c_void_p, c_void_p, c_double, c_int, c_int]
                                     + [c_void_p] * 10),
+    "libp_pcg_jacobi_csr_cg1_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int, c_int]
+                                   + [c_void_p] * 9),
     "libp_pcg_gather_pap_f64": (c_int, [c_int] + [c_void_p] * 8),
     "libp_pcg_jacobi_update_f64": (c_int, [c_int] + [c_void_p] * 9),
     "libp_pcg_relative_tol_f64": (c_int, [c_double, c_void_p, c_void_p]),

@@ -152,6 +152,9 @@ settings_t ellipticDefaultSettings() {
                "Fused coarse Jacobi-PCG in its single-reduction (Chronopoulos-Gear) form: two launches per "
                "iteration instead of three",
                {"TRUE", "FALSE"});
+  s.newSetting("MULTIGRID COARSE ASSEMBLED", "TRUE",
+               "Single-reduction coarse Jacobi-PCG on the assembled N=1 operator (CSR): one launch per iteration",
+               {"TRUE", "FALSE"});
   s.newSetting("MULTIGRID COARSE PERSISTENT", "FALSE",
                "Fused coarse Jacobi-PCG as one persistent kernel with grid barriers (no host reads); slower "
                "than the replayed graphs on a 32^3 box (31.5 against 30.0 ms per p-MG solve)",

@@ -505,6 +505,17 @@ class coarseJacobiPCG_t {
     std::string cg1s = "TRUE";
     elliptic->settings.getSetting("MULTIGRID COARSE SINGLE REDUCTION", cg1s);
     cg1 = fusedHex1 && !persistent && cg1s == "TRUE";
+    // single-reduction iterations on the assembled N=1 operator: one launch each
+    if (cg1 && elliptic->settings.compareSetting("MULTIGRID COARSE ASSEMBLED", "TRUE") &&
+        !elliptic->settings.compareSetting("MULTIGRID COARSE SOLVER", "AMG")) {
+      const csrHost_t A = ellipticAssembleCSR(*elliptic);
+      LIBP_ABORT("assembled coarse operator: size mismatch", A.n != elliptic->Ndofs || A.m != elliptic->Ndofs);
+      platform_t& p = elliptic->platform;
+      o_Crs = p.malloc(A.rs);
+      o_Cci = p.malloc(A.ci);
+      o_Cv = p.malloc(A.v);
+      cg1csr = true;
+    }
     // the reference's base level: parAlmond AMG on the assembled operator
     // (ellipticPreconMultiGrid.cpp:104-131), one V-cycle per coarse solve
     if (elliptic->settings.compareSetting("MULTIGRID COARSE SOLVER", "AMG")) {
@@ -522,6 +533,10 @@ class coarseJacobiPCG_t {
   int persistentBlocks = 64;    // 0: one block per CU
   bool iterationsOnDevice = false;  // lastIterations is still in o_s (persistent solve)
   bool cg1 = false;
+  bool cg1csr = false;  // cg1 on the assembled operator (o_Crs, o_Cci, o_Cv; w in o_w, o_w1)
+  deviceMemory<dlong> o_Crs, o_Cci;
+  deviceMemory<double> o_Cv, o_w1;
+  deviceMemory<char> o_ws2;  // two reduction workspaces (the assembled form alternates them)
   deviceMemory<double> o_s, o_z, o_p, o_Ap, o_AqL;
   deviceMemory<unsigned> o_bar;
   // single-reduction form: w, the parity pairs of p, s, r (r[0] is o_r), u, owner flags
@@ -680,6 +695,10 @@ int coarseJacobiPCG_t::solveCG1Hex1(deviceMemory<double>& o_X) {
       o_u2[k] = p.malloc<double>(N);
     }
     o_r1 = p.malloc<double>(N);
+    if (cg1csr) {
+      o_w1 = p.malloc<double>(N);
+      o_ws2 = deviceMemory<char>::alloc(2 * (size_t)libp_reduce_ws_bytes());
+    }
     // the first local copy of each gathered node writes its updates
     const ogs::ogs_t& og = elliptic->ogsMasked;
     std::vector<unsigned char> own((size_t)m.Nelements * m.Np, 0);
@@ -714,7 +733,13 @@ void coarseJacobiPCG_t::enqueueCG1Chunk(deviceMemory<double>& o_X, int itStart, 
   double* R2[2] = {o_r.ptr(), o_r1.ptr()};
   double* U2[2] = {o_u2[0].ptr(), o_u2[1].ptr()};
   const int first = itStart == 0;
+  double* W2[2] = {o_w.ptr(), o_w1.ptr()};
   auto call = [&](int mode, int parity) {
+    if (cg1csr) {
+      LIBP_CALL(libp_pcg_jacobi_csr_cg1_f64(N, o_Crs.ptr(), o_Cci.ptr(), o_Cv.ptr(), jacobi->o_invDiagA.ptr(), tol,
+                                            mode, parity, o_X.ptr(), W2, P2, S2, R2, U2, o_s.ptr(), o_ws2.ptr(), st));
+      return;
+    }
     LIBP_CALL(libp_pcg_jacobi_hex1_cg1_f64(
         m.Nelements, elliptic->o_GlobalToLocal.ptr(), o_owner.ptr(), m.o_wJ.ptr(), m.o_ggeo.ptr(), m.o_D.ptr(),
         elliptic->lambda, N, elliptic->ogsMasked.o_rowStartsT.ptr(), elliptic->ogsMasked.o_colIdsT.ptr(),
@@ -727,7 +752,7 @@ void coarseJacobiPCG_t::enqueueCG1Chunk(deviceMemory<double>& o_X, int itStart, 
       const int itc = itStart + c;
       call(itc == 0 ? 1 : 2, itc & 1);
     }
-    call(3, 0);  // the stopping test of the last iteration
+    call(3, (itStart + n - 1) & 1);  // the stopping test of the last iteration (of this parity)
   };
   if (!useGraph) {
     launch();

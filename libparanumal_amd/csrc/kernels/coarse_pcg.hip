@@ -302,48 +302,11 @@ struct CG1Vec {
   double *p, *s, *r, *u;
 };
 
-__global__ void __launch_bounds__(256) coarse_cg1_ax_kernel(
-    const int E, const int* __restrict__ G2L, const unsigned char* __restrict__ owner, const double* __restrict__ wJ,
-    const double* __restrict__ ggeo, const double* __restrict__ D, const double lambda, const int mode,
-    const int P, const double tol2, const int nb, const double* __restrict__ invD, const double* __restrict__ w,
-    double* __restrict__ x, const CG1Vec cur, const CG1Vec nxt, double* s, const ReduceWS* __restrict__ ws,
-    double* __restrict__ AqL) {
-  // mode 0: start (u = invD r, x = 0, no update); 1: first update (beta = 0); 2: later updates
-  __shared__ double s_ab[2];
-  __shared__ int s_stop;
-  if (s[8] != 0.0) return;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int e = t >> 3, n = t & 7;
-  const bool live = e < E;
-  Hex1Node v{};
-  int g = -1;
-  bool own = false;
-  double uo = 0, wo = 0, po = 0, so = 0, xo = 0, ro = 0, dg = 0;
-  if (live) {
-    g = G2L[(size_t)e * 8 + n];
-    if (g >= 0) {
-      own = owner[(size_t)e * 8 + n] != 0;
-      ro = cur.r[g];
-      dg = invD[g];
-      if (mode > 0) {
-        uo = cur.u[g];
-        wo = w[g];
-        if (own) xo = x[g];
-        if (mode > 1) {
-          po = cur.p[g];
-          so = cur.s[g];
-        }
-      }
-    }
-    const double* G = ggeo + (size_t)e * 48 + n;
-    v.G00 = G[0];
-    v.G01 = G[8];
-    v.G02 = G[16];
-    v.G11 = G[24];
-    v.G12 = G[32];
-    v.G22 = G[40];
-    v.W = wJ[(size_t)e * 8 + n];
-  }
+// The step scalars of a single-reduction iteration from the previous launch's
+// r.u, w.u, r.r partials: every block sums them in the same order and reaches
+// the same alpha, beta and stop decision; block 0 records them in s.
+__device__ inline void cg1_scalars(const int mode, const int P, const double tol2, const int nb, double* s,
+                                   const ReduceWS* __restrict__ ws, double* s_ab, int& s_stop) {
   if (threadIdx.x < 64) {
     double ga = 0, de = 0, rr = 0;
     if (mode > 0) {
@@ -386,6 +349,51 @@ __global__ void __launch_bounds__(256) coarse_cg1_ax_kernel(
     }
   }
   __syncthreads();
+}
+
+__global__ void __launch_bounds__(256) coarse_cg1_ax_kernel(
+    const int E, const int* __restrict__ G2L, const unsigned char* __restrict__ owner, const double* __restrict__ wJ,
+    const double* __restrict__ ggeo, const double* __restrict__ D, const double lambda, const int mode,
+    const int P, const double tol2, const int nb, const double* __restrict__ invD, const double* __restrict__ w,
+    double* __restrict__ x, const CG1Vec cur, const CG1Vec nxt, double* s, const ReduceWS* __restrict__ ws,
+    double* __restrict__ AqL) {
+  // mode 0: start (u = invD r, x = 0, no update); 1: first update (beta = 0); 2: later updates
+  __shared__ double s_ab[2];
+  __shared__ int s_stop;
+  if (s[8] != 0.0) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = t >> 3, n = t & 7;
+  const bool live = e < E;
+  Hex1Node v{};
+  int g = -1;
+  bool own = false;
+  double uo = 0, wo = 0, po = 0, so = 0, xo = 0, ro = 0, dg = 0;
+  if (live) {
+    g = G2L[(size_t)e * 8 + n];
+    if (g >= 0) {
+      own = owner[(size_t)e * 8 + n] != 0;
+      ro = cur.r[g];
+      dg = invD[g];
+      if (mode > 0) {
+        uo = cur.u[g];
+        wo = w[g];
+        if (own) xo = x[g];
+        if (mode > 1) {
+          po = cur.p[g];
+          so = cur.s[g];
+        }
+      }
+    }
+    const double* G = ggeo + (size_t)e * 48 + n;
+    v.G00 = G[0];
+    v.G01 = G[8];
+    v.G02 = G[16];
+    v.G11 = G[24];
+    v.G12 = G[32];
+    v.G22 = G[40];
+    v.W = wJ[(size_t)e * 8 + n];
+  }
+  cg1_scalars(mode, P, tol2, nb, s, ws, s_ab, s_stop);
   if (s_stop) return;  // the same decision in every block
   double un = 0.0;
   if (live && g >= 0) {
@@ -464,6 +472,137 @@ __global__ void __launch_bounds__(64) coarse_cg1_check_kernel(const int nb, doub
   if (threadIdx.x == 0) {
     s[5] = rr;
     if (s[7] > 0.0 && rr <= s[7]) s[8] = 1.0;
+  }
+}
+
+// The single-reduction iteration on the assembled operator (CSR rows of the
+// gathered dofs, columns ascending): one launch per iteration. A row's eight
+// lanes recompute the updated u at every column from the previous iterate
+// (read from the buffers of parity P) and form w = A u for the row; lane 0
+// writes the row's updated vectors to parity P^1 and adds the row to the r.u,
+// w.u, r.r partials. Every site evaluates the update with the same
+// uncontracted expressions, so a column's u agrees bit for bit with the u its
+// own row stores. Modes and scalars as in coarse_cg1_ax_kernel; mode 0 writes
+// u and w of parity P in place (nothing reads them in that launch). The
+// partials of the previous launch are read from wsIn and this launch's are
+// written to wsOut, a different workspace: the blocks of one launch need not
+// all be resident at once, so one block may finish before another starts.
+constexpr int CSR_THREADS = 512, CSR_LANES = 8, CSR_ROWS = CSR_THREADS / CSR_LANES;
+
+__global__ void __launch_bounds__(CSR_THREADS) coarse_cg1_csr_kernel(
+    const int Nrows, const int* __restrict__ rowStarts, const int* __restrict__ colIds,
+    const double* __restrict__ vals, const int mode, const int P, const double tol2, const int nb,
+    const double* __restrict__ invD, double* __restrict__ x, const CG1Vec cur, const CG1Vec nxt,
+    double* __restrict__ wcur, double* __restrict__ wnxt, double* s, const ReduceWS* __restrict__ wsIn,
+    ReduceWS* __restrict__ wsOut) {
+#pragma clang fp contract(off)
+  __shared__ double s_ab[2];
+  __shared__ int s_stop;
+  __shared__ double s_w[16];
+  if (s[8] != 0.0) return;
+  // u at gathered dof j after this iteration's update, from its loaded
+  // r, w, s and invD
+  auto unew = [&](double ro, double wo, double so, double dg, double alpha, double beta) -> double {
+    if (mode == 0) return 1.0 * dg * ro;
+    const double sn = (mode == 1) ? wo : wo + beta * so;
+    const double rn = ro - alpha * sn;
+    return 1.0 * dg * rn;
+  };
+  const int sub = threadIdx.x & (CSR_LANES - 1);
+  double ga = 0, de = 0, rr = 0, alpha = 0, beta = 0;
+  bool first = true;
+  for (int g0 = blockIdx.x * CSR_ROWS; g0 < Nrows; g0 += gridDim.x * CSR_ROWS) {
+    const int g = g0 + threadIdx.x / CSR_LANES;
+    const bool row = g < Nrows;
+    // every load of the pass is issued before the step scalars are formed:
+    // none of them depends on the scalars
+    int a = 0, b = 0;
+    double uo = 0, wo = 0, ro = 0, dg = 0, po = 0, so = 0, xo = 0;
+    if (row) {
+      a = rowStarts[g];
+      b = rowStarts[g + 1];
+      if (sub == 0) {
+        ro = cur.r[g];
+        dg = invD[g];
+        if (mode > 0) {
+          uo = cur.u[g];
+          wo = wcur[g];
+          xo = x[g];
+          if (mode > 1) {
+            po = cur.p[g];
+            so = cur.s[g];
+          }
+        }
+      }
+    }
+    int c[4];
+    double v[4], cr[4], cw[4], cs[4], cd[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int jk = a + sub + CSR_LANES * k;
+      c[k] = (jk < b) ? colIds[jk] : -1;
+      v[k] = (jk < b) ? vals[jk] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int j = c[k];
+      cr[k] = (j >= 0) ? cur.r[j] : 0.0;
+      cd[k] = (j >= 0) ? invD[j] : 0.0;
+      cw[k] = (j >= 0 && mode > 0) ? wcur[j] : 0.0;
+      cs[k] = (j >= 0 && mode > 1) ? cur.s[j] : 0.0;
+    }
+    if (first) {
+      cg1_scalars(mode, P, tol2, nb, s, wsIn, s_ab, s_stop);
+      if (s_stop) return;  // the same decision in every block
+      alpha = s_ab[0];
+      beta = s_ab[1];
+      first = false;
+    }
+    double acc = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (c[k] >= 0) acc += v[k] * unew(cr[k], cw[k], cs[k], cd[k], alpha, beta);
+    for (int jk = a + sub + 4 * CSR_LANES; jk < b; jk += CSR_LANES) {
+      const int j = colIds[jk];
+      acc += vals[jk] * unew(cur.r[j], mode > 0 ? wcur[j] : 0.0, mode > 1 ? cur.s[j] : 0.0, invD[j], alpha, beta);
+    }
+    // the row's sum over its lanes (xor butterfly: every lane holds the same value)
+#pragma unroll
+    for (int m = 1; m < CSR_LANES; m <<= 1) acc += __shfl_xor(acc, m, 64);
+    if (row && sub == 0) {
+      const double wv = acc;
+      if (mode == 0) {
+        const double un = unew(ro, 0.0, 0.0, dg, alpha, beta);
+        x[g] = 0.0;
+        cur.u[g] = un;
+        wcur[g] = wv;
+        ga += ro * un;
+        de += wv * un;
+        rr += ro * ro;
+      } else {
+        const double pn = (mode == 1) ? uo : uo + beta * po;
+        const double sn = (mode == 1) ? wo : wo + beta * so;
+        const double rn = ro - alpha * sn;
+        const double un = 1.0 * dg * rn;
+        x[g] = xo + alpha * pn;
+        nxt.p[g] = pn;
+        nxt.s[g] = sn;
+        nxt.r[g] = rn;
+        nxt.u[g] = un;
+        wnxt[g] = wv;
+        ga += rn * un;
+        de += wv * un;
+        rr += rn * rn;
+      }
+    }
+  }
+  const double b0 = block_sum(ga, s_w);
+  const double b1 = block_sum(de, s_w);
+  const double b2 = block_sum(rr, s_w);
+  if (threadIdx.x == 0) {
+    wsOut->part[0][blockIdx.x] = b0;
+    wsOut->part[1][blockIdx.x] = b1;
+    wsOut->part[2][blockIdx.x] = b2;
   }
 }
 
@@ -782,4 +921,30 @@ extern "C" int libp_pcg_jacobi_hex1_cg1_f64(int Nelements, const int* GlobalToLo
   hipLaunchKernelGGL(coarse_cg1_gather_kernel, dim3(nb), dim3(RBS), 0, st, Nrows, rowStarts, colIds, AqL, out.r,
                      out.u, w, scalars, (ReduceWS*)ws);
   return launch_status("pcg_jacobi_hex1_cg1");
+}
+
+extern "C" int libp_pcg_jacobi_csr_cg1_f64(int Nrows, const int* rowStarts, const int* colIds, const double* vals,
+                                           const double* invDiag, double tol, int mode, int parity, double* x,
+                                           double* const* w2, double* const* p2, double* const* s2,
+                                           double* const* r2, double* const* u2, double* scalars, void* ws,
+                                           void* stream) {
+  if (Nrows <= 0) return LIBP_OK;
+  if (!rowStarts || !colIds || !vals || !invDiag || !x || !w2 || !p2 || !s2 || !r2 || !u2 || !scalars || !ws)
+    return arg_error("pcg_jacobi_csr_cg1", "null pointer");
+  if (mode < 0 || mode > 3 || (parity & ~1)) return arg_error("pcg_jacobi_csr_cg1", "mode 0..3, parity 0 or 1");
+  hipStream_t st = as_stream(stream);
+  const int nb = (Nrows + CSR_ROWS - 1) / CSR_ROWS < RMAXB ? (Nrows + CSR_ROWS - 1) / CSR_ROWS : RMAXB;
+  ReduceWS* w = (ReduceWS*)ws;  // two workspaces, used by parity
+  const int a = parity, b = parity ^ 1;
+  if (mode == 3) {
+    // iteration (mode 1 or 2) of parity a wrote its partials to w[b]
+    hipLaunchKernelGGL(coarse_cg1_check_kernel, dim3(1), dim3(64), 0, st, nb, scalars, (const ReduceWS*)(w + b));
+    return launch_status("pcg_jacobi_csr_cg1 (check)");
+  }
+  const CG1Vec cur{p2[a], s2[a], r2[a], u2[a]}, nxt{p2[b], s2[b], r2[b], u2[b]};
+  // mode 0 writes w[a], which iteration 0 (mode 1, parity a) reads
+  hipLaunchKernelGGL(coarse_cg1_csr_kernel, dim3(nb), dim3(CSR_THREADS), 0, st, Nrows, rowStarts, colIds, vals, mode,
+                     parity, tol * tol, nb, invDiag, x, cur, nxt, w2[a], w2[b], scalars, (const ReduceWS*)(w + a),
+                     w + (mode == 0 ? a : b));
+  return launch_status("pcg_jacobi_csr_cg1");
 }

@@ -650,6 +650,57 @@ def test_coarse_single_reduction_matches_three_kernel():
         else:
             assert abs(one.geti("coarseIterations") - three.geti("coarseIterations")) <= 1
 
+def test_coarse_assembled_matches_element_form():
+    """The single-reduction coarse Jacobi-PCG on the assembled N=1 operator
+    (one launch per iteration) against its element form: with a fixed count
+    of coarse iterations the p-MG solves agree to rounding and take the same
+    outer iterations; with a tolerance the coarse counts agree to +-1.
+    Repeated solves are bit-identical. The 42^3-element case has 68,921
+    coarse rows, more than one pass of the kernel's 1024 blocks of 64 rows."""
+    import libparanumal_amd as lp
+    cases = [(0.0, {"BOX NX": 4}), (1e-2, {"BOX NX": 4}),
+             (1e-2, {"BOX NX": 42, "BOX NY": 42, "BOX NZ": 42, "POLYNOMIAL DEGREE": 3,
+                                       "MULTIGRID DEGREES": "3,1"})]
+    for ctol, kw in cases:
+        base = mg_settings("FP32", **{"LINEAR SOLVER STOPPING CRITERION": "RELATIVE",
+                                      "MULTIGRID COARSE TOLERANCE": ctol, "MULTIGRID COARSE MAXIT": 40}, **kw)
+        elem = lp.Elliptic(dict(base, **{"MULTIGRID COARSE ASSEMBLED": "FALSE"}))
+        asm = lp.Elliptic(dict(base, **{"MULTIGRID COARSE ASSEMBLED": "TRUE"}))
+        a = elem.run()
+        runs = [asm.run() for _ in range(3)]
+        b = runs[0]
+        print("assembled coarse", ctol, kw.get("BOX NX"), a["iterations"], b["iterations"], a["norm"], b["norm"],
+              elem.geti("coarseIterations"), asm.geti("coarseIterations"), flush=True)
+        assert all(r["norm"] == b["norm"] and r["iterations"] == b["iterations"] for r in runs)
+        assert abs(a["iterations"] - b["iterations"]) <= (0 if ctol == 0 else 1)
+        assert abs(a["norm"] - b["norm"]) <= 1e-9 * abs(a["norm"])
+        assert b["r_final"] <= 1e-8 * b["r0"] * (1 + 1e-12)
+        if ctol == 0:
+            assert asm.geti("coarseIterations") == elem.geti("coarseIterations") == 40
+            x1, x3 = asm.solution(), elem.solution()
+            assert np.max(np.abs(x1 - x3)) <= 1e-9 * np.max(np.abs(x3))
+        else:
+            assert abs(asm.geti("coarseIterations") - elem.geti("coarseIterations")) <= 1
+
+
+def test_coarse_assembled_vcycle_vs_oracle():
+    """The FP64 V-cycle with the assembled single-reduction coarse solve (15
+    coarse iterations) against the oracle's FP64 V-cycle."""
+    import ctypes
+    import libparanumal_amd as lp
+    s = mg_settings("FP64", **{"MULTIGRID COARSE ASSEMBLED": "TRUE"})
+    o = OracleElliptic(s)
+    o.mg_setup()
+    p = lp.Elliptic(s)
+    r = np.random.default_rng(17).standard_normal(o.geti("Ngather"))
+    ref = o.mg_apply(r)
+    rd = dev(r)
+    z = torch.zeros_like(rd)
+    p.precon(rd, z)
+    ctypes.CDLL("libamdhip64.so").hipStreamSynchronize(ctypes.c_void_p(p.stream))
+    assert relerr(z.cpu().numpy(), ref) < 1e-9
+
+
 def test_coarse_persistent_vcycle_vs_oracle():
     """The V-cycle tests above run the default chunked coarse solve; this one runs
     the persistent one-launch coarse solve (15 coarse iterations) against the

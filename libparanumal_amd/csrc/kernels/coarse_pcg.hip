@@ -487,7 +487,7 @@ __global__ void __launch_bounds__(64) coarse_cg1_check_kernel(const int nb, doub
 // partials of the previous launch are read from wsIn and this launch's are
 // written to wsOut, a different workspace: the blocks of one launch need not
 // all be resident at once, so one block may finish before another starts.
-constexpr int CSR_THREADS = 512, CSR_LANES = 8, CSR_ROWS = CSR_THREADS / CSR_LANES;
+constexpr int CSR_THREADS = 1024, CSR_LANES = 8, CSR_ROWS = CSR_THREADS / CSR_LANES;
 
 __global__ void __launch_bounds__(CSR_THREADS) coarse_cg1_csr_kernel(
     const int Nrows, const int* __restrict__ rowStarts, const int* __restrict__ colIds,

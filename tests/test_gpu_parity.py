@@ -655,11 +655,11 @@ def test_coarse_assembled_matches_element_form():
     (one launch per iteration) against its element form: with a fixed count
     of coarse iterations the p-MG solves agree to rounding and take the same
     outer iterations; with a tolerance the coarse counts agree to +-1.
-    Repeated solves are bit-identical. The 42^3-element case has 68,921
-    coarse rows, more than one pass of the kernel's 1024 blocks of 64 rows."""
+    Repeated solves are bit-identical. The 52^3-element case has 132,651
+    coarse rows, more than one pass of the kernel's 1024 blocks of 128 rows."""
     import libparanumal_amd as lp
     cases = [(0.0, {"BOX NX": 4}), (1e-2, {"BOX NX": 4}),
-             (1e-2, {"BOX NX": 42, "BOX NY": 42, "BOX NZ": 42, "POLYNOMIAL DEGREE": 3,
+             (1e-2, {"BOX NX": 52, "BOX NY": 52, "BOX NZ": 52, "POLYNOMIAL DEGREE": 3,
                                        "MULTIGRID DEGREES": "3,1"})]
     for ctol, kw in cases:
         base = mg_settings("FP32", **{"LINEAR SOLVER STOPPING CRITERION": "RELATIVE",

@@ -487,7 +487,7 @@ __global__ void __launch_bounds__(64) coarse_cg1_check_kernel(const int nb, doub
 // partials of the previous launch are read from wsIn and this launch's are
 // written to wsOut, a different workspace: the blocks of one launch need not
 // all be resident at once, so one block may finish before another starts.
-constexpr int CSR_THREADS = 1024, CSR_LANES = 8, CSR_ROWS = CSR_THREADS / CSR_LANES;
+constexpr int CSR_THREADS = 1024, CSR_LANES = 8, CSR_ROWS = CSR_THREADS / CSR_LANES, CSR_UNROLL = 32 / CSR_LANES;
 
 __global__ void __launch_bounds__(CSR_THREADS) coarse_cg1_csr_kernel(
     const int Nrows, const int* __restrict__ rowStarts, const int* __restrict__ colIds,
@@ -535,16 +535,16 @@ __global__ void __launch_bounds__(CSR_THREADS) coarse_cg1_csr_kernel(
         }
       }
     }
-    int c[4];
-    double v[4], cr[4], cw[4], cs[4], cd[4];
+    int c[CSR_UNROLL];
+    double v[CSR_UNROLL], cr[CSR_UNROLL], cw[CSR_UNROLL], cs[CSR_UNROLL], cd[CSR_UNROLL];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < CSR_UNROLL; ++k) {
       const int jk = a + sub + CSR_LANES * k;
       c[k] = (jk < b) ? colIds[jk] : -1;
       v[k] = (jk < b) ? vals[jk] : 0.0;
     }
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < CSR_UNROLL; ++k) {
       const int j = c[k];
       cr[k] = (j >= 0) ? cur.r[j] : 0.0;
       cd[k] = (j >= 0) ? invD[j] : 0.0;
@@ -560,9 +560,9 @@ __global__ void __launch_bounds__(CSR_THREADS) coarse_cg1_csr_kernel(
     }
     double acc = 0;
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < CSR_UNROLL; ++k)
       if (c[k] >= 0) acc += v[k] * unew(cr[k], cw[k], cs[k], cd[k], alpha, beta);
-    for (int jk = a + sub + 4 * CSR_LANES; jk < b; jk += CSR_LANES) {
+    for (int jk = a + sub + CSR_UNROLL * CSR_LANES; jk < b; jk += CSR_LANES) {
       const int j = colIds[jk];
       acc += vals[jk] * unew(cur.r[j], mode > 0 ? wcur[j] : 0.0, mode > 1 ? cur.s[j] : 0.0, invD[j], alpha, beta);
     }

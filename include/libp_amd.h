@@ -527,6 +527,16 @@ int libp_elliptic_csr(void* h, int* rowStarts, int* cols, double* vals);
 int libp_elliptic_amg_setup(void* h);
 int libp_elliptic_amg_vcycle(void* h, const double* b, double* x);
 int libp_elliptic_amg_info(void* h, double* out, int nout);
+/* p-MG level access, for tests that compare each level operation with the CPU
+ * restatement. ndofs returns level `level`'s gathered length (or -status) and the
+ * level precision in *bits (32 or 64). op: 0 out = A in (MGLevel::Operator,
+ * ellipticPreconMultiGridLevel.cpp:30-32); 1 out = R in (coarsen, :41-82); 2 out +=
+ * P in (prolongate, :84-125); 3 smoother (:127-206; in = rhs, out = x updated in
+ * place, flag = x is zero); 4 out = inverse diagonal. Device vectors of the level
+ * precision. lambda returns the Chebyshev interval {lambda1, lambda0} (:361-368). */
+long libp_elliptic_mg_level_ndofs(void* h, int level, int* bits);
+int libp_elliptic_mg_level_op(void* h, int level, int op, const void* in, void* out, int flag);
+int libp_elliptic_mg_level_lambda(void* h, int level, double* lambda);
 /* LINEAR architecture: Gram matrices of one least-squares sample (device FP32 r
  * and eh; host G [nIn][nIn] = X^T X, B [Np][nIn] = T^T X with X the BF16 network
  * input of r and T[e][n] = bf16(eh / s_e) at the element's nodes). */

@@ -168,6 +168,9 @@ SIGNATURES = {
     "libp_elliptic_amg_setup": (c_int, [c_void_p]),
     "libp_elliptic_amg_vcycle": (c_int, [c_void_p, c_void_p, c_void_p]),
     "libp_elliptic_amg_info": (c_int, [c_void_p, c_void_p, c_int]),
+    "libp_elliptic_mg_level_ndofs": (c_long, [c_void_p, c_int, ctypes.POINTER(c_int)]),
+    "libp_elliptic_mg_level_op": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int]),
+    "libp_elliptic_mg_level_lambda": (c_int, [c_void_p, c_int, ctypes.POINTER(c_double)]),
     "libp_elliptic_run": (c_int, [c_void_p, ctypes.POINTER(c_double), c_int]),
     "libp_elliptic_solution": (c_int, [c_void_p, c_void_p]),
 }
@@ -338,6 +341,25 @@ class Elliptic:
         check(lib().libp_elliptic_learned_linear_gram(self._h, r.data_ptr(), eh.data_ptr(), G.ctypes.data,
                                                       B.ctypes.data), "learned_linear_gram")
         return G, B
+
+    # p-MG level access (libp_elliptic_mg_level_*): operations on device vectors
+    MG_OPERATOR, MG_COARSEN, MG_PROLONGATE, MG_SMOOTH, MG_INVDIAG = range(5)
+
+    def mg_level_ndofs(self, level):
+        bits = c_int()
+        n = lib().libp_elliptic_mg_level_ndofs(self._h, level, ctypes.byref(bits))
+        if n < 0:
+            check(-n, "mg_level_ndofs")
+        return int(n), bits.value
+
+    def mg_level_op(self, level, op, vin, vout, flag=0):
+        check(lib().libp_elliptic_mg_level_op(self._h, level, op, vin.data_ptr() if vin is not None else None,
+                                              vout.data_ptr(), int(flag)), "mg_level_op")
+
+    def mg_level_lambda(self, level):
+        out = (c_double * 2)()
+        check(lib().libp_elliptic_mg_level_lambda(self._h, level, out), "mg_level_lambda")
+        return out[0], out[1]
 
     def learned_status(self):
         return lib().libp_elliptic_learned_status(self._h).decode()

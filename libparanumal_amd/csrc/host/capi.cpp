@@ -215,6 +215,41 @@ extern "C" int libp_elliptic_solution(void* p, double* xL_host) {
   });
 }
 
+/* ------------------------- p-MG level access (tests) ------------------------- */
+namespace {
+MultiGridPrecon& multigridOf(Handle* h) {
+  LIBP_ABORT("null handle", !h);
+  auto* mg = dynamic_cast<MultiGridPrecon*>(h->elliptic->precon.get());
+  LIBP_ABORT("the handle has no p-MG preconditioner", !mg);
+  return *mg;
+}
+}  // namespace
+
+extern "C" long libp_elliptic_mg_level_ndofs(void* p, int level, int* bits) {
+  long n = -1;
+  int st = guarded("libp_elliptic_mg_level_ndofs", [&]() {
+    MultiGridPrecon& mg = multigridOf((Handle*)p);
+    if (bits) *bits = multigridPrecisionBits(mg);
+    n = multigridLevelNdofs(mg, level);
+  });
+  return st == LIBP_OK ? n : -st;
+}
+
+extern "C" int libp_elliptic_mg_level_op(void* p, int level, int op, const void* in, void* out, int flag) {
+  Handle* h = (Handle*)p;
+  return guarded("libp_elliptic_mg_level_op", [&]() {
+    LIBP_ABORT("null pointer", (!in && op != 4) || !out);
+    multigridLevelOp(multigridOf(h), level, op, in, out, flag);
+  });
+}
+
+extern "C" int libp_elliptic_mg_level_lambda(void* p, int level, double* lambda) {
+  return guarded("libp_elliptic_mg_level_lambda", [&]() {
+    LIBP_ABORT("null pointer", !lambda);
+    multigridLevelLambda(multigridOf((Handle*)p), level, lambda);
+  });
+}
+
 /* ------------------------- learned fine smoother ------------------------- */
 namespace {
 learnedSmoother_t* learnedOf(Handle* h) {

@@ -125,6 +125,15 @@ int multigridCoarseIterations(MultiGridPrecon& mg);
 // FP32 inverse diagonal and operator of the fine p-MG level (FP32 preconditioner only)
 deviceMemory<float> multigridFineInvDiag32(MultiGridPrecon& mg);
 void multigridApplyFineOperator32(MultiGridPrecon& mg, deviceMemory<float>& x, deviceMemory<float>& y);
+// Level access for tests: 32 or 64 (the level precision), the gathered length of
+// level l, one operation of level l (0 A, 1 coarsen to l+1, 2 prolongate from l+1
+// (out += P in), 3 smooth (in = rhs, out = x in place, flag = x is zero), 4 the
+// inverse diagonal) on device vectors of that precision, and its Chebyshev
+// interval {lambda1, lambda0}.
+int multigridPrecisionBits(MultiGridPrecon& mg);
+long multigridLevelNdofs(MultiGridPrecon& mg, int l);
+void multigridLevelOp(MultiGridPrecon& mg, int l, int op, const void* in, void* out, int flag);
+void multigridLevelLambda(MultiGridPrecon& mg, int l, double* lambda);
 
 /* Training of the learned fine smoother (no reference counterpart). Pairs
  * (r, eh) come from FPCG solves preconditioned by this p-MG with the Chebyshev

@@ -1236,6 +1236,61 @@ void multigridApplyFineOperator32(MultiGridPrecon& mg, deviceMemory<float>& x, d
   m->levels[0]->Operator(x, y);
 }
 
+namespace {
+template <typename T>
+void levelOp(multigrid_t<T>& m, int l, int op, const void* in, void* out, int flag) {
+  LIBP_ABORT("no such p-MG level", l < 0 || l + 1 >= (int)m.ellipticLevels.size());
+  MGLevel<T>& L = *m.levels[l];
+  const dlong Nf = m.ellipticLevels[l]->Ndofs, Nc = m.ellipticLevels[l + 1]->Ndofs;
+  T* pin = const_cast<T*>(static_cast<const T*>(in));
+  T* pout = static_cast<T*>(out);
+  deviceMemory<T> x = deviceMemory<T>::wrap(pin, op == 2 ? Nc : Nf);
+  deviceMemory<T> y = deviceMemory<T>::wrap(pout, op == 1 ? Nc : Nf);
+  switch (op) {
+    case 0: L.Operator(x, y); break;
+    case 1: L.coarsen(x, y); break;
+    case 2: L.prolongate(x, y); break;
+    case 3: L.smooth(x, y, flag != 0); break;
+    case 4:
+      LIBP_HIP(hipMemcpyAsync(pout, L.o_invDiagA.ptr(), Nf * sizeof(T), hipMemcpyDeviceToDevice,
+                              m.fine.platform.stream));
+      break;
+    default: LIBP_FORCE_ABORT("unknown p-MG level operation");
+  }
+}
+}  // namespace
+
+int multigridPrecisionBits(MultiGridPrecon& mg) {
+  if (dynamic_cast<multigrid_t<float>*>(mg.impl.get())) return 32;
+  if (dynamic_cast<multigrid_t<double>*>(mg.impl.get())) return 64;
+  return 0;
+}
+
+long multigridLevelNdofs(MultiGridPrecon& mg, int l) {
+  std::vector<std::shared_ptr<elliptic_t>>* lv = nullptr;
+  if (auto* m = dynamic_cast<multigrid_t<float>*>(mg.impl.get())) lv = &m->ellipticLevels;
+  if (auto* m = dynamic_cast<multigrid_t<double>*>(mg.impl.get())) lv = &m->ellipticLevels;
+  if (!lv || l < 0 || l >= (int)lv->size()) return -1;
+  return (*lv)[l]->Ndofs;
+}
+
+void multigridLevelOp(MultiGridPrecon& mg, int l, int op, const void* in, void* out, int flag) {
+  if (auto* m = dynamic_cast<multigrid_t<float>*>(mg.impl.get())) return levelOp(*m, l, op, in, out, flag);
+  if (auto* m = dynamic_cast<multigrid_t<double>*>(mg.impl.get())) return levelOp(*m, l, op, in, out, flag);
+  LIBP_FORCE_ABORT("no p-MG preconditioner");
+}
+
+void multigridLevelLambda(MultiGridPrecon& mg, int l, double* lambda) {
+  auto get = [&](auto* m) {
+    LIBP_ABORT("no such p-MG level", l < 0 || l >= (int)m->levels.size());
+    lambda[0] = m->levels[l]->lambda1;
+    lambda[1] = m->levels[l]->lambda0;
+  };
+  if (auto* m = dynamic_cast<multigrid_t<float>*>(mg.impl.get())) return get(m);
+  if (auto* m = dynamic_cast<multigrid_t<double>*>(mg.impl.get())) return get(m);
+  LIBP_FORCE_ABORT("no p-MG preconditioner");
+}
+
 int multigridCoarseIterations(MultiGridPrecon& mg) {
   if (auto* m = dynamic_cast<multigrid_t<float>*>(mg.impl.get())) return m->coarse->iterations();
   if (auto* m = dynamic_cast<multigrid_t<double>*>(mg.impl.get())) return m->coarse->iterations();

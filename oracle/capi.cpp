@@ -231,6 +231,53 @@ int orc_mg_setup(void* p) {
 
 void orc_mg_apply(void* p, const double* r, double* z) { ((Handle*)p)->mg->apply(r, z); }
 
+// Gathered length of level l of the handle's multigrid (-1 if there is no such level).
+long orc_mg_level_ndofs(void* p, int l) {
+  Handle* h = (Handle*)p;
+  if (!h->mg || l < 0 || l >= (int)h->mg->levels.size()) return -1;
+  return h->mg->levels[l]->Ndofs;
+}
+
+// One operation of p-multigrid level l (0 <= l < number of levels - 1):
+//   op 0: out = A_l in;  op 1: out = R in (coarsen l -> l+1);
+//   op 2: out += P in (prolongate l+1 -> l);  op 3: smoother, in = rhs, out = x
+//   (updated in place, flag = x is zero);  op 4: out = invDiag of level l.
+int orc_mg_level_op(void* p, int l, int op, const double* in, double* out, int flag) {
+  Handle* h = (Handle*)p;
+  if (!h->mg || l < 0 || l + 1 >= (int)h->mg->levels.size()) {
+    g_err = "no such multigrid level";
+    return -1;
+  }
+  const MGLevelO& L = h->mg->mg[l];
+  switch (op) {
+    case 0: {
+      std::vector<double> scratch;
+      L.el->Operator(in, out, scratch);
+      return 0;
+    }
+    case 1: coarsen(L, in, out); return 0;
+    case 2: prolongate(L, in, out); return 0;
+    case 3:
+      if (L.stype == 1) smoothChebyshev(L, in, out, flag != 0);
+      else smoothJacobi(L, in, out, flag != 0);
+      return 0;
+    case 4: std::copy(L.invDiagA.begin(), L.invDiagA.end(), out); return 0;
+    default: g_err = "unknown multigrid level operation"; return -1;
+  }
+}
+
+// rho(D^-1 A) of the handle's own operator by the smoother setup's 10-step
+// Arnoldi, started from the drand48 stream seeded with `seed`.
+double orc_max_eig(void* p, long seed) {
+  Handle* h = (Handle*)p;
+  std::vector<double> d;
+  h->el.BuildOperatorDiagonal(d);
+  for (auto& v : d) v = 1.0 / v;
+  Drand48 r;
+  r.seed(seed);
+  return maxEigSmoothAx(h->el, d, r);
+}
+
 double orc_mg_level_d(void* p, int l, const char* name) {
   Handle* h = (Handle*)p;
   std::string n(name);

@@ -216,6 +216,14 @@ struct MultiGrid {
   std::vector<std::vector<double>> rhsL, xL;
 };
 
+/* p-multigrid level operations (ellipticPreconMultiGridLevel.cpp), exposed so
+ * that tests can pin each one against closed forms on small dense systems. */
+double maxEigSmoothAx(const Elliptic& el, const std::vector<double>& invDiagA, Drand48& drng);  // :393-473
+void smoothChebyshev(const MGLevelO& L, const double* r, double* X, bool xIsZero);             // :154-206
+void smoothJacobi(const MGLevelO& L, const double* r, double* X, bool xIsZero);                // :135-152
+void coarsen(const MGLevelO& L, const double* X, double* Rx);                                  // :41-82
+void prolongate(const MGLevelO& L, const double* Xc, double* Px);                              // :84-125
+
 /* End-to-end run (ellipticRun.cpp:30-248). Returns the solution norm. */
 struct RunResult {
   double norm = 0;

@@ -302,7 +302,7 @@ struct IdOp : OpFn {
 }  // namespace
 
 /* MGLevel::maxEigSmoothAx (ellipticPreconMultiGridLevel.cpp:393-473). */
-static double maxEigSmoothAx(const Elliptic& el, const std::vector<double>& invDiagA, Drand48& drng) {
+double maxEigSmoothAx(const Elliptic& el, const std::vector<double>& invDiagA, Drand48& drng) {
   const dlong N = el.Ndofs;
   int k = 10;
   if (k > N) k = N;
@@ -422,7 +422,7 @@ void MultiGrid::Setup(Elliptic& fine, GlibcRand& rng, Drand48& drng, const std::
   }
 }
 
-static void smoothChebyshev(const MGLevelO& L, const double* r, double* X, bool xIsZero) {
+void smoothChebyshev(const MGLevelO& L, const double* r, double* X, bool xIsZero) {
   const Elliptic& el = *L.el;
   const dlong N = el.Ndofs;
   const double theta = 0.5 * (L.lambda1 + L.lambda0);
@@ -455,7 +455,7 @@ static void smoothChebyshev(const MGLevelO& L, const double* r, double* X, bool 
   axpy(N, 1.0, d.data(), 1.0, X);
 }
 
-static void smoothJacobi(const MGLevelO& L, const double* r, double* X, bool xIsZero) {
+void smoothJacobi(const MGLevelO& L, const double* r, double* X, bool xIsZero) {
   const Elliptic& el = *L.el;
   const dlong N = el.Ndofs;
   if (xIsZero) {
@@ -470,7 +470,7 @@ static void smoothJacobi(const MGLevelO& L, const double* r, double* X, bool xIs
 
 /* ellipticPartialPreconCoarsenHex3D (okl/ellipticPreconCoarsenHex3D.okl:209-294)
  * applied through MGLevel::coarsen (ellipticPreconMultiGridLevel.cpp:41-82). */
-static void coarsen(const MGLevelO& L, const double* X, double* Rx) {
+void coarsen(const MGLevelO& L, const double* X, double* Rx) {
   const Elliptic& F = *L.el;
   const Elliptic& C = *L.elC;
   const int NqF = F.mesh.Nq, NqC = L.Nc + 1, NpF = F.mesh.Np, NpC = NqC * NqC * NqC;
@@ -511,7 +511,7 @@ static void coarsen(const MGLevelO& L, const double* X, double* Rx) {
 
 /* ellipticPartialPreconProlongateHex3D (okl/ellipticPreconProlongateHex3D.okl:216-301)
  * applied through MGLevel::prolongate (ellipticPreconMultiGridLevel.cpp:84-125). */
-static void prolongate(const MGLevelO& L, const double* Xc, double* Px) {
+void prolongate(const MGLevelO& L, const double* Xc, double* Px) {
   const Elliptic& F = *L.el;
   const Elliptic& C = *L.elC;
   const int NqF = F.mesh.Nq, NqC = L.Nc + 1, NpF = F.mesh.Np, NpC = NqC * NqC * NqC;

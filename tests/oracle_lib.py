@@ -48,6 +48,9 @@ def load():
             "orc_mg_setup": (c_int, [c_void_p]),
             "orc_mg_apply": (None, [c_void_p, c_void_p, c_void_p]),
             "orc_mg_level_d": (c_double, [c_void_p, c_int, c_char_p]),
+            "orc_mg_level_ndofs": (c_long, [c_void_p, c_int]),
+            "orc_mg_level_op": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int]),
+            "orc_max_eig": (c_double, [c_void_p, c_long]),
             "orc_run": (c_double, [c_char_p, ctypes.POINTER(c_int), ctypes.POINTER(c_long)]),
             "orc_gll": (None, [c_int, c_void_p, c_void_p]),
             "orc_dmatrix": (None, [c_int, c_void_p]),
@@ -166,6 +169,28 @@ class OracleElliptic:
         r = np.ascontiguousarray(r, dtype=np.float64)
         self.L.orc_mg_apply(self.h, r.ctypes.data, z.ctypes.data)
         return z
+
+    def mg_level_ndofs(self, l):
+        return int(self.L.orc_mg_level_ndofs(self.h, l))
+
+    def mg_level_d(self, l, name):
+        return float(self.L.orc_mg_level_d(self.h, l, name.encode()))
+
+    # operations of p-MG level l (orc_mg_level_op)
+    MG_OPERATOR, MG_COARSEN, MG_PROLONGATE, MG_SMOOTH, MG_INVDIAG = range(5)
+
+    def mg_level_op(self, l, op, vin, out=None, flag=0):
+        n_out = {0: l, 1: l + 1, 2: l, 3: l, 4: l}[op]
+        if out is None:
+            out = np.zeros(self.mg_level_ndofs(n_out))
+        out = np.ascontiguousarray(out, dtype=np.float64)
+        vin = np.ascontiguousarray(vin if vin is not None else np.zeros(1), dtype=np.float64)
+        if self.L.orc_mg_level_op(self.h, l, op, vin.ctypes.data, out.ctypes.data, int(flag)) != 0:
+            raise RuntimeError(self.L.orc_last_error().decode())
+        return out
+
+    def max_eig(self, seed=0):
+        return float(self.L.orc_max_eig(self.h, seed))
 
 
 def learned_dims(Np, Nfp, H):

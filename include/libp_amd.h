@@ -254,28 +254,6 @@ int libp_pcg_jacobi_csr_cg1_f64(int Nrows, const int* rowStarts, const int* colI
                                 double* const* p2, double* const* s2, double* const* r2, double* const* u2,
                                 double* scalars, void* ws, void* stream);
 
-/* The whole fused Jacobi-PCG of libp_pcg_jacobi_hex1_iteration_f64 in one
- * launch (the same counterpart: linearSolverPCG.cpp:107-171 with
- * ellipticPreconJacobi.cpp:42-51 and the N=1 ellipticAxHex3D.okl:156-295).
- * Grid barriers separate the phases, so all blocks must be resident at once:
- * the block count is capped at the device's occupancy limit. Every block sums
- * the block partials in the same order, so all blocks stop together. On entry r holds the right-hand
- * side; x receives the solution from a zero guess and r the final residual.
- * Stops when r.r <= tol^2 r0.r0, or after maxit iterations (tol = 0 runs
- * maxit). blocks: 0 = one per compute unit; more than fit at once is an error
- * (libp_pcg_jacobi_hex1_solve_blocks gives the limit). On return, once the
- * stream has run: scalars[ITERS] = iterations, scalars[RDOTR] = final r.r,
- * scalars[STOP] = 1 if converged. barrier: 4 words of device memory owned by
- * the caller (the call clears the arrival count and the error word);
- * barrier[2] becomes nonzero if a grid barrier timed out, in which case the
- * results are invalid. Calls that overlap on one barrier array race. */
-int libp_pcg_jacobi_hex1_solve_f64(int Nelements, const int* GlobalToLocal, const double* wJ, const double* ggeo,
-                                   const double* D, double lambda, int Nrows, const int* rowStarts,
-                                   const int* colIds, const double* invDiag, double tol, int maxit, int blocks,
-                                   double* x, double* r, double* z, double* p, double* Ap, double* AqL,
-                                   double* scalars, unsigned* barrier, void* ws, void* stream);
-int libp_pcg_jacobi_hex1_solve_blocks(int* maxBlocks);
-
 /* Non-blocking solvers (libs/linearSolver/linearSolverNBPCG.cpp,
  * linearSolverNBFPCG.cpp): fused update + multi-dot kernels replacing
  * update1NBPCG/update2NBPCG (okl/linearSolverUpdateNBPCG.okl) and

@@ -84,10 +84,6 @@ SIGNATURES = {
                                    c_void_p]),
     "libp_pcg_jacobi_hex1_iteration_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int,
                                                    c_void_p, c_void_p, c_void_p, c_int] + [c_void_p] * 9),
-    "libp_pcg_jacobi_hex1_solve_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int,
-                                               c_void_p, c_void_p, c_void_p, c_double, c_int, c_int]
-                                      + [c_void_p] * 10),
-    "libp_pcg_jacobi_hex1_solve_blocks": (c_int, [c_void_p]),
     "libp_pcg_jacobi_hex1_check_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
     "libp_pcg_jacobi_hex1_cg1_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int,
                                              c_void_p, c_void_p, c_void_p, c_double, c_int, c_int]

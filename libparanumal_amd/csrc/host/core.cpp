@@ -155,12 +155,6 @@ settings_t ellipticDefaultSettings() {
   s.newSetting("MULTIGRID COARSE ASSEMBLED", "TRUE",
                "Single-reduction coarse Jacobi-PCG on the assembled N=1 operator (CSR): one launch per iteration",
                {"TRUE", "FALSE"});
-  s.newSetting("MULTIGRID COARSE PERSISTENT", "FALSE",
-               "Fused coarse Jacobi-PCG as one persistent kernel with grid barriers (no host reads); slower "
-               "than the replayed graphs on a 32^3 box (31.5 against 30.0 ms per p-MG solve)",
-               {"TRUE", "FALSE"});
-  s.newSetting("MULTIGRID COARSE PERSISTENT BLOCKS", "64",
-               "Blocks of the persistent coarse solve (0: one per compute unit)");
   s.newSetting("MULTIGRID COARSE SOLVER", "JACOBI-PCG",
                "Base level of the p-MG: FP64 Jacobi-PCG to MULTIGRID COARSE TOLERANCE, or one V-cycle of "
                "smoothed-aggregation AMG on the assembled operator (the reference's parAlmond)",

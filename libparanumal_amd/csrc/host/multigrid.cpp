@@ -496,15 +496,10 @@ class coarseJacobiPCG_t {
     std::string graph = "TRUE";
     elliptic->settings.getSetting("MULTIGRID COARSE GRAPH", graph);
     useGraph = graph == "TRUE";
-    // the whole fused solve in one persistent launch (no host reads)
-    std::string persist = "FALSE";
-    elliptic->settings.getSetting("MULTIGRID COARSE PERSISTENT", persist);
-    persistent = fusedHex1 && persist == "TRUE";
-    elliptic->settings.getSetting("MULTIGRID COARSE PERSISTENT BLOCKS", persistentBlocks);
     // one fused reduction per iteration (Chronopoulos-Gear), two launches
     std::string cg1s = "TRUE";
     elliptic->settings.getSetting("MULTIGRID COARSE SINGLE REDUCTION", cg1s);
-    cg1 = fusedHex1 && !persistent && cg1s == "TRUE";
+    cg1 = fusedHex1 && cg1s == "TRUE";
     // single-reduction iterations on the assembled N=1 operator: one launch each
     if (cg1 && elliptic->settings.compareSetting("MULTIGRID COARSE ASSEMBLED", "TRUE") &&
         !elliptic->settings.compareSetting("MULTIGRID COARSE SOLVER", "AMG")) {
@@ -529,16 +524,12 @@ class coarseJacobiPCG_t {
   }
   bool fusedHex1 = false;
   bool useGraph = true;
-  bool persistent = false;
-  int persistentBlocks = 64;    // 0: one block per CU
-  bool iterationsOnDevice = false;  // lastIterations is still in o_s (persistent solve)
   bool cg1 = false;
   bool cg1csr = false;  // cg1 on the assembled operator (o_Crs, o_Cci, o_Cv; w in o_w, o_w1)
   deviceMemory<dlong> o_Crs, o_Cci;
   deviceMemory<double> o_Cv, o_w1;
   deviceMemory<char> o_ws2;  // two reduction workspaces (the assembled form alternates them)
   deviceMemory<double> o_s, o_z, o_p, o_Ap, o_AqL;
-  deviceMemory<unsigned> o_bar;
   // single-reduction form: w, the parity pairs of p, s, r (r[0] is o_r), u, owner flags
   deviceMemory<double> o_w, o_p2[2], o_s2[2], o_r1, o_u2[2];
   deviceMemory<unsigned char> o_owner;
@@ -554,10 +545,9 @@ class coarseJacobiPCG_t {
   int solveFusedHex1(deviceMemory<double>& o_X);
   void enqueueCG1Chunk(deviceMemory<double>& o_X, int itStart, int n);
   int solveCG1Hex1(deviceMemory<double>& o_X);
-  void solvePersistentHex1(deviceMemory<double>& o_X);
   void allocFused();
-  // iterations of the last coarse solve (reads them back after a persistent solve)
-  int iterations();
+  // iterations of the last coarse solve
+  int iterations() const { return lastIterations; }
   template <typename T>
   void solve(deviceMemory<T>& o_rhs, deviceMemory<T>& o_X);
 };
@@ -575,35 +565,6 @@ void coarseJacobiPCG_t::allocFused() {
   o_p = p.malloc<double>(N);
   o_Ap = p.malloc<double>(N);
   o_AqL = p.malloc<double>((size_t)m.Nelements * m.Np);
-  o_bar = p.malloc<unsigned>(4);
-  LIBP_HIP(hipMemsetAsync(o_bar.ptr(), 0, 4 * sizeof(unsigned), p.stream));
-}
-
-/* The fused iterations of solveFusedHex1 as one persistent kernel
- * (libp_pcg_jacobi_hex1_solve_f64): the same stopping rule, with every
- * reduction summed in a fixed order on the device; nothing is read back. */
-void coarseJacobiPCG_t::solvePersistentHex1(deviceMemory<double>& o_X) {
-  platform_t& p = elliptic->platform;
-  allocFused();
-  const mesh_t& m = elliptic->mesh;
-  LIBP_CALL(libp_pcg_jacobi_hex1_solve_f64(
-      m.Nelements, elliptic->o_GlobalToLocal.ptr(), m.o_wJ.ptr(), m.o_ggeo.ptr(), m.o_D.ptr(), elliptic->lambda,
-      elliptic->Ndofs, elliptic->ogsMasked.o_rowStartsT.ptr(), elliptic->ogsMasked.o_colIdsT.ptr(),
-      jacobi->o_invDiagA.ptr(), tol, maxit, persistentBlocks, o_X.ptr(), o_r.ptr(), o_z.ptr(), o_p.ptr(),
-      o_Ap.ptr(), o_AqL.ptr(), o_s.ptr(), o_bar.ptr(), p.o_reduceWS.ptr(), p.s()));
-  iterationsOnDevice = true;
-}
-
-int coarseJacobiPCG_t::iterations() {
-  if (iterationsOnDevice) {
-    platform_t& p = elliptic->platform;
-    unsigned err = 0;
-    LIBP_HIP(hipMemcpyAsync(&err, o_bar.ptr() + 2, sizeof(unsigned), hipMemcpyDeviceToHost, p.stream));
-    lastIterations = (int)p.readScalars(o_s.ptr() + LIBP_PCG_ITERS, 1)[0];
-    LIBP_ABORT("coarse Jacobi-PCG: a grid barrier of the persistent solve timed out", err != 0);
-    iterationsOnDevice = false;
-  }
-  return lastIterations;
 }
 
 int coarseJacobiPCG_t::solveFusedHex1(deviceMemory<double>& o_X) {
@@ -785,11 +746,6 @@ void coarseJacobiPCG_t::solve<double>(deviceMemory<double>& o_rhs, deviceMemory<
     return;
   }
   LIBP_HIP(hipMemcpyAsync(o_r.ptr(), o_rhs.ptr(), N * sizeof(double), hipMemcpyDeviceToDevice, p.stream));
-  iterationsOnDevice = false;
-  if (persistent) {
-    solvePersistentHex1(o_X);
-    return;
-  }
   if (cg1) {
     lastIterations = solveCG1Hex1(o_X);
     return;
@@ -806,12 +762,9 @@ void coarseJacobiPCG_t::solve<float>(deviceMemory<float>& o_rhs, deviceMemory<fl
   platform_t& p = elliptic->platform;
   const dlong N = elliptic->Ndofs;
   LIBP_CALL(libp_convert_f32_to_f64(N, o_rhs.ptr(), o_r.ptr(), p.s()));
-  iterationsOnDevice = false;
   if (amg) {
     amg->vcycle(o_r.ptr(), o_x.ptr());
     lastIterations = 1;
-  } else if (persistent) {
-    solvePersistentHex1(o_x);
   } else if (cg1) {
     lastIterations = solveCG1Hex1(o_x);
   } else if (fusedHex1) {

@@ -606,200 +606,6 @@ __global__ void __launch_bounds__(CSR_THREADS) coarse_cg1_csr_kernel(
   }
 }
 
-// ---------------------------------------------------------------------------
-// The whole coarse solve in one launch.
-//
-// Phases A, B and C above become phases of one persistent kernel separated by
-// grid barriers, so a coarse solve costs one launch and no host read. Each
-// block writes its partial sums; after the barrier every block sums all
-// partials in the same fixed order, so every block holds bit-identical
-// scalars and takes the same stop decision (no block leaves the loop alone).
-// The barrier is a counter plus a generation word; the wait is bounded: a
-// block that waits longer than spinLimit polls sets the error word and
-// leaves, and then so does every other block at its next barrier.
-// The grid is capped at the number of blocks the device holds at once (the
-// occupancy limit), so on an idle device every block is resident from the
-// start. hipLaunchCooperativeKernel, which would guarantee that, was refused
-// ("cannot be performed in the present state") in a process while another
-// process on the same GPU had made cooperative launches, so it is not used.
-constexpr int CP_THREADS = 512;
-
-struct CoarsePcgArgs {
-  int E, Nrows, maxit;
-  double lambda, tol2;  // tol2 = tol^2 (stop when r.r <= tol2 r0.r0); 0 runs maxit iterations
-  long spinLimit;
-  const int *G2L, *rowStarts, *colIds;
-  const double *wJ, *ggeo, *D, *invD;
-  double *x, *r, *z, *p, *Ap, *AqL, *s;
-  ReduceWS* ws;
-  unsigned* bar;  // [0] arrivals (zero at launch), [1] generation, [2] error
-};
-
-// One arrival per block: an acq_rel add on the counter (writes back this
-// block's stores, and the last block acquires everyone's). The waiters poll
-// the generation word with relaxed loads, which invalidate nothing, and take
-// one acquire fence when it moves (an acquire on every poll would invalidate
-// the L2 the working blocks are reading, once per poll).
-__device__ inline bool cp_grid_sync(unsigned* bar, unsigned& gen, const long spinLimit, int* s_ok) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int ok = 1;
-    const unsigned arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (arrived == gridDim.x - 1) {
-      __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&bar[1], gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      long spins = 0;
-      while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-        if (++spins > spinLimit) {
-          ok = 0;
-          __hip_atomic_store(&bar[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-      }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    }
-    *s_ok = ok;
-  }
-  gen += 1u;
-  __syncthreads();
-  return *s_ok != 0;
-}
-
-// block partial of NV sums into ws->part[slot + k][blockIdx.x]
-template <int NV>
-__device__ inline void cp_store_partials(ReduceWS* ws, const int slot, const double (&v)[NV], double* s_w) {
-#pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    const double b = block_sum(v[k], s_w);
-    if (threadIdx.x == 0) ws->part[slot + k][blockIdx.x] = b;
-  }
-}
-
-// sum of all blocks' partials in a fixed order, returned to every thread
-template <int NV>
-__device__ inline void cp_sum_partials(ReduceWS* ws, const int slot, double (&out)[NV], double* s_w,
-                                       double* s_out) {
-#pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    double acc = 0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
-      acc += __hip_atomic_load(&ws->part[slot + k][b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const double t = block_sum(acc, s_w);
-    if (threadIdx.x == 0) s_out[k] = t;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < NV; ++k) out[k] = s_out[k];
-  __syncthreads();
-}
-
-__global__ void __launch_bounds__(CP_THREADS) coarse_pcg_persistent_kernel(const CoarsePcgArgs a) {
-  __shared__ double s_w[CP_THREADS / 64];
-  __shared__ double s_out[2];
-  __shared__ int s_ok;
-  __shared__ unsigned s_gen;
-  if (threadIdx.x == 0) s_gen = __hip_atomic_load(&a.bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __syncthreads();
-  unsigned gen = s_gen;
-  const long T = (long)gridDim.x * CP_THREADS;
-  const long tid = (long)blockIdx.x * CP_THREADS + threadIdx.x;
-  // element lanes padded to whole wavefronts: the loop bound is uniform in a wave
-  const long lanes = ((8L * a.E + 63) / 64) * 64;
-
-  // x = 0, z = invD r, r.r and r.z of the incoming residual
-  {
-    double v[2] = {0, 0};
-    for (long g = tid; g < a.Nrows; g += T) {
-      const double rn = a.r[g], zn = 1.0 * a.invD[g] * rn;
-      a.x[g] = 0.0;
-      a.z[g] = zn;
-      v[0] += rn * rn;
-      v[1] += rn * zn;
-    }
-    cp_store_partials<2>(a.ws, 0, v, s_w);
-  }
-  if (!cp_grid_sync(a.bar, gen, a.spinLimit, &s_ok)) return;
-  double sums[2];
-  cp_sum_partials<2>(a.ws, 0, sums, s_w, s_out);
-  const double rdotr0 = sums[0];
-  double rdotr = rdotr0, rdotz = sums[1], beta = 0.0;
-  double TOL = a.tol2 * rdotr0;
-  if (!(TOL > 0.0)) TOL = 1e-300;
-  int it = 0;
-  if (rdotr0 != 0.0) {
-    for (; it < a.maxit; ++it) {
-      const bool first = it == 0;
-      // A: AqL = A_e (z + beta p)
-      for (long t = tid; t < lanes; t += T) {
-        const int e = (int)(t >> 3), n = (int)(t & 7);
-        const bool live = e < a.E;
-        Hex1Node v{};
-        if (live) v = hex1_load(e, n, a.G2L, a.wJ, a.ggeo, a.z, a.p, beta, !first);
-        const double r = hex1_apply(v, n, a.D, a.lambda);
-        if (live) a.AqL[(size_t)e * 8 + n] = r;
-      }
-      if (!cp_grid_sync(a.bar, gen, a.spinLimit, &s_ok)) return;
-      // B: Ap = gather(AqL), p = z + beta p, p.Ap
-      {
-        double v[1] = {0};
-        for (long g = tid; g < a.Nrows; g += T) {
-          const int s0 = a.rowStarts[g], s1 = a.rowStarts[g + 1];
-          double ap = 0;
-          for (int j = s0; j < s1; ++j) ap += a.AqL[a.colIds[j]];
-          const double pn = first ? a.z[g] : a.z[g] + beta * a.p[g];
-          a.p[g] = pn;
-          a.Ap[g] = ap;
-          v[0] += pn * ap;
-        }
-        cp_store_partials<1>(a.ws, 2, v, s_w);
-      }
-      if (!cp_grid_sync(a.bar, gen, a.spinLimit, &s_ok)) return;
-      double pap[1];
-      cp_sum_partials<1>(a.ws, 2, pap, s_w, s_out);
-      const double alpha = rdotz / pap[0];
-      // C: x += alpha p, r -= alpha Ap, z = invD r; r.r and r.z (rows as in B)
-      {
-        double v[2] = {0, 0};
-        for (long g = tid; g < a.Nrows; g += T) {
-          a.x[g] += alpha * a.p[g];
-          const double rn = a.r[g] - alpha * a.Ap[g];
-          a.r[g] = rn;
-          const double zn = 1.0 * a.invD[g] * rn;
-          a.z[g] = zn;
-          v[0] += rn * rn;
-          v[1] += rn * zn;
-        }
-        cp_store_partials<2>(a.ws, 0, v, s_w);
-      }
-      if (!cp_grid_sync(a.bar, gen, a.spinLimit, &s_ok)) return;
-      cp_sum_partials<2>(a.ws, 0, sums, s_w, s_out);
-      rdotr = sums[0];
-      beta = sums[1] / rdotz;
-      rdotz = sums[1];
-      if (rdotr <= TOL) {
-        ++it;
-        break;
-      }
-    }
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    a.s[0] = rdotz;
-    a.s[5] = rdotr;
-    a.s[7] = TOL;
-    a.s[8] = (rdotr <= TOL) ? 1.0 : 0.0;
-    a.s[9] = (double)it;
-  }
-}
-
-__global__ void coarse_barrier_reset_kernel(unsigned* bar) {
-  if (threadIdx.x == 0) {
-    bar[0] = 0u;
-    bar[2] = 0u;
-  }
-}
-
 }  // namespace libp_amd
 
 using namespace libp_amd;
@@ -831,67 +637,6 @@ extern "C" int libp_pcg_jacobi_hex1_check_f64(int Nrows, double* scalars, void* 
   hipLaunchKernelGGL(coarse_check_kernel, dim3(1), dim3(64), 0, as_stream(stream), coarse_row_blocks(Nrows), scalars,
                      (const ReduceWS*)ws);
   return launch_status("pcg_jacobi_hex1_check");
-}
-
-extern "C" int libp_pcg_jacobi_hex1_solve_f64(int Nelements, const int* GlobalToLocal, const double* wJ,
-                                              const double* ggeo, const double* D, double lambda, int Nrows,
-                                              const int* rowStarts, const int* colIds, const double* invDiag,
-                                              double tol, int maxit, int blocks, double* x, double* r, double* z,
-                                              double* p, double* Ap, double* AqL, double* scalars,
-                                              unsigned* barrier, void* ws, void* stream) {
-  if (Nelements <= 0 || Nrows <= 0) return LIBP_OK;
-  if (!GlobalToLocal || !wJ || !ggeo || !D || !rowStarts || !colIds || !invDiag || !x || !r || !z || !p || !Ap ||
-      !AqL || !scalars || !barrier || !ws)
-    return arg_error("pcg_jacobi_hex1_solve", "null pointer");
-  if (maxit < 0 || !(tol >= 0.0)) return arg_error("pcg_jacobi_hex1_solve", "maxit < 0 or tol < 0");
-  int dev = 0, cus = 0, perCU = 0;
-  if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, coarse_pcg_persistent_kernel, CP_THREADS, 0) !=
-          hipSuccess)
-    return launch_status("pcg_jacobi_hex1_solve (device query)");
-  const int maxBlocks = std::min(cus * perCU, RMAXB);
-  if (blocks <= 0) blocks = std::min(cus, maxBlocks);
-  if (blocks > maxBlocks) return arg_error("pcg_jacobi_hex1_solve", "more blocks than can be resident at once");
-  CoarsePcgArgs a;
-  a.E = Nelements;
-  a.Nrows = Nrows;
-  a.maxit = maxit;
-  a.lambda = lambda;
-  a.tol2 = tol * tol;
-  a.spinLimit = 1L << 22;
-  a.G2L = GlobalToLocal;
-  a.rowStarts = rowStarts;
-  a.colIds = colIds;
-  a.wJ = wJ;
-  a.ggeo = ggeo;
-  a.D = D;
-  a.invD = invDiag;
-  a.x = x;
-  a.r = r;
-  a.z = z;
-  a.p = p;
-  a.Ap = Ap;
-  a.AqL = AqL;
-  a.s = scalars;
-  a.ws = (ReduceWS*)ws;
-  a.bar = barrier;
-  hipStream_t st = as_stream(stream);
-  hipLaunchKernelGGL(coarse_barrier_reset_kernel, dim3(1), dim3(64), 0, st, barrier);
-  hipLaunchKernelGGL(coarse_pcg_persistent_kernel, dim3(blocks), dim3(CP_THREADS), 0, st, a);
-  return launch_status("pcg_jacobi_hex1_solve");
-}
-
-extern "C" int libp_pcg_jacobi_hex1_solve_blocks(int* maxBlocks) {
-  if (!maxBlocks) return arg_error("pcg_jacobi_hex1_solve_blocks", "null pointer");
-  int dev = 0, cus = 0, perCU = 0;
-  if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, coarse_pcg_persistent_kernel, CP_THREADS, 0) !=
-          hipSuccess)
-    return launch_status("pcg_jacobi_hex1_solve_blocks");
-  *maxBlocks = std::min(cus * perCU, RMAXB);
-  return LIBP_OK;
 }
 
 extern "C" int libp_pcg_jacobi_hex1_cg1_f64(int Nelements, const int* GlobalToLocal, const unsigned char* owner,

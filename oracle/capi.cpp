@@ -27,6 +27,7 @@ struct Handle {
   Mesh mesh;
   Elliptic el;
   std::unique_ptr<MultiGrid> mg;
+  std::unique_ptr<AmgO> amg;  // parAlmond AMG on the handle's own assembled operator
   std::vector<double> invDiag;
   std::vector<double> scratch;
   std::string err;
@@ -276,6 +277,117 @@ double orc_max_eig(void* p, long seed) {
   Drand48 r;
   r.seed(seed);
   return maxEigSmoothAx(h->el, d, r);
+}
+
+/* ---- parAlmond AMG ---- */
+namespace {
+// which = 0: the AMG on the handle's own operator (orc_amg_setup); 1: the p-MG's base level
+const AmgO* amgOf(Handle* h, int which) {
+  if (which == 0) return h->amg.get();
+  return h->mg ? h->mg->amg.get() : nullptr;
+}
+}  // namespace
+
+// Assembled operator (BuildOperatorMatrixContinuousHex3D) of the handle: returns nnz;
+// fills rowStarts [Ngather+1], cols, vals when non-null.
+long orc_assembled_csr(void* p, int* rs, int* ci, double* v) {
+  Handle* h = (Handle*)p;
+  try {
+    CsrO A = BuildOperatorMatrixContinuousHex3D(h->el);
+    if (rs) std::copy(A.rs.begin(), A.rs.end(), rs);
+    if (ci) std::copy(A.ci.begin(), A.ci.end(), ci);
+    if (v) std::copy(A.v.begin(), A.v.end(), v);
+    return A.nnz();
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+// AMG setup on the handle's own assembled operator, drand48 seeded with `seed`.
+// Returns the number of levels (-1 on error).
+int orc_amg_setup(void* p, long seed) {
+  Handle* h = (Handle*)p;
+  try {
+    Drand48 d;
+    d.seed(seed);
+    h->amg.reset(new AmgO());
+    h->amg->Setup(BuildOperatorMatrixContinuousHex3D(h->el), h->s, d, h->el.allNeumann != 0,
+                  h->el.allNeumannPenalty);
+    return (int)h->amg->levels.size();
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+int orc_amg_levels(void* p, int which) {
+  const AmgO* a = amgOf((Handle*)p, which);
+  return a ? (int)a->levels.size() : -1;
+}
+
+double orc_amg_level_d(void* p, int which, int l, const char* name) {
+  const AmgO* a = amgOf((Handle*)p, which);
+  if (!a || l < 0 || l >= (int)a->levels.size()) return NAN;
+  const AmgLevelO& L = a->levels[l];
+  std::string n(name);
+  if (n == "rho") return L.rho;
+  if (n == "lambda") return L.lambda;
+  if (n == "lambda0") return L.lambda0;
+  if (n == "lambda1") return L.lambda1;
+  return NAN;
+}
+
+// Named array of AMG level l; dtype 0 float64, 1 int32, 2 int64. Returns the length.
+long orc_amg_level_array(void* p, int which, int l, const char* name, void* out, int* dtype) {
+  const AmgO* a = amgOf((Handle*)p, which);
+  if (!a || l < 0 || l >= (int)a->levels.size()) return -1;
+  const AmgLevelO& L = a->levels[l];
+  std::string n(name);
+  *dtype = 0;
+  if (n == "A_v") return copy_out(L.A.v, out);
+  if (n == "P_v") return copy_out(L.P.v, out);
+  if (n == "R_v") return copy_out(L.R.v, out);
+  if (n == "diagInv") return copy_out(L.diagInv, out);
+  if (n == "coarseInv") return copy_out(a->coarseInv, out);
+  if (n == "weights") return copy_out(L.weights, out);
+  if (n == "null") return copy_out(a->null, out);
+  *dtype = 1;
+  if (n == "states") return copy_out(L.states, out);
+  if (n == "A_rs") return copy_out(L.A.rs, out);
+  if (n == "A_ci") return copy_out(L.A.ci, out);
+  if (n == "P_rs") return copy_out(L.P.rs, out);
+  if (n == "P_ci") return copy_out(L.P.ci, out);
+  if (n == "R_rs") return copy_out(L.R.rs, out);
+  if (n == "R_ci") return copy_out(L.R.ci, out);
+  if (n == "C_rs") return copy_out(L.Crs, out);
+  if (n == "C_ci") return copy_out(L.Cci, out);
+  *dtype = 2;
+  if (n == "FineToCoarse") return copy_out(L.FineToCoarse, out);
+  *dtype = -1;
+  return -1;
+}
+
+// x = V(b), one AMG V-cycle from level 0 (x need not be initialised)
+int orc_amg_vcycle(void* p, int which, const double* b, double* x) {
+  const AmgO* a = amgOf((Handle*)p, which);
+  if (!a) {
+    g_err = "no AMG";
+    return -1;
+  }
+  a->vcycle(0, b, x);
+  return 0;
+}
+
+// one smoothing of AMG level l (x updated in place; flag = x is zero)
+int orc_amg_smooth(void* p, int which, int l, const double* b, double* x, int flag) {
+  const AmgO* a = amgOf((Handle*)p, which);
+  if (!a || l < 0 || l + 1 >= (int)a->levels.size()) {
+    g_err = "no such AMG level";
+    return -1;
+  }
+  a->smooth(l, b, x, flag != 0);
+  return 0;
 }
 
 double orc_mg_level_d(void* p, int l, const char* name) {

@@ -203,7 +203,40 @@ struct PCGResult {
 PCGResult pcg_solve(dlong N, OpFn& A, OpFn& M, double* x, double* r, double tol, int maxit,
                     bool flexible, bool stopRhsNorm, bool pureRelative = false);
 
+/* parAlmond AMG (amg.cpp), one rank */
+struct CsrO {
+  dlong n = 0, m = 0;
+  std::vector<dlong> rs, ci;  // rows sorted, columns ascending in each row
+  std::vector<double> v;
+  dlong nnz() const { return rs.empty() ? 0 : rs[n]; }
+  void spmv(double alpha, const double* x, double beta, double* y) const;  // y = alpha A x + beta y
+};
+struct AmgLevelO {
+  CsrO A, P, R;                      // P, R empty on the coarsest level
+  std::vector<double> diagA, diagInv;
+  double rho = 0;                    // parCSR::rhoDinvA
+  std::vector<dlong> Crs, Cci;       // strong graph
+  std::vector<hlong> FineToCoarse;   // aggregate of each row
+  std::vector<double> weights;       // MIS-2 weights: drand48 draw + strong column count
+  std::vector<int> states;           // MIS-2 result: 1 root, -1 removed
+  int stype = 1;                     // 1 Chebyshev, 0 damped Jacobi
+  int ChebyshevIterations = 2;
+  double lambda = 0, lambda0 = 0, lambda1 = 0;
+};
+struct AmgO {
+  std::vector<AmgLevelO> levels;
+  std::vector<double> null;          // null vector of the coarsest level
+  std::vector<double> coarseInv;     // dense inverse of the coarsest operator, row-major
+  void Setup(const CsrO& A, const Settings& s, Drand48& drng, bool nullSpace, double nullSpacePenalty);
+  void smooth(int k, const double* b, double* x, bool xIsZero) const;
+  void vcycle(int k, const double* b, double* x) const;
+};
+// ellipticBuildOperatorMatrixContinuous.cpp:697-880 (hex, one rank)
+CsrO BuildOperatorMatrixContinuousHex3D(const Elliptic& el);
+std::vector<double> denseInverse(int n, std::vector<double> M);
+
 struct MultiGrid {
+  std::unique_ptr<AmgO> amg;   // base level: parAlmond AMG when MULTIGRID COARSE SOLVER is AMG
   std::vector<std::unique_ptr<Elliptic>> levels;   // levels[0] = fine (shallow copy of the fine elliptic)
   std::vector<MGLevelO> mg;
   std::vector<std::vector<double>> invDiagCoarse;

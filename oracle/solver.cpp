@@ -409,6 +409,13 @@ void MultiGrid::Setup(Elliptic& fine, GlibcRand& rng, Drand48& drng, const std::
     levels.emplace_back(elC);
     mg.back().elC = elC;
   }
+  // base level: the reference's parAlmond AMG on the assembled N=1 operator
+  // (ellipticPreconMultiGrid.cpp:119-150), continuing the drand48 stream
+  if (s.has("MULTIGRID COARSE SOLVER") && s.compare("MULTIGRID COARSE SOLVER", "AMG")) {
+    const Elliptic& C = *levels.back();
+    amg.reset(new AmgO());
+    amg->Setup(BuildOperatorMatrixContinuousHex3D(C), s, drng, C.allNeumann != 0, C.allNeumannPenalty);
+  }
   // coarse Jacobi-PCG data
   std::vector<double> diagC;
   levels.back()->BuildOperatorDiagonal(diagC);
@@ -554,6 +561,10 @@ void prolongate(const MGLevelO& L, const double* Xc, double* Px) {
  * pure relative tolerance coarseTol. */
 void MultiGrid::vcycle(int k, std::vector<double>& rhs, std::vector<double>& x) {
   const int base = (int)levels.size() - 1;
+  if (k == base && amg) {
+    amg->vcycle(0, rhs.data(), x.data());
+    return;
+  }
   if (k == base) {
     Elliptic& C = *levels[base];
     EllOp A(&C);

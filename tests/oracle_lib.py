@@ -51,6 +51,13 @@ def load():
             "orc_mg_level_ndofs": (c_long, [c_void_p, c_int]),
             "orc_mg_level_op": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int]),
             "orc_max_eig": (c_double, [c_void_p, c_long]),
+            "orc_assembled_csr": (c_long, [c_void_p, c_void_p, c_void_p, c_void_p]),
+            "orc_amg_setup": (c_int, [c_void_p, c_long]),
+            "orc_amg_levels": (c_int, [c_void_p, c_int]),
+            "orc_amg_level_d": (c_double, [c_void_p, c_int, c_int, c_char_p]),
+            "orc_amg_level_array": (c_long, [c_void_p, c_int, c_int, c_char_p, c_void_p, ctypes.POINTER(c_int)]),
+            "orc_amg_vcycle": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+            "orc_amg_smooth": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int]),
             "orc_run": (c_double, [c_char_p, ctypes.POINTER(c_int), ctypes.POINTER(c_long)]),
             "orc_gll": (None, [c_int, c_void_p, c_void_p]),
             "orc_dmatrix": (None, [c_int, c_void_p]),
@@ -191,6 +198,64 @@ class OracleElliptic:
 
     def max_eig(self, seed=0):
         return float(self.L.orc_max_eig(self.h, seed))
+
+    # parAlmond AMG: which = 0 the AMG on this handle's operator, 1 the p-MG's base level
+    def assembled_csr(self):
+        nnz = self.L.orc_assembled_csr(self.h, None, None, None)
+        if nnz < 0:
+            raise RuntimeError(self.L.orc_last_error().decode())
+        rs = np.empty(self.geti("Ngather") + 1, dtype=np.int32)
+        ci = np.empty(nnz, dtype=np.int32)
+        v = np.empty(nnz)
+        self.L.orc_assembled_csr(self.h, rs.ctypes.data, ci.ctypes.data, v.ctypes.data)
+        return rs, ci, v
+
+    def amg_setup(self, seed=0):
+        n = self.L.orc_amg_setup(self.h, seed)
+        if n < 0:
+            raise RuntimeError(self.L.orc_last_error().decode())
+        return n
+
+    def amg_levels(self, which=0):
+        return int(self.L.orc_amg_levels(self.h, which))
+
+    def amg_level_d(self, l, name, which=0):
+        return float(self.L.orc_amg_level_d(self.h, which, l, name.encode()))
+
+    def amg_level_array(self, l, name, which=0):
+        dt = c_int()
+        n = self.L.orc_amg_level_array(self.h, which, l, name.encode(), None, ctypes.byref(dt))
+        if n < 0:
+            raise KeyError(name)
+        out = np.empty(n, dtype=_DT[dt.value])
+        self.L.orc_amg_level_array(self.h, which, l, name.encode(), out.ctypes.data, ctypes.byref(dt))
+        return out
+
+    def amg_matrix(self, l, which="A", amg=0):
+        """scipy CSR of level l's A, P or R."""
+        import scipy.sparse as sp
+        rs = self.amg_level_array(l, which + "_rs", amg)
+        ci = self.amg_level_array(l, which + "_ci", amg)
+        v = self.amg_level_array(l, which + "_v", amg)
+        n = rs.size - 1
+        m = (ci.max() + 1) if ci.size else 0
+        if which == "A":
+            m = n
+        return sp.csr_matrix((v, ci, rs), shape=(n, m))
+
+    def amg_vcycle(self, b, which=0):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros_like(b)
+        if self.L.orc_amg_vcycle(self.h, which, b.ctypes.data, x.ctypes.data) != 0:
+            raise RuntimeError(self.L.orc_last_error().decode())
+        return x
+
+    def amg_smooth(self, l, b, x, x_is_zero, which=0):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.ascontiguousarray(x, dtype=np.float64).copy()
+        if self.L.orc_amg_smooth(self.h, which, l, b.ctypes.data, x.ctypes.data, int(x_is_zero)) != 0:
+            raise RuntimeError(self.L.orc_last_error().decode())
+        return x
 
 
 def learned_dims(Np, Nfp, H):

@@ -93,10 +93,42 @@ def ax_rate(lp, settings, steps=20, warmup=5):
             "elements": E, "N": N, "gathered_dofs": Ng}
 
 
+def cpu_info():
+    """CPU model, sockets and cores of this host (lscpu, else /proc/cpuinfo)."""
+    import subprocess
+    info = {"model": "unknown", "sockets": None, "cores_per_socket": None, "threads_per_core": None}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                v = v.strip()
+                info[keys[k.strip()]] = int(v) if v.isdigit() else v
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if info["model"] == "unknown":
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        info["model"] = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    info["visible_cpus"] = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return info
+
+
 def cpu_baseline(reps=20):
-    """Oracle partial Ax on a 16^3 x N=7 sample of the workload: OpenMP over
-    elements on every thread OpenMP is given (the reference's OpenMP mode runs
-    its @outer element loop that way), and serial on 1 thread (its Serial mode)."""
+    """The oracle (the reference's Serial-mode path restated in C++) on a 16^3 x N=7
+    sample of the workload: the partial Ax (the metric), the gather, the whole
+    operator and a Jacobi-PCG solve to the reference's stopping rule. OpenMP over
+    elements and gather rows on every thread OpenMP is given (the reference's
+    OpenMP mode runs its @outer loops that way), and the Ax serially on 1 thread
+    (its Serial mode)."""
+    import ctypes
     from tests import oracle_lib
     s = workload_settings(nx=16)
     t0 = time.time()
@@ -107,19 +139,21 @@ def cpu_baseline(reps=20):
     t_omp = L.orc_time_partial_ax(o.h, reps, 1)
     threads = int(L.orc_omp_threads())
     ng = o.geti("Ngather")
-    cpu = "unknown"
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    cpu = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
+    nl = o.geti("Nelements") * o.geti("Np")
+    it = ctypes.c_int(0)
+    t_g = L.orc_time_leg(o.h, 0, reps, 1, ctypes.byref(it))
+    t_op = L.orc_time_leg(o.h, 1, max(3, reps // 4), 1, ctypes.byref(it))
+    t_solve = L.orc_time_leg(o.h, 2, 1, 1, ctypes.byref(it))
+    cpu = cpu_info()
     return {"value": ng / t_omp / 1e9, "unit": "GDOF/s", "cores": threads, "kind": "port",
             "sample": f"oracle partial Ax, 16^3 elements N=7 ({ng} gathered dofs), median of {reps} applications, "
-                      f"OpenMP over elements on {threads} threads of {cpu}; setup {setup:.1f}s",
-            "serial": {"value": ng / t / 1e9, "unit": "GDOF/s", "cores": 1}}
+                      f"OpenMP over elements on {threads} threads of {cpu['model']}; setup {setup:.1f}s",
+            "cpu": cpu,
+            "serial": {"value": ng / t / 1e9, "unit": "GDOF/s", "cores": 1},
+            "gather": {"ms": t_g * 1e3, "GBs": (12 * nl + 12 * ng) / t_g / 1e9, "threads": threads},
+            "operator": {"ms": t_op * 1e3, "gdofs": ng / t_op / 1e9, "threads": threads},
+            "solve": {"seconds": t_solve, "iterations": it.value, "threads": threads,
+                      "solver": "PCG+Jacobi, stop |r|<=max(1e-8|r0|,1e-8), 16^3 N=7 sample"}}
 
 
 LEARNED_WEIGHTS_LINEAR = os.path.join(ROOT, "libparanumal_amd", "weights", "learned_hex_n7_linear.bin")
@@ -284,6 +318,11 @@ def main():
     ms_gather = timed(lambda: lp.check(L.libp_ogs_gather_f64(Ng, rs, ci, AqL.data_ptr(), Aq.data_ptr(), el.stream)),
                       max(10, args.steps // 2), 2)
 
+    def two_pass():
+        el.partial_ax(q, AqL)
+        lp.check(L.libp_ogs_gather_f64(Ng, rs, ci, AqL.data_ptr(), Aq.data_ptr(), el.stream))
+    ms_op2 = timed(two_pass, max(10, args.steps // 2), 2)
+
     bytes_ax = 68 * NL + 8 * Ng          # algorithmic bytes per Ax launch
     flops_ax = NL * (12 * (7 + 1) + 18)  # algorithmic flops per Ax launch
     bytes_gather = 12 * NL + 12 * Ng
@@ -306,32 +345,62 @@ def main():
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs / HBM_PEAK_GBS, "traffic": measured_traffic(args.nx)},
         "ax_gflops": flops_ax / (ms_ax * 1e-3) / 1e9,
-        "operator": {"ms": ms_op, "gdofs": Ng / (ms_op * 1e-3) / 1e9},
+        "operator": {"ms": ms_op, "gdofs": Ng / (ms_op * 1e-3) / 1e9,
+                     "form": "split" if el.geti("splitOperator") == 1 else "two-pass",
+                     "frac_of_two_pass_bytes": (80 * NL + 20 * Ng) / (ms_op * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "operator_two_pass": {"ms": ms_op2, "gdofs": Ng / (ms_op2 * 1e-3) / 1e9,
+                              "frac_of_two_pass_bytes": (80 * NL + 20 * Ng) / (ms_op2 * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "gather": {"ms": ms_gather, "GBs": bytes_gather / (ms_gather * 1e-3) / 1e9},
         "setup_seconds": setup_s,
     }
-    # achievable HBM: z = a x + b y over three 1 GiB FP64 arrays with the library's own streaming kernel
-    n = (1 << 30) // 8
+    # achievable HBM: 16-byte non-temporal read-only sweep and copy (libp_hbm_probe_f64),
+    # and the library's own streaming BLAS-1 pattern z = a x + b y for comparison
+    n = (1 << 31) // 8
     xs = torch.rand(n, dtype=torch.float64, device="cuda")
-    ys = torch.rand(n, dtype=torch.float64, device="cuda")
-    zs = torch.empty_like(xs)
-    ms_tri = timed(lambda: lp.check(L.libp_zaxpy_f64(n, 1.5, xs.data_ptr(), 0.5, ys.data_ptr(), zs.data_ptr(),
-                                                    el.stream)), 20, 3)
-    result["hbm_stream_probe"] = {"kernel": "libp_zaxpy_f64, 3 x 1 GiB (2 reads + 1 write)", "ms": ms_tri,
-                                  "GBs": 3 * n * 8 / (ms_tri * 1e-3) / 1e9}
-    result["roofline"]["frac_of_measured_stream"] = gbs / result["hbm_stream_probe"]["GBs"]
-    del xs, ys, zs
+    ys = torch.empty_like(xs)
+    ms_read = timed(lambda: lp.check(L.libp_hbm_probe_f64(0, n, xs.data_ptr(), ys.data_ptr(), el.stream)), 20, 3)
+    ms_copy = timed(lambda: lp.check(L.libp_hbm_probe_f64(1, n // 2, xs.data_ptr(), ys.data_ptr(), el.stream)), 20, 3)
+    ms_tri = timed(lambda: lp.check(L.libp_zaxpy_f64(n // 3, 1.5, xs.data_ptr(), 0.5, xs.data_ptr() + 8 * (n // 3),
+                                                    ys.data_ptr(), el.stream)), 20, 3)
+    result["hbm_stream_probe"] = {
+        "read": {"kernel": "libp_hbm_probe_f64 read, 2 GiB", "ms": ms_read, "GBs": 8 * n / (ms_read * 1e-3) / 1e9},
+        "copy": {"kernel": "libp_hbm_probe_f64 copy, 1 GiB -> 1 GiB", "ms": ms_copy,
+                 "GBs": 8 * n / (ms_copy * 1e-3) / 1e9},
+        "zaxpy": {"kernel": "libp_zaxpy_f64, 3 x 0.67 GiB (2 reads + 1 write)", "ms": ms_tri,
+                  "GBs": 3 * 8 * (n // 3) / (ms_tri * 1e-3) / 1e9}}
+    best = max(v["GBs"] for v in result["hbm_stream_probe"].values())
+    result["roofline"]["measured_stream_GBs"] = best
+    result["roofline"]["frac_of_measured_stream"] = gbs / best
+    del xs, ys
     if args.copy_probe:
         result["hbm_copy_probe"] = copy_probe(torch)
     if not args.no_solve:
         r = el.run()
-        result["solve"] = {"solver": "PCG+Jacobi, stop |r|<=max(1e-8|r0|,1e-8)", "iterations": r["iterations"],
-                           "seconds": r["seconds"], "r0": r["r0"], "r_final": r["r_final"],
-                           "error_l2_vs_exact": r["error_l2"], "solution_norm": r["norm"]}
+        result["solve"] = {"solver": "PCG+Jacobi, stop |r|<=max(1e-8|r0|,1e-8) (the reference's rule)",
+                           "iterations": r["iterations"], "seconds": r["seconds"],
+                           "ms_per_iteration": 1e3 * r["seconds"] / max(r["iterations"], 1), "r0": r["r0"],
+                           "r_final": r["r_final"], "error_max_vs_exact": r["error_l2"], "solution_norm": r["norm"]}
     el.close()
+    if not args.no_solve:
+        el = lp.Elliptic(dict(workload_settings(nx=args.nx), **{"LINEAR SOLVER STOPPING CRITERION": "RELATIVE"}))
+        r = el.run()
+        result["solve_relative"] = {"solver": "PCG+Jacobi, stop |r|<=1e-8|r0| (pure relative)",
+                                    "iterations": r["iterations"], "seconds": r["seconds"],
+                                    "ms_per_iteration": 1e3 * r["seconds"] / max(r["iterations"], 1),
+                                    "r0": r["r0"], "r_final": r["r_final"], "error_max_vs_exact": r["error_l2"],
+                                    "solution_norm": r["norm"]}
+        el.close()
     if not args.no_extra:
         # configuration 4: 16^3 elements, N=15, curved, lambda=1
         result["ax_n15"] = ax_rate(lp, N15_SETTINGS)
+        el15 = lp.Elliptic(N15_SETTINGS)
+        r = el15.run()
+        result["solve_n15"] = {"workload": "configuration 4: curved 16^3 box, N=15, lambda=1",
+                               "solver": "PCG+Jacobi, stop |r|<=max(1e-8|r0|,1e-8)", "iterations": r["iterations"],
+                               "seconds": r["seconds"], "r0": r["r0"], "r_final": r["r_final"],
+                               "error_max_vs_exact": r["error_l2"], "solution_norm": r["norm"],
+                               "gathered_dofs": el15.geti("Ngather")}
+        el15.close()
         # configuration 3: FPCG + p-MG 7->3->1 in FP32 and FP64, same problem
         import numpy as np
         info32, x32 = pmg_solve(lp, pmg_settings(precision="FP32"))

@@ -45,6 +45,36 @@ const char* libp_last_error(void);
 int libp_ax_hex_partial_f64(int N, int Nelements, const int* elementList, const int* GlobalToLocal,
                             const double* wJ, const double* ggeo, const double* D, double lambda,
                             const double* q, double* AqL, void* stream);
+/* The whole C0 operator A q = Q^T A_L Q q (ellipticOperator.cpp:29-107, i.e.
+ * ellipticAxHex3D.okl:156-295 followed by the ogs gather, ogsKernels.okl:86-122)
+ * in a split form, N <= 14. libp_ax_hex_split_f64 stores the rows of
+ * element-interior nodes (one copy each) straight into the gathered Aq and the
+ * element-boundary nodes into AqFace, e * Nface + the node's place among the
+ * element's boundary nodes in local order (Nface = libp_ax_hex_face_nodes(N));
+ * libp_ogs_gather_face_f64 then sums the remaining rows, whose copies are listed
+ * as AqFace indices in ascending local order (faceRowStarts over the NfaceRows
+ * rows faceRows). The result is bit-identical to the partial Ax and the full
+ * gather. papPartials, when not null, receives libp_ax_hex_split_blocks(N,
+ * Nelements) partial sums of q.Aq over the interior rows for
+ * libp_pcg_gather_face_pap_f64, which gathers the boundary rows, adds their
+ * p.Ap partials after those and sets scalars[PAP] and scalars[ALPHA] as
+ * libp_pcg_gather_pap_f64 does; partials holds nAxPartials +
+ * libp_ogs_gather_face_blocks(NfaceRows) doubles. */
+/* HBM bandwidth probe (no reference counterpart; measurement only). mode 0
+ * reads x[0:n] once with 16-byte non-temporal loads and writes 2048 block sums to
+ * y; mode 1 copies y[0:n] = x[0:n] the same way. n even, x and y 16-byte aligned. */
+int libp_hbm_probe_f64(int mode, long n, const double* x, double* y, void* stream);
+int libp_ax_hex_face_nodes(int N);
+int libp_ax_hex_split_blocks(int N, int Nelements);
+int libp_ax_hex_split_f64(int N, int Nelements, const int* elementList, const int* GlobalToLocal, const double* wJ,
+                          const double* ggeo, const double* D, double lambda, const double* q, double* Aq,
+                          double* AqFace, double* papPartials, void* stream);
+int libp_ogs_gather_face_blocks(int NfaceRows);
+int libp_ogs_gather_face_f64(int NfaceRows, const int* faceRows, const int* faceRowStarts, const int* faceCols,
+                             const double* AqFace, double* Aq, void* stream);
+int libp_pcg_gather_face_pap_f64(int NfaceRows, const int* faceRows, const int* faceRowStarts,
+                                 const int* faceCols, const double* AqFace, double* Ap, const double* p,
+                                 int nAxPartials, double* partials, double* scalars, void* stream);
 /* FP32 variant (no reference counterpart; the reference fixes pfloat at build time,
  * include/types.h:31-45) used by the p-multigrid smoother. */
 int libp_ax_hex_partial_f32(int N, int Nelements, const int* elementList, const int* GlobalToLocal,
@@ -505,6 +535,17 @@ int libp_elliptic_csr(void* h, int* rowStarts, int* cols, double* vals);
 int libp_elliptic_amg_setup(void* h);
 int libp_elliptic_amg_vcycle(void* h, const double* b, double* x);
 int libp_elliptic_amg_info(void* h, double* out, int nout);
+/* Host copy of a named array of AMG level `level`, for tests that compare the setup
+ * with the CPU restatement (parAlmondAMGSetup.cpp:35-144). which: 0 the AMG of
+ * libp_elliptic_amg_setup, 1 the base level of the handle's p-MG ([MULTIGRID COARSE
+ * SOLVER] AMG). Names: A_rs, A_ci, P_rs, P_ci, R_rs, R_ci, aggregate (int32);
+ * A_v, P_v, R_v, invDiag, null, rho = {rho, lambda0, lambda1, lambda} (float64).
+ * out may be null to query *length; *dtype is 0 float64 or 1 int32. */
+int libp_elliptic_amg_level_array(void* h, int which, int level, const char* name, void* out, long* length,
+                                  int* dtype);
+/* One smoothing of level `level` of the libp_elliptic_amg_setup AMG
+ * (amgLevel::smooth, parAlmondAMGLevel.cpp:63-72): device b and x (updated in place). */
+int libp_elliptic_amg_smooth(void* h, int level, const double* b, double* x, int x_is_zero);
 /* p-MG level access, for tests that compare each level operation with the CPU
  * restatement. ndofs returns level `level`'s gathered length (or -status) and the
  * level precision in *bits (32 or 64). op: 0 out = A in (MGLevel::Operator,

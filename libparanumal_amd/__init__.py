@@ -30,6 +30,13 @@ SIGNATURES = {
                                       c_void_p]),
     "libp_ax_hex_partial_mfma_n15": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double,
                                              c_void_p, c_void_p, c_void_p]),
+    "libp_hbm_probe_f64": (c_int, [c_int, c_long, c_void_p, c_void_p, c_void_p]),
+    "libp_ax_hex_face_nodes": (c_int, [c_int]),
+    "libp_ax_hex_split_blocks": (c_int, [c_int, c_int]),
+    "libp_ax_hex_split_f64": (c_int, [c_int, c_int] + [c_void_p] * 5 + [c_double] + [c_void_p] * 5),
+    "libp_ogs_gather_face_blocks": (c_int, [c_int]),
+    "libp_ogs_gather_face_f64": (c_int, [c_int] + [c_void_p] * 6),
+    "libp_pcg_gather_face_pap_f64": (c_int, [c_int] + [c_void_p] * 6 + [c_int] + [c_void_p] * 3),
     "libp_ogs_gather_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "libp_ogs_gather_f32": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "libp_ogs_scatter_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
@@ -164,6 +171,9 @@ SIGNATURES = {
     "libp_elliptic_amg_setup": (c_int, [c_void_p]),
     "libp_elliptic_amg_vcycle": (c_int, [c_void_p, c_void_p, c_void_p]),
     "libp_elliptic_amg_info": (c_int, [c_void_p, c_void_p, c_int]),
+    "libp_elliptic_amg_level_array": (c_int, [c_void_p, c_int, c_int, c_char_p, c_void_p, ctypes.POINTER(c_long),
+                                              ctypes.POINTER(c_int)]),
+    "libp_elliptic_amg_smooth": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int]),
     "libp_elliptic_mg_level_ndofs": (c_long, [c_void_p, c_int, ctypes.POINTER(c_int)]),
     "libp_elliptic_mg_level_op": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int]),
     "libp_elliptic_mg_level_lambda": (c_int, [c_void_p, c_int, ctypes.POINTER(c_double)]),
@@ -326,6 +336,22 @@ class Elliptic:
         L = int(out[0])
         return {"levels": L, "rows": [int(out[1 + 2 * l]) for l in range(L)],
                 "nnz": [int(out[2 + 2 * l]) for l in range(L)]}
+
+    def amg_level_array(self, level, name, which=0):
+        """Host copy (numpy) of a named array of AMG level `level` (which: 0 the
+        amg_setup AMG, 1 the p-MG base level)."""
+        import numpy as np
+        n, dt = c_long(), c_int()
+        check(lib().libp_elliptic_amg_level_array(self._h, which, level, name.encode(), None, ctypes.byref(n),
+                                                  ctypes.byref(dt)), "amg_level_array")
+        out = np.empty(n.value, dtype=np.float64 if dt.value == 0 else np.int32)
+        check(lib().libp_elliptic_amg_level_array(self._h, which, level, name.encode(), out.ctypes.data,
+                                                  ctypes.byref(n), ctypes.byref(dt)), "amg_level_array")
+        return out
+
+    def amg_smooth(self, level, b, x, x_is_zero):
+        check(lib().libp_elliptic_amg_smooth(self._h, level, b.data_ptr(), x.data_ptr(), int(x_is_zero)),
+              "amg_smooth")
 
     def learned_linear_gram(self, r, eh):
         """Gram matrices (G = X^T X, B = T^T X; numpy FP64) of one least-squares sample of the

@@ -100,27 +100,43 @@ std::vector<double> diagonal(const csrHost_t& A) {
   return d;
 }
 
-// spectral radius of D^{-1} A by power iteration (A SPD: D^{-1} A is similar to
-// D^{-1/2} A D^{-1/2}); the start vector comes from the erand48 stream
+// parCSR::rhoDinvA (parAlmondparCSR.cpp:364-460): a 10-step Arnoldi on D^{-1} A
+// from an erand48 start (the reference's drand48 stream); rho is the largest
+// |eigenvalue| of the Hessenberg matrix
 double rhoDinvA(const csrHost_t& A, const std::vector<double>& invD, unsigned short xs[3]) {
-  std::vector<double> x(A.n), y(A.n);
-  for (auto& v : x) v = erand48(xs);
-  double rho = 0;
-  for (int it = 0; it < 40; ++it) {
-    double nx = 0;
-    for (double v : x) nx += v * v;
-    nx = std::sqrt(nx);
-    if (!(nx > 0)) break;
-    for (dlong i = 0; i < A.n; ++i) {
-      double s = 0;
-      for (dlong k = A.rs[i]; k < A.rs[i + 1]; ++k) s += A.v[k] * x[A.ci[k]];
-      y[i] = invD[i] * s / nx;
+  const dlong N = A.n;
+  const int k = (int)std::min<dlong>(10, N);
+  if (k == 0) return 0.0;
+  std::vector<double> H((size_t)k * k, 0.0);
+  std::vector<std::vector<double>> V(k + 1, std::vector<double>(N));
+  for (auto& v : V[0]) v = erand48(xs);
+  double nv = 0.0;
+  for (double v : V[0]) nv += v * v;
+  nv = std::sqrt(nv);
+  for (auto& v : V[0]) v *= 1.0 / nv;
+  for (int j = 0; j < k; ++j) {
+    std::vector<double>& w = V[j + 1];
+    for (dlong i = 0; i < N; ++i) {
+      double s = 0.0;
+      for (dlong q = A.rs[i]; q < A.rs[i + 1]; ++q) s += A.v[q] * V[j][A.ci[q]];
+      w[i] = s * invD[i];
     }
-    double ny = 0;
-    for (double v : y) ny += v * v;
-    rho = std::sqrt(ny);
-    x.swap(y);
+    for (int i = 0; i <= j; ++i) {  // modified Gram-Schmidt
+      double hij = 0.0;
+      for (dlong n = 0; n < N; ++n) hij += V[i][n] * w[n];
+      for (dlong n = 0; n < N; ++n) w[n] -= hij * V[i][n];
+      H[i + (size_t)j * k] = hij;
+    }
+    if (j + 1 < k) {
+      double nj = 0.0;
+      for (double v : w) nj += v * v;
+      nj = std::sqrt(nj);
+      H[j + 1 + (size_t)j * k] = nj;
+      for (auto& v : w) v *= 1.0 / nj;
+    }
   }
+  double rho = 0.0;
+  for (auto& l : hessenbergEigenvalues(k, H)) rho = std::max(rho, std::abs(l));
   return rho;
 }
 
@@ -232,17 +248,25 @@ std::vector<dlong> formAggregates(const csrHost_t& C, unsigned short xs[3], dlon
   return agg;
 }
 
-// symmetric strength (parAlmondStrongGraph.cpp:147-230): |a_ij| > theta sqrt(|a_ii a_jj|), diagonal kept
-csrHost_t strongGraph(const csrHost_t& A, const std::vector<double>& d, double theta) {
+// strength of connection (parAlmondStrongGraph.cpp:39-222), diagonal kept:
+// symmetric |a_ij| > theta sqrt(|a_ii a_jj|), or Ruge-Stuben -s_i a_ij > theta max_k(-s_i a_ik)
+csrHost_t strongGraph(const csrHost_t& A, const std::vector<double>& d, double theta, bool rugeStuben) {
   csrHost_t C;
   C.n = A.n;
   C.m = A.m;
   C.rs.assign(A.n + 1, 0);
   for (dlong i = 0; i < A.n; ++i) {
     const double aii = std::fabs(d[i]);
+    const double sign = (d[i] >= 0) ? 1.0 : -1.0;
+    double maxOD = 0.0;
+    if (rugeStuben)
+      for (dlong k = A.rs[i]; k < A.rs[i + 1]; ++k)
+        if (A.ci[k] != i) maxOD = std::max(maxOD, -sign * A.v[k]);
     for (dlong k = A.rs[i]; k < A.rs[i + 1]; ++k) {
       const dlong j = A.ci[k];
-      if (j == i || std::fabs(A.v[k]) > theta * std::sqrt(aii * std::fabs(d[j]))) {
+      const bool strong = rugeStuben ? (-sign * A.v[k] > theta * maxOD)
+                                     : (std::fabs(A.v[k]) > theta * std::sqrt(aii * std::fabs(d[j])));
+      if (j == i || strong) {
         C.ci.push_back(j);
         C.v.push_back(1.0);
       }
@@ -250,6 +274,12 @@ csrHost_t strongGraph(const csrHost_t& A, const std::vector<double>& d, double t
     C.rs[i + 1] = (dlong)C.ci.size();
   }
   return C;
+}
+
+std::vector<double> inverseDiagonal(const std::vector<double>& d) {
+  std::vector<double> inv(d.size());
+  for (size_t i = 0; i < d.size(); ++i) inv[i] = (d[i] != 0.0) ? 1.0 / d[i] : 0.0;
+  return inv;
 }
 
 int lanesFor(const csrHost_t& A) {
@@ -288,41 +318,63 @@ csrHost_t ellipticAssembleCSR(elliptic_t& el) {
       if (r < 0) continue;
       for (int b = 0; b < Np; ++b) {
         const dlong c = g2l[(size_t)e * Np + b];
-        if (c >= 0) t.push_back({r, c, Ae[((size_t)e * Np + a) * Np + b]});
+        const double v = Ae[((size_t)e * Np + a) * Np + b];
+        // entries below the reference's nonZeroThreshold are dropped (:832-834)
+        if (c >= 0 && std::fabs(v) >= 1e-7) t.push_back({r, c, v});
       }
     }
   return fromTriplets(t, el.Ndofs, el.Ndofs);
 }
 
+amgSolver_t::options_t amgSolver_t::optionsFrom(const settings_t& settings) {
+  options_t o;
+  std::string v;
+  settings.getSetting("PARALMOND CYCLE", v);
+  LIBP_ABORT("PARALMOND CYCLE " << v << " is not supported by this build (VCYCLE only)",
+             !v.empty() && v != "VCYCLE");
+  o.smoothedAggregation = !settings.compareSetting("PARALMOND AGGREGATION", "UNSMOOTHED");
+  o.rugeStuben = settings.compareSetting("PARALMOND STRENGTH", "RUGESTUBEN");
+  o.chebyshev = !settings.compareSetting("PARALMOND SMOOTHER", "DAMPEDJACOBI");
+  settings.getSetting("PARALMOND CHEBYSHEV DEGREE", o.chebDegree);
+  LIBP_ABORT("PARALMOND CHEBYSHEV DEGREE must be at least 1", o.chebyshev && o.chebDegree < 1);
+  return o;
+}
+
 amgSolver_t::amgSolver_t(elliptic_t& el, unsigned short xs[3]) : platform(el.platform) {
-  LIBP_ABORT("AMG: the all-Neumann operator (singular) is not supported", el.allNeumann);
-  build(ellipticAssembleCSR(el), xs);
+  opt = optionsFrom(el.settings);
+  build(ellipticAssembleCSR(el), xs, el.allNeumann != 0, el.allNeumannPenalty);
 }
 
-amgSolver_t::amgSolver_t(platform_t& _platform, const csrHost_t& A, unsigned short xs[3]) : platform(_platform) {
-  build(A, xs);
+amgSolver_t::amgSolver_t(platform_t& _platform, const csrHost_t& A, unsigned short xs[3], const options_t& o,
+                         bool nullSpace, double nullSpacePenalty)
+    : platform(_platform), opt(o) {
+  build(A, xs, nullSpace, nullSpacePenalty);
 }
 
-void amgSolver_t::build(const csrHost_t& A0, unsigned short xs[3]) {
+void amgSolver_t::build(const csrHost_t& A0, unsigned short xs[3], bool nullSpace, double nullSpacePenalty) {
   levels.clear();
   levels.emplace_back();
   levels[0].A = A0;
-  std::vector<double> null(A0.n, 1.0);
-  double theta = 0.08;  // halved on every level (parAlmondAMGSetup.cpp:86-119)
-  for (;;) {
+  // parCSR::diagSetup of the fine matrix (its Arnoldi draws come first)
+  levels[0].invDiag = inverseDiagonal(diagonal(A0));
+  levels[0].rho = rhoDinvA(A0, levels[0].invDiag, xs);
+  null.assign(A0.n, 1.0 / std::sqrt((double)A0.n));  // ellipticPreconMultiGrid.cpp:140-146
+  // theta 0.08 halved on every level (symmetric) or 0.5 (Ruge-Stuben), parAlmondAMGSetup.cpp:86-119
+  double theta = opt.rugeStuben ? 0.5 : 0.08;
+  dlong n = A0.n;
+  while (n > targetSize) {
     level_t& L = levels.back();
     const csrHost_t& A = L.A;
-    const std::vector<double> d = diagonal(A);
-    L.invDiag.resize(A.n);
-    for (dlong i = 0; i < A.n; ++i) L.invDiag[i] = (d[i] != 0.0) ? 1.0 / d[i] : 0.0;
-    L.rho = rhoDinvA(A, L.invDiag, xs);
+    // amgLevel::setupSmoother (parAlmondAMGLevel.cpp:76-84)
     L.lambda1 = L.rho;
     L.lambda0 = L.rho / 10.;
-    if (A.n <= targetSize) break;
-    // aggregates, tentative prolongator with normalised columns, smoothing
-    const csrHost_t C = strongGraph(A, d, theta);
+    L.lambda = (4. / 3.) / L.rho;
+    // strength, aggregates, tentative prolongator with normalised columns
+    const std::vector<double> d = diagonal(A);
+    const csrHost_t C = strongGraph(A, d, theta, opt.rugeStuben);
     dlong nc = 0;
-    const std::vector<dlong> agg = formAggregates(C, xs, nc);
+    L.aggregate = formAggregates(C, xs, nc);
+    const std::vector<dlong>& agg = L.aggregate;
     std::vector<double> nullc(nc, 0.0);
     for (dlong i = 0; i < A.n; ++i) nullc[agg[i]] += null[i] * null[i];
     for (auto& v : nullc) v = std::sqrt(v);
@@ -338,30 +390,40 @@ void amgSolver_t::build(const csrHost_t& A0, unsigned short xs[3]) {
       T.v[i] = null[i] / nullc[agg[i]];
     }
     T.rs[A.n] = A.n;
-    // P = T - omega D^{-1} A T, omega = 4/3 / rho
-    csrHost_t AT = multiply(A, T);
-    const double omega = (4. / 3.) / L.rho;
-    std::vector<triplet_t> t;
-    for (dlong i = 0; i < A.n; ++i) {
-      t.push_back({i, T.ci[i], T.v[i]});
-      for (dlong k = AT.rs[i]; k < AT.rs[i + 1]; ++k) t.push_back({i, AT.ci[k], -omega * L.invDiag[i] * AT.v[k]});
+    csrHost_t Ac;
+    if (opt.smoothedAggregation) {
+      // P = T - omega D^{-1} A T, omega = 4/3 / rho; Ac = R (A P)
+      csrHost_t AT = multiply(A, T);
+      const double omega = (4. / 3.) / L.rho;
+      std::vector<triplet_t> t;
+      for (dlong i = 0; i < A.n; ++i) {
+        t.push_back({i, T.ci[i], T.v[i]});
+        for (dlong k = AT.rs[i]; k < AT.rs[i + 1]; ++k) t.push_back({i, AT.ci[k], -omega * L.invDiag[i] * AT.v[k]});
+      }
+      L.P = fromTriplets(t, A.n, nc);
+      L.R = transpose(L.P);
+      Ac = multiply(L.R, multiply(A, L.P));
+    } else {
+      // P = T; (T^T A T)_IJ = sum over i in I, j in J of T_iI a_ij T_jJ (parAlmondGalerkinProd.cpp)
+      L.P = T;
+      L.R = transpose(T);
+      std::vector<triplet_t> t;
+      t.reserve(A.nnz());
+      for (dlong i = 0; i < A.n; ++i)
+        for (dlong k = A.rs[i]; k < A.rs[i + 1]; ++k)
+          t.push_back({agg[i], agg[A.ci[k]], A.v[k] * T.v[i] * T.v[A.ci[k]]});
+      Ac = fromTriplets(t, nc, nc);
     }
-    L.P = fromTriplets(t, A.n, nc);
-    L.R = transpose(L.P);
-    csrHost_t Ac = multiply(L.R, multiply(A, L.P));
-    const dlong n = A.n;
     levels.emplace_back();
-    levels.back().A = std::move(Ac);
+    level_t& Lc = levels.back();
+    Lc.A = std::move(Ac);
+    Lc.invDiag = inverseDiagonal(diagonal(Lc.A));
+    Lc.rho = rhoDinvA(Lc.A, Lc.invDiag, xs);
     null = std::move(nullc);
-    theta *= 0.5;
-    const dlong ncNew = levels.back().A.n;
-    if (ncNew <= targetSize || n < 2 * ncNew) {
-      level_t& Lc = levels.back();
-      const std::vector<double> dc = diagonal(Lc.A);
-      Lc.invDiag.resize(Lc.A.n);
-      for (dlong i = 0; i < Lc.A.n; ++i) Lc.invDiag[i] = (dc[i] != 0.0) ? 1.0 / dc[i] : 0.0;
-      break;
-    }
+    if (!opt.rugeStuben) theta *= 0.5;
+    const dlong ncNew = Lc.A.n;
+    if (ncNew <= targetSize || n < 2 * ncNew) break;  // coarse enough, or coarsening stalls
+    n = ncNew;
   }
   // coarsest level: dense inverse by Gauss-Jordan with partial pivoting (parAlmondCoarseExact.cpp)
   const csrHost_t& Ac = levels.back().A;
@@ -371,6 +433,9 @@ void amgSolver_t::build(const csrHost_t& A0, unsigned short xs[3]) {
     I[(size_t)i * nc + i] = 1.0;
     for (dlong k = Ac.rs[i]; k < Ac.rs[i + 1]; ++k) M[(size_t)i * nc + Ac.ci[k]] = Ac.v[k];
   }
+  if (nullSpace)  // exactSolver_t::setup, parAlmondCoarseExact.cpp:154-160
+    for (dlong i = 0; i < nc; ++i)
+      for (dlong j = 0; j < nc; ++j) M[(size_t)i * nc + j] += nullSpacePenalty * null[i] * null[j];
   for (dlong c = 0; c < nc; ++c) {
     dlong piv = c;
     for (dlong r = c + 1; r < nc; ++r)
@@ -430,14 +495,26 @@ void amgSolver_t::spmv(const deviceMemory<dlong>& rs, const deviceMemory<dlong>&
   LIBP_CALL(libp_csr_spmv_f64(n, rs.ptr(), ci.ptr(), v.ptr(), alpha, x, beta, z, y, lanes, platform.s()));
 }
 
-// parCSR::smoothChebyshev (parAlmondAMGSmoother.cpp:68-160) with the vector
-// updates fused as in the p-MG smoother
-void amgSolver_t::smooth(level_t& L, const double* b, double* x, bool xIsZero) {
+// amgLevel::smooth (parAlmondAMGLevel.cpp:63-72): parCSR::smoothChebyshev
+// (parAlmondAMGSmoother.cpp:68-160) with the vector updates fused as in the p-MG
+// smoother, or parCSR::smoothDampedJacobi (:36-66)
+void amgSolver_t::smooth(size_t l, const double* b, double* x, bool xIsZero) {
+  level_t& L = levels[l];
   const dlong N = L.A.n;
+  void* s = platform.s();
+  if (!opt.chebyshev) {
+    if (xIsZero) {
+      LIBP_CALL(libp_amxpy_f64(N, L.lambda, L.o_invDiag.ptr(), b, 0.0, x, s));
+      return;
+    }
+    // x += lambda D^{-1} (b - A x)
+    spmv(L.o_Ars, L.o_Aci, L.o_Av, N, L.lanesA, -1.0, x, 1.0, b, L.o_res.ptr());
+    LIBP_CALL(libp_amxpy_f64(N, L.lambda, L.o_invDiag.ptr(), L.o_res.ptr(), 1.0, x, s));
+    return;
+  }
   const double theta = 0.5 * (L.lambda1 + L.lambda0), delta = 0.5 * (L.lambda1 - L.lambda0);
   const double sigma = theta / delta;
   double rho_n = 1. / sigma;
-  void* s = platform.s();
   if (xIsZero) {
     LIBP_CALL(libp_cheb_start_f64(N, L.o_invDiag.ptr(), b, nullptr, 1.0 / theta, L.o_res.ptr(), L.o_d.ptr(), x, 1, s));
   } else {
@@ -445,7 +522,7 @@ void amgSolver_t::smooth(level_t& L, const double* b, double* x, bool xIsZero) {
     LIBP_CALL(libp_cheb_start_f64(N, L.o_invDiag.ptr(), b, L.o_Ad.ptr(), 1.0 / theta, L.o_res.ptr(), L.o_d.ptr(), x,
                                   0, s));
   }
-  for (int k = 0; k < chebDegree; ++k) {
+  for (int k = 0; k < opt.chebDegree; ++k) {
     spmv(L.o_Ars, L.o_Aci, L.o_Av, N, L.lanesA, 1.0, L.o_d.ptr(), 0.0, nullptr, L.o_Ad.ptr());
     const double rho_np1 = 1.0 / (2. * sigma - rho_n);
     LIBP_CALL(libp_cheb_step_f64(N, L.o_invDiag.ptr(), L.o_Ad.ptr(), 2.0 * rho_np1 / delta, rho_np1 * rho_n,
@@ -462,14 +539,14 @@ void amgSolver_t::cycle(size_t l, const double* b, double* x) {
     return;
   }
   level_t& Lc = levels[l + 1];
-  smooth(L, b, x, true);
+  smooth(l, b, x, true);
   // res = b - A x ; bc = R res
   spmv(L.o_Ars, L.o_Aci, L.o_Av, L.A.n, L.lanesA, -1.0, x, 1.0, b, L.o_res.ptr());
   spmv(L.o_Rrs, L.o_Rci, L.o_Rv, L.R.n, L.lanesR, 1.0, L.o_res.ptr(), 0.0, nullptr, Lc.o_b.ptr());
   cycle(l + 1, Lc.o_b.ptr(), Lc.o_x.ptr());
   // x += P xc
   spmv(L.o_Prs, L.o_Pci, L.o_Pv, L.P.n, L.lanesP, 1.0, Lc.o_x.ptr(), 1.0, x, x);
-  smooth(L, b, x, false);
+  smooth(l, b, x, false);
 }
 
 void amgSolver_t::vcycle(const double* b, double* x) { cycle(0, b, x); }

@@ -78,6 +78,8 @@ extern "C" long libp_elliptic_geti(void* p, const char* name) {
   if (n == "NrowBlocksT") return e.ogsMasked.NrowBlocksT;
   if (n == "NrowBlocksN") return e.ogsMasked.NrowBlocksN;
   if (n == "lastIterations") return h->last.iterations;
+  if (n == "splitOperator") return e.SplitGather() ? 1 : 0;
+  if (n == "faceRows") return e.SplitGather() ? e.SplitGather()->NfaceRows : -1;
   if (n == "coarseIterations") {
     if (auto* mg = dynamic_cast<MultiGridPrecon*>(e.precon.get())) return multigridCoarseIterations(*mg);
     return -1;
@@ -427,6 +429,63 @@ extern "C" int libp_elliptic_amg_info(void* p, double* out, int nout) {
       v.push_back((double)L.A.nnz());
     }
     for (int i = 0; i < nout && i < (int)v.size(); ++i) out[i] = v[i];
+  });
+}
+
+namespace {
+amgSolver_t* amgOf(Handle* h, int which) {
+  LIBP_ABORT("null handle", !h);
+  amgSolver_t* a = nullptr;
+  if (which == 0) {
+    a = h->amg.get();
+  } else if (auto* mg = dynamic_cast<MultiGridPrecon*>(h->elliptic->precon.get())) {
+    a = multigridBaseAmg(*mg);
+  }
+  LIBP_ABORT("no AMG (which = " << which << ")", !a);
+  return a;
+}
+template <typename T>
+void hostOut(const std::vector<T>& v, void* out, long* length, int dt, int* dtype) {
+  if (out) std::memcpy(out, v.data(), v.size() * sizeof(T));
+  *length = (long)v.size();
+  *dtype = dt;
+}
+}  // namespace
+
+extern "C" int libp_elliptic_amg_level_array(void* p, int which, int level, const char* name, void* out,
+                                             long* length, int* dtype) {
+  Handle* h = (Handle*)p;
+  return guarded("libp_elliptic_amg_level_array", [&]() {
+    LIBP_ABORT("null pointer", !name || !length || !dtype);
+    amgSolver_t* a = amgOf(h, which);
+    LIBP_ABORT("no AMG level " << level, level < 0 || level >= (int)a->levels.size());
+    const amgSolver_t::level_t& L = a->levels[level];
+    const std::string n(name);
+    if (n == "A_rs") return hostOut(L.A.rs, out, length, 1, dtype);
+    if (n == "A_ci") return hostOut(L.A.ci, out, length, 1, dtype);
+    if (n == "A_v") return hostOut(L.A.v, out, length, 0, dtype);
+    if (n == "P_rs") return hostOut(L.P.rs, out, length, 1, dtype);
+    if (n == "P_ci") return hostOut(L.P.ci, out, length, 1, dtype);
+    if (n == "P_v") return hostOut(L.P.v, out, length, 0, dtype);
+    if (n == "R_rs") return hostOut(L.R.rs, out, length, 1, dtype);
+    if (n == "R_ci") return hostOut(L.R.ci, out, length, 1, dtype);
+    if (n == "R_v") return hostOut(L.R.v, out, length, 0, dtype);
+    if (n == "aggregate") return hostOut(L.aggregate, out, length, 1, dtype);
+    if (n == "invDiag") return hostOut(L.invDiag, out, length, 0, dtype);
+    if (n == "null") return hostOut(a->null, out, length, 0, dtype);
+    if (n == "rho") return hostOut(std::vector<double>{L.rho, L.lambda0, L.lambda1, L.lambda}, out, length, 0, dtype);
+    LIBP_FORCE_ABORT("no AMG array named " << n);
+  });
+}
+
+extern "C" int libp_elliptic_amg_smooth(void* p, int level, const double* b, double* x, int x_is_zero) {
+  Handle* h = (Handle*)p;
+  return guarded("libp_elliptic_amg_smooth", [&]() {
+    amgSolver_t* a = amgOf(h, 0);
+    LIBP_ABORT("no smoother on AMG level " << level, level < 0 || level + 1 >= (int)a->levels.size());
+    LIBP_ABORT("null pointer", !b || !x);
+    a->smooth(level, b, x, x_is_zero != 0);
+    h->platform.finish();
   });
 }
 

@@ -148,6 +148,10 @@ settings_t ellipticDefaultSettings() {
                "(F)PCG in fused launches: p.Ap in the operator's gather; with Jacobi, z = invDiag.*r and r.z in "
                "the update",
                {"TRUE", "FALSE"});
+  s.newSetting("OPERATOR SPLIT GATHER", "TRUE",
+               "FP64 operator with element-interior rows written by the Ax kernel and only the element-boundary "
+               "rows gathered (bit-identical to the two-pass Ax + gather)",
+               {"TRUE", "FALSE"});
   s.newSetting("MULTIGRID COARSE SINGLE REDUCTION", "TRUE",
                "Fused coarse Jacobi-PCG in its single-reduction (Chronopoulos-Gear) form: two launches per "
                "iteration instead of three",

@@ -21,6 +21,7 @@ void elliptic_t::Setup(platform_t& _platform, mesh_t& _mesh, settings_t& _settin
   NBCTypes = _NBCTypes;
   BCType = _BCType.clone();
   BoundarySetup();
+  SplitGather();
   SetupPrecon();
 }
 
@@ -37,6 +38,8 @@ void elliptic_t::SetupPrecon() {
 
 void elliptic_t::BoundarySetup() {
   const dlong NL = mesh.Nelements * mesh.Np;
+  split.reset();
+  splitTried = false;
   allNeumann = (lambda == 0) ? 1 : 0;
   allNeumannPenalty = 1.;
   // node-wise flags (ellipticBoundarySetup.cpp:53-65); box faces all carry one flag,
@@ -103,12 +106,78 @@ void elliptic_t::PartialAx(deviceMemory<dfloat>& o_q, deviceMemory<dfloat>& o_Aq
                                     o_AqL_.ptr(), platform.s()));
 }
 
+/* The split form of the operator: a row whose copy is interior to its element
+ * has that one copy, so the Ax kernel stores it straight into Aq; the face
+ * gather sums the rows of element-boundary nodes from the compact face
+ * E-vector. Every row is one kind or the other on a conforming mesh; a mesh
+ * where it is not aborts. Bit-identical to PartialAx + Gather. */
+splitGather_t* elliptic_t::SplitGather() {
+  if (splitTried) return split.get();
+  splitTried = true;
+  if (!settings.compareSetting("OPERATOR SPLIT GATHER", "TRUE") || mesh.N < 1 || mesh.N > 14) return nullptr;
+  const int N = mesh.N, Nq = N + 1, Np = mesh.Np;
+  const int Nface = libp_ax_hex_face_nodes(N);
+  std::vector<int> faceIdx(Np, -1);
+  for (int k = 0, f = 0; k < Nq; ++k)
+    for (int j = 0; j < Nq; ++j)
+      for (int i = 0; i < Nq; ++i) {
+        const bool interior = i > 0 && i < N && j > 0 && j < N && k > 0 && k < N;
+        if (!interior) faceIdx[i + j * Nq + k * Nq * Nq] = f++;
+      }
+  const memory<dlong>& rs = ogsMasked.rowStartsT;
+  const memory<dlong>& ci = ogsMasked.colIdsT;
+  std::vector<dlong> rows, starts{0}, cols;
+  for (dlong r = 0; r < ogsMasked.Ngather; ++r) {
+    const dlong a = rs[r], b = rs[r + 1];
+    if (b - a == 1 && faceIdx[ci[a] % Np] < 0) continue;  // an interior node: written by the Ax kernel
+    rows.push_back(r);
+    for (dlong g = a; g < b; ++g) {
+      const int f = faceIdx[ci[g] % Np];
+      LIBP_ABORT("split operator: an element-interior node shares its row (non-conforming mesh)", f < 0);
+      cols.push_back((ci[g] / Np) * Nface + f);
+    }
+    starts.push_back((dlong)cols.size());
+  }
+  auto sp = std::make_shared<splitGather_t>();
+  sp->Nface = Nface;
+  sp->NfaceRows = (dlong)rows.size();
+  sp->o_rows = platform.malloc<dlong>(std::max<size_t>(rows.size(), 1));
+  if (!rows.empty()) sp->o_rows.copyFrom(rows.data(), rows.size(), 0, platform.stream);
+  sp->o_rowStarts = platform.malloc<dlong>(starts.size());
+  sp->o_rowStarts.copyFrom(starts.data(), starts.size(), 0, platform.stream);
+  sp->o_cols = platform.malloc<dlong>(std::max<size_t>(cols.size(), 1));
+  if (!cols.empty()) sp->o_cols.copyFrom(cols.data(), cols.size(), 0, platform.stream);
+  sp->o_AqFace = platform.malloc<dfloat>((size_t)mesh.Nelements * Nface);
+  sp->nAxPartials = libp_ax_hex_split_blocks(N, mesh.NlocalGatherElements);
+  sp->o_papPart = platform.malloc<double>(sp->nAxPartials + libp_ogs_gather_face_blocks(sp->NfaceRows) + 1);
+  platform.finish();
+  split = sp;
+  return split.get();
+}
+
 void elliptic_t::Operator(deviceMemory<dfloat>& o_q, deviceMemory<dfloat>& o_Aq) {
+  if (splitGather_t* sp = SplitGather()) {
+    LIBP_CALL(libp_ax_hex_split_f64(mesh.N, mesh.NlocalGatherElements, nullptr, o_GlobalToLocal.ptr(), mesh.o_wJ.ptr(),
+                                    mesh.o_ggeo.ptr(), mesh.o_D.ptr(), lambda, o_q.ptr(), o_Aq.ptr(),
+                                    sp->o_AqFace.ptr(), nullptr, platform.s()));
+    LIBP_CALL(libp_ogs_gather_face_f64(sp->NfaceRows, sp->o_rows.ptr(), sp->o_rowStarts.ptr(), sp->o_cols.ptr(),
+                                       sp->o_AqFace.ptr(), o_Aq.ptr(), platform.s()));
+    return;
+  }
   PartialAx(o_q, o_AqL);
   ogsMasked.Gather(o_Aq, o_AqL, 1, ogs::Add, ogs::Trans);
 }
 
 bool elliptic_t::OperatorPAP(deviceMemory<dfloat>& o_q, deviceMemory<dfloat>& o_Aq, double* scalars, void* ws) {
+  if (splitGather_t* sp = SplitGather()) {
+    LIBP_CALL(libp_ax_hex_split_f64(mesh.N, mesh.NlocalGatherElements, nullptr, o_GlobalToLocal.ptr(), mesh.o_wJ.ptr(),
+                                    mesh.o_ggeo.ptr(), mesh.o_D.ptr(), lambda, o_q.ptr(), o_Aq.ptr(),
+                                    sp->o_AqFace.ptr(), sp->o_papPart.ptr(), platform.s()));
+    LIBP_CALL(libp_pcg_gather_face_pap_f64(sp->NfaceRows, sp->o_rows.ptr(), sp->o_rowStarts.ptr(), sp->o_cols.ptr(),
+                                           sp->o_AqFace.ptr(), o_Aq.ptr(), o_q.ptr(), sp->nAxPartials,
+                                           sp->o_papPart.ptr(), scalars, platform.s()));
+    return true;
+  }
   PartialAx(o_q, o_AqL);
   LIBP_CALL(libp_pcg_gather_pap_f64(ogsMasked.NlocalT, ogsMasked.o_rowStartsT.ptr(), ogsMasked.o_colIdsT.ptr(),
                                     o_AqL.ptr(), o_Aq.ptr(), o_q.ptr(), scalars, ws, platform.s()));

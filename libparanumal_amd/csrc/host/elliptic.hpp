@@ -16,6 +16,20 @@ struct ellipticRunResult_t {
   double maxError = NAN;  // max nodal |u_h - u| for the sine data set
 };
 
+/* Maps of the split operator (libp_ax_hex_split_f64 + libp_ogs_gather_face_f64):
+ * the gathered rows that have element-boundary copies, their copies as indices
+ * into the compact face E-vector (e * Nface + face index, ascending local order as
+ * in the ogs CSR), the face E-vector and the p.Ap partials of the fused PCG step.
+ * Rows of element-interior nodes have one copy and are written by the Ax kernel. */
+struct splitGather_t {
+  int Nface = 0;
+  dlong NfaceRows = 0;
+  deviceMemory<dlong> o_rows, o_rowStarts, o_cols;
+  deviceMemory<dfloat> o_AqFace;
+  deviceMemory<double> o_papPart;
+  int nAxPartials = 0;
+};
+
 class elliptic_t : public operator_t {
  public:
   platform_t platform;
@@ -45,6 +59,8 @@ class elliptic_t : public operator_t {
   deviceMemory<dlong> o_GlobalToLocal;
   deviceMemory<dfloat> o_AqL;
   deviceMemory<float> o_AqL32;
+  std::shared_ptr<splitGather_t> split;  // built on first use ([OPERATOR SPLIT GATHER] TRUE, N <= 14)
+  bool splitTried = false;
   dlong Ndofs = 0, Nhalo = 0;
   precon_t precon;
 
@@ -71,6 +87,8 @@ class elliptic_t : public operator_t {
   // the operator's gather fused with the PCG p.Ap (libp_pcg_gather_pap_f64)
   bool OperatorPAP(deviceMemory<dfloat>& o_q, deviceMemory<dfloat>& o_Aq, double* scalars, void* ws) override;
   void PartialAx(deviceMemory<dfloat>& o_q, deviceMemory<dfloat>& o_AqL);
+  // the split operator's maps, or null when the two-pass form is used
+  splitGather_t* SplitGather();
   // FP32 operator used by the p-MG smoother (no reference counterpart)
   void OperatorF32(deviceMemory<float>& o_q, deviceMemory<float>& o_Aq);
 
@@ -130,6 +148,9 @@ void multigridApplyFineOperator32(MultiGridPrecon& mg, deviceMemory<float>& x, d
 // (out += P in), 3 smooth (in = rhs, out = x in place, flag = x is zero), 4 the
 // inverse diagonal) on device vectors of that precision, and its Chebyshev
 // interval {lambda1, lambda0}.
+class amgSolver_t;
+// the AMG base level of a p-MG preconditioner ([MULTIGRID COARSE SOLVER] AMG), or null
+amgSolver_t* multigridBaseAmg(MultiGridPrecon& mg);
 int multigridPrecisionBits(MultiGridPrecon& mg);
 long multigridLevelNdofs(MultiGridPrecon& mg, int l);
 void multigridLevelOp(MultiGridPrecon& mg, int l, int op, const void* in, void* out, int flag);

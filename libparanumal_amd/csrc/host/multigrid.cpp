@@ -20,8 +20,8 @@ namespace libp {
 
 /* Eigenvalues of a k x k upper Hessenberg matrix (column-major H[i+j*k]) by
  * complex single-shift QR with deflation; stands in for dgeev in
- * ellipticPreconMultiGridLevel.cpp:456-470. */
-static std::vector<std::complex<double>> hessenbergEigenvalues(int n, const std::vector<double>& H) {
+ * ellipticPreconMultiGridLevel.cpp:456-470 and parAlmondparCSR.cpp:444-446. */
+std::vector<std::complex<double>> hessenbergEigenvalues(int n, const std::vector<double>& H) {
   typedef std::complex<double> cd;
   std::vector<cd> A(n * n);
   for (int i = 0; i < n; ++i)
@@ -512,12 +512,10 @@ class coarseJacobiPCG_t {
       cg1csr = true;
     }
     // the reference's base level: parAlmond AMG on the assembled operator
-    // (ellipticPreconMultiGrid.cpp:104-131), one V-cycle per coarse solve
-    if (elliptic->settings.compareSetting("MULTIGRID COARSE SOLVER", "AMG")) {
-      LIBP_ABORT("MULTIGRID COARSE SOLVER AMG needs a Dirichlet or screened problem (not all-Neumann)",
-                 elliptic->allNeumann);
+    // (ellipticPreconMultiGrid.cpp:119-150), one V-cycle per coarse solve; on the
+    // all-Neumann operator the coarsest matrix carries the null-space penalty
+    if (elliptic->settings.compareSetting("MULTIGRID COARSE SOLVER", "AMG"))
       amg = std::make_shared<amgSolver_t>(*elliptic, xs);
-    }
   }
   ~coarseJacobiPCG_t() {
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
@@ -1242,6 +1240,12 @@ void multigridLevelLambda(MultiGridPrecon& mg, int l, double* lambda) {
   if (auto* m = dynamic_cast<multigrid_t<float>*>(mg.impl.get())) return get(m);
   if (auto* m = dynamic_cast<multigrid_t<double>*>(mg.impl.get())) return get(m);
   LIBP_FORCE_ABORT("no p-MG preconditioner");
+}
+
+amgSolver_t* multigridBaseAmg(MultiGridPrecon& mg) {
+  if (auto* m = dynamic_cast<multigrid_t<float>*>(mg.impl.get())) return m->coarse->amg.get();
+  if (auto* m = dynamic_cast<multigrid_t<double>*>(mg.impl.get())) return m->coarse->amg.get();
+  return nullptr;
 }
 
 int multigridCoarseIterations(MultiGridPrecon& mg) {

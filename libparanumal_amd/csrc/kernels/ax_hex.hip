@@ -18,7 +18,18 @@
 //    the dominant arrays);
 //  * the E-vector result is written with non-temporal stores (it is consumed
 //    by the gather kernel, not re-read by this one).
-#include "kcommon.hpp"
+//
+// The split output form (OUT = 1) serves the whole C0 operator A = Q^T A_L Q
+// (ellipticOperator.cpp:29-107) without writing the full E-vector: a node
+// interior to its element has exactly one copy, so its gathered row is the
+// element's value itself and is stored straight into the gathered output
+// (0 + a = a: bit-identical to the gather). Only the element-boundary ("face")
+// nodes go to a compact face E-vector, e * Nface + faceIndex(n), which a gather
+// over the shared rows alone then sums (ogs_gather_face_kernel). The ids of
+// the element's nodes wait in LDS for the output stage (the kernel has no
+// registers to spare at N=7). With PAP, the block also forms its share of
+// p.Ap over the interior rows from the q and Aq values it holds.
+#include "reduce.hpp"
 
 namespace libp_amd {
 
@@ -38,23 +49,45 @@ struct AxShape {
   static constexpr int waves = (Nq == 8 && f32) ? 8 : ((Nq <= 8) ? 4 : 2);
 };
 
+// Number of element-boundary nodes of an Nq^3 element, and the position of
+// node (i,j,k) among them in local order (the node index minus the interior
+// nodes before it); -1 for an interior node.
+__host__ __device__ constexpr int hex_face_nodes(int Nq) { return Nq * Nq * Nq - (Nq - 2) * (Nq - 2) * (Nq - 2); }
+__device__ inline int interior_before(int x, int Nq) { return min(max(x - 1, 0), Nq - 2); }
+__device__ inline bool hex_interior(int i, int j, int k, int Nq) {
+  return i > 0 && i < Nq - 1 && j > 0 && j < Nq - 1 && k > 0 && k < Nq - 1;
+}
+__device__ inline int hex_face_index(int i, int j, int k, int Nq) {
+  const int n = i + j * Nq + k * Nq * Nq;
+  const bool kin = k > 0 && k < Nq - 1, jin = j > 0 && j < Nq - 1;
+  const int before = interior_before(k, Nq) * (Nq - 2) * (Nq - 2) +
+                     (kin ? interior_before(j, Nq) * (Nq - 2) + (jin ? interior_before(i, Nq) : 0) : 0);
+  return n - before;
+}
+
 // MODE 0: q is a gathered vector indexed through GlobalToLocal (-1 -> 0).
 // MODE 1: q is an E-vector (element-local input, no gather).
+// OUT 0: Aq is the E-vector. OUT 1 (MODE 0 only): interior nodes go to the
+// gathered Ag, element-boundary nodes to the face E-vector Aq; PAP adds one
+// partial of sum q Ag over the interior rows per block, pap[pap0 + block].
 // Every element slot of a block is a real element: the launcher runs the
 // whole blocks first and the remainder with NE=1 (element offset e0). A
 // clamped/predicated tail inside this kernel defeats the compiler's uniform
 // address analysis and roughly quadruples register use.
-template <int Nq, int NE, typename T, int MODE>
+template <int Nq, int NE, typename T, int MODE, int OUT = 0, bool PAP = false>
 __global__ void __launch_bounds__(NE * Nq * Nq) __attribute__((amdgpu_waves_per_eu(AxShape<Nq, T>::waves, 8)))
     ax_hex_kernel(const int e0, const int* __restrict__ elementList, const int* __restrict__ G2L,
                   const T* __restrict__ wJ, const T* __restrict__ ggeo, const T* __restrict__ D, const T lambda,
-                  const T* __restrict__ q, T* __restrict__ Aq) {
+                  const T* __restrict__ q, T* __restrict__ Aq, T* __restrict__ Ag = nullptr,
+                  double* __restrict__ pap = nullptr, const int pap0 = 0) {
   using S = AxShape<Nq, T>;
   constexpr int NqNq = S::NqNq, Np = S::Np, threads = NE * NqNq;
   __shared__ T s_D[Nq][Nq + 1];
   __shared__ T s_q[NE][Nq][Nq + 1];
   __shared__ T s_Gqr[NE][Nq][Nq + 1];
   __shared__ T s_Gqs[NE][Nq][Nq + 1];
+  __shared__ int s_id[OUT == 1 ? NE : 1][OUT == 1 ? Nq : 1][OUT == 1 ? NqNq : 1];
+  __shared__ T s_out[OUT == 1 ? NE : 1][OUT == 1 ? Nq : 1][OUT == 1 ? NqNq : 1];
 
   const int t = threadIdx.x;
   const int es = (NE == 1) ? 0 : t / NqNq;
@@ -73,6 +106,7 @@ __global__ void __launch_bounds__(NE * Nq * Nq) __attribute__((amdgpu_waves_per_
     if (MODE == 0) {
       const int id = __builtin_nontemporal_load(G2L + base + k * NqNq);
       r_q[k] = (id >= 0) ? q[id] : T(0);
+      if (OUT == 1) s_id[es][k][ij] = id;
     } else {
       r_q[k] = __builtin_nontemporal_load(q + base + k * NqNq);
     }
@@ -119,9 +153,52 @@ __global__ void __launch_bounds__(NE * Nq * Nq) __attribute__((amdgpu_waves_per_
     r_Aq[k] += Auk;
   }
 
-  T* out = Aq + base;
+  if (OUT == 0) {
+    T* out = Aq + base;
 #pragma unroll
-  for (int k = 0; k < Nq; ++k) __builtin_nontemporal_store(r_Aq[k], out + k * NqNq);
+    for (int k = 0; k < Nq; ++k) __builtin_nontemporal_store(r_Aq[k], out + k * NqNq);
+  } else {
+    // The results go through LDS first: stored with the regular pattern of the
+    // E-vector store, the registers die there. Scattering straight from the
+    // registers (one branch and address per k) made the compiler spill ~500
+    // VGPRs at N=7.
+    constexpr int Nface = hex_face_nodes(Nq);
+    double acc = 0;
+#pragma unroll
+    for (int k = 0; k < Nq; ++k) {
+      s_out[es][k][ij] = r_Aq[k];
+      if (PAP) acc += hex_interior(i, j, k, Nq) ? (double)r_q[k] * (double)r_Aq[k] : 0.0;
+    }
+    __syncthreads();
+    T* fout = Aq + (size_t)element * Nface;
+    for (int k = 0; k < Nq; ++k) {
+      const T v = s_out[es][k][ij];
+      if (hex_interior(i, j, k, Nq)) {
+        const int id = s_id[es][k][ij];
+        if (id >= 0) Ag[id] = v;
+      } else {
+        __builtin_nontemporal_store(v, fout + hex_face_index(i, j, k, Nq));
+      }
+    }
+    if constexpr (PAP) {
+      if constexpr (threads % 64 == 0) {
+        __shared__ double s_w[threads / 64];
+        const double b = block_sum(acc, s_w);
+        if (t == 0) pap[pap0 + blockIdx.x] = b;
+      } else {  // partial wavefront (N <= 4): a tree in LDS, no cross-lane reads of idle lanes
+        constexpr int P = (threads <= 64) ? 64 : 256;
+        __shared__ double s_red[P];
+        s_red[t] = acc;
+        for (int z = threads + t; z < P; z += threads) s_red[z] = 0.0;
+#pragma unroll
+        for (int off = P / 2; off > 0; off >>= 1) {
+          __syncthreads();
+          if (t < off) s_red[t] += s_red[t + off];
+        }
+        if (t == 0) pap[pap0 + blockIdx.x] = s_red[0];
+      }
+    }
+  }
 }
 
 template <int Nq, typename T, int MODE>
@@ -138,6 +215,40 @@ static int launch_ax(int Nelements, const int* elist, const int* g2l, const T* w
     hipLaunchKernelGGL((ax_hex_kernel<Nq, 1, T, MODE>), dim3(rest), dim3(S::NqNq), 0, st, full * S::NE, elist,
                        g2l, wJ, ggeo, D, lambda, q, Aq);
   return launch_status("ax_hex");
+}
+
+// Split output (OUT = 1): interior rows into Ag, boundary nodes into the face
+// E-vector AqF, and with pap one p.Ap partial per block (split_blocks of them).
+template <int Nq>
+static int split_blocks(int Nelements) {
+  using S = AxShape<Nq, double>;
+  return Nelements / S::NE + ((S::NE > 1) ? Nelements % S::NE : 0);
+}
+
+template <int Nq>
+static int launch_ax_split(int Nelements, const int* elist, const int* g2l, const double* wJ, const double* ggeo,
+                           const double* D, double lambda, const double* q, double* AqF, double* Ag, double* pap,
+                           hipStream_t st) {
+  using S = AxShape<Nq, double>;
+  if (Nelements <= 0) return LIBP_OK;
+  const int full = Nelements / S::NE;
+  const int rest = Nelements - full * S::NE;
+  if (pap) {
+    if (full > 0)
+      hipLaunchKernelGGL((ax_hex_kernel<Nq, S::NE, double, 0, 1, true>), dim3(full), dim3(S::threads), 0, st, 0,
+                         elist, g2l, wJ, ggeo, D, lambda, q, AqF, Ag, pap, 0);
+    if (S::NE > 1 && rest > 0)
+      hipLaunchKernelGGL((ax_hex_kernel<Nq, 1, double, 0, 1, true>), dim3(rest), dim3(S::NqNq), 0, st,
+                         full * S::NE, elist, g2l, wJ, ggeo, D, lambda, q, AqF, Ag, pap, full);
+  } else {
+    if (full > 0)
+      hipLaunchKernelGGL((ax_hex_kernel<Nq, S::NE, double, 0, 1, false>), dim3(full), dim3(S::threads), 0, st, 0,
+                         elist, g2l, wJ, ggeo, D, lambda, q, AqF, Ag, nullptr, 0);
+    if (S::NE > 1 && rest > 0)
+      hipLaunchKernelGGL((ax_hex_kernel<Nq, 1, double, 0, 1, false>), dim3(rest), dim3(S::NqNq), 0, st,
+                         full * S::NE, elist, g2l, wJ, ggeo, D, lambda, q, AqF, Ag, nullptr, 0);
+  }
+  return launch_status("ax_hex_split");
 }
 
 template <typename T, int MODE>
@@ -189,4 +300,62 @@ extern "C" int libp_ax_hex_partial_f32(int N, int Nelements, const int* elementL
 extern "C" int libp_ax_hex_local_f64(int N, int Nelements, const double* wJ, const double* ggeo, const double* D,
                                      double lambda, const double* qL, double* AqL, void* stream) {
   return dispatch_ax<double, 1>(N, Nelements, nullptr, nullptr, wJ, ggeo, D, lambda, qL, AqL, stream);
+}
+
+extern "C" int libp_ax_hex_split_blocks(int N, int Nelements) {
+  switch (N) {
+#define LIBP_AX_CASE(NN) \
+  case NN:               \
+    return split_blocks<NN + 1>(Nelements);
+    LIBP_AX_CASE(1)
+    LIBP_AX_CASE(2)
+    LIBP_AX_CASE(3)
+    LIBP_AX_CASE(4)
+    LIBP_AX_CASE(5)
+    LIBP_AX_CASE(6)
+    LIBP_AX_CASE(7)
+    LIBP_AX_CASE(8)
+    LIBP_AX_CASE(9)
+    LIBP_AX_CASE(10)
+    LIBP_AX_CASE(11)
+    LIBP_AX_CASE(12)
+    LIBP_AX_CASE(13)
+    LIBP_AX_CASE(14)
+#undef LIBP_AX_CASE
+    default:
+      return -1;
+  }
+}
+
+extern "C" int libp_ax_hex_face_nodes(int N) { return (N >= 1 && N <= 15) ? hex_face_nodes(N + 1) : -1; }
+
+extern "C" int libp_ax_hex_split_f64(int N, int Nelements, const int* elementList, const int* GlobalToLocal,
+                                     const double* wJ, const double* ggeo, const double* D, double lambda,
+                                     const double* q, double* Aq, double* AqFace, double* papPartials,
+                                     void* stream) {
+  if (!GlobalToLocal || !wJ || !ggeo || !D || !q || !Aq || !AqFace) return arg_error("ax_hex_split", "null pointer");
+  hipStream_t st = as_stream(stream);
+  switch (N) {
+#define LIBP_AX_CASE(NN) \
+  case NN:               \
+    return launch_ax_split<NN + 1>(Nelements, elementList, GlobalToLocal, wJ, ggeo, D, lambda, q, AqFace, Aq, \
+                                   papPartials, st);
+    LIBP_AX_CASE(1)
+    LIBP_AX_CASE(2)
+    LIBP_AX_CASE(3)
+    LIBP_AX_CASE(4)
+    LIBP_AX_CASE(5)
+    LIBP_AX_CASE(6)
+    LIBP_AX_CASE(7)
+    LIBP_AX_CASE(8)
+    LIBP_AX_CASE(9)
+    LIBP_AX_CASE(10)
+    LIBP_AX_CASE(11)
+    LIBP_AX_CASE(12)
+    LIBP_AX_CASE(13)
+    LIBP_AX_CASE(14)
+#undef LIBP_AX_CASE
+    default:
+      return arg_error("ax_hex_split", "polynomial degree must be in [1,14]");
+  }
 }

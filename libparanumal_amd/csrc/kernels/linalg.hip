@@ -302,6 +302,47 @@ __global__ void __launch_bounds__(256) cheb_step_kernel(const int N, const T* __
 
 using namespace libp_amd;
 
+// HBM probes (no reference counterpart; measurement only): a read-only sweep
+// that sums x into one partial per block, and a copy y = x. Each thread moves
+// 16 bytes per access (two doubles) with non-temporal loads and stores, four
+// accesses in flight per loop step; the grid is a fixed 2048 blocks of 256.
+typedef double hbm_v2d __attribute__((ext_vector_type(2)));
+constexpr int HBM_PROBE_BLOCKS = 2048;
+
+__global__ void __launch_bounds__(256) hbm_read_kernel(const long n2, const hbm_v2d* __restrict__ x,
+                                                       double* __restrict__ part) {
+  __shared__ double s_w[4];
+  const long S = (long)gridDim.x * blockDim.x;
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  double acc = 0;
+  for (; i + 3 * S < n2; i += 4 * S) {
+    const hbm_v2d a = __builtin_nontemporal_load(x + i), b = __builtin_nontemporal_load(x + i + S);
+    const hbm_v2d c = __builtin_nontemporal_load(x + i + 2 * S), d = __builtin_nontemporal_load(x + i + 3 * S);
+    acc += (a.x + a.y) + (b.x + b.y) + (c.x + c.y) + (d.x + d.y);
+  }
+  for (; i < n2; i += S) {
+    const hbm_v2d a = __builtin_nontemporal_load(x + i);
+    acc += a.x + a.y;
+  }
+  const double b = block_sum(acc, s_w);
+  if (threadIdx.x == 0) part[blockIdx.x] = b;
+}
+
+__global__ void __launch_bounds__(256) hbm_copy_kernel(const long n2, const hbm_v2d* __restrict__ x,
+                                                       hbm_v2d* __restrict__ y) {
+  const long S = (long)gridDim.x * blockDim.x;
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  for (; i + 3 * S < n2; i += 4 * S) {
+    const hbm_v2d a = __builtin_nontemporal_load(x + i), b = __builtin_nontemporal_load(x + i + S);
+    const hbm_v2d c = __builtin_nontemporal_load(x + i + 2 * S), d = __builtin_nontemporal_load(x + i + 3 * S);
+    __builtin_nontemporal_store(a, y + i);
+    __builtin_nontemporal_store(b, y + i + S);
+    __builtin_nontemporal_store(c, y + i + 2 * S);
+    __builtin_nontemporal_store(d, y + i + 3 * S);
+  }
+  for (; i < n2; i += S) __builtin_nontemporal_store(__builtin_nontemporal_load(x + i), y + i);
+}
+
 extern "C" long libp_reduce_ws_bytes(void) { return (long)sizeof(ReduceWS); }
 
 extern "C" int libp_inner_prod_f64(int N, const double* x, const double* y, double* result, void* ws, void* stream) {
@@ -443,4 +484,20 @@ extern "C" int libp_cheb_start_f64(int N, const double* invDiag, const double* r
 extern "C" int libp_cheb_step_f64(int N, const double* invDiag, const double* Ad, double c1, double c2, double* RES,
                                   double* d, double* X, void* stream) {
   LIBP_LAUNCH_1D(cheb_step_kernel<double>, N, N, invDiag, Ad, c1, c2, RES, d, X);
+}
+
+extern "C" int libp_hbm_probe_f64(int mode, long n, const double* x, double* y, void* stream) {
+  if (n <= 0) return LIBP_OK;
+  if (!x || !y || (n & 1) || ((uintptr_t)x & 15) || ((uintptr_t)y & 15))
+    return arg_error("hbm_probe", "null or unaligned pointer, or odd length");
+  if (mode == 0) {
+    hipLaunchKernelGGL(hbm_read_kernel, dim3(HBM_PROBE_BLOCKS), dim3(256), 0, as_stream(stream), n / 2,
+                       (const hbm_v2d*)x, y);
+  } else if (mode == 1) {
+    hipLaunchKernelGGL(hbm_copy_kernel, dim3(HBM_PROBE_BLOCKS), dim3(256), 0, as_stream(stream), n / 2,
+                       (const hbm_v2d*)x, (hbm_v2d*)y);
+  } else {
+    return arg_error("hbm_probe", "mode must be 0 (read) or 1 (copy)");
+  }
+  return launch_status("hbm_probe");
 }

@@ -203,6 +203,84 @@ __global__ void __launch_bounds__(256) gather_pap_finish_kernel(const int nb, do
   }
 }
 
+// Gather of the element-boundary rows of the split operator (ax_hex_kernel
+// OUT = 1): gv[rows[f]] = the sum of the row's copies in the face E-vector v,
+// in ascending local order from zero as in ogs_gather_kernel, so bit-identical
+// to the full gather. Rows of element-interior nodes were written by the Ax
+// kernel itself. PAP: one partial of sum y[row] gv[row] per block into part.
+template <bool PAP>
+__global__ void __launch_bounds__(256) ogs_gather_face_kernel(const int Nrows, const int* __restrict__ rows,
+                                                              const int* __restrict__ rowStarts,
+                                                              const int* __restrict__ colIds,
+                                                              const double* __restrict__ v, double* __restrict__ gv,
+                                                              const double* __restrict__ y,
+                                                              const double* __restrict__ s,
+                                                              double* __restrict__ part) {
+  __shared__ double s_w[4];
+  if (PAP && s[8] != 0.0) return;
+  const long S = (long)gridDim.x * blockDim.x;
+  double acc = 0;
+  for (long f0 = blockIdx.x * (long)blockDim.x + threadIdx.x; f0 < Nrows; f0 += 2 * S) {
+    int a[2], len[2], row[2];
+    double yv[2] = {0.0, 0.0};
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const long f = f0 + r * S;
+      row[r] = (f < Nrows) ? rows[f] : 0;
+      a[r] = (f < Nrows) ? rowStarts[f] : 0;
+      len[r] = (f < Nrows) ? rowStarts[f + 1] - a[r] : 0;
+    }
+    if (PAP) {
+#pragma unroll
+      for (int r = 0; r < 2; ++r)
+        if (f0 + r * S < Nrows) yv[r] = y[row[r]];
+    }
+    int c[2][8];
+    double x[2][8];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) c[r][k] = (k < len[r]) ? colIds[a[r] + k] : 0;
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) x[r][k] = (k < len[r]) ? v[c[r][k]] : 0.0;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      if (f0 + r * S >= Nrows) continue;
+      double val = 0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (k < len[r]) val += x[r][k];
+      for (int g = a[r] + 8; g < a[r] + len[r]; ++g) val += v[colIds[g]];
+      gv[row[r]] = val;
+      if (PAP) acc += yv[r] * val;
+    }
+  }
+  if (PAP) {
+    const double b = block_sum(acc, s_w);
+    if (threadIdx.x == 0) part[blockIdx.x] = b;
+  }
+}
+
+// p.Ap of the split operator: the Ax kernel's interior partials and the face
+// gather's partials, summed in a fixed order; then alpha as in
+// gather_pap_finish_kernel.
+__global__ void __launch_bounds__(1024) split_pap_finish_kernel(const int nb, double* __restrict__ s,
+                                                                const double* __restrict__ part) {
+  __shared__ double s_w[16];
+  if (s[8] != 0.0) return;
+  double t = 0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) t += part[i];
+  const double pap = block_sum(t, s_w);
+  if (threadIdx.x == 0) {
+    s[4] = pap;
+    s[2] = s[0] / pap;
+  }
+}
+
+inline unsigned face_gather_blocks(int Nrows) { return stream_blocks(Nrows, 1024, 4096); }
+
 template <typename T>
 __global__ void __launch_bounds__(256) ogs_scatter_kernel(const int Nrows, const int* __restrict__ rowStarts,
                                                           const int* __restrict__ colIds, const T* __restrict__ gv,
@@ -311,4 +389,31 @@ extern "C" int libp_ogs_gather_cheb_step_f32(int Nrows, const int* rowStarts, co
                                              const float* invDiag, float c1, float c2, float* RES, float* d, float* X,
                                              void* stream) {
   return gather_cheb<float, false>(Nrows, rowStarts, colIds, v, invDiag, nullptr, c1, c2, RES, d, X, stream);
+}
+
+extern "C" int libp_ogs_gather_face_blocks(int NfaceRows) { return NfaceRows > 0 ? (int)face_gather_blocks(NfaceRows) : 0; }
+
+extern "C" int libp_ogs_gather_face_f64(int NfaceRows, const int* faceRows, const int* faceRowStarts,
+                                        const int* faceCols, const double* AqFace, double* Aq, void* stream) {
+  if (NfaceRows <= 0) return LIBP_OK;
+  if (!faceRows || !faceRowStarts || !faceCols || !AqFace || !Aq) return arg_error("ogs_gather_face", "null pointer");
+  hipLaunchKernelGGL(ogs_gather_face_kernel<false>, dim3(face_gather_blocks(NfaceRows)), dim3(256), 0,
+                     as_stream(stream), NfaceRows, faceRows, faceRowStarts, faceCols, AqFace, Aq, nullptr, nullptr,
+                     nullptr);
+  return launch_status("ogs_gather_face");
+}
+
+extern "C" int libp_pcg_gather_face_pap_f64(int NfaceRows, const int* faceRows, const int* faceRowStarts,
+                                            const int* faceCols, const double* AqFace, double* Ap, const double* p,
+                                            int nAxPartials, double* partials, double* scalars, void* stream) {
+  if (!faceRows || !faceRowStarts || !faceCols || !AqFace || !Ap || !p || !partials || !scalars || nAxPartials < 0 ||
+      NfaceRows < 0)
+    return arg_error("pcg_gather_face_pap", "null pointer or negative count");
+  const unsigned blocks = NfaceRows > 0 ? face_gather_blocks(NfaceRows) : 0;
+  if (blocks > 0)
+    hipLaunchKernelGGL(ogs_gather_face_kernel<true>, dim3(blocks), dim3(256), 0, as_stream(stream), NfaceRows,
+                       faceRows, faceRowStarts, faceCols, AqFace, Ap, p, scalars, partials + nAxPartials);
+  hipLaunchKernelGGL(split_pap_finish_kernel, dim3(1), dim3(1024), 0, as_stream(stream), nAxPartials + (int)blocks,
+                     scalars, partials);
+  return launch_status("pcg_gather_face_pap");
 }

@@ -479,6 +479,58 @@ double orc_time_partial_ax(void* p, int reps, int parallel) {
   return t[t.size() / 2];
 }
 
+// CPU baseline legs on the handle's mesh: leg 0 the gather (Trans) of an
+// E-vector, 1 the whole operator (partial Ax + gather), 2 a Jacobi-PCG solve of
+// the handle's right-hand side with the reference's stopping rule. Returns the
+// median seconds of `reps` runs; *iters receives the solve's iteration count.
+double orc_time_leg(void* p, int leg, int reps, int parallel, int* iters) {
+  Handle* h = (Handle*)p;
+  Elliptic& el = h->el;
+  const size_t NL = (size_t)h->mesh.Nelements * h->mesh.Np;
+  el.parallel = parallel != 0;
+  std::vector<double> q(el.Ndofs), Aq(el.Ndofs), vL(NL), t;
+  for (size_t i = 0; i < q.size(); ++i) q[i] = std::sin(0.37 * (double)i);
+  for (size_t i = 0; i < NL; ++i) vL[i] = std::cos(0.11 * (double)i);
+  std::vector<double> rL, r0(el.Ndofs), diagA, invD;
+  if (leg == 2) {
+    el.Rhs(rL);
+    el.ogsMasked.gatherLocal.GatherAdd(r0.data(), rL.data(), true);
+    el.BuildOperatorDiagonal(diagA);
+    invD.resize(diagA.size());
+    for (size_t n = 0; n < diagA.size(); ++n) invD[n] = 1.0 / diagA[n];
+  }
+  struct A_ : OpFn {
+    Elliptic* el;
+    std::vector<double> s;
+    void apply(const double* a, double* b) override { el->Operator(a, b, s); }
+  } A;
+  A.el = &el;
+  struct M_ : OpFn {
+    const std::vector<double>* inv;
+    void apply(const double* a, double* b) override {
+      for (size_t n = 0; n < inv->size(); ++n) b[n] = 1.0 * (*inv)[n] * a[n];
+    }
+  } M;
+  M.inv = &invD;
+  for (int r = 0; r < reps; ++r) {
+    auto a = std::chrono::steady_clock::now();
+    if (leg == 0) {
+      el.ogsMasked.gatherLocal.GatherAdd(Aq.data(), vL.data(), true, parallel != 0);
+    } else if (leg == 1) {
+      el.Operator(q.data(), Aq.data(), h->scratch);
+    } else {
+      std::vector<double> x(el.Ndofs, 0.0), rr = r0;
+      PCGResult res = pcg_solve(el.Ndofs, A, M, x.data(), rr.data(), 1e-8, 5000, false, false, false);
+      if (iters) *iters = res.iters;
+    }
+    auto b = std::chrono::steady_clock::now();
+    t.push_back(std::chrono::duration<double>(b - a).count());
+  }
+  el.parallel = false;
+  std::sort(t.begin(), t.end());
+  return t[t.size() / 2];
+}
+
 /* ---- learned smoother oracle ---- */
 long orc_learned_trace(void* p, int* out) {
   Handle* h = (Handle*)p;

@@ -183,8 +183,8 @@ void Elliptic::PartialAx(const double* q, double* Aq, bool parallel) const {
 /* elliptic_t::Operator, C0 (ellipticOperator.cpp:29-107): Aq = Q^T A_L Q q. */
 void Elliptic::Operator(const double* q, double* Aq, std::vector<double>& AqL) const {
   AqL.resize((size_t)mesh.Nelements * mesh.Np);
-  PartialAx(q, AqL.data());
-  ogsMasked.gatherLocal.GatherAdd(Aq, AqL.data(), true);
+  PartialAx(q, AqL.data(), parallel);
+  ogsMasked.gatherLocal.GatherAdd(Aq, AqL.data(), true, parallel);
 }
 
 /* ellipticBuildOperatorDiagonal.cpp:53-78 + :998-1057 (hex C0). */

@@ -163,10 +163,13 @@ void OgsOperator::setupRowBlocks() {
 /* Host gather with Op=Add (ogsOperator.cpp:64-113): sum from zero in
  * ascending column order. trans=true uses the T (all copies) rows. */
 template <typename T>
-void OgsOperator::GatherAdd(T* gv, const T* v, bool trans) const {
+void OgsOperator::GatherAdd(T* gv, const T* v, bool trans, bool parallel) const {
   const dlong Nrows = trans ? NrowsT : NrowsN;
   const dlong* rs = trans ? rowStartsT.data() : rowStartsN.data();
   const dlong* ci = trans ? colIdsT.data() : colIdsN.data();
+  // rows are independent: the reference's host loop runs them under OpenMP
+  // (ogsOperator.cpp:87); each row still sums in ascending column order
+#pragma omp parallel for schedule(static) if (parallel)
   for (dlong n = 0; n < Nrows; ++n) {
     T val = 0;
     for (dlong g = rs[n]; g < rs[n + 1]; ++g) val += v[ci[g]];
@@ -184,8 +187,8 @@ void OgsOperator::Scatter(T* v, const T* gv, bool trans) const {
     for (dlong g = rs[n]; g < rs[n + 1]; ++g) v[ci[g]] = gv[n];
 }
 
-template void OgsOperator::GatherAdd<double>(double*, const double*, bool) const;
-template void OgsOperator::GatherAdd<float>(float*, const float*, bool) const;
+template void OgsOperator::GatherAdd<double>(double*, const double*, bool, bool) const;
+template void OgsOperator::GatherAdd<float>(float*, const float*, bool, bool) const;
 template void OgsOperator::Scatter<double>(double*, const double*, bool) const;
 template void OgsOperator::Scatter<float>(float*, const float*, bool) const;
 template void OgsOperator::Scatter<dlong>(dlong*, const dlong*, bool) const;

@@ -126,7 +126,7 @@ struct OgsOperator {
   std::vector<dlong> blockRowStartsN, blockRowStartsT;
   static const int gatherNodesPerBlock = 512;   // include/ogs/ogsOperator.hpp:122-123
   void setupRowBlocks();                        // ogsOperator.cpp:495-560
-  template <typename T> void GatherAdd(T* gv, const T* v, bool trans) const;  // :64-113
+  template <typename T> void GatherAdd(T* gv, const T* v, bool trans, bool parallel = false) const;  // :64-113
   template <typename T> void Scatter(T* v, const T* gv, bool trans) const;     // :225-266
 };
 
@@ -168,6 +168,7 @@ struct Elliptic {
   std::vector<double> weight, weightG;
   std::vector<dlong> GlobalToLocal;
   dlong Ndofs = 0;
+  bool parallel = false;  // Operator runs the element and row loops under OpenMP (CPU baseline timing)
 
   void Setup(const Mesh& mesh, const Settings& s, double lambda, GlibcRand& rng);
   void BoundarySetup(GlibcRand& rng);            // ellipticBoundarySetup.cpp:30-140

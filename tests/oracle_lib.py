@@ -66,6 +66,7 @@ def load():
             "orc_drand48_seq": (None, [c_long, c_int, c_void_p]),
             "orc_hess_eigs": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
             "orc_time_partial_ax": (c_double, [c_void_p, c_int, c_int]),
+            "orc_time_leg": (c_double, [c_void_p, c_int, c_int, c_int, ctypes.POINTER(c_int)]),
             "orc_partial_ax_omp": (None, [c_void_p, c_void_p, c_void_p]),
             "orc_omp_threads": (c_int, []),
             "orc_learned_trace": (c_long, [c_void_p, c_void_p]),

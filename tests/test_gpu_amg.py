@@ -154,3 +154,109 @@ def test_pmg_with_amg_base_level():
     assert ra["r_final"] <= 1e-8 * ra["r0"] * (1 + 1e-12)
     assert abs(ra["norm"] - rb["norm"]) < 1e-7 * rb["norm"]
     assert ra["iterations"] <= rb["iterations"] + 3
+
+
+# ---------------------------------------------------------------------------
+# against the oracle's restatement of parAlmond (oracle/amg.cpp, pinned in
+# tests/test_oracle_amg.py): aggregates bit for bit, matrices and rho to rounding
+# ---------------------------------------------------------------------------
+def cube(n, **extra):
+    s = {"MESH FILE": "BOX", "MESH DIMENSION": 3, "ELEMENT TYPE": 12, "BOX NX": n, "BOX NY": n, "BOX NZ": n,
+         "BOX DIMX": 2, "BOX DIMY": 2, "BOX DIMZ": 2, "BOX BOUNDARY FLAG": 1, "POLYNOMIAL DEGREE": 1,
+         "LAMBDA": 0.0, "DATA FILE": "data/ellipticSine3D.h", "BOX CURVED MAP AMPLITUDE": 0.05}
+    s.update(extra)
+    return s
+
+
+def compare_hierarchies(el, o, which):
+    """Level by level: sizes, aggregates (bit for bit), A, P, R (1e-12) and rho (1e-10)."""
+    nl = o.amg_levels(which)
+    info = el.amg_info() if which == 0 else None
+    if info is not None:
+        assert info["levels"] == nl
+    for l in range(nl):
+        for m in ("A",) + (("P", "R") if l + 1 < nl else ()):
+            rs, ci, v = (el.amg_level_array(l, m + "_" + k, which) for k in ("rs", "ci", "v"))
+            ors, oci, ov = (o.amg_level_array(l, m + "_" + k, which) for k in ("rs", "ci", "v"))
+            assert np.array_equal(rs, ors) and np.array_equal(ci, oci), (l, m)
+            assert np.max(np.abs(v - ov)) <= 1e-12 * np.max(np.abs(ov)), (l, m)
+        rho = el.amg_level_array(l, "rho", which)[0]
+        assert abs(rho - o.amg_level_d(l, "rho", which)) <= 1e-10 * rho
+        if l + 1 < nl:
+            agg = el.amg_level_array(l, "aggregate", which)
+            assert np.array_equal(agg, o.amg_level_array(l, "FineToCoarse", which).astype(np.int32)), l
+
+
+@pytest.mark.parametrize("opts", [{}, {"PARALMOND AGGREGATION": "UNSMOOTHED", "PARALMOND SMOOTHER": "DAMPEDJACOBI"},
+                                  {"PARALMOND STRENGTH": "RUGESTUBEN", "PARALMOND CHEBYSHEV DEGREE": 3}])
+def test_amg_setup_and_cycle_match_oracle(opts):
+    from tests.oracle_lib import OracleElliptic
+    s = cube(16, **opts)
+    o = OracleElliptic(s)
+    o.amg_setup(0)
+    el = lp().Elliptic(s)
+    el.amg_setup()
+    compare_hierarchies(el, o, 0)
+    n = el.geti("Ngather")
+    b = np.random.default_rng(4).standard_normal(n)
+    # one smoothing from zero and from a guess, then the whole V-cycle
+    for x0, zero in ((np.zeros(n), True), (np.random.default_rng(5).standard_normal(n), False)):
+        xd = dev(x0)
+        el.amg_smooth(0, dev(b), xd, zero)
+        ref = o.amg_smooth(0, b, x0, zero)
+        assert np.max(np.abs(xd.cpu().numpy() - ref)) <= 1e-12 * np.max(np.abs(ref))
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    el.amg_vcycle(dev(b), x)
+    sync(el)
+    ref = o.amg_vcycle(b)
+    assert np.max(np.abs(x.cpu().numpy() - ref)) <= 1e-11 * np.max(np.abs(ref))
+
+
+def test_amg_rejects_unsupported_cycle():
+    el = lp().Elliptic(cube(4, **{"PARALMOND CYCLE": "KCYCLE"}))
+    with pytest.raises(lp().LibpError, match="VCYCLE only"):
+        el.amg_setup()
+
+
+def test_pmg_amg_base_matches_oracle_harness_settings():
+    """testEllipticHex_C0_Multigrid as the reference composes it: the harness settings
+    (test/testElliptic.py:34-72, PARALMOND AGGREGATION UNSMOOTHED) with the p-MG
+    ending in parAlmond AMG. The base level's hierarchy matches the oracle's (the
+    drand48 stream continues from the p-MG levels' Arnoldi starts), the solve takes
+    the oracle's iterations +-1 and gives its norm to 1e-10."""
+    from tests import oracle_lib
+    from tests.oracle_lib import OracleElliptic
+    from tests.test_oracle import ref_settings
+    s = ref_settings(PRECONDITIONER="MULTIGRID", **{"LINEAR SOLVER": "FPCG", "MULTIGRID COARSE SOLVER": "AMG",
+                                                    "MULTIGRID PRECISION": "FP64"})
+    o = OracleElliptic(dict(s, **{"MULTIGRID COARSENING": "CUSTOM", "MULTIGRID DEGREES": "4,2,1"}))
+    o.mg_setup()
+    el = lp().Elliptic(dict(s, **{"MULTIGRID COARSENING": "CUSTOM", "MULTIGRID DEGREES": "4,2,1"}))
+    compare_hierarchies(el, o, 1)
+    onorm, oit, _ = oracle_lib.run(s)
+    r = lp().Elliptic(s).run()
+    print(f"harness p-MG + AMG: GPU {r['iterations']} its norm {r['norm']:.15f}; oracle {oit} its {onorm:.15f}",
+          flush=True)
+    assert abs(r["iterations"] - oit) <= 1
+    assert abs(r["norm"] - onorm) <= 1e-10 * onorm
+    assert abs(r["norm"] - 0.353553400508458) < 1e-5
+
+
+def test_pmg_amg_base_hierarchy_and_vcycle_match_oracle():
+    """Curved 12^3 box, p-MG 3 -> 1 in FP64 with the AMG base level (1331 rows, one
+    coarsening): the base hierarchy matches the oracle's and so does one V-cycle."""
+    from tests.oracle_lib import OracleElliptic
+    s = cube(12, **{"POLYNOMIAL DEGREE": 3, "LAMBDA": 1.0, "PRECONDITIONER": "MULTIGRID", "LINEAR SOLVER": "FPCG",
+                    "MULTIGRID COARSENING": "CUSTOM", "MULTIGRID DEGREES": "3,1", "MULTIGRID PRECISION": "FP64",
+                    "MULTIGRID COARSE SOLVER": "AMG", "PARALMOND AGGREGATION": "UNSMOOTHED"})
+    o = OracleElliptic(s)
+    o.mg_setup()
+    assert o.amg_levels(1) == 2
+    el = lp().Elliptic(s)
+    compare_hierarchies(el, o, 1)
+    r = np.random.default_rng(8).standard_normal(o.geti("Ngather"))
+    z = torch.zeros(r.size, dtype=torch.float64, device="cuda")
+    el.precon(dev(r), z)
+    sync(el)
+    ref = o.mg_apply(r)
+    assert np.max(np.abs(z.cpu().numpy() - ref)) <= 1e-10 * np.max(np.abs(ref))

@@ -264,10 +264,40 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="also time a 1 GiB device-to-device copy (achievable HBM bandwidth; "
                          "the reference kernel for scaling FETCH_SIZE/WRITE_SIZE counters)")
+    ap.add_argument("--counters", action="store_true",
+                    help="only a few launches of the 48^3 Ax, split operator and gathers plus a 1 GiB probe copy "
+                         "(the program for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes)")
     ap.add_argument("--n15-only", action="store_true",
                     help="only the N=15 MFMA Ax of configuration 4 (for counter runs; with --copy-probe for the "
                          "traffic calibration)")
     args = ap.parse_args()
+    if args.counters:
+        # a few launches of each 48^3 operator kernel plus a 1 GiB probe copy whose
+        # bytes are known: the program that FETCH_SIZE / WRITE_SIZE passes profile
+        import torch
+
+        import libparanumal_amd as lp
+        el = lp.Elliptic(workload_settings(nx=args.nx))
+        L = lp.lib()
+        Ng, NL = el.geti("Ngather"), el.geti("Nelements") * el.geti("Np")
+        q = torch.rand(Ng, dtype=torch.float64, device="cuda") * 2 - 1
+        AqL = torch.empty(NL, dtype=torch.float64, device="cuda")
+        Aq = torch.empty(Ng, dtype=torch.float64, device="cuda")
+        rs, _, _, _ = el.array_info("rowStartsT")
+        ci, _, _, _ = el.array_info("colIdsT")
+        for _ in range(3):
+            el.partial_ax(q, AqL)
+            lp.check(L.libp_ogs_gather_f64(Ng, rs, ci, AqL.data_ptr(), Aq.data_ptr(), el.stream))
+            el.operator(q, Aq)
+        n = (1 << 30) // 8
+        xs = torch.rand(n, dtype=torch.float64, device="cuda")
+        ys = torch.empty_like(xs)
+        for _ in range(3):
+            lp.check(L.libp_hbm_probe_f64(1, n, xs.data_ptr(), ys.data_ptr(), el.stream))
+        torch.cuda.synchronize()
+        print(json.dumps({"counters_run": True, "gathered_dofs": Ng, "local_nodes": NL,
+                          "face_rows": el.geti("faceRows"), "probe_copy_bytes": 2 * 8 * n}))
+        return
     if args.n15_only:
         import torch
 

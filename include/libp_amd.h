@@ -49,12 +49,16 @@ int libp_ax_hex_partial_f64(int N, int Nelements, const int* elementList, const 
  * ellipticAxHex3D.okl:156-295 followed by the ogs gather, ogsKernels.okl:86-122)
  * in a split form, N <= 14. libp_ax_hex_split_f64 stores the rows of
  * element-interior nodes (one copy each) straight into the gathered Aq and the
- * element-boundary nodes into AqFace, e * Nface + the node's place among the
- * element's boundary nodes in local order (Nface = libp_ax_hex_face_nodes(N));
- * libp_ogs_gather_face_f64 then sums the remaining rows, whose copies are listed
- * as AqFace indices in ascending local order (faceRowStarts over the NfaceRows
- * rows faceRows). The result is bit-identical to the partial Ax and the full
- * gather. papPartials, when not null, receives libp_ax_hex_split_blocks(N,
+ * element-boundary nodes into AqFace; libp_ogs_gather_face_f64 then sums the
+ * remaining rows (faceRowStarts over the NfaceRows rows faceRows, copies in
+ * ascending local order). Two layouts of AqFace: with faceSlots null, boundary
+ * node f of element e (f = its place among the element's Nface =
+ * libp_ax_hex_face_nodes(N) boundary nodes in local order) goes to e * Nface + f
+ * and faceCols lists each row's copies as such indices; with faceSlots (one int
+ * per e * Nface + f) it goes to AqFace[faceSlots[...]], the copy's place in the
+ * row-ordered copy list, and faceCols is null (row f's copies are AqFace[
+ * faceRowStarts[f] .. faceRowStarts[f+1])). The result is bit-identical to the
+ * partial Ax and the full gather. papPartials, when not null, receives libp_ax_hex_split_blocks(N,
  * Nelements) partial sums of q.Aq over the interior rows for
  * libp_pcg_gather_face_pap_f64, which gathers the boundary rows, adds their
  * p.Ap partials after those and sets scalars[PAP] and scalars[ALPHA] as
@@ -68,7 +72,7 @@ int libp_ax_hex_face_nodes(int N);
 int libp_ax_hex_split_blocks(int N, int Nelements);
 int libp_ax_hex_split_f64(int N, int Nelements, const int* elementList, const int* GlobalToLocal, const double* wJ,
                           const double* ggeo, const double* D, double lambda, const double* q, double* Aq,
-                          double* AqFace, double* papPartials, void* stream);
+                          double* AqFace, const int* faceSlots, double* papPartials, void* stream);
 int libp_ogs_gather_face_blocks(int NfaceRows);
 int libp_ogs_gather_face_f64(int NfaceRows, const int* faceRows, const int* faceRowStarts, const int* faceCols,
                              const double* AqFace, double* Aq, void* stream);

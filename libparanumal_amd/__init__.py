@@ -33,7 +33,7 @@ SIGNATURES = {
     "libp_hbm_probe_f64": (c_int, [c_int, c_long, c_void_p, c_void_p, c_void_p]),
     "libp_ax_hex_face_nodes": (c_int, [c_int]),
     "libp_ax_hex_split_blocks": (c_int, [c_int, c_int]),
-    "libp_ax_hex_split_f64": (c_int, [c_int, c_int] + [c_void_p] * 5 + [c_double] + [c_void_p] * 5),
+    "libp_ax_hex_split_f64": (c_int, [c_int, c_int] + [c_void_p] * 5 + [c_double] + [c_void_p] * 6),
     "libp_ogs_gather_face_blocks": (c_int, [c_int]),
     "libp_ogs_gather_face_f64": (c_int, [c_int] + [c_void_p] * 6),
     "libp_pcg_gather_face_pap_f64": (c_int, [c_int] + [c_void_p] * 6 + [c_int] + [c_void_p] * 3),

@@ -152,6 +152,11 @@ settings_t ellipticDefaultSettings() {
                "FP64 operator with element-interior rows written by the Ax kernel and only the element-boundary "
                "rows gathered (bit-identical to the two-pass Ax + gather)",
                {"TRUE", "FALSE"});
+  s.newSetting("OPERATOR FACE LAYOUT", "SLOT",
+               "Split operator: element-boundary results stored in the gather's row order (SLOT: the Ax kernel "
+               "scatters, the gather reads contiguous copies) or in element order (ELEMENT: the gather reads "
+               "through column ids)",
+               {"SLOT", "ELEMENT"});
   s.newSetting("MULTIGRID COARSE SINGLE REDUCTION", "TRUE",
                "Fused coarse Jacobi-PCG in its single-reduction (Chronopoulos-Gear) form: two launches per "
                "iteration instead of three",

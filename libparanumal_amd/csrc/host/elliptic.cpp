@@ -126,7 +126,9 @@ splitGather_t* elliptic_t::SplitGather() {
       }
   const memory<dlong>& rs = ogsMasked.rowStartsT;
   const memory<dlong>& ci = ogsMasked.colIdsT;
-  std::vector<dlong> rows, starts{0}, cols;
+  const bool slotLayout = !settings.compareSetting("OPERATOR FACE LAYOUT", "ELEMENT");
+  const size_t NfaceL = (size_t)mesh.Nelements * Nface;
+  std::vector<dlong> rows, starts{0}, cols, slots;
   for (dlong r = 0; r < ogsMasked.Ngather; ++r) {
     const dlong a = rs[r], b = rs[r + 1];
     if (b - a == 1 && faceIdx[ci[a] % Np] < 0) continue;  // an interior node: written by the Ax kernel
@@ -138,16 +140,30 @@ splitGather_t* elliptic_t::SplitGather() {
     }
     starts.push_back((dlong)cols.size());
   }
+  const dlong Ncopies = (dlong)cols.size();
+  if (slotLayout) {
+    // each boundary node's slot in the row-ordered copy list; masked nodes (no row)
+    // all write one trash slot past the end
+    slots.assign(NfaceL, Ncopies);
+    for (dlong s = 0; s < Ncopies; ++s) slots[cols[s]] = s;
+  }
   auto sp = std::make_shared<splitGather_t>();
   sp->Nface = Nface;
   sp->NfaceRows = (dlong)rows.size();
+  sp->slotLayout = slotLayout;
   sp->o_rows = platform.malloc<dlong>(std::max<size_t>(rows.size(), 1));
   if (!rows.empty()) sp->o_rows.copyFrom(rows.data(), rows.size(), 0, platform.stream);
   sp->o_rowStarts = platform.malloc<dlong>(starts.size());
   sp->o_rowStarts.copyFrom(starts.data(), starts.size(), 0, platform.stream);
-  sp->o_cols = platform.malloc<dlong>(std::max<size_t>(cols.size(), 1));
-  if (!cols.empty()) sp->o_cols.copyFrom(cols.data(), cols.size(), 0, platform.stream);
-  sp->o_AqFace = platform.malloc<dfloat>((size_t)mesh.Nelements * Nface);
+  if (slotLayout) {
+    sp->o_slots = platform.malloc<dlong>(std::max<size_t>(NfaceL, 1));
+    if (NfaceL) sp->o_slots.copyFrom(slots.data(), NfaceL, 0, platform.stream);
+    sp->o_AqFace = platform.malloc<dfloat>((size_t)Ncopies + 1);
+  } else {
+    sp->o_cols = platform.malloc<dlong>(std::max<size_t>(cols.size(), 1));
+    if (!cols.empty()) sp->o_cols.copyFrom(cols.data(), cols.size(), 0, platform.stream);
+    sp->o_AqFace = platform.malloc<dfloat>(NfaceL);
+  }
   sp->nAxPartials = libp_ax_hex_split_blocks(N, mesh.NlocalGatherElements);
   sp->o_papPart = platform.malloc<double>(sp->nAxPartials + libp_ogs_gather_face_blocks(sp->NfaceRows) + 1);
   platform.finish();
@@ -159,7 +175,7 @@ void elliptic_t::Operator(deviceMemory<dfloat>& o_q, deviceMemory<dfloat>& o_Aq)
   if (splitGather_t* sp = SplitGather()) {
     LIBP_CALL(libp_ax_hex_split_f64(mesh.N, mesh.NlocalGatherElements, nullptr, o_GlobalToLocal.ptr(), mesh.o_wJ.ptr(),
                                     mesh.o_ggeo.ptr(), mesh.o_D.ptr(), lambda, o_q.ptr(), o_Aq.ptr(),
-                                    sp->o_AqFace.ptr(), nullptr, platform.s()));
+                                    sp->o_AqFace.ptr(), sp->o_slots.ptr(), nullptr, platform.s()));
     LIBP_CALL(libp_ogs_gather_face_f64(sp->NfaceRows, sp->o_rows.ptr(), sp->o_rowStarts.ptr(), sp->o_cols.ptr(),
                                        sp->o_AqFace.ptr(), o_Aq.ptr(), platform.s()));
     return;
@@ -172,7 +188,7 @@ bool elliptic_t::OperatorPAP(deviceMemory<dfloat>& o_q, deviceMemory<dfloat>& o_
   if (splitGather_t* sp = SplitGather()) {
     LIBP_CALL(libp_ax_hex_split_f64(mesh.N, mesh.NlocalGatherElements, nullptr, o_GlobalToLocal.ptr(), mesh.o_wJ.ptr(),
                                     mesh.o_ggeo.ptr(), mesh.o_D.ptr(), lambda, o_q.ptr(), o_Aq.ptr(),
-                                    sp->o_AqFace.ptr(), sp->o_papPart.ptr(), platform.s()));
+                                    sp->o_AqFace.ptr(), sp->o_slots.ptr(), sp->o_papPart.ptr(), platform.s()));
     LIBP_CALL(libp_pcg_gather_face_pap_f64(sp->NfaceRows, sp->o_rows.ptr(), sp->o_rowStarts.ptr(), sp->o_cols.ptr(),
                                            sp->o_AqFace.ptr(), o_Aq.ptr(), o_q.ptr(), sp->nAxPartials,
                                            sp->o_papPart.ptr(), scalars, platform.s()));

@@ -17,14 +17,17 @@ struct ellipticRunResult_t {
 };
 
 /* Maps of the split operator (libp_ax_hex_split_f64 + libp_ogs_gather_face_f64):
- * the gathered rows that have element-boundary copies, their copies as indices
- * into the compact face E-vector (e * Nface + face index, ascending local order as
- * in the ogs CSR), the face E-vector and the p.Ap partials of the fused PCG step.
+ * the gathered rows that have element-boundary copies and, in the slot layout,
+ * each boundary node's place in the row-ordered list of copies (ascending local
+ * order within a row, as in the ogs CSR), or in the element layout each row's
+ * copies as indices e * Nface + face index; the face E-vector and the p.Ap
+ * partials of the fused PCG step.
  * Rows of element-interior nodes have one copy and are written by the Ax kernel. */
 struct splitGather_t {
   int Nface = 0;
   dlong NfaceRows = 0;
-  deviceMemory<dlong> o_rows, o_rowStarts, o_cols;
+  bool slotLayout = true;  // AqFace in row-ordered copy slots (o_slots) instead of element order (o_cols)
+  deviceMemory<dlong> o_rows, o_rowStarts, o_cols, o_slots;
   deviceMemory<dfloat> o_AqFace;
   deviceMemory<double> o_papPart;
   int nAxPartials = 0;

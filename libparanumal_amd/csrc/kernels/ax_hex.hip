@@ -79,7 +79,8 @@ __global__ void __launch_bounds__(NE * Nq * Nq) __attribute__((amdgpu_waves_per_
     ax_hex_kernel(const int e0, const int* __restrict__ elementList, const int* __restrict__ G2L,
                   const T* __restrict__ wJ, const T* __restrict__ ggeo, const T* __restrict__ D, const T lambda,
                   const T* __restrict__ q, T* __restrict__ Aq, T* __restrict__ Ag = nullptr,
-                  double* __restrict__ pap = nullptr, const int pap0 = 0) {
+                  double* __restrict__ pap = nullptr, const int pap0 = 0,
+                  const int* __restrict__ fslot = nullptr) {
   using S = AxShape<Nq, T>;
   constexpr int NqNq = S::NqNq, Np = S::Np, threads = NE * NqNq;
   __shared__ T s_D[Nq][Nq + 1];
@@ -177,7 +178,11 @@ __global__ void __launch_bounds__(NE * Nq * Nq) __attribute__((amdgpu_waves_per_
         const int id = s_id[es][k][ij];
         if (id >= 0) Ag[id] = v;
       } else {
-        __builtin_nontemporal_store(v, fout + hex_face_index(i, j, k, Nq));
+        const int fi = hex_face_index(i, j, k, Nq);
+        if (fslot)  // slot layout: the copy's place in the row-ordered copy list
+          Aq[__builtin_nontemporal_load(fslot + (size_t)element * Nface + fi)] = v;
+        else
+          __builtin_nontemporal_store(v, fout + fi);
       }
     }
     if constexpr (PAP) {
@@ -228,7 +233,7 @@ static int split_blocks(int Nelements) {
 template <int Nq>
 static int launch_ax_split(int Nelements, const int* elist, const int* g2l, const double* wJ, const double* ggeo,
                            const double* D, double lambda, const double* q, double* AqF, double* Ag, double* pap,
-                           hipStream_t st) {
+                           const int* fslot, hipStream_t st) {
   using S = AxShape<Nq, double>;
   if (Nelements <= 0) return LIBP_OK;
   const int full = Nelements / S::NE;
@@ -236,17 +241,17 @@ static int launch_ax_split(int Nelements, const int* elist, const int* g2l, cons
   if (pap) {
     if (full > 0)
       hipLaunchKernelGGL((ax_hex_kernel<Nq, S::NE, double, 0, 1, true>), dim3(full), dim3(S::threads), 0, st, 0,
-                         elist, g2l, wJ, ggeo, D, lambda, q, AqF, Ag, pap, 0);
+                         elist, g2l, wJ, ggeo, D, lambda, q, AqF, Ag, pap, 0, fslot);
     if (S::NE > 1 && rest > 0)
       hipLaunchKernelGGL((ax_hex_kernel<Nq, 1, double, 0, 1, true>), dim3(rest), dim3(S::NqNq), 0, st,
-                         full * S::NE, elist, g2l, wJ, ggeo, D, lambda, q, AqF, Ag, pap, full);
+                         full * S::NE, elist, g2l, wJ, ggeo, D, lambda, q, AqF, Ag, pap, full, fslot);
   } else {
     if (full > 0)
       hipLaunchKernelGGL((ax_hex_kernel<Nq, S::NE, double, 0, 1, false>), dim3(full), dim3(S::threads), 0, st, 0,
-                         elist, g2l, wJ, ggeo, D, lambda, q, AqF, Ag, nullptr, 0);
+                         elist, g2l, wJ, ggeo, D, lambda, q, AqF, Ag, nullptr, 0, fslot);
     if (S::NE > 1 && rest > 0)
       hipLaunchKernelGGL((ax_hex_kernel<Nq, 1, double, 0, 1, false>), dim3(rest), dim3(S::NqNq), 0, st,
-                         full * S::NE, elist, g2l, wJ, ggeo, D, lambda, q, AqF, Ag, nullptr, 0);
+                         full * S::NE, elist, g2l, wJ, ggeo, D, lambda, q, AqF, Ag, nullptr, 0, fslot);
   }
   return launch_status("ax_hex_split");
 }
@@ -331,15 +336,15 @@ extern "C" int libp_ax_hex_face_nodes(int N) { return (N >= 1 && N <= 15) ? hex_
 
 extern "C" int libp_ax_hex_split_f64(int N, int Nelements, const int* elementList, const int* GlobalToLocal,
                                      const double* wJ, const double* ggeo, const double* D, double lambda,
-                                     const double* q, double* Aq, double* AqFace, double* papPartials,
-                                     void* stream) {
+                                     const double* q, double* Aq, double* AqFace, const int* faceSlots,
+                                     double* papPartials, void* stream) {
   if (!GlobalToLocal || !wJ || !ggeo || !D || !q || !Aq || !AqFace) return arg_error("ax_hex_split", "null pointer");
   hipStream_t st = as_stream(stream);
   switch (N) {
 #define LIBP_AX_CASE(NN) \
   case NN:               \
     return launch_ax_split<NN + 1>(Nelements, elementList, GlobalToLocal, wJ, ggeo, D, lambda, q, AqFace, Aq, \
-                                   papPartials, st);
+                                   papPartials, faceSlots, st);
     LIBP_AX_CASE(1)
     LIBP_AX_CASE(2)
     LIBP_AX_CASE(3)

@@ -237,10 +237,17 @@ __global__ void __launch_bounds__(256) ogs_gather_face_kernel(const int Nrows, c
     }
     int c[2][8];
     double x[2][8];
+    if (colIds) {
 #pragma unroll
-    for (int r = 0; r < 2; ++r)
+      for (int r = 0; r < 2; ++r)
 #pragma unroll
-      for (int k = 0; k < 8; ++k) c[r][k] = (k < len[r]) ? colIds[a[r] + k] : 0;
+        for (int k = 0; k < 8; ++k) c[r][k] = (k < len[r]) ? colIds[a[r] + k] : 0;
+    } else {  // slot layout: the row's copies are v[a .. a + len)
+#pragma unroll
+      for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) c[r][k] = a[r] + k;
+    }
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
@@ -252,7 +259,7 @@ __global__ void __launch_bounds__(256) ogs_gather_face_kernel(const int Nrows, c
 #pragma unroll
       for (int k = 0; k < 8; ++k)
         if (k < len[r]) val += x[r][k];
-      for (int g = a[r] + 8; g < a[r] + len[r]; ++g) val += v[colIds[g]];
+      for (int g = a[r] + 8; g < a[r] + len[r]; ++g) val += v[colIds ? colIds[g] : g];
       gv[row[r]] = val;
       if (PAP) acc += yv[r] * val;
     }
@@ -396,7 +403,7 @@ extern "C" int libp_ogs_gather_face_blocks(int NfaceRows) { return NfaceRows > 0
 extern "C" int libp_ogs_gather_face_f64(int NfaceRows, const int* faceRows, const int* faceRowStarts,
                                         const int* faceCols, const double* AqFace, double* Aq, void* stream) {
   if (NfaceRows <= 0) return LIBP_OK;
-  if (!faceRows || !faceRowStarts || !faceCols || !AqFace || !Aq) return arg_error("ogs_gather_face", "null pointer");
+  if (!faceRows || !faceRowStarts || !AqFace || !Aq) return arg_error("ogs_gather_face", "null pointer");
   hipLaunchKernelGGL(ogs_gather_face_kernel<false>, dim3(face_gather_blocks(NfaceRows)), dim3(256), 0,
                      as_stream(stream), NfaceRows, faceRows, faceRowStarts, faceCols, AqFace, Aq, nullptr, nullptr,
                      nullptr);
@@ -406,7 +413,7 @@ extern "C" int libp_ogs_gather_face_f64(int NfaceRows, const int* faceRows, cons
 extern "C" int libp_pcg_gather_face_pap_f64(int NfaceRows, const int* faceRows, const int* faceRowStarts,
                                             const int* faceCols, const double* AqFace, double* Ap, const double* p,
                                             int nAxPartials, double* partials, double* scalars, void* stream) {
-  if (!faceRows || !faceRowStarts || !faceCols || !AqFace || !Ap || !p || !partials || !scalars || nAxPartials < 0 ||
+  if (!faceRows || !faceRowStarts || !AqFace || !Ap || !p || !partials || !scalars || nAxPartials < 0 ||
       NfaceRows < 0)
     return arg_error("pcg_gather_face_pap", "null pointer or negative count");
   const unsigned blocks = NfaceRows > 0 ? face_gather_blocks(NfaceRows) : 0;

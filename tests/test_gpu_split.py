@@ -36,11 +36,12 @@ def lp():
 
 
 @pytest.mark.parametrize("N", list(range(1, 15)))
-@pytest.mark.parametrize("flag", [1, 2])
-def test_split_operator_bit_identical_to_two_pass(N, flag):
+@pytest.mark.parametrize("flag,layout", [(1, "SLOT"), (2, "SLOT"), (1, "ELEMENT"), (2, "ELEMENT")])
+def test_split_operator_bit_identical_to_two_pass(N, flag, layout):
     nx = 3 if N <= 8 else 2
     s = box(nx, N, flag=flag)
-    a = lp().Elliptic(dict(s, **{"OPERATOR SPLIT GATHER": "TRUE"}))
+    a = lp().Elliptic(dict(s, **{"OPERATOR SPLIT GATHER": "TRUE", "OPERATOR FACE LAYOUT": layout}))
+    assert a.geti("splitOperator") == 1
     b = lp().Elliptic(dict(s, **{"OPERATOR SPLIT GATHER": "FALSE"}))
     n = a.geti("Ngather")
     q = torch.rand(n, dtype=torch.float64, device="cuda", generator=torch.Generator("cuda").manual_seed(N)) - 0.5
@@ -80,7 +81,8 @@ def face_maps(o):
 
 
 @pytest.mark.parametrize("N", [1, 3, 7])
-def test_split_kernels_through_c_abi_vs_oracle(N):
+@pytest.mark.parametrize("layout", ["ELEMENT", "SLOT"])
+def test_split_kernels_through_c_abi_vs_oracle(N, layout):
     """libp_ax_hex_split_f64 / libp_ogs_gather_face_f64 / libp_pcg_gather_face_pap_f64
     called directly with the oracle's arrays and maps built here."""
     o = OracleElliptic(box(3, N))
@@ -96,23 +98,36 @@ def test_split_kernels_through_c_abi_vs_oracle(N):
     g2l, wJ, ggeo, D = dev(o.array("GlobalToLocal")), dev(o.array("wJ")), dev(o.array("ggeo")), dev(o.array("D"))
     qd = dev(q)
     Aq = torch.full((Ng,), float("nan"), dtype=torch.float64, device="cuda")
-    AqF = torch.full((E * Nface,), float("nan"), dtype=torch.float64, device="cuda")
+    ncop = cols.size
+    if layout == "SLOT":  # each boundary node's place in the row-ordered copy list; masked nodes -> trash slot
+        slots = np.full(E * Nface, ncop, np.int32)
+        slots[cols] = np.arange(ncop, dtype=np.int32)
+        sl = dev(slots)
+        AqF = torch.full((ncop + 1,), float("nan"), dtype=torch.float64, device="cuda")
+    else:
+        sl = None
+        AqF = torch.full((E * Nface,), float("nan"), dtype=torch.float64, device="cuda")
     nax = L.libp_ax_hex_split_blocks(N, E)
     nf = L.libp_ogs_gather_face_blocks(rows.size)
     part = torch.zeros(nax + nf, dtype=torch.float64, device="cuda")
     st = torch.cuda.current_stream().cuda_stream
     lp().check(L.libp_ax_hex_split_f64(N, E, None, g2l.data_ptr(), wJ.data_ptr(), ggeo.data_ptr(), D.data_ptr(), 0.7,
-                                       qd.data_ptr(), Aq.data_ptr(), AqF.data_ptr(), part.data_ptr(), st))
+                                       qd.data_ptr(), Aq.data_ptr(), AqF.data_ptr(),
+                                       sl.data_ptr() if sl is not None else None, part.data_ptr(), st))
     torch.cuda.synchronize()
-    # face E-vector = the oracle's E-vector at the boundary nodes
+    # face E-vector = the oracle's E-vector at the boundary nodes (in row order for the slot layout)
     AqLe = AqL.reshape(E, Np)
     face_ref = AqLe[:, ~interior].ravel()
     got = AqF.cpu().numpy()
+    if layout == "SLOT":
+        face_ref, got = face_ref[cols], got[:ncop]
     assert np.max(np.abs(got - face_ref)) <= 1e-12 * np.max(np.abs(AqL))
-    rd, sd, cd = dev(rows), dev(starts), dev(cols)
+    rd, sd = dev(rows), dev(starts)
+    cd = dev(cols) if layout == "ELEMENT" else None
+    cptr = cd.data_ptr() if cd is not None else None
     scal = torch.zeros(16, dtype=torch.float64, device="cuda")
     scal[0] = 2.0  # r.z of the iteration: alpha = 2 / p.Ap
-    lp().check(L.libp_pcg_gather_face_pap_f64(rows.size, rd.data_ptr(), sd.data_ptr(), cd.data_ptr(), AqF.data_ptr(),
+    lp().check(L.libp_pcg_gather_face_pap_f64(rows.size, rd.data_ptr(), sd.data_ptr(), cptr, AqF.data_ptr(),
                                               Aq.data_ptr(), qd.data_ptr(), nax, part.data_ptr(), scal.data_ptr(), st))
     torch.cuda.synchronize()
     y = Aq.cpu().numpy()
@@ -123,18 +138,19 @@ def test_split_kernels_through_c_abi_vs_oracle(N):
     assert abs(float(scal[2]) - 2.0 / pap) <= 1e-15 * abs(2.0 / pap)
     # the plain face gather gives the same rows
     Aq2 = Aq.clone()
-    lp().check(L.libp_ogs_gather_face_f64(rows.size, rd.data_ptr(), sd.data_ptr(), cd.data_ptr(), AqF.data_ptr(),
+    lp().check(L.libp_ogs_gather_face_f64(rows.size, rd.data_ptr(), sd.data_ptr(), cptr, AqF.data_ptr(),
                                           Aq2.data_ptr(), st))
     torch.cuda.synchronize()
     assert torch.equal(Aq, Aq2)
 
 
 @pytest.mark.parametrize("N", [3, 7])
-def test_split_pcg_matches_two_pass(N):
+@pytest.mark.parametrize("layout", ["SLOT", "ELEMENT"])
+def test_split_pcg_matches_two_pass(N, layout):
     """Jacobi-PCG with the split operator and its fused p.Ap: the same iterates as the
     two-pass form up to the order of the p.Ap partial sums."""
     s = box(4, N, lam=0.0, curve=0.0, **{"LINEAR SOLVER": "PCG", "PRECONDITIONER": "JACOBI"})
-    ra = lp().Elliptic(dict(s, **{"OPERATOR SPLIT GATHER": "TRUE"})).run()
+    ra = lp().Elliptic(dict(s, **{"OPERATOR SPLIT GATHER": "TRUE", "OPERATOR FACE LAYOUT": layout})).run()
     rb = lp().Elliptic(dict(s, **{"OPERATOR SPLIT GATHER": "FALSE"})).run()
     print(f"N={N}: split {ra['iterations']} its {ra['norm']:.15f}; two-pass {rb['iterations']} its {rb['norm']:.15f}",
           flush=True)

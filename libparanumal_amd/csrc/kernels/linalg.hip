@@ -5,6 +5,13 @@
 #include "reduce.hpp"
 
 namespace libp_amd {
+template <typename... P>
+inline bool aligned16(const P*... ptrs) {
+  return (((((uintptr_t)ptrs) & 15) == 0) && ...);
+}
+}  // namespace libp_amd
+
+namespace libp_amd {
 
 /* ------------------------------ reductions ------------------------------ */
 template <int KIND>  // 0 x.y, 1 x.x, 2 sum x, 3 w.x.y
@@ -226,6 +233,104 @@ __global__ void __launch_bounds__(RBS) pcg_update_finish_kernel(const int nb, do
 // and the r.z of the next iteration (:107-124). Its finisher writes r.r,
 // counts the iteration, raises STOP once r.r <= TOL, shifts r.z (RDOTZ2 <-
 // RDOTZ1 <- r.z) and sets beta = r.z / old r.z.
+// The same update with 16-byte accesses: each thread takes two pairs of rows per
+// step (four rows, all loads issued before any store), so four times the bytes of
+// the one-row loop are in flight per wave; the last row of an odd N is done by
+// one thread after the loop. Needs 16-byte aligned vectors.
+typedef double pcg_v2d __attribute__((ext_vector_type(2)));
+
+__device__ inline void jacobi_pair(const pcg_v2d d, const pcg_v2d pv, const pcg_v2d ap, pcg_v2d& xv, pcg_v2d& rv,
+                                   pcg_v2d& zv, const double alpha, double& a0, double& a1) {
+  xv.x += alpha * pv.x;
+  xv.y += alpha * pv.y;
+  rv.x -= alpha * ap.x;
+  rv.y -= alpha * ap.y;
+  zv.x = 1.0 * d.x * rv.x;
+  zv.y = 1.0 * d.y * rv.y;
+  a0 += rv.x * rv.x;
+  a0 += rv.y * rv.y;
+  a1 += rv.x * zv.x;
+  a1 += rv.y * zv.y;
+}
+
+__global__ void __launch_bounds__(RBS) pcg_jacobi_update2_kernel(const int N, const double* __restrict__ invDiag,
+                                                                 const double* __restrict__ p,
+                                                                 const double* __restrict__ Ap,
+                                                                 double* __restrict__ x, double* __restrict__ r,
+                                                                 double* __restrict__ z, const double* __restrict__ s,
+                                                                 ReduceWS* __restrict__ ws) {
+  __shared__ double s_w[16];
+  if (s[8] != 0.0) return;
+  const double alpha = s[2];
+  const long N2 = N / 2, S = (long)gridDim.x * blockDim.x;
+  const pcg_v2d *d2 = (const pcg_v2d*)invDiag, *p2 = (const pcg_v2d*)p, *A2 = (const pcg_v2d*)Ap;
+  pcg_v2d *x2 = (pcg_v2d*)x, *r2 = (pcg_v2d*)r, *z2 = (pcg_v2d*)z;
+  double a0 = 0, a1 = 0;
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  for (; i + S < N2; i += 2 * S) {
+    const pcg_v2d da = d2[i], db = d2[i + S], pa = p2[i], pb = p2[i + S], Aa = A2[i], Ab = A2[i + S];
+    pcg_v2d xa = x2[i], xb = x2[i + S], ra = r2[i], rb = r2[i + S], za, zb;
+    jacobi_pair(da, pa, Aa, xa, ra, za, alpha, a0, a1);
+    jacobi_pair(db, pb, Ab, xb, rb, zb, alpha, a0, a1);
+    x2[i] = xa;
+    x2[i + S] = xb;
+    r2[i] = ra;
+    r2[i + S] = rb;
+    z2[i] = za;
+    z2[i + S] = zb;
+  }
+  for (; i < N2; i += S) {
+    pcg_v2d xa = x2[i], ra = r2[i], za;
+    jacobi_pair(d2[i], p2[i], A2[i], xa, ra, za, alpha, a0, a1);
+    x2[i] = xa;
+    r2[i] = ra;
+    z2[i] = za;
+  }
+  if ((N & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+    const int n = N - 1;
+    double rn = r[n];
+    x[n] += alpha * p[n];
+    rn -= alpha * Ap[n];
+    r[n] = rn;
+    const double zn = 1.0 * invDiag[n] * rn;
+    z[n] = zn;
+    a0 += rn * rn;
+    a1 += rn * zn;
+  }
+  double v[2] = {a0, a1};
+  store_block_partials<2>(ws, v, s_w);
+}
+
+// p = z + beta p with 16-byte accesses, two pairs per thread per step
+__global__ void __launch_bounds__(256) pcg_update_p2_kernel(const int N, const int first, const double* __restrict__ z,
+                                                            double* __restrict__ p, const double* __restrict__ s) {
+  if (s[8] != 0.0) return;
+  const double beta = first ? 0.0 : s[3];
+  const long N2 = N / 2, S = (long)gridDim.x * blockDim.x;
+  const pcg_v2d* z2 = (const pcg_v2d*)z;
+  pcg_v2d* p2 = (pcg_v2d*)p;
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (beta != 0) {
+    for (; i + S < N2; i += 2 * S) {
+      const pcg_v2d za = z2[i], zb = z2[i + S], pa = p2[i], pb = p2[i + S];
+      p2[i] = 1.0 * za + beta * pa;
+      p2[i + S] = 1.0 * zb + beta * pb;
+    }
+    for (; i < N2; i += S) p2[i] = 1.0 * z2[i] + beta * p2[i];
+  } else {
+    for (; i + S < N2; i += 2 * S) {
+      const pcg_v2d za = z2[i], zb = z2[i + S];
+      p2[i] = 1.0 * za;
+      p2[i + S] = 1.0 * zb;
+    }
+    for (; i < N2; i += S) p2[i] = 1.0 * z2[i];
+  }
+  if ((N & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+    const int n = N - 1;
+    p[n] = (beta != 0) ? 1.0 * z[n] + beta * p[n] : 1.0 * z[n];
+  }
+}
+
 __global__ void __launch_bounds__(RBS) pcg_jacobi_update_kernel(const int N, const double* __restrict__ invDiag,
                                                                 const double* __restrict__ p,
                                                                 const double* __restrict__ Ap,
@@ -442,14 +547,23 @@ extern "C" int libp_pcg_jacobi_update_f64(int N, const double* invDiag, const do
                                           double* x, double* r, double* z, double* scalars, void* ws, void* stream) {
   if (!invDiag || !p || !Ap || !x || !r || !z || !scalars || !ws) return arg_error("pcg_jacobi_update", "null pointer");
   const unsigned nb = reduce_blocks(N);
-  hipLaunchKernelGGL(pcg_jacobi_update_kernel, dim3(nb), dim3(RBS), 0, as_stream(stream), N, invDiag, p, Ap, x, r, z,
-                     scalars, (ReduceWS*)ws);
+  if (aligned16(invDiag, p, Ap, x, r, z))
+    hipLaunchKernelGGL(pcg_jacobi_update2_kernel, dim3(nb), dim3(RBS), 0, as_stream(stream), N, invDiag, p, Ap, x, r,
+                       z, scalars, (ReduceWS*)ws);
+  else
+    hipLaunchKernelGGL(pcg_jacobi_update_kernel, dim3(nb), dim3(RBS), 0, as_stream(stream), N, invDiag, p, Ap, x, r,
+                       z, scalars, (ReduceWS*)ws);
   hipLaunchKernelGGL(pcg_jacobi_update_finish_kernel, dim3(1), dim3(RBS), 0, as_stream(stream), (int)nb, scalars,
                      (const ReduceWS*)ws);
   return launch_status("pcg_jacobi_update");
 }
 extern "C" int libp_pcg_update_p_f64(int N, int first_iter, const double* z, double* p, const double* scalars,
                                      void* stream) {
+  if (N > 0 && z && p && scalars && aligned16(z, p)) {
+    hipLaunchKernelGGL(pcg_update_p2_kernel, dim3(stream_blocks((N + 1) / 2, 256 * 4, 8192)), dim3(256), 0,
+                       as_stream(stream), N, first_iter, z, p, scalars);
+    return launch_status("pcg_update_p");
+  }
   LIBP_LAUNCH_1D(pcg_update_p_kernel, N, N, first_iter, z, p, scalars);
 }
 extern "C" int libp_pcg_pap_f64(int N, const double* p, const double* Ap, double* scalars, void* ws, void* stream) {

@@ -255,10 +255,15 @@ def test_learned_pmg_solve_and_gain_over_untrained(trained):
     r0 = q.run()
     print("untrained", r0, q.learned_info(), q.learned_status(), flush=True)
     assert r0["r_final"] <= 1e-8 * r0["r0"] * (1 + 1e-12)
-    if not info["fallback"]:
-        assert r["iterations"] < r0["iterations"]
-    # same discrete solution as the Chebyshev smoother
+    # training gains over the untrained network, whether or not the solve ends on the
+    # Chebyshev fallback (which replaces the network only after it stalls)
+    assert r["iterations"] < r0["iterations"]
+    # same discrete solution as the Chebyshev smoother; the MLP does not beat it in
+    # iterations (it is nonlinear, so FPCG loses its Krylov acceleration), which is
+    # why the linear architecture is the one held to a wall-time gain
     c = lp().Elliptic(solve_settings(**{"MULTIGRID SMOOTHER": "CHEBYSHEV"})).run()
+    print(f"MLP {r['iterations']} its (fallback {int(info['fallback'])}), untrained {r0['iterations']}, "
+          f"Chebyshev {c['iterations']}", flush=True)
     assert abs(r["norm"] - c["norm"]) < 1e-7 * c["norm"]
 
 

@@ -81,6 +81,11 @@ __global__ void __launch_bounds__(NE * Nq * Nq) __attribute__((amdgpu_waves_per_
                   const T* __restrict__ q, T* __restrict__ Aq, T* __restrict__ Ag = nullptr,
                   double* __restrict__ pap = nullptr, const int pap0 = 0,
                   const int* __restrict__ fslot = nullptr) {
+  // a*b + c contracts only within one source expression, so every
+  // instantiation (E-vector, split, split with p.Ap) rounds alike: with
+  // contraction across expressions the backend may fuse x*y + z*w as
+  // fma(x, y, z*w) in one instantiation and fma(z, w, x*y) in another
+#pragma clang fp contract(on)
   using S = AxShape<Nq, T>;
   constexpr int NqNq = S::NqNq, Np = S::Np, threads = NE * NqNq;
   __shared__ T s_D[Nq][Nq + 1];

@@ -169,9 +169,8 @@ int libp_convert_f32_to_f64(int N, const float* x, double* y, void* stream);
  * (linAlgInnerProd.okl:28-100, linAlgNorm2.okl, linAlgSum.okl) plus the
  * host copy of linAlg.cpp:160-224: the result stays in device memory at
  * *result. ws is a device workspace of at least libp_reduce_ws_bytes()
- * bytes; its trailing counter must be zero before the first call (the
- * kernels leave it zero). Deterministic: fixed block partition, partials
- * summed in block order. */
+ * bytes holding one partial per block; it needs no initialisation.
+ * Deterministic: fixed block partition, partials summed in block order. */
 long libp_reduce_ws_bytes(void);
 int libp_inner_prod_f64(int N, const double* x, const double* y, double* result, void* ws, void* stream);
 int libp_weighted_inner_prod_f64(int N, const double* w, const double* x, const double* y, double* result,
@@ -222,6 +221,20 @@ int libp_pcg_gather_pap_f64(int Nrows, const int* rowStarts, const int* colIds, 
  * libp_pcg_update_f64, then RDOTZ2 <- RDOTZ1 <- r.z and BETA = r.z / RDOTZ2. */
 int libp_pcg_jacobi_update_f64(int N, const double* invDiag, const double* p, const double* Ap, double* x, double* r,
                                double* z, double* scalars, void* ws, void* stream);
+/* Allocation-free variant of libp_pcg_jacobi_update_f64 (no counterpart in the
+ * reference): for callers that cannot allocate device memory, such as code under
+ * HIP graph capture, tight solve loops and harnesses that forbid hipMalloc. The
+ * block partials go to a 32 KiB static device array (one per GPU, held until the
+ * process ends) instead of a caller workspace; nothing is allocated. Results are
+ * bit-identical to libp_pcg_jacobi_update_f64. The partial count is capped at 1024
+ * blocks, so every N in [0, LIBP_NOALLOC_MAX_N] fits; other N are rejected with
+ * LIBP_ERR_ARG. All calls on one GPU share that GPU's array: calls that overlap
+ * on one GPU (two streams, or two threads) race and must be serialised by the
+ * caller. The stream must belong to the current device. */
+#define LIBP_NOALLOC_MAX_N 2147483647L
+long libp_pcg_jacobi_update_noalloc_max_n(void);
+int libp_pcg_jacobi_update_noalloc_f64(int N, const double* invDiag, const double* p, const double* Ap, double* x,
+                                       double* r, double* z, double* scalars, void* stream);
 /* scalars[TOL] = tol^2 * scalars[RDOTR] (1e-300 when that is 0), and
  * scalars[STOP] = 1 when scalars[RDOTR] is 0: the pure relative stopping rule
  * of a solve from a zero guess, formed on the device instead of after a host

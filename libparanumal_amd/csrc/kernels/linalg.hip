@@ -2,6 +2,8 @@
 // kernels and Chebyshev smoother steps, FP64 and FP32.
 //
 // Reductions use the deterministic two-stage scheme of reduce.hpp.
+#include <mutex>
+
 #include "reduce.hpp"
 
 namespace libp_amd {
@@ -543,19 +545,77 @@ extern "C" int libp_pcg_relative_tol_f64(double tol, double* scalars, void* stre
   return launch_status("pcg_relative_tol");
 }
 
+static void pcg_jacobi_update_launch(int N, const double* invDiag, const double* p, const double* Ap, double* x,
+                                     double* r, double* z, double* scalars, ReduceWS* ws, hipStream_t st) {
+  const unsigned nb = reduce_blocks(N);
+  if (aligned16(invDiag, p, Ap, x, r, z))
+    hipLaunchKernelGGL(pcg_jacobi_update2_kernel, dim3(nb), dim3(RBS), 0, st, N, invDiag, p, Ap, x, r, z, scalars,
+                       ws);
+  else
+    hipLaunchKernelGGL(pcg_jacobi_update_kernel, dim3(nb), dim3(RBS), 0, st, N, invDiag, p, Ap, x, r, z, scalars, ws);
+  hipLaunchKernelGGL(pcg_jacobi_update_finish_kernel, dim3(1), dim3(RBS), 0, st, (int)nb, scalars,
+                     (const ReduceWS*)ws);
+}
+
 extern "C" int libp_pcg_jacobi_update_f64(int N, const double* invDiag, const double* p, const double* Ap,
                                           double* x, double* r, double* z, double* scalars, void* ws, void* stream) {
   if (!invDiag || !p || !Ap || !x || !r || !z || !scalars || !ws) return arg_error("pcg_jacobi_update", "null pointer");
-  const unsigned nb = reduce_blocks(N);
-  if (aligned16(invDiag, p, Ap, x, r, z))
-    hipLaunchKernelGGL(pcg_jacobi_update2_kernel, dim3(nb), dim3(RBS), 0, as_stream(stream), N, invDiag, p, Ap, x, r,
-                       z, scalars, (ReduceWS*)ws);
-  else
-    hipLaunchKernelGGL(pcg_jacobi_update_kernel, dim3(nb), dim3(RBS), 0, as_stream(stream), N, invDiag, p, Ap, x, r,
-                       z, scalars, (ReduceWS*)ws);
-  hipLaunchKernelGGL(pcg_jacobi_update_finish_kernel, dim3(1), dim3(RBS), 0, as_stream(stream), (int)nb, scalars,
-                     (const ReduceWS*)ws);
+  if (N < 0) return arg_error("pcg_jacobi_update", "N < 0");
+  pcg_jacobi_update_launch(N, invDiag, p, Ap, x, r, z, scalars, (ReduceWS*)ws, as_stream(stream));
   return launch_status("pcg_jacobi_update");
+}
+
+// Scratch of libp_pcg_jacobi_update_noalloc_f64: the block partials of the
+// fused update in a fixed device array instead of a caller workspace. Every
+// GPU the library runs on holds its own copy (32 KiB) until the process ends.
+// Host code never passes the array's name to a kernel (on the host it is a
+// stand-in, not a device address): each device's address comes from
+// hipGetSymbolAddress, once per device with that device current.
+static __device__ __align__(256) ReduceWS scratch;
+
+namespace {
+constexpr int kScratchMaxDevices = 64;
+std::mutex g_scratch_mutex;
+ReduceWS* g_scratch_ptr[kScratchMaxDevices] = {};
+
+int scratch_for_current_device(ReduceWS** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    set_last_error("pcg_jacobi_update_noalloc", hipGetErrorString(e));
+    return LIBP_ERR_HIP;
+  }
+  if (dev < 0 || dev >= kScratchMaxDevices) return arg_error("pcg_jacobi_update_noalloc", "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      set_last_error("pcg_jacobi_update_noalloc: hipGetSymbolAddress", hipGetErrorString(e));
+      return LIBP_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = (ReduceWS*)ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return LIBP_OK;
+}
+}  // namespace
+
+extern "C" long libp_pcg_jacobi_update_noalloc_max_n(void) { return LIBP_NOALLOC_MAX_N; }
+
+extern "C" int libp_pcg_jacobi_update_noalloc_f64(int N, const double* invDiag, const double* p, const double* Ap,
+                                                  double* x, double* r, double* z, double* scalars, void* stream) {
+  if (!invDiag || !p || !Ap || !x || !r || !z || !scalars)
+    return arg_error("pcg_jacobi_update_noalloc", "null pointer");
+  if (N < 0 || (long)N > LIBP_NOALLOC_MAX_N)
+    return arg_error("pcg_jacobi_update_noalloc", "N outside [0, LIBP_NOALLOC_MAX_N]");
+  if ((long)reduce_blocks(N) * (long)sizeof(double) > (long)sizeof(scratch.part[0]))
+    return arg_error("pcg_jacobi_update_noalloc", "block partials exceed the static scratch");
+  ReduceWS* ws = nullptr;
+  const int st = scratch_for_current_device(&ws);
+  if (st != LIBP_OK) return st;
+  pcg_jacobi_update_launch(N, invDiag, p, Ap, x, r, z, scalars, ws, as_stream(stream));
+  return launch_status("pcg_jacobi_update_noalloc");
 }
 extern "C" int libp_pcg_update_p_f64(int N, int first_iter, const double* z, double* p, const double* scalars,
                                      void* stream) {

@@ -148,9 +148,10 @@ settings_t ellipticDefaultSettings() {
                "(F)PCG in fused launches: p.Ap in the operator's gather; with Jacobi, z = invDiag.*r and r.z in "
                "the update",
                {"TRUE", "FALSE"});
-  s.newSetting("OPERATOR SPLIT GATHER", "TRUE",
+  s.newSetting("OPERATOR SPLIT GATHER", "FALSE",
                "FP64 operator with element-interior rows written by the Ax kernel and only the element-boundary "
-               "rows gathered (bit-identical to the two-pass Ax + gather)",
+               "rows gathered (bit-identical to the two-pass Ax + gather; slower on the 48^3 N=7 box: its "
+               "scattered 8-byte stores cost more than the E-vector round trip they save)",
                {"TRUE", "FALSE"});
   s.newSetting("OPERATOR FACE LAYOUT", "SLOT",
                "Split operator: element-boundary results stored in the gather's row order (SLOT: the Ax kernel "

@@ -421,8 +421,9 @@ def main():
         "gather": {"ms": ms_gather, "GBs": bytes_gather / (ms_gather * 1e-3) / 1e9},
         "setup_seconds": setup_s,
     }
-    # achievable HBM: 16-byte non-temporal read-only sweep and copy (libp_hbm_probe_f64),
-    # and the library's own streaming BLAS-1 pattern z = a x + b y for comparison
+    # achievable HBM: a read-only sweep (8-byte non-temporal loads, 65536 blocks) and a
+    # copy (16-byte non-temporal, libp_hbm_probe_f64), and the library's own streaming
+    # BLAS-1 pattern z = a x + b y for comparison
     n = (1 << 31) // 8
     xs = torch.rand(n, dtype=torch.float64, device="cuda")
     ys = torch.empty_like(xs)
@@ -431,8 +432,9 @@ def main():
     ms_tri = timed(lambda: lp.check(L.libp_zaxpy_f64(n // 3, 1.5, xs.data_ptr(), 0.5, xs.data_ptr() + 8 * (n // 3),
                                                     ys.data_ptr(), el.stream)), 20, 3)
     result["hbm_stream_probe"] = {
-        "read": {"kernel": "libp_hbm_probe_f64 read, 2 GiB", "ms": ms_read, "GBs": 8 * n / (ms_read * 1e-3) / 1e9},
-        "copy": {"kernel": "libp_hbm_probe_f64 copy, 1 GiB -> 1 GiB", "ms": ms_copy,
+        "read": {"kernel": "libp_hbm_probe_f64 read (8-byte non-temporal loads), 2 GiB", "ms": ms_read,
+                 "GBs": 8 * n / (ms_read * 1e-3) / 1e9},
+        "copy": {"kernel": "libp_hbm_probe_f64 copy (16-byte non-temporal), 1 GiB -> 1 GiB", "ms": ms_copy,
                  "GBs": 8 * n / (ms_copy * 1e-3) / 1e9},
         "zaxpy": {"kernel": "libp_zaxpy_f64, 3 x 0.67 GiB (2 reads + 1 write)", "ms": ms_tri,
                   "GBs": 3 * 8 * (n // 3) / (ms_tri * 1e-3) / 1e9}}

@@ -65,8 +65,9 @@ int libp_ax_hex_partial_f64(int N, int Nelements, const int* elementList, const 
  * libp_pcg_gather_pap_f64 does; partials holds nAxPartials +
  * libp_ogs_gather_face_blocks(NfaceRows) doubles. */
 /* HBM bandwidth probe (no reference counterpart; measurement only). mode 0
- * reads x[0:n] once with 16-byte non-temporal loads and writes 2048 block sums to
- * y; mode 1 copies y[0:n] = x[0:n] the same way. n even, x and y 16-byte aligned. */
+ * reads x[0:n] once with 8-byte non-temporal loads on 65536 blocks and writes
+ * the 65536 block sums to y (n >= 65536); mode 1 copies y[0:n] = x[0:n] with
+ * 16-byte non-temporal loads and stores. n even, x and y 16-byte aligned. */
 int libp_hbm_probe_f64(int mode, long n, const double* x, double* y, void* stream);
 int libp_ax_hex_face_nodes(int N);
 int libp_ax_hex_split_blocks(int N, int Nelements);
@@ -235,6 +236,33 @@ int libp_pcg_jacobi_update_f64(int N, const double* invDiag, const double* p, co
 long libp_pcg_jacobi_update_noalloc_max_n(void);
 int libp_pcg_jacobi_update_noalloc_f64(int N, const double* invDiag, const double* p, const double* Ap, double* x,
                                        double* r, double* z, double* scalars, void* stream);
+/* A Jacobi-PCG iteration on the C0 hex operator in three passes: the p update
+ * (libp_pcg_update_p_f64), the Ax with p.Ap, and the gather fused with the
+ * update (no stored Ap). Replaces, for PCG with the Jacobi preconditioner,
+ * the Operator call of linearSolverPCG.cpp:130 (ellipticOperator.cpp:29-107:
+ * partialAxKernel, then the ogs gather), the p.Ap dot at :134-140 and updatePCG
+ * with the preconditioner and r.z at :142-160 (ellipticPreconJacobi.cpp:42-51).
+ *
+ * libp_pcg_ax_hex_pap_f64: the partial Ax of ellipticAxHex3D.okl:156-295
+ * applied to the gathered p (through GlobalToLocal) into the E-vector AqL,
+ * then scalars[PAP] = p.Ap summed from the local nodes (sum of p_L * AqL,
+ * which equals p.Ap in exact arithmetic) and scalars[ALPHA] =
+ * scalars[RDOTZ1] / p.Ap. papPartials holds libp_ax_hex_split_blocks(N,
+ * Nelements) doubles. N in [1,14].
+ *
+ * libp_pcg_gather_jacobi_update_f64: per gathered row, Ap = the sum of the
+ * row's copies of AqL (the gather's arithmetic), then x += alpha p,
+ * r -= alpha Ap, z = invDiag.*r, r.r and r.z; ITERS, STOP, RDOTZ1/RDOTZ2 and
+ * BETA as in libp_pcg_jacobi_update_f64. partials holds
+ * libp_pcg_gather_jacobi_partials(Nrows) doubles. Both return at once while
+ * scalars[STOP] is set. */
+int libp_pcg_ax_hex_pap_f64(int N, int Nelements, const int* elementList, const int* GlobalToLocal, const double* wJ,
+                            const double* ggeo, const double* D, double lambda, const double* p, double* AqL,
+                            double* papPartials, double* scalars, void* stream);
+long libp_pcg_gather_jacobi_partials(int Nrows);
+int libp_pcg_gather_jacobi_update_f64(int Nrows, const int* rowStarts, const int* colIds, const double* AqL,
+                                      const double* invDiag, const double* p, double* x, double* r, double* z,
+                                      double* scalars, double* partials, void* stream);
 /* scalars[TOL] = tol^2 * scalars[RDOTR] (1e-300 when that is 0), and
  * scalars[STOP] = 1 when scalars[RDOTR] is 0: the pure relative stopping rule
  * of a solve from a zero guess, formed on the device instead of after a host

@@ -100,6 +100,9 @@ SIGNATURES = {
     "libp_pcg_gather_pap_f64": (c_int, [c_int] + [c_void_p] * 8),
     "libp_pcg_jacobi_update_f64": (c_int, [c_int] + [c_void_p] * 9),
     "libp_pcg_jacobi_update_noalloc_f64": (c_int, [c_int] + [c_void_p] * 8),
+    "libp_pcg_gather_jacobi_update_f64": (c_int, [c_int] + [c_void_p] * 11),
+    "libp_pcg_gather_jacobi_partials": (c_long, [c_int]),
+    "libp_pcg_ax_hex_pap_f64": (c_int, [c_int, c_int] + [c_void_p] * 5 + [c_double] + [c_void_p] * 5),
     "libp_pcg_jacobi_update_noalloc_max_n": (c_long, []),
     "libp_pcg_relative_tol_f64": (c_int, [c_double, c_void_p, c_void_p]),
     "libp_pcg_rdotz_from_f32_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,

@@ -148,6 +148,15 @@ settings_t ellipticDefaultSettings() {
                "(F)PCG in fused launches: p.Ap in the operator's gather; with Jacobi, z = invDiag.*r and r.z in "
                "the update",
                {"TRUE", "FALSE"});
+  s.newSetting("LINEAR SOLVER CHECK EVERY", "4",
+               "(F)PCG with a Jacobi preconditioner: queue this many iterations between host reads of r.r; the "
+               "kernels stop themselves at the tolerance (device STOP flag), so the iterates and the iteration "
+               "count are those of a test after every iteration");
+  s.newSetting("PCG FUSED OPERATOR", "TRUE",
+               "PCG + Jacobi in three passes per iteration: the p update, the Ax kernel with p.Ap, and the gather "
+               "fused with the x, r, z update (N <= 14, two-pass operator); FALSE runs the gather (with p.Ap) and "
+               "the update as separate passes",
+               {"TRUE", "FALSE"});
   s.newSetting("OPERATOR SPLIT GATHER", "FALSE",
                "FP64 operator with element-interior rows written by the Ax kernel and only the element-boundary "
                "rows gathered (bit-identical to the two-pass Ax + gather; slower on the 48^3 N=7 box: its "

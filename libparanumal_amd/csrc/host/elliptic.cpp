@@ -26,14 +26,21 @@ void elliptic_t::Setup(platform_t& _platform, mesh_t& _mesh, settings_t& _settin
 }
 
 void elliptic_t::SetupPrecon() {
-  if (settings.compareSetting("PRECONDITIONER", "JACOBI"))
+  if (settings.compareSetting("PRECONDITIONER", "JACOBI")) {
     precon.Setup<JacobiPrecon>(*this);
-  else if (settings.compareSetting("PRECONDITIONER", "MULTIGRID"))
+    // the fused Jacobi-PCG iteration's reduction partials, allocated here rather
+    // than inside a timed solve
+    if (settings.compareSetting("PCG FUSED OPERATOR", "TRUE") && mesh.N >= 1 && mesh.N <= 14) {
+      o_axPartials = platform.malloc<double>(std::max(libp_ax_hex_split_blocks(mesh.N, mesh.NlocalGatherElements), 1));
+      o_gatherPartials = platform.malloc<double>(std::max<long>(libp_pcg_gather_jacobi_partials(ogsMasked.NlocalT), 2));
+    }
+  } else if (settings.compareSetting("PRECONDITIONER", "MULTIGRID")) {
     precon.Setup<MultiGridPrecon>(*this);
-  else if (settings.compareSetting("PRECONDITIONER", "NONE"))
+  } else if (settings.compareSetting("PRECONDITIONER", "NONE")) {
     precon.Setup<IdentityPrecon>(Ndofs, platform);
-  else
+  } else {
     LIBP_FORCE_ABORT("PRECONDITIONER " << settings.getSetting("PRECONDITIONER") << " is not supported on this path");
+  }
 }
 
 void elliptic_t::BoundarySetup() {
@@ -182,6 +189,25 @@ void elliptic_t::Operator(deviceMemory<dfloat>& o_q, deviceMemory<dfloat>& o_Aq)
   }
   PartialAx(o_q, o_AqL);
   ogsMasked.Gather(o_Aq, o_AqL, 1, ogs::Add, ogs::Trans);
+}
+
+bool elliptic_t::OperatorJacobiPCG(deviceMemory<dfloat>& o_p, deviceMemory<dfloat>& o_x, deviceMemory<dfloat>& o_r,
+                                   deviceMemory<dfloat>& o_z, const deviceMemory<dfloat>& o_invDiag,
+                                   double* scalars) {
+  if (!settings.compareSetting("PCG FUSED OPERATOR", "TRUE") || mesh.N < 1 || mesh.N > 14 || SplitGather())
+    return false;
+  if (!o_gatherPartials.isInitialized()) {
+    o_axPartials = platform.malloc<double>(std::max(libp_ax_hex_split_blocks(mesh.N, mesh.NlocalGatherElements), 1));
+    o_gatherPartials = platform.malloc<double>(std::max<long>(libp_pcg_gather_jacobi_partials(ogsMasked.NlocalT), 2));
+  }
+  LIBP_CALL(libp_pcg_ax_hex_pap_f64(mesh.N, mesh.NlocalGatherElements, nullptr, o_GlobalToLocal.ptr(),
+                                    mesh.o_wJ.ptr(), mesh.o_ggeo.ptr(), mesh.o_D.ptr(), lambda, o_p.ptr(),
+                                    o_AqL.ptr(), o_axPartials.ptr(), scalars, platform.s()));
+  LIBP_CALL(libp_pcg_gather_jacobi_update_f64(ogsMasked.NlocalT, ogsMasked.o_rowStartsT.ptr(),
+                                              ogsMasked.o_colIdsT.ptr(), o_AqL.ptr(), o_invDiag.ptr(), o_p.ptr(),
+                                              o_x.ptr(), o_r.ptr(), o_z.ptr(), scalars, o_gatherPartials.ptr(),
+                                              platform.s()));
+  return true;
 }
 
 bool elliptic_t::OperatorPAP(deviceMemory<dfloat>& o_q, deviceMemory<dfloat>& o_Aq, double* scalars, void* ws) {

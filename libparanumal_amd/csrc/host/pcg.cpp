@@ -12,6 +12,7 @@ pcg::pcg(dlong _N, dlong _Nhalo, platform_t& _platform, settings_t& _settings, c
     : linearSolverBase_t(_N, _Nhalo, _platform, _settings, _comm) {
   flexible = settings.compareSetting("LINEAR SOLVER", "FPCG");
   fuseJacobi = !settings.compareSetting("PCG JACOBI FUSED", "FALSE");
+  if (settings.hasSetting("LINEAR SOLVER CHECK EVERY")) settings.getSetting("LINEAR SOLVER CHECK EVERY", checkEvery);
   const dlong Ntotal = N + Nhalo;
   o_p = platform.malloc<dfloat>(Ntotal);
   o_z = platform.malloc<dfloat>(Ntotal);
@@ -25,7 +26,12 @@ pcg::pcg(dlong _N, dlong _Nhalo, platform_t& _platform, settings_t& _settings, c
   LIBP_CALL(libp_set_f64(Ntotal, 0.0, o_Ap.ptr(), platform.s()));
 }
 
-/* One PCG iteration (linearSolverPCG.cpp:107-171). With [PCG JACOBI FUSED]
+/* One PCG iteration (linearSolverPCG.cpp:107-171). With a Jacobi
+ * preconditioner, a plain iteration and an operator that has the fused form
+ * (operator_t::OperatorJacobiPCG), an iteration is three passes: p = z + beta
+ * p; the Ax kernel with p.Ap and alpha; the gather fused with the x, r, z
+ * update, r.r, r.z and beta (Ap is never stored).
+ * Otherwise, with [PCG JACOBI FUSED]
  * the operator's gather also forms p.Ap whatever the preconditioner, and with
  * a Jacobi preconditioner and a plain (not flexible) iteration the launches
  * fuse further:
@@ -46,6 +52,7 @@ void pcg::iteration(operator_t& linearOperator, operator_t& precon, diagonalPrec
       LIBP_CALL(libp_pcg_jacobi_rdotz_f64(N, jac->invDiagonal().ptr(), o_r.ptr(), o_z.ptr(), o_Ap.ptr(), 0, s, ws,
                                           st));
     LIBP_CALL(libp_pcg_update_p_f64(N, it == 0, o_z.ptr(), o_p.ptr(), s, st));
+    if (linearOperator.OperatorJacobiPCG(o_p, o_x, o_r, o_z, jac->invDiagonal(), s)) return;
     if (!linearOperator.OperatorPAP(o_p, o_Ap, s, ws)) {
       linearOperator.Operator(o_p, o_Ap);
       LIBP_CALL(libp_pcg_pap_f64(N, o_p.ptr(), o_Ap.ptr(), s, ws, st));
@@ -103,7 +110,9 @@ int pcg::Solve(operator_t& linearOperator, operator_t& precon, deviceMemory<dflo
   diagonalPrecon_t* jac = dynamic_cast<diagonalPrecon_t*>(inner);
   if (jac && jac->needsZeroMean()) jac = nullptr;
 
-  if (checkEvery > 1) {
+  // Chunked convergence checks need device-side scalars only: a Jacobi (or no)
+  // preconditioner, no per-iteration callbacks, no history or printing.
+  if (checkEvery > 1 && jac && !recordHistory && !verbose) {
     // chunked: the kernels stop themselves once rdotr <= TOL (device STOP flag)
     int iter = 0;
     if (rdotr0 != 0.0) {

@@ -22,6 +22,21 @@ class operator_t {
     (void)ws;
     return false;
   }
+  // One Jacobi-PCG iteration after the p update, with p.Ap formed by the Ax
+  // kernel (libp_pcg_ax_hex_pap_f64) and the operator's gather fused into the
+  // update (libp_pcg_gather_jacobi_update_f64: x += alpha p, r -= alpha Ap,
+  // z = invDiag.*r, r.r, r.z and beta; Ap never stored). Returns false when
+  // the operator has no such form.
+  virtual bool OperatorJacobiPCG(deviceMemory<dfloat>& o_p, deviceMemory<dfloat>& o_x, deviceMemory<dfloat>& o_r,
+                                 deviceMemory<dfloat>& o_z, const deviceMemory<dfloat>& o_invDiag, double* scalars) {
+    (void)o_p;
+    (void)o_x;
+    (void)o_r;
+    (void)o_z;
+    (void)o_invDiag;
+    (void)scalars;
+    return false;
+  }
   // For a preconditioner: z = M r together with the PCG scalars of
   // libp_pcg_rdotz_f64 (r.z, and z.Ap when flexible) in one pass over z.
   // Returns false when there is no fused form.

@@ -65,9 +65,34 @@ __device__ inline int hex_face_index(int i, int j, int k, int Nq) {
   return n - before;
 }
 
+// Block sum of one double per thread into part[blockIdx.x] (thread 0 writes).
+template <int threads>
+__device__ inline void block_partial(const double acc, double* __restrict__ part) {
+  const int t = threadIdx.x;
+  if constexpr (threads % 64 == 0) {
+    __shared__ double s_w[threads / 64];
+    const double b = block_sum(acc, s_w);
+    if (t == 0) part[blockIdx.x] = b;
+  } else {  // partial wavefront (N <= 4): a tree in LDS, no cross-lane reads of idle lanes
+    constexpr int P = (threads <= 64) ? 64 : 256;
+    __shared__ double s_red[P];
+    s_red[t] = acc;
+    for (int z = threads + t; z < P; z += threads) s_red[z] = 0.0;
+#pragma unroll
+    for (int off = P / 2; off > 0; off >>= 1) {
+      __syncthreads();
+      if (t < off) s_red[t] += s_red[t + off];
+    }
+    if (t == 0) part[blockIdx.x] = s_red[0];
+  }
+}
+
 // MODE 0: q is a gathered vector indexed through GlobalToLocal (-1 -> 0).
 // MODE 1: q is an E-vector (element-local input, no gather).
-// OUT 0: Aq is the E-vector. OUT 1 (MODE 0 only): interior nodes go to the
+// OUT 0: Aq is the E-vector; with PAP (MODE 0, the PCG form), each block
+// also stores sum over its local nodes of q_L * (A_L q)_L, which summed over
+// all blocks is p.Ap = p^T Q^T A_L Q p for a gathered q = p, and the kernel
+// returns at once while scal[8] (STOP) is set. OUT 1 (MODE 0 only): interior nodes go to the
 // gathered Ag, element-boundary nodes to the face E-vector Aq; PAP adds one
 // partial of sum q Ag over the interior rows per block, pap[pap0 + block].
 // Every element slot of a block is a real element: the launcher runs the
@@ -80,7 +105,7 @@ __global__ void __launch_bounds__(NE * Nq * Nq) __attribute__((amdgpu_waves_per_
                   const T* __restrict__ wJ, const T* __restrict__ ggeo, const T* __restrict__ D, const T lambda,
                   const T* __restrict__ q, T* __restrict__ Aq, T* __restrict__ Ag = nullptr,
                   double* __restrict__ pap = nullptr, const int pap0 = 0,
-                  const int* __restrict__ fslot = nullptr) {
+                  const int* __restrict__ fslot = nullptr, const double* __restrict__ scal = nullptr) {
   // a*b + c contracts only within one source expression, so every
   // instantiation (E-vector, split, split with p.Ap) rounds alike: with
   // contraction across expressions the backend may fuse x*y + z*w as
@@ -95,6 +120,7 @@ __global__ void __launch_bounds__(NE * Nq * Nq) __attribute__((amdgpu_waves_per_
   __shared__ int s_id[OUT == 1 ? NE : 1][OUT == 1 ? Nq : 1][OUT == 1 ? NqNq : 1];
   __shared__ T s_out[OUT == 1 ? NE : 1][OUT == 1 ? Nq : 1][OUT == 1 ? NqNq : 1];
 
+  if (scal != nullptr && scal[8] != 0.0) return;  // a PCG step after convergence (STOP)
   const int t = threadIdx.x;
   const int es = (NE == 1) ? 0 : t / NqNq;
   const int ij = (NE == 1) ? t : t - es * NqNq;
@@ -163,6 +189,12 @@ __global__ void __launch_bounds__(NE * Nq * Nq) __attribute__((amdgpu_waves_per_
     T* out = Aq + base;
 #pragma unroll
     for (int k = 0; k < Nq; ++k) __builtin_nontemporal_store(r_Aq[k], out + k * NqNq);
+    if constexpr (PAP) {
+      double acc = 0;
+#pragma unroll
+      for (int k = 0; k < Nq; ++k) acc += (double)r_q[k] * (double)r_Aq[k];
+      block_partial<threads>(acc, pap + pap0);
+    }
   } else {
     // The results go through LDS first: stored with the regular pattern of the
     // E-vector store, the registers die there. Scattering straight from the
@@ -190,24 +222,7 @@ __global__ void __launch_bounds__(NE * Nq * Nq) __attribute__((amdgpu_waves_per_
           __builtin_nontemporal_store(v, fout + fi);
       }
     }
-    if constexpr (PAP) {
-      if constexpr (threads % 64 == 0) {
-        __shared__ double s_w[threads / 64];
-        const double b = block_sum(acc, s_w);
-        if (t == 0) pap[pap0 + blockIdx.x] = b;
-      } else {  // partial wavefront (N <= 4): a tree in LDS, no cross-lane reads of idle lanes
-        constexpr int P = (threads <= 64) ? 64 : 256;
-        __shared__ double s_red[P];
-        s_red[t] = acc;
-        for (int z = threads + t; z < P; z += threads) s_red[z] = 0.0;
-#pragma unroll
-        for (int off = P / 2; off > 0; off >>= 1) {
-          __syncthreads();
-          if (t < off) s_red[t] += s_red[t + off];
-        }
-        if (t == 0) pap[pap0 + blockIdx.x] = s_red[0];
-      }
-    }
+    if constexpr (PAP) block_partial<threads>(acc, pap + pap0);
   }
 }
 
@@ -259,6 +274,42 @@ static int launch_ax_split(int Nelements, const int* elist, const int* g2l, cons
                          full * S::NE, elist, g2l, wJ, ggeo, D, lambda, q, AqF, Ag, nullptr, 0, fslot);
   }
   return launch_status("ax_hex_split");
+}
+
+// p.Ap from the partials of the PCG-form Ax kernel, summed in block order,
+// then alpha = s[RDOTZ1] / p.Ap (linearSolverPCG.cpp:134-140).
+__global__ void __launch_bounds__(1024) ax_pap_finish_kernel(const int nb, double* __restrict__ s,
+                                                             const double* __restrict__ part) {
+  __shared__ double s_w[16];
+  if (s[8] != 0.0) return;
+  double t = 0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) t += part[i];
+  const double pap = block_sum(t, s_w);
+  if (threadIdx.x == 0) {
+    s[4] = pap;
+    s[2] = s[0] / pap;
+  }
+}
+
+// Partial Ax of p (gathered, through GlobalToLocal) into the E-vector with
+// p.Ap from the local nodes and alpha (MODE 0, OUT 0, PAP).
+template <int Nq>
+static int launch_ax_pap(int Nelements, const int* elist, const int* g2l, const double* wJ, const double* ggeo,
+                         const double* D, double lambda, const double* p, double* AqL, double* part, double* s,
+                         hipStream_t st) {
+  using S = AxShape<Nq, double>;
+  if (Nelements <= 0) return LIBP_OK;
+  const int full = Nelements / S::NE;
+  const int rest = Nelements - full * S::NE;
+  if (full > 0)
+    hipLaunchKernelGGL((ax_hex_kernel<Nq, S::NE, double, 0, 0, true>), dim3(full), dim3(S::threads), 0, st, 0, elist,
+                       g2l, wJ, ggeo, D, lambda, p, AqL, nullptr, part, 0, nullptr, s);
+  if (S::NE > 1 && rest > 0)
+    hipLaunchKernelGGL((ax_hex_kernel<Nq, 1, double, 0, 0, true>), dim3(rest), dim3(S::NqNq), 0, st, full * S::NE,
+                       elist, g2l, wJ, ggeo, D, lambda, p, AqL, nullptr, part, full, nullptr, s);
+  hipLaunchKernelGGL(ax_pap_finish_kernel, dim3(1), dim3(1024), 0, st, split_blocks<Nq>(Nelements), s,
+                     (const double*)part);
+  return launch_status("pcg_ax_hex_pap");
 }
 
 template <typename T, int MODE>
@@ -367,5 +418,37 @@ extern "C" int libp_ax_hex_split_f64(int N, int Nelements, const int* elementLis
 #undef LIBP_AX_CASE
     default:
       return arg_error("ax_hex_split", "polynomial degree must be in [1,14]");
+  }
+}
+
+extern "C" int libp_pcg_ax_hex_pap_f64(int N, int Nelements, const int* elementList, const int* GlobalToLocal,
+                                       const double* wJ, const double* ggeo, const double* D, double lambda,
+                                       const double* p, double* AqL, double* papPartials, double* scalars,
+                                       void* stream) {
+  if (!GlobalToLocal || !wJ || !ggeo || !D || !p || !AqL || !papPartials || !scalars)
+    return arg_error("pcg_ax_hex_pap", "null pointer");
+  hipStream_t st = as_stream(stream);
+  switch (N) {
+#define LIBP_AX_CASE(NN) \
+  case NN:               \
+    return launch_ax_pap<NN + 1>(Nelements, elementList, GlobalToLocal, wJ, ggeo, D, lambda, p, AqL, papPartials, \
+                                 scalars, st);
+    LIBP_AX_CASE(1)
+    LIBP_AX_CASE(2)
+    LIBP_AX_CASE(3)
+    LIBP_AX_CASE(4)
+    LIBP_AX_CASE(5)
+    LIBP_AX_CASE(6)
+    LIBP_AX_CASE(7)
+    LIBP_AX_CASE(8)
+    LIBP_AX_CASE(9)
+    LIBP_AX_CASE(10)
+    LIBP_AX_CASE(11)
+    LIBP_AX_CASE(12)
+    LIBP_AX_CASE(13)
+    LIBP_AX_CASE(14)
+#undef LIBP_AX_CASE
+    default:
+      return arg_error("pcg_ax_hex_pap", "polynomial degree must be in [1,14]");
   }
 }

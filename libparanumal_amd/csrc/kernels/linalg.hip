@@ -410,27 +410,29 @@ __global__ void __launch_bounds__(256) cheb_step_kernel(const int N, const T* __
 using namespace libp_amd;
 
 // HBM probes (no reference counterpart; measurement only): a read-only sweep
-// that sums x into one partial per block, and a copy y = x. Each thread moves
-// 16 bytes per access (two doubles) with non-temporal loads and stores, four
-// accesses in flight per loop step; the grid is a fixed 2048 blocks of 256.
+// that sums x into one partial per block, and a copy y = x. A sweep over grid
+// sizes, access widths and cache hints on MI355X gave the highest read rate
+// (7.27 TB/s over 4 GiB) with 8-byte non-temporal loads, eight in flight per
+// thread, on 65536 blocks of 256 threads, and the highest copy rate (5.18 TB/s)
+// with 16-byte non-temporal loads and stores on 16384 blocks.
 typedef double hbm_v2d __attribute__((ext_vector_type(2)));
-constexpr int HBM_PROBE_BLOCKS = 2048;
+constexpr int HBM_READ_BLOCKS = 65536;
+constexpr int HBM_COPY_BLOCKS = 16384;
 
-__global__ void __launch_bounds__(256) hbm_read_kernel(const long n2, const hbm_v2d* __restrict__ x,
+__global__ void __launch_bounds__(256) hbm_read_kernel(const long n, const double* __restrict__ x,
                                                        double* __restrict__ part) {
   __shared__ double s_w[4];
   const long S = (long)gridDim.x * blockDim.x;
   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
   double acc = 0;
-  for (; i + 3 * S < n2; i += 4 * S) {
-    const hbm_v2d a = __builtin_nontemporal_load(x + i), b = __builtin_nontemporal_load(x + i + S);
-    const hbm_v2d c = __builtin_nontemporal_load(x + i + 2 * S), d = __builtin_nontemporal_load(x + i + 3 * S);
-    acc += (a.x + a.y) + (b.x + b.y) + (c.x + c.y) + (d.x + d.y);
+  for (; i + 7 * S < n; i += 8 * S) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = __builtin_nontemporal_load(x + i + u * S);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc += v[u];
   }
-  for (; i < n2; i += S) {
-    const hbm_v2d a = __builtin_nontemporal_load(x + i);
-    acc += a.x + a.y;
-  }
+  for (; i < n; i += S) acc += __builtin_nontemporal_load(x + i);
   const double b = block_sum(acc, s_w);
   if (threadIdx.x == 0) part[blockIdx.x] = b;
 }
@@ -665,10 +667,10 @@ extern "C" int libp_hbm_probe_f64(int mode, long n, const double* x, double* y, 
   if (!x || !y || (n & 1) || ((uintptr_t)x & 15) || ((uintptr_t)y & 15))
     return arg_error("hbm_probe", "null or unaligned pointer, or odd length");
   if (mode == 0) {
-    hipLaunchKernelGGL(hbm_read_kernel, dim3(HBM_PROBE_BLOCKS), dim3(256), 0, as_stream(stream), n / 2,
-                       (const hbm_v2d*)x, y);
+    if (n < HBM_READ_BLOCKS) return arg_error("hbm_probe", "read mode needs n >= 65536 (y holds the block sums)");
+    hipLaunchKernelGGL(hbm_read_kernel, dim3(HBM_READ_BLOCKS), dim3(256), 0, as_stream(stream), n, x, y);
   } else if (mode == 1) {
-    hipLaunchKernelGGL(hbm_copy_kernel, dim3(HBM_PROBE_BLOCKS), dim3(256), 0, as_stream(stream), n / 2,
+    hipLaunchKernelGGL(hbm_copy_kernel, dim3(HBM_COPY_BLOCKS), dim3(256), 0, as_stream(stream), n / 2,
                        (const hbm_v2d*)x, (hbm_v2d*)y);
   } else {
     return arg_error("hbm_probe", "mode must be 0 (read) or 1 (copy)");

@@ -288,6 +288,116 @@ __global__ void __launch_bounds__(1024) split_pap_finish_kernel(const int nb, do
 
 inline unsigned face_gather_blocks(int Nrows) { return stream_blocks(Nrows, 1024, 4096); }
 
+// The operator's gather fused with the Jacobi-PCG update, for a PCG step whose
+// Ax kernel already formed p.Ap and alpha (libp_pcg_ax_hex_pap_f64). Per row, Ap = the sum of the row's copies in
+// ascending order from zero (the arithmetic of ogs_gather_kernel), used at
+// once: x += alpha p, r -= alpha Ap, z = invDiag.*r, with the expressions of
+// pcg_jacobi_update_kernel (linearSolverPCG.cpp:142-160 with
+// ellipticPreconJacobi.cpp:42-51). Ap is never stored. Each block stores its
+// r.r and r.z partials at part[b] and part[nb + b] (nb = grid size);
+// pcg_gather_jacobi_finish_kernel sums them in block order. Both return at
+// once while s[8] (STOP) is set. The grid gives each thread about
+// GATHER_JACOBI_ROWS rows (two per loop step): a row is a chain of dependent
+// loads, so the gather needs many rows in flight across the machine. On the
+// 48^3, N=7 Jacobi-PCG solve 2, 4, 8 and 16 rows per thread gave 1.638, 1.586,
+// 1.556 and 1.561 ms per iteration; a grid capped at 2048 blocks (~36 rows
+// per thread) made this kernel slower than the gather and update apart.
+#ifndef LIBP_GATHER_JACOBI_ROWS
+#define LIBP_GATHER_JACOBI_ROWS 8
+#endif
+constexpr int GATHER_JACOBI_ROWS = LIBP_GATHER_JACOBI_ROWS;
+inline unsigned gather_jacobi_blocks(int Nrows) {
+  return stream_blocks(((long)Nrows + GATHER_JACOBI_ROWS - 1) / GATHER_JACOBI_ROWS, 256, 1L << 20);
+}
+
+__global__ void __launch_bounds__(256) pcg_gather_jacobi_kernel(const int Nrows, const int* __restrict__ rowStarts,
+                                                                const int* __restrict__ colIds,
+                                                                const double* __restrict__ v,
+                                                                const double* __restrict__ invDiag,
+                                                                const double* __restrict__ p, double* __restrict__ x,
+                                                                double* __restrict__ r, double* __restrict__ z,
+                                                                const double* __restrict__ s,
+                                                                double* __restrict__ part) {
+  __shared__ double s_w[16];
+  if (s[8] != 0.0) return;
+  const double alpha = s[2];
+  const long S = (long)gridDim.x * blockDim.x;
+  double a0 = 0, a1 = 0;
+  for (long row0 = blockIdx.x * (long)blockDim.x + threadIdx.x; row0 < Nrows; row0 += 2 * S) {
+    int a[2], len[2];
+    double pv[2], xv[2], rv[2], dv[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const long row = row0 + q * S;
+      const bool in = row < Nrows;
+      a[q] = in ? rowStarts[row] : 0;
+      len[q] = in ? rowStarts[row + 1] - a[q] : 0;
+      pv[q] = in ? p[row] : 0.0;
+      xv[q] = in ? x[row] : 0.0;
+      rv[q] = in ? r[row] : 0.0;
+      dv[q] = in ? invDiag[row] : 0.0;
+    }
+    int c[2][8];
+    double val8[2][8];
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) c[q][k] = (k < len[q]) ? colIds[a[q] + k] : 0;
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) val8[q][k] = (k < len[q]) ? v[c[q][k]] : 0.0;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const long row = row0 + q * S;
+      if (row >= Nrows) continue;
+      double Ap = 0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (k < len[q]) Ap += val8[q][k];
+      for (int g = a[q] + 8; g < a[q] + len[q]; ++g) Ap += v[colIds[g]];
+      double xn = xv[q], rn = rv[q];
+      xn += alpha * pv[q];
+      rn -= alpha * Ap;
+      const double zn = 1.0 * dv[q] * rn;
+      x[row] = xn;
+      r[row] = rn;
+      z[row] = zn;
+      a0 += rn * rn;
+      a1 += rn * zn;
+    }
+  }
+  const double b0 = block_sum(a0, s_w);
+  const double b1 = block_sum(a1, s_w);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = b0;
+    part[gridDim.x + blockIdx.x] = b1;
+  }
+}
+
+// r.r and r.z from the partials in block order, then ITERS, STOP and the
+// r.z / beta bookkeeping of pcg_jacobi_update_finish_kernel.
+__global__ void __launch_bounds__(1024) pcg_gather_jacobi_finish_kernel(const int nb, double* __restrict__ s,
+                                                                       const double* __restrict__ part) {
+  __shared__ double s_w[16];
+  if (s[8] != 0.0) return;
+  double t0 = 0, t1 = 0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    t0 += part[i];
+    t1 += part[nb + i];
+  }
+  const double rr = block_sum(t0, s_w);
+  const double rz = block_sum(t1, s_w);
+  if (threadIdx.x == 0) {
+    s[5] = rr;
+    s[9] += 1.0;
+    if (s[7] > 0.0 && rr <= s[7]) s[8] = 1.0;
+    s[1] = s[0];
+    s[0] = rz;
+    s[3] = rz / s[1];
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) ogs_scatter_kernel(const int Nrows, const int* __restrict__ rowStarts,
                                                           const int* __restrict__ colIds, const T* __restrict__ gv,
@@ -423,4 +533,21 @@ extern "C" int libp_pcg_gather_face_pap_f64(int NfaceRows, const int* faceRows, 
   hipLaunchKernelGGL(split_pap_finish_kernel, dim3(1), dim3(1024), 0, as_stream(stream), nAxPartials + (int)blocks,
                      scalars, partials);
   return launch_status("pcg_gather_face_pap");
+}
+
+extern "C" long libp_pcg_gather_jacobi_partials(int Nrows) { return Nrows >= 0 ? 2L * gather_jacobi_blocks(Nrows) : -1; }
+
+extern "C" int libp_pcg_gather_jacobi_update_f64(int Nrows, const int* rowStarts, const int* colIds,
+                                                 const double* AqL, const double* invDiag, const double* p, double* x,
+                                                 double* r, double* z, double* scalars, double* partials,
+                                                 void* stream) {
+  if (Nrows < 0) return arg_error("pcg_gather_jacobi_update", "negative row count");
+  if (!rowStarts || !colIds || !AqL || !invDiag || !p || !x || !r || !z || !scalars || !partials)
+    return arg_error("pcg_gather_jacobi_update", "null pointer");
+  const unsigned nb = gather_jacobi_blocks(Nrows);
+  hipLaunchKernelGGL(pcg_gather_jacobi_kernel, dim3(nb), dim3(256), 0, as_stream(stream), Nrows, rowStarts, colIds,
+                     AqL, invDiag, p, x, r, z, (const double*)scalars, partials);
+  hipLaunchKernelGGL(pcg_gather_jacobi_finish_kernel, dim3(1), dim3(1024), 0, as_stream(stream), (int)nb, scalars,
+                     (const double*)partials);
+  return launch_status("pcg_gather_jacobi_update");
 }

@@ -1,0 +1,164 @@
+"""The fused Jacobi-PCG iteration against the oracle and the separate kernels.
+
+After the p update, libp_pcg_ax_hex_pap_f64 runs the partial Ax of p and forms
+p.Ap from the local nodes and alpha; libp_pcg_gather_jacobi_update_f64 gathers
+A p row by row and applies the update of linearSolverPCG.cpp:142-160 with the
+Jacobi preconditioner (ellipticPreconJacobi.cpp:42-51) without storing Ap.
+Checked through the C ABI against the oracle's partial Ax and operator
+(solvers/elliptic/okl/ellipticAxHex3D.okl:156-295, ellipticOperator.cpp:29-107)
+and, bit for bit, against the separate kernels (libp_ax_hex_partial_f64,
+libp_ogs_gather_f64 + libp_pcg_jacobi_update_f64); then whole solves with and
+without the fused form."""
+import numpy as np
+import pytest
+
+from tests.oracle_lib import OracleElliptic
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def lp():
+    import libparanumal_amd
+    return libparanumal_amd
+
+
+def box(nx, N, curve=0.05, lam=0.5, **kw):
+    s = {"MESH FILE": "BOX", "MESH DIMENSION": 3, "ELEMENT TYPE": 12, "BOX NX": nx, "BOX NY": nx + 1, "BOX NZ": nx,
+         "BOX DIMX": 2, "BOX DIMY": 2, "BOX DIMZ": 2, "BOX BOUNDARY FLAG": 1, "POLYNOMIAL DEGREE": N,
+         "BOX CURVED MAP AMPLITUDE": curve, "LAMBDA": lam, "DATA FILE": "data/ellipticSine3D.h",
+         "DISCRETIZATION": "CONTINUOUS", "LINEAR SOLVER": "PCG", "PRECONDITIONER": "JACOBI"}
+    s.update(kw)
+    return s
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 7])
+def test_fused_iteration_through_c_abi(N):
+    o = OracleElliptic(box(3, N))
+    L = lp().lib()
+    E, Np, Ng = o.geti("Nelements"), o.geti("Np"), o.geti("Ngather")
+    NL = E * Np
+    rs, ci = o.array("rowStartsT"), o.array("colIdsT")
+    assert rs.size == Ng + 1
+    rng = np.random.default_rng(10 * N)
+    p = rng.uniform(-1, 1, Ng)
+    x0 = rng.uniform(-1, 1, Ng)
+    r0 = rng.uniform(-1, 1, Ng)
+    invD = 1.0 / o.diagonal()
+    rdotz = 1.9
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    g2l, wJ, ggeo, D = dev(o.array("GlobalToLocal")), dev(o.array("wJ")), dev(o.array("ggeo")), dev(o.array("D"))
+    pd, invd, rsd, cid = dev(p), dev(invD), dev(rs), dev(ci)
+    AqL = torch.full((NL,), float("nan"), dtype=torch.float64, device="cuda")
+    part = torch.zeros(L.libp_ax_hex_split_blocks(N, E), dtype=torch.float64, device="cuda")
+    s = torch.zeros(16, dtype=torch.float64, device="cuda")
+    s[0] = rdotz
+    st = torch.cuda.current_stream().cuda_stream
+    lp().check(L.libp_pcg_ax_hex_pap_f64(N, E, None, g2l.data_ptr(), wJ.data_ptr(), ggeo.data_ptr(), D.data_ptr(),
+                                         0.5, pd.data_ptr(), AqL.data_ptr(), part.data_ptr(), s.data_ptr(), st))
+    AqL2 = torch.full((NL,), float("nan"), dtype=torch.float64, device="cuda")
+    lp().check(L.libp_ax_hex_partial_f64(N, E, None, g2l.data_ptr(), wJ.data_ptr(), ggeo.data_ptr(), D.data_ptr(),
+                                         0.5, pd.data_ptr(), AqL2.data_ptr(), st))
+    torch.cuda.synchronize()
+    # the same E-vector as the plain partial Ax, bit for bit, and the oracle's to 1e-12
+    assert torch.equal(AqL, AqL2)
+    AqL_ref = o.partial_ax(p)
+    assert np.max(np.abs(AqL.cpu().numpy() - AqL_ref)) <= 1e-12 * np.max(np.abs(AqL_ref))
+    Ap = o.operator(p)
+    pap = float(p @ Ap)
+    sh = s.cpu().numpy()
+    print(f"N={N}: p.Ap {sh[4]:.17e} oracle {pap:.17e}", flush=True)
+    assert abs(sh[4] - pap) <= 1e-12 * abs(pap)
+    assert abs(sh[2] - rdotz / sh[4]) <= 1e-15 * abs(sh[2])
+
+    # the fused gather + update against numpy, and bit for bit against the separate kernels
+    ws = torch.empty(L.libp_reduce_ws_bytes(), dtype=torch.uint8, device="cuda")
+    gpart = torch.zeros(L.libp_pcg_gather_jacobi_partials(Ng), dtype=torch.float64, device="cuda")
+    xa, ra, za = dev(x0), dev(r0), torch.full((Ng,), float("nan"), dtype=torch.float64, device="cuda")
+    sa = s.clone()
+    lp().check(L.libp_pcg_gather_jacobi_update_f64(Ng, rsd.data_ptr(), cid.data_ptr(), AqL.data_ptr(),
+                                                   invd.data_ptr(), pd.data_ptr(), xa.data_ptr(), ra.data_ptr(),
+                                                   za.data_ptr(), sa.data_ptr(), gpart.data_ptr(), st))
+    Apd = torch.empty(Ng, dtype=torch.float64, device="cuda")
+    lp().check(L.libp_ogs_gather_f64(Ng, rsd.data_ptr(), cid.data_ptr(), AqL.data_ptr(), Apd.data_ptr(), st))
+    xb, rb, zb = dev(x0), dev(r0), torch.full((Ng,), float("nan"), dtype=torch.float64, device="cuda")
+    sb = s.clone()
+    lp().check(L.libp_pcg_jacobi_update_f64(Ng, invd.data_ptr(), pd.data_ptr(), Apd.data_ptr(), xb.data_ptr(),
+                                            rb.data_ptr(), zb.data_ptr(), sb.data_ptr(), ws.data_ptr(), st))
+    torch.cuda.synchronize()
+    assert torch.equal(xa, xb) and torch.equal(ra, rb) and torch.equal(za, zb)
+    alpha = sh[2]
+    x_ref, r_ref = x0 + alpha * p, r0 - alpha * Ap
+    assert np.max(np.abs(xa.cpu().numpy() - x_ref)) <= 4e-15 * np.max(np.abs(x_ref))
+    assert np.max(np.abs(ra.cpu().numpy() - r_ref)) <= 1e-12 * np.max(np.abs(r_ref))
+    assert np.max(np.abs(za.cpu().numpy() - invD * r_ref)) <= 1e-12 * np.max(np.abs(invD * r_ref))
+    sah, sbh = sa.cpu().numpy(), sb.cpu().numpy()
+    rr, rz = float(r_ref @ r_ref), float(r_ref @ (invD * r_ref))
+    assert abs(sah[5] - rr) <= 1e-11 * rr and abs(sah[0] - rz) <= 1e-11 * abs(rz)
+    assert abs(sah[5] - sbh[5]) <= 1e-13 * rr and abs(sah[0] - sbh[0]) <= 1e-13 * abs(rz)
+    assert sah[1] == rdotz and sah[9] == 1.0 and sah[8] == 0.0
+    assert abs(sah[3] - sah[0] / rdotz) <= 1e-15 * abs(sah[3])
+
+
+def test_stop_flag_skips_both_kernels():
+    """With scalars[STOP] set, neither kernel writes anything (chunked solves queue
+    iterations past convergence)."""
+    N = 3
+    o = OracleElliptic(box(3, N))
+    L = lp().lib()
+    E, Np, Ng = o.geti("Nelements"), o.geti("Np"), o.geti("Ngather")
+    rs, ci = o.array("rowStartsT"), o.array("colIdsT")
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    g2l, wJ, ggeo, D = dev(o.array("GlobalToLocal")), dev(o.array("wJ")), dev(o.array("ggeo")), dev(o.array("D"))
+    z = torch.ones(Ng, dtype=torch.float64, device="cuda")
+    AqL = torch.full((E * Np,), 7.0, dtype=torch.float64, device="cuda")
+    part = torch.zeros(L.libp_ax_hex_split_blocks(N, E), dtype=torch.float64, device="cuda")
+    s = torch.zeros(16, dtype=torch.float64, device="cuda")
+    s[8] = 1.0
+    s0 = s.clone()
+    st = torch.cuda.current_stream().cuda_stream
+    lp().check(L.libp_pcg_ax_hex_pap_f64(N, E, None, g2l.data_ptr(), wJ.data_ptr(), ggeo.data_ptr(), D.data_ptr(),
+                                         0.0, z.data_ptr(), AqL.data_ptr(), part.data_ptr(), s.data_ptr(), st))
+    gpart = torch.zeros(L.libp_pcg_gather_jacobi_partials(Ng), dtype=torch.float64, device="cuda")
+    x, r, zz = z.clone(), z.clone(), z.clone()
+    lp().check(L.libp_pcg_gather_jacobi_update_f64(Ng, dev(rs).data_ptr(), dev(ci).data_ptr(), AqL.data_ptr(),
+                                                   z.data_ptr(), z.data_ptr(), x.data_ptr(), r.data_ptr(),
+                                                   zz.data_ptr(), s.data_ptr(), gpart.data_ptr(), st))
+    torch.cuda.synchronize()
+    assert bool((AqL == 7.0).all())
+    assert bool((x == 1.0).all()) and bool((r == 1.0).all()) and bool((zz == 1.0).all())
+    assert torch.equal(s, s0)
+
+
+def test_fused_kernels_reject_bad_arguments():
+    L = lp().lib()
+    v = torch.zeros(64, dtype=torch.float64, device="cuda")
+    ptr = v.data_ptr()
+    assert L.libp_pcg_ax_hex_pap_f64(15, 1, None, ptr, ptr, ptr, ptr, 0.0, ptr, ptr, ptr, ptr, None) == 1
+    assert b"degree" in L.libp_last_error()
+    assert L.libp_pcg_gather_jacobi_update_f64(-1, ptr, ptr, ptr, ptr, ptr, ptr, ptr, ptr, ptr, ptr, None) == 1
+    assert L.libp_pcg_gather_jacobi_update_f64(8, ptr, ptr, ptr, ptr, ptr, ptr, ptr, ptr, ptr, None, None) == 1
+    assert L.libp_pcg_gather_jacobi_partials(-1) == -1
+
+
+@pytest.mark.parametrize("N,nx,curve,lam", [(4, 5, 0.0, 0.0), (7, 4, 0.05, 1.0), (3, 6, 0.05, 0.0)])
+@pytest.mark.parametrize("stop", ["ABS/REL-INITRESID", "RELATIVE"])
+def test_fused_solve_matches_four_launch_form_and_oracle(N, nx, curve, lam, stop):
+    """Whole PCG+Jacobi solves (elliptic_t::Run): the fused iteration against the
+    four-launch form ([PCG FUSED OPERATOR] FALSE) and the oracle: iterations within
+    one, solution norms to 1e-10."""
+    from tests import oracle_lib
+    s = box(nx, N, curve=curve, lam=lam, **{"LINEAR SOLVER STOPPING CRITERION": stop})
+    a = lp().Elliptic(dict(s, **{"PCG FUSED OPERATOR": "TRUE"}))
+    ra = a.run()
+    a.close()
+    b = lp().Elliptic(dict(s, **{"PCG FUSED OPERATOR": "FALSE"}))
+    rb = b.run()
+    b.close()
+    onorm, oit, _ = oracle_lib.run(s)
+    print(f"N={N} {nx}^3 {stop}: fused {ra['iterations']} its {ra['norm']:.15f}; four-launch {rb['iterations']} "
+          f"its {rb['norm']:.15f}; oracle {oit} its {onorm:.15f}", flush=True)
+    assert abs(ra["iterations"] - rb["iterations"]) <= 1 and abs(ra["iterations"] - oit) <= 1
+    assert abs(ra["norm"] - rb["norm"]) <= 1e-10 * abs(rb["norm"])
+    assert abs(ra["norm"] - onorm) <= 1e-10 * abs(onorm)

@@ -4,16 +4,19 @@ benchmarks.
     python -m libparanumal_amd.rocprof_summary stats  <dir> [--top 12] [--cmd "..."]
     python -m libparanumal_amd.rocprof_summary kernel <dir> --kernel <substring> [--grid G] [--cmd "..."]
     python -m libparanumal_amd.rocprof_summary kernels <dir> --match <sub1,sub2,...> [--cmd "..."]
+        (substrings separated by "|" instead when they contain commas)
     python -m libparanumal_amd.rocprof_summary traffic --fetch <dir> --write <dir>
-           --kernel <substring> --copy-bytes <B> [--cmd-fetch ...] [--cmd-write ...] [--json out.json]
+           --kernel <substring> --copy-bytes <B> [--copy-kernel <substring>] [--grid G]
+           [--cmd-fetch ...] [--cmd-write ...] [--json out.json]
 
 `stats` reads `*_kernel_stats.csv` (from `rocprofv3 --kernel-trace --stats
 --output-format csv`). `traffic` reads `*_counter_collection.csv` from two
 `--pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs. Both counters are in KiB per
 dispatch. Each is calibrated against a plain device-to-device copy of known
-size in the same run: the factor is (copy bytes) / (counter KiB x 1024 of the
-copy dispatch). The kernel's HBM bytes per dispatch are its counter times that
-factor.
+size in the same run (by default the largest dispatch of the library's HBM
+copy probe, hbm_copy_kernel, which `bench.py --counters` runs on 1 GiB): the
+factor is (copy bytes) / (counter KiB x 1024 of the copy dispatch). The
+kernel's HBM bytes per dispatch are its counter times that factor.
 """
 import argparse
 import csv
@@ -55,28 +58,36 @@ def stats_text(d, top=12, cmd=None):
 
 
 def kernel_durations(d, kernel, grid=None):
-    """Durations (ns) of the dispatches of one kernel from *kernel_trace.csv,
-    optionally only those with Grid_Size_X == grid."""
+    """Durations (ns) of the dispatches of one kernel from *kernel_trace.csv in
+    start order, optionally only those with Grid_Size_X == grid."""
     out = []
     for r in csv.DictReader(open(_one(os.path.join(d, "*kernel_trace.csv")))):
         if kernel not in r["Kernel_Name"]:
             continue
         if grid is not None and int(r["Grid_Size_X"]) != grid:
             continue
-        out.append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+        out.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]) - int(r["Start_Timestamp"])))
     if not out:
         raise SystemExit(f"no dispatch of {kernel!r} with grid {grid}")
-    return out
+    return [dur for _, dur in sorted(out)]
 
 
-def kernel_text(d, kernel, grid=None, cmd=None):
-    v = sorted(kernel_durations(d, kernel, grid))
+def _line(label, v):
+    v = sorted(v)
+    return (f"{label}: dispatches {len(v)}  mean {sum(v) / len(v) / 1e3:.1f} us  median {v[len(v) // 2] / 1e3:.1f} us  "
+            f"min {v[0] / 1e3:.1f} us  max {v[-1] / 1e3:.1f} us")
+
+
+def kernel_text(d, kernel, grid=None, cmd=None, first=None, count=None):
+    v = kernel_durations(d, kernel, grid)
     out = []
     if cmd:
         out.append(f"command: {cmd}")
     out.append(f"kernel: {kernel}" + (f"  grid size: {grid}" if grid is not None else ""))
-    out.append(f"dispatches {len(v)}  mean {sum(v) / len(v) / 1e3:.1f} us  median {v[len(v) // 2] / 1e3:.1f} us  "
-               f"min {v[0] / 1e3:.1f} us  max {v[-1] / 1e3:.1f} us")
+    out.append(_line("all", v))
+    if first is not None:
+        w = v[first:first + (count or len(v))]
+        out.append(_line(f"dispatches {first}..{first + len(w) - 1} (in start order)", w))
     return "\n".join(out)
 
 
@@ -100,12 +111,15 @@ def kernels_text(d, subs, cmd=None):
     return "\n".join(out)
 
 
-def counters(d, counter):
-    """{kernel name: [value per dispatch]} for one counter."""
+def counters(d, counter, grid=None):
+    """{kernel name: [value per dispatch]} for one counter (only dispatches of
+    grid size `grid` when given)."""
     f = _one(os.path.join(d, "*counter_collection.csv"))
     agg = {}
     for r in csv.DictReader(open(f)):
         if r.get("Counter_Name") != counter:
+            continue
+        if grid is not None and int(r.get("Grid_Size", r.get("Grid_Size_X", -1))) != grid:
             continue
         agg.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
     if not agg:
@@ -125,13 +139,12 @@ def _median(v):
     return v[len(v) // 2]
 
 
-def traffic(fetch_dir, write_dir, kernel, copy_bytes, copy_kernel="copyBuffer"):
+def traffic(fetch_dir, write_dir, kernel, copy_bytes, copy_kernel="hbm_copy_kernel", grid=None):
     res = {"kernel": None, "copy_bytes": copy_bytes}
     total = 0.0
     for tag, d, counter in (("fetch", fetch_dir, "FETCH_SIZE"), ("write", write_dir, "WRITE_SIZE")):
-        agg = counters(d, counter)
-        name, vals = _pick(agg, kernel)
-        _, cvals = _pick(agg, copy_kernel)
+        name, vals = _pick(counters(d, counter, grid), kernel)
+        _, cvals = _pick(counters(d, counter), copy_kernel)
         copy_kib = max(cvals)  # the large calibration copy is the biggest copy dispatch
         factor = copy_bytes / (copy_kib * 1024.0)
         kib = _median(vals)
@@ -154,6 +167,8 @@ def main(argv=None):
     k.add_argument("dir")
     k.add_argument("--kernel", required=True)
     k.add_argument("--grid", type=int)
+    k.add_argument("--first", type=int, help="also summarise the dispatches from this index (start order)")
+    k.add_argument("--count", type=int)
     k.add_argument("--cmd")
     ks = sub.add_parser("kernels")
     ks.add_argument("dir")
@@ -164,6 +179,8 @@ def main(argv=None):
     t.add_argument("--write", required=True)
     t.add_argument("--kernel", required=True)
     t.add_argument("--copy-bytes", type=float, required=True)
+    t.add_argument("--copy-kernel", default="hbm_copy_kernel")
+    t.add_argument("--grid", type=int)
     t.add_argument("--cmd-fetch")
     t.add_argument("--cmd-write")
     t.add_argument("--json")
@@ -172,12 +189,12 @@ def main(argv=None):
         print(stats_text(a.dir, a.top, a.cmd))
         return 0
     if a.mode == "kernel":
-        print(kernel_text(a.dir, a.kernel, a.grid, a.cmd))
+        print(kernel_text(a.dir, a.kernel, a.grid, a.cmd, a.first, a.count))
         return 0
     if a.mode == "kernels":
-        print(kernels_text(a.dir, a.match.split(","), a.cmd))
+        print(kernels_text(a.dir, a.match.split("|" if "|" in a.match else ","), a.cmd))
         return 0
-    r = traffic(a.fetch, a.write, a.kernel, a.copy_bytes)
+    r = traffic(a.fetch, a.write, a.kernel, a.copy_bytes, a.copy_kernel, a.grid)
     r["commands"] = {"fetch": a.cmd_fetch, "write": a.cmd_write}
     if a.json:
         with open(a.json, "w") as f:

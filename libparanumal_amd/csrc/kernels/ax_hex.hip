@@ -133,17 +133,25 @@ __global__ void __launch_bounds__(NE * Nq * Nq) __attribute__((amdgpu_waves_per_
 
   T r_q[Nq], r_Aq[Nq];
   const size_t base = (size_t)element * Np + ij;
+  if (MODE == 0) {
+    // all Nq index loads in flight, then all Nq value loads: the value loads are
+    // unconditional (a masked node, id -1, reads q[0] and takes 0), since a load
+    // under (id >= 0) compiles to a branch that waits for each load in turn
+    int ids[Nq];
 #pragma unroll
-  for (int k = 0; k < Nq; ++k) {
-    if (MODE == 0) {
-      const int id = __builtin_nontemporal_load(G2L + base + k * NqNq);
-      r_q[k] = (id >= 0) ? q[id] : T(0);
-      if (OUT == 1) s_id[es][k][ij] = id;
-    } else {
-      r_q[k] = __builtin_nontemporal_load(q + base + k * NqNq);
+    for (int k = 0; k < Nq; ++k) ids[k] = __builtin_nontemporal_load(G2L + base + k * NqNq);
+#pragma unroll
+    for (int k = 0; k < Nq; ++k) {
+      const T v = q[max(ids[k], 0)];
+      r_q[k] = (ids[k] >= 0) ? v : T(0);
+      if (OUT == 1) s_id[es][k][ij] = ids[k];
     }
-    r_Aq[k] = 0;
+  } else {
+#pragma unroll
+    for (int k = 0; k < Nq; ++k) r_q[k] = __builtin_nontemporal_load(q + base + k * NqNq);
   }
+#pragma unroll
+  for (int k = 0; k < Nq; ++k) r_Aq[k] = 0;
   const T* gb = ggeo + (size_t)element * 6 * Np + ij;
   const T* wb = wJ + (size_t)element * Np + ij;
   __syncthreads();

@@ -62,7 +62,10 @@ __global__ void __launch_bounds__(256, 1) ax_hex_mfma_n15_kernel(const int* __re
     for (int u = 0; u < 16; ++u) id[u] = __builtin_nontemporal_load(G2L + eb + t + 256 * u);
     double v[16];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) v[u] = (id[u] >= 0) ? q[id[u]] : 0.0;
+    for (int u = 0; u < 16; ++u) {
+      const double qv = q[max(id[u], 0)];  // unconditional: masked nodes read q[0] and take 0
+      v[u] = (id[u] >= 0) ? qv : 0.0;
+    }
 #pragma unroll
     for (int u = 0; u < 16; ++u) s_q[t + 256 * u] = v[u];
   }

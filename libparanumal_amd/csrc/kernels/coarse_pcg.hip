@@ -41,7 +41,10 @@ __device__ inline Hex1Node hex1_load(const int e, const int n, const int* __rest
                                      const bool withP) {
   Hex1Node v;
   const int g = G2L[(size_t)e * 8 + n];
-  v.q = (g < 0) ? 0.0 : (withP ? z[g] + beta * p[g] : z[g]);
+  const int gc = max(g, 0);  // unconditional loads: a masked node reads entry 0 and takes 0
+  const double zv = z[gc];
+  const double pv = withP ? p[gc] : 0.0;
+  v.q = (g < 0) ? 0.0 : (withP ? zv + beta * pv : zv);
   const double* G = ggeo + (size_t)e * 48 + n;
   v.G00 = G[0];
   v.G01 = G[8];
@@ -187,9 +190,9 @@ __global__ void __launch_bounds__(RBS) coarse_gather_pap_kernel(const int Nrows,
       int c[8];
       double xv[8];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) c[k] = (k < len) ? colIds[a + k] : 0;
+      for (int k = 0; k < 8; ++k) c[k] = colIds[row_copy(a, len, k)];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) xv[k] = (k < len) ? AqL[c[k]] : 0.0;
+      for (int k = 0; k < 8; ++k) xv[k] = AqL[c[k]];
 #pragma unroll
       for (int k = 0; k < 8; ++k)
         if (k < len) ap += xv[k];
@@ -441,9 +444,9 @@ __global__ void __launch_bounds__(RBS) coarse_cg1_gather_kernel(const int Nrows,
       int c[8];
       double xv[8];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) c[k] = (k < len) ? colIds[a + k] : 0;
+      for (int k = 0; k < 8; ++k) c[k] = colIds[row_copy(a, len, k)];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) xv[k] = (k < len) ? AqL[c[k]] : 0.0;
+      for (int k = 0; k < 8; ++k) xv[k] = AqL[c[k]];
 #pragma unroll
       for (int k = 0; k < 8; ++k)
         if (k < len) wg += xv[k];
@@ -537,19 +540,27 @@ __global__ void __launch_bounds__(CSR_THREADS) coarse_cg1_csr_kernel(
     }
     int c[CSR_UNROLL];
     double v[CSR_UNROLL], cr[CSR_UNROLL], cw[CSR_UNROLL], cs[CSR_UNROLL], cd[CSR_UNROLL];
+    // the loads are unconditional, at positions clamped into the row (a row
+    // holds at least its diagonal; a lane past the last row reads entry 0), so
+    // they are all in flight together; entries past the row's end are dropped
+    // from the sum through c[k] = -1
+    int cj[CSR_UNROLL];
 #pragma unroll
     for (int k = 0; k < CSR_UNROLL; ++k) {
       const int jk = a + sub + CSR_LANES * k;
-      c[k] = (jk < b) ? colIds[jk] : -1;
-      v[k] = (jk < b) ? vals[jk] : 0.0;
+      const int jc = min(jk, max(b - 1, 0));
+      cj[k] = colIds[jc];
+      const double vk = vals[jc];
+      c[k] = (jk < b) ? cj[k] : -1;
+      v[k] = (jk < b) ? vk : 0.0;
     }
 #pragma unroll
     for (int k = 0; k < CSR_UNROLL; ++k) {
-      const int j = c[k];
-      cr[k] = (j >= 0) ? cur.r[j] : 0.0;
-      cd[k] = (j >= 0) ? invD[j] : 0.0;
-      cw[k] = (j >= 0 && mode > 0) ? wcur[j] : 0.0;
-      cs[k] = (j >= 0 && mode > 1) ? cur.s[j] : 0.0;
+      const int j = cj[k];
+      cr[k] = cur.r[j];
+      cd[k] = invD[j];
+      cw[k] = (mode > 0) ? wcur[j] : 0.0;
+      cs[k] = (mode > 1) ? cur.s[j] : 0.0;
     }
     if (first) {
       cg1_scalars(mode, P, tol2, nb, s, wsIn, s_ab, s_stop);

@@ -38,6 +38,24 @@ inline unsigned stream_blocks(long N, int bs, long cap = 65536) {
   return (unsigned)b;
 }
 
+// Position of copy k of a CSR row [a, a + len), clamped into the row: the
+// gather kernels load the first eight copies of a row unconditionally (copies
+// past the row's end re-read its last copy, which is not summed), so all the
+// loads are independent and in flight together. A load under `k < len` is
+// compiled as a branch that waits for each load before the next one is issued.
+// A row with len == 0 (and a thread past the last row, a = len = 0) reads an
+// earlier valid entry; the arrays hold at least one entry.
+__device__ inline int row_copy(const int a, const int len, const int k) { return min(a + k, max(a + len - 1, 0)); }
+
+// The largest v over the 64 lanes of a wavefront, as a scalar: a loop over
+// k < wave_max_uniform(len) is a uniform branch, so the loads it guards need
+// not wait for one another. Every lane of the wavefront must be active.
+__device__ inline int wave_max_uniform(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
 // Wavefront (64-lane) sum of a double via DPP-friendly shuffles.
 __device__ inline double wave_sum(double v) {
 #pragma unroll

@@ -12,6 +12,7 @@
 //
 // Correction at gathered node g (rows of the masked gather):
 //   c[g] = (1/|row g|) sum_{copies j} s_{e(j)} y[j] + omega D^{-1}_g r[g].
+#include "gather_block.hpp"
 #include "reduce.hpp"
 
 namespace libp_amd {
@@ -56,7 +57,9 @@ __global__ void __launch_bounds__(256) learned_features_kernel(const int E, cons
 #pragma unroll
   for (int i = 0; i < NPER; ++i) {
     const int g = gid[i];
-    z[i] = (g >= 0) ? invD[g] * r[g] : 0.f;
+    const int gc = max(g, 0);  // unconditional loads: a missing node reads entry 0 and takes 0
+    const float dv = invD[gc], rv = r[gc];
+    z[i] = (g >= 0) ? dv * rv : 0.f;
   }
 #pragma unroll
   for (int i = 0; i < NPER; ++i) ss += (double)z[i] * z[i];
@@ -77,59 +80,38 @@ __global__ void __launch_bounds__(256) learned_features_kernel(const int E, cons
 // ACC: c += the correction (the smoother's x += c in the same pass; the sum
 // rounds once, as the separate axpy it replaces does).
 template <int SHIFT, bool ACC = false>
-__global__ void __launch_bounds__(256) learned_assemble_kernel(const int Nrows, const int* __restrict__ rowStarts,
-                                                               const int* __restrict__ colIds, const int Np,
-                                                               const float* __restrict__ Y, const long ldy,
-                                                               const float* __restrict__ scale,
-                                                               const float* __restrict__ omega,
-                                                               const float* __restrict__ r,
-                                                               const float* __restrict__ invD, float* __restrict__ c) {
+__global__ void __launch_bounds__(GB_THREADS) learned_assemble_kernel(const int Nrows, const int* __restrict__ rowStarts,
+                                                                      const int* __restrict__ colIds, const int Np,
+                                                                      const float* __restrict__ Y, const long ldy,
+                                                                      const float* __restrict__ scale,
+                                                                      const float* __restrict__ omega,
+                                                                      const float* __restrict__ r,
+                                                                      const float* __restrict__ invD,
+                                                                      float* __restrict__ c) {
+  // rows in row blocks (gather_block_sums_by): the copies' s_e * y values staged in
+  // LDS, each row summed from zero in ascending order
+  __shared__ float s_v[GB_CAP];
   const float om = omega ? *omega : 0.f;
-  auto elem = [&](int id) { return SHIFT >= 0 ? (id >> SHIFT) : id / Np; };
-  // two rows per step (row and row + S), every load of both issued before the
-  // sums, as in the gather; each row's arithmetic is unchanged
-  const long S = (long)gridDim.x * blockDim.x;
-  for (long g0 = blockIdx.x * (long)blockDim.x + threadIdx.x; g0 < Nrows; g0 += 2 * S) {
-    int a[2], len[2];
+  auto copy_value = [=](int id) {
+    const int e = SHIFT >= 0 ? (id >> SHIFT) : id / Np;
+    const int n = id - e * Np;
+    return scale[e] * Y[(long)e * ldy + n];
+  };
+  const long nrb = gather_row_blocks(Nrows);
+  for (long rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+    GatherRows<float> g;
+    gather_block_sums_by(rb, Nrows, rowStarts, colIds, copy_value, s_v, g);
 #pragma unroll
-    for (int r2 = 0; r2 < 2; ++r2) {
-      const long g = g0 + r2 * S;
-      a[r2] = (g < Nrows) ? rowStarts[g] : 0;
-      len[r2] = (g < Nrows) ? rowStarts[g + 1] - a[r2] : 0;
-    }
-    int col[2][8];
-    float y[2][8], sc[2][8];
-#pragma unroll
-    for (int r2 = 0; r2 < 2; ++r2)
-#pragma unroll
-      for (int k = 0; k < 8; ++k) col[r2][k] = (k < len[r2]) ? colIds[a[r2] + k] : 0;
-#pragma unroll
-    for (int r2 = 0; r2 < 2; ++r2)
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const int e = elem(col[r2][k]), n = col[r2][k] - e * Np;
-        y[r2][k] = (k < len[r2]) ? Y[(long)e * ldy + n] : 0.f;
-        sc[r2][k] = (k < len[r2]) ? scale[e] : 0.f;
-      }
-#pragma unroll
-    for (int r2 = 0; r2 < 2; ++r2) {
-      const long g = g0 + r2 * S;
-      if (g >= Nrows) continue;
-      float acc = 0.f;
-#pragma unroll
-      for (int k = 0; k < 8; ++k)
-        if (k < len[r2]) acc += sc[r2][k] * y[r2][k];
-      for (int j = a[r2] + 8; j < a[r2] + len[r2]; ++j) {
-        const int id = colIds[j];
-        const int e = elem(id), n = id - e * Np;
-        acc += scale[e] * Y[(long)e * ldy + n];
-      }
-      float v = (len[r2] > 0 ? acc / (float)len[r2] : 0.f);
-      if (omega) v += om * invD[g] * r[g];
+    for (int u = 0; u < 2; ++u) {
+      if (!g.in[u]) continue;
+      const long row = g.row[u];
+      const int len = g.len[u];
+      float v = (len > 0 ? g.val[u] / (float)len : 0.f);
+      if (omega) v += om * invD[row] * r[row];
       if (ACC)
-        c[g] = 1.f * v + 1.f * c[g];
+        c[row] = 1.f * v + 1.f * c[row];
       else
-        c[g] = v;
+        c[row] = v;
     }
   }
 }
@@ -235,7 +217,9 @@ __global__ void __launch_bounds__(256) learned_scatter_grad_kernel(const int E, 
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < NL; i += (long)gridDim.x * blockDim.x) {
     const int e = (int)(i / Np), n = (int)(i - (long)e * Np);
     const int g = G2L[i];
-    const float v = (g >= 0) ? gc[g] * scale[e] * invLen[g] : 0.f;
+    const int gs = max(g, 0);
+    const float gv = gc[gs], lv = invLen[gs];
+    const float v = (g >= 0) ? gv * scale[e] * lv : 0.f;
     dY[(long)e * lddy + n] = f2bf_l(v);
   }
 }
@@ -321,7 +305,7 @@ static int assemble(int Nrows, const int* rowStarts, const int* colIds, int Np, 
   if (Nrows <= 0) return LIBP_OK;
   if (!rowStarts || !colIds || !Y || !scale || !r || !invDiag || !c) return arg_error("learned_assemble", "null pointer");
   if (Np <= 0 || ldy < Np) return arg_error("learned_assemble", "bad size");
-  const dim3 grid(stream_blocks(Nrows, 1024)), block(256);
+  const dim3 grid(row_block_grid(Nrows)), block(GB_THREADS);
   hipStream_t st = as_stream(stream);
   if (Np == 512)
     hipLaunchKernelGGL((learned_assemble_kernel<9, ACC>), grid, block, 0, st, Nrows, rowStarts, colIds, Np, Y, ldy,

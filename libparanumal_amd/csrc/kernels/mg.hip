@@ -26,11 +26,19 @@ __global__ void __launch_bounds__(NqF * NqF) mg_coarsen_kernel(const int NqC, co
   const int element = elementList ? elementList[e] : e;
   const int t = threadIdx.x, i = t % NqF, j = t / NqF;
   const int NpC = NqC * NqC * NqC;
+  // index loads first, then unconditional value loads (masked nodes read entry 0 and take 0):
+  // loads under (id >= 0) compile to branches that wait for each load in turn
+  int ids[NqF];
+#pragma unroll
+  for (int k = 0; k < NqF; ++k) ids[k] = G2L[(size_t)element * NpF + k * NqF * NqF + t];
+#pragma unroll
   for (int k = 0; k < NqF; ++k) {
-    const int id = G2L[(size_t)element * NpF + k * NqF * NqF + t];
+    const int id = max(ids[k], 0);
     // w: the inverse-multiplicity weight of the fine residual (ellipticPreconMultiGridLevel.cpp:51),
     // applied here instead of in a separate pass (the same single rounding)
-    s_f[k][j][i] = (id >= 0) ? (w ? T(1) * w[id] * qf[id] : qf[id]) : T(0);
+    const T qv = qf[id];
+    const T wv = w ? w[id] : T(1);
+    s_f[k][j][i] = (ids[k] >= 0) ? (w ? T(1) * wv * qv : qv) : T(0);
   }
   if (t < NqF * NqC) s_P[t / NqC][t % NqC] = P[t];
   __syncthreads();
@@ -71,11 +79,20 @@ __global__ void __launch_bounds__(NqF * NqF) mg_prolong_kernel(const int NqC, co
   const int element = elementList ? elementList[e] : e;
   const int t = threadIdx.x, i = t % NqF, j = t / NqF;
   const int NpC = NqC * NqC * NqC;
-  if (i < NqC && j < NqC)
-    for (int k = 0; k < NqC; ++k) {
-      const int id = G2LC[(size_t)element * NpC + i + j * NqC + k * NqC * NqC];
-      s_ccc[k][j][i] = (id >= 0) ? qcIn[id] : T(0);
-    }
+  if (i < NqC && j < NqC) {
+    // index loads first, then unconditional value loads (a masked node reads entry 0 and
+    // takes 0); NqC < NqF, and k < NqC is the same for every lane
+    int ids[NqF - 1];
+#pragma unroll
+    for (int k = 0; k < NqF - 1; ++k)
+      if (k < NqC) ids[k] = G2LC[(size_t)element * NpC + i + j * NqC + k * NqC * NqC];
+#pragma unroll
+    for (int k = 0; k < NqF - 1; ++k)
+      if (k < NqC) {
+        const T qv = qcIn[max(ids[k], 0)];
+        s_ccc[k][j][i] = (ids[k] >= 0) ? qv : T(0);
+      }
+  }
   if (t < NqF * NqC) s_P[t / NqC][t % NqC] = P[t];
   __syncthreads();
   if (j < NqC)

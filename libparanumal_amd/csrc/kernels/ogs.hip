@@ -10,55 +10,67 @@
 // which is the reference's order, so Gather results are bit-identical.
 #include <algorithm>
 
+#include "gather_block.hpp"
 #include "reduce.hpp"
 
 namespace libp_amd {
 
-// Each thread takes two rows (row and row + S, S = threads in the grid) per
-// step and issues the loads of both before summing either: a row is a chain
-// of three dependent loads (row start, column ids, values), and two chains in
-// flight per thread keep more bytes in flight. The grid gives each thread
-// about four rows. Each sum runs from zero over the row in ascending order.
+// The first eight copies of two CSR rows (a[r], len[r]) for the gather
+// kernels: the index loads, then the value loads, each up to the longest row
+// of the wavefront, all issued before any is used. A load under the lane's
+// own `k < len` is compiled as a branch that waits for each load in turn, and
+// loading all eight for every row issues 8x the loads short rows need: the
+// bound is the wavefront's longest row, a scalar, and the positions are
+// clamped into each row. colIds null: the copies of a row are v[a, a + len)
+// (the split operator's slot layout). Every lane of the wavefront must be
+// active (the gather loops step by whole wavefronts).
+template <typename T>
+__device__ inline void gather_rows2(const int* __restrict__ colIds, const T* __restrict__ v, const int (&a)[2],
+                                    const int (&len)[2], T (&x)[2][8]) {
+  const int wl = wave_max_uniform(max(len[0], len[1]));
+  int c[2][8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+      c[r][k] = (k < wl) ? (colIds ? colIds[row_copy(a[r], len[r], k)] : row_copy(a[r], len[r], k)) : 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+#pragma unroll
+    for (int r = 0; r < 2; ++r) x[r][k] = (k < wl) ? v[c[r][k]] : T(0);
+}
+
+// A workgroup takes row blocks of GB_ROWS rows (gather_block_sums: the copies
+// staged in LDS, each row summed from zero in ascending order, the reference's
+// order, so results are bit-identical to a thread-per-row gather).
 // OP 0: gv = sum; OP 1: gv = r - sum (the residual r - A x of an operator
 // whose last step is this gather); OP 2: gv += sum. Each fused form rounds
 // once, as the separate axpy it replaces does, so results are bit-identical.
 template <typename T, int OP = 0>
-__global__ void __launch_bounds__(256) ogs_gather_kernel(const int Nrows, const int* __restrict__ rowStarts,
-                                                         const int* __restrict__ colIds, const T* __restrict__ v,
-                                                         const T* __restrict__ r_vec, T* __restrict__ gv) {
-  const long S = (long)gridDim.x * blockDim.x;
-  for (long row0 = blockIdx.x * (long)blockDim.x + threadIdx.x; row0 < Nrows; row0 += 2 * S) {
-    int a[2], len[2];
+__global__ void __launch_bounds__(GB_THREADS) ogs_gather_kernel(const int Nrows, const int* __restrict__ rowStarts,
+                                                                const int* __restrict__ colIds,
+                                                                const T* __restrict__ v, const T* __restrict__ r_vec,
+                                                                T* __restrict__ gv) {
+  __shared__ T s_v[GB_CAP];
+  const long nrb = gather_row_blocks(Nrows);
+  for (long rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
     T rr[2] = {T(0), T(0)};
+    if (OP != 0) {  // the row's other operand, loaded before the gather's loads complete
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      const long row = row0 + r * S;
-      a[r] = (row < Nrows) ? rowStarts[row] : 0;
-      len[r] = (row < Nrows) ? rowStarts[row + 1] - a[r] : 0;
-      if (OP == 1 && row < Nrows) rr[r] = r_vec[row];
+      for (int u = 0; u < 2; ++u) {
+        const long row = min(rb * GB_ROWS + threadIdx.x + u * GB_THREADS, (long)Nrows - 1);
+        rr[u] = (OP == 1) ? r_vec[row] : gv[row];
+      }
     }
-    int c[2][8];
-    T x[2][8];
+    GatherRows<T> g;
+    gather_block_sums(rb, Nrows, rowStarts, colIds, v, s_v, g);
 #pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-      for (int k = 0; k < 8; ++k) c[r][k] = (k < len[r]) ? colIds[a[r] + k] : 0;
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-      for (int k = 0; k < 8; ++k) x[r][k] = (k < len[r]) ? v[c[r][k]] : T(0);
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      const long row = row0 + r * S;
-      if (row >= Nrows) continue;
-      T val = 0;
-#pragma unroll
-      for (int k = 0; k < 8; ++k)
-        if (k < len[r]) val += x[r][k];
-      for (int g = a[r] + 8; g < a[r] + len[r]; ++g) val += v[colIds[g]];
-      if (OP == 0) gv[row] = val;
-      if (OP == 1) gv[row] = T(1) * rr[r] + T(-1) * val;
-      if (OP == 2) gv[row] = T(1) * val + T(1) * gv[row];
+    for (int u = 0; u < 2; ++u) {
+      if (!g.in[u]) continue;
+      const T val = g.val[u];
+      if (OP == 0) gv[g.row[u]] = val;
+      if (OP == 1) gv[g.row[u]] = T(1) * rr[u] + T(-1) * val;
+      if (OP == 2) gv[g.row[u]] = T(1) * val + T(1) * rr[u];
     }
   }
 }
@@ -93,42 +105,21 @@ __device__ inline void cheb_row_update(const long row, const T val, const T* __r
 }
 
 template <typename T, bool START>
-__global__ void __launch_bounds__(256) ogs_gather_cheb_kernel(const int Nrows, const int* __restrict__ rowStarts,
-                                                              const int* __restrict__ colIds,
-                                                              const T* __restrict__ v, const T* __restrict__ invDiag,
-                                                              const T* __restrict__ rvec, const T c1, const T c2,
-                                                              T* __restrict__ RES, T* __restrict__ d,
-                                                              T* __restrict__ X) {
-  const long S = (long)gridDim.x * blockDim.x;
-  for (long row0 = blockIdx.x * (long)blockDim.x + threadIdx.x; row0 < Nrows; row0 += 2 * S) {
-    int a[2], len[2];
+__global__ void __launch_bounds__(GB_THREADS) ogs_gather_cheb_kernel(const int Nrows, const int* __restrict__ rowStarts,
+                                                                     const int* __restrict__ colIds,
+                                                                     const T* __restrict__ v,
+                                                                     const T* __restrict__ invDiag,
+                                                                     const T* __restrict__ rvec, const T c1,
+                                                                     const T c2, T* __restrict__ RES,
+                                                                     T* __restrict__ d, T* __restrict__ X) {
+  __shared__ T s_v[GB_CAP];
+  const long nrb = gather_row_blocks(Nrows);
+  for (long rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+    GatherRows<T> g;
+    gather_block_sums(rb, Nrows, rowStarts, colIds, v, s_v, g);
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      const long row = row0 + r * S;
-      a[r] = (row < Nrows) ? rowStarts[row] : 0;
-      len[r] = (row < Nrows) ? rowStarts[row + 1] - a[r] : 0;
-    }
-    int c[2][8];
-    T x[2][8];
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-      for (int k = 0; k < 8; ++k) c[r][k] = (k < len[r]) ? colIds[a[r] + k] : 0;
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-      for (int k = 0; k < 8; ++k) x[r][k] = (k < len[r]) ? v[c[r][k]] : T(0);
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      const long row = row0 + r * S;
-      if (row >= Nrows) continue;
-      T val = 0;
-#pragma unroll
-      for (int k = 0; k < 8; ++k)
-        if (k < len[r]) val += x[r][k];
-      for (int g = a[r] + 8; g < a[r] + len[r]; ++g) val += v[colIds[g]];
-      cheb_row_update<T, START>(row, val, invDiag, rvec, c1, c2, RES, d, X);
-    }
+    for (int u = 0; u < 2; ++u)
+      if (g.in[u]) cheb_row_update<T, START>(g.row[u], g.val[u], invDiag, rvec, c1, c2, RES, d, X);
   }
 }
 
@@ -142,47 +133,29 @@ __global__ void __launch_bounds__(256) ogs_gather_cheb_kernel(const int Nrows, c
 // (linearSolverPCG.cpp:134-140). Both return at once when scalars[STOP] is set.
 constexpr int GATHER_PAP_MAXB = RNPART * RMAXB;
 
-__global__ void __launch_bounds__(256) ogs_gather_pap_kernel(const int Nrows, const int* __restrict__ rowStarts,
-                                                             const int* __restrict__ colIds,
-                                                             const double* __restrict__ v, double* __restrict__ gv,
-                                                             const double* __restrict__ y,
-                                                             const double* __restrict__ s,
-                                                             ReduceWS* __restrict__ ws) {
+__global__ void __launch_bounds__(GB_THREADS) ogs_gather_pap_kernel(const int Nrows, const int* __restrict__ rowStarts,
+                                                                    const int* __restrict__ colIds,
+                                                                    const double* __restrict__ v,
+                                                                    double* __restrict__ gv,
+                                                                    const double* __restrict__ y,
+                                                                    const double* __restrict__ s,
+                                                                    ReduceWS* __restrict__ ws) {
+  __shared__ double s_v[GB_CAP];
   __shared__ double s_w[16];
   if (s[8] != 0.0) return;
-  const long S = (long)gridDim.x * blockDim.x;
+  const long nrb = gather_row_blocks(Nrows);
   double acc = 0;
-  for (long row0 = blockIdx.x * (long)blockDim.x + threadIdx.x; row0 < Nrows; row0 += 2 * S) {
-    int a[2], len[2];
+  for (long rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
     double yv[2];
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      const long row = row0 + r * S;
-      a[r] = (row < Nrows) ? rowStarts[row] : 0;
-      len[r] = (row < Nrows) ? rowStarts[row + 1] - a[r] : 0;
-      yv[r] = (row < Nrows) ? y[row] : 0.0;
-    }
-    int c[2][8];
-    double x[2][8];
+    for (int u = 0; u < 2; ++u) yv[u] = y[min(rb * GB_ROWS + threadIdx.x + u * GB_THREADS, (long)Nrows - 1)];
+    GatherRows<double> g;
+    gather_block_sums(rb, Nrows, rowStarts, colIds, v, s_v, g);
 #pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-      for (int k = 0; k < 8; ++k) c[r][k] = (k < len[r]) ? colIds[a[r] + k] : 0;
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-      for (int k = 0; k < 8; ++k) x[r][k] = (k < len[r]) ? v[c[r][k]] : 0.0;
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      const long row = row0 + r * S;
-      if (row >= Nrows) continue;
-      double val = 0;
-#pragma unroll
-      for (int k = 0; k < 8; ++k)
-        if (k < len[r]) val += x[r][k];
-      for (int g = a[r] + 8; g < a[r] + len[r]; ++g) val += v[colIds[g]];
-      gv[row] = val;
-      acc += yv[r] * val;
+    for (int u = 0; u < 2; ++u) {
+      if (!g.in[u]) continue;
+      gv[g.row[u]] = g.val[u];
+      acc += yv[u] * g.val[u];
     }
   }
   const double b = block_sum(acc, s_w);
@@ -220,38 +193,28 @@ __global__ void __launch_bounds__(256) ogs_gather_face_kernel(const int Nrows, c
   if (PAP && s[8] != 0.0) return;
   const long S = (long)gridDim.x * blockDim.x;
   double acc = 0;
-  for (long f0 = blockIdx.x * (long)blockDim.x + threadIdx.x; f0 < Nrows; f0 += 2 * S) {
+  for (long w0 = blockIdx.x * (long)blockDim.x + (threadIdx.x & ~63); w0 < Nrows; w0 += 2 * S) {
+    const long f0 = w0 + (threadIdx.x & 63);
     int a[2], len[2], row[2];
     double yv[2] = {0.0, 0.0};
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
       const long f = f0 + r * S;
-      row[r] = (f < Nrows) ? rows[f] : 0;
-      a[r] = (f < Nrows) ? rowStarts[f] : 0;
-      len[r] = (f < Nrows) ? rowStarts[f + 1] - a[r] : 0;
+      const long fc = min(f, (long)Nrows - 1);  // clamped: the loads of both rows in flight together
+      const int rw = rows[fc], s0 = rowStarts[fc], s1 = rowStarts[fc + 1];
+      row[r] = (f < Nrows) ? rw : 0;
+      a[r] = (f < Nrows) ? s0 : 0;
+      len[r] = (f < Nrows) ? s1 - s0 : 0;
     }
     if (PAP) {
 #pragma unroll
-      for (int r = 0; r < 2; ++r)
-        if (f0 + r * S < Nrows) yv[r] = y[row[r]];
+      for (int r = 0; r < 2; ++r) {
+        const double yc = y[row[r]];
+        yv[r] = (f0 + r * S < Nrows) ? yc : 0.0;
+      }
     }
-    int c[2][8];
     double x[2][8];
-    if (colIds) {
-#pragma unroll
-      for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int k = 0; k < 8; ++k) c[r][k] = (k < len[r]) ? colIds[a[r] + k] : 0;
-    } else {  // slot layout: the row's copies are v[a .. a + len)
-#pragma unroll
-      for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int k = 0; k < 8; ++k) c[r][k] = a[r] + k;
-    }
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-      for (int k = 0; k < 8; ++k) x[r][k] = (k < len[r]) ? v[c[r][k]] : 0.0;
+    gather_rows2(colIds, v, a, len, x);  // colIds null: slot layout, copies v[a .. a + len)
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
       if (f0 + r * S >= Nrows) continue;
@@ -296,70 +259,53 @@ inline unsigned face_gather_blocks(int Nrows) { return stream_blocks(Nrows, 1024
 // ellipticPreconJacobi.cpp:42-51). Ap is never stored. Each block stores its
 // r.r and r.z partials at part[b] and part[nb + b] (nb = grid size);
 // pcg_gather_jacobi_finish_kernel sums them in block order. Both return at
-// once while s[8] (STOP) is set. The grid gives each thread about
-// GATHER_JACOBI_ROWS rows (two per loop step): a row is a chain of dependent
-// loads, so the gather needs many rows in flight across the machine. On the
-// 48^3, N=7 Jacobi-PCG solve 2, 4, 8 and 16 rows per thread gave 1.638, 1.586,
-// 1.556 and 1.561 ms per iteration; a grid capped at 2048 blocks (~36 rows
-// per thread) made this kernel slower than the gather and update apart.
-#ifndef LIBP_GATHER_JACOBI_ROWS
-#define LIBP_GATHER_JACOBI_ROWS 8
-#endif
-constexpr int GATHER_JACOBI_ROWS = LIBP_GATHER_JACOBI_ROWS;
+// once while s[8] (STOP) is set. The rows run in row blocks (gather_block_sums);
+// a workgroup loops over row blocks, so the grid, and the partials, stay at
+// most GATHER_JACOBI_MAXB.
+constexpr long GATHER_JACOBI_MAXB = 8192;
 inline unsigned gather_jacobi_blocks(int Nrows) {
-  return stream_blocks(((long)Nrows + GATHER_JACOBI_ROWS - 1) / GATHER_JACOBI_ROWS, 256, 1L << 20);
+  return (unsigned)std::max<long>(1, std::min<long>(gather_row_blocks(Nrows), GATHER_JACOBI_MAXB));
 }
 
-__global__ void __launch_bounds__(256) pcg_gather_jacobi_kernel(const int Nrows, const int* __restrict__ rowStarts,
-                                                                const int* __restrict__ colIds,
-                                                                const double* __restrict__ v,
-                                                                const double* __restrict__ invDiag,
-                                                                const double* __restrict__ p, double* __restrict__ x,
-                                                                double* __restrict__ r, double* __restrict__ z,
-                                                                const double* __restrict__ s,
-                                                                double* __restrict__ part) {
+__global__ void __launch_bounds__(GB_THREADS) pcg_gather_jacobi_kernel(const int Nrows,
+                                                                       const int* __restrict__ rowStarts,
+                                                                       const int* __restrict__ colIds,
+                                                                       const double* __restrict__ v,
+                                                                       const double* __restrict__ invDiag,
+                                                                       const double* __restrict__ p,
+                                                                       double* __restrict__ x,
+                                                                       double* __restrict__ r,
+                                                                       double* __restrict__ z,
+                                                                       const double* __restrict__ s,
+                                                                       double* __restrict__ part) {
+  __shared__ double s_v[GB_CAP];
   __shared__ double s_w[16];
   if (s[8] != 0.0) return;
   const double alpha = s[2];
-  const long S = (long)gridDim.x * blockDim.x;
+  const long nrb = gather_row_blocks(Nrows);
   double a0 = 0, a1 = 0;
-  for (long row0 = blockIdx.x * (long)blockDim.x + threadIdx.x; row0 < Nrows; row0 += 2 * S) {
-    int a[2], len[2];
+  for (long rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+    // the rows' own operands, loaded before the gather's loads complete
     double pv[2], xv[2], rv[2], dv[2];
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const long row = row0 + q * S;
-      const bool in = row < Nrows;
-      a[q] = in ? rowStarts[row] : 0;
-      len[q] = in ? rowStarts[row + 1] - a[q] : 0;
-      pv[q] = in ? p[row] : 0.0;
-      xv[q] = in ? x[row] : 0.0;
-      rv[q] = in ? r[row] : 0.0;
-      dv[q] = in ? invDiag[row] : 0.0;
+    for (int u = 0; u < 2; ++u) {
+      const long rc = min(rb * GB_ROWS + threadIdx.x + u * GB_THREADS, (long)Nrows - 1);
+      pv[u] = p[rc];
+      xv[u] = x[rc];
+      rv[u] = r[rc];
+      dv[u] = invDiag[rc];
     }
-    int c[2][8];
-    double val8[2][8];
+    GatherRows<double> g;
+    gather_block_sums(rb, Nrows, rowStarts, colIds, v, s_v, g);
 #pragma unroll
-    for (int q = 0; q < 2; ++q)
-#pragma unroll
-      for (int k = 0; k < 8; ++k) c[q][k] = (k < len[q]) ? colIds[a[q] + k] : 0;
-#pragma unroll
-    for (int q = 0; q < 2; ++q)
-#pragma unroll
-      for (int k = 0; k < 8; ++k) val8[q][k] = (k < len[q]) ? v[c[q][k]] : 0.0;
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const long row = row0 + q * S;
-      if (row >= Nrows) continue;
-      double Ap = 0;
-#pragma unroll
-      for (int k = 0; k < 8; ++k)
-        if (k < len[q]) Ap += val8[q][k];
-      for (int g = a[q] + 8; g < a[q] + len[q]; ++g) Ap += v[colIds[g]];
-      double xn = xv[q], rn = rv[q];
-      xn += alpha * pv[q];
+    for (int u = 0; u < 2; ++u) {
+      if (!g.in[u]) continue;
+      const long row = g.row[u];
+      const double Ap = g.val[u];
+      double xn = xv[u], rn = rv[u];
+      xn += alpha * pv[u];
       rn -= alpha * Ap;
-      const double zn = 1.0 * dv[q] * rn;
+      const double zn = 1.0 * dv[u] * rn;
       x[row] = xn;
       r[row] = rn;
       z[row] = zn;
@@ -413,8 +359,8 @@ template <typename T, int OP = 0>
 static int gather(int Nrows, const int* rs, const int* ci, const T* v, T* gv, void* stream, const T* r = nullptr) {
   if (Nrows <= 0) return LIBP_OK;
   if (!rs || !ci || !v || !gv || (OP == 1 && !r)) return arg_error("ogs_gather", "null pointer");
-  hipLaunchKernelGGL((ogs_gather_kernel<T, OP>), dim3(stream_blocks(Nrows, 1024)), dim3(256), 0, as_stream(stream),
-                     Nrows, rs, ci, v, r, gv);
+  hipLaunchKernelGGL((ogs_gather_kernel<T, OP>), dim3(row_block_grid(Nrows)), dim3(GB_THREADS), 0,
+                     as_stream(stream), Nrows, rs, ci, v, r, gv);
   return launch_status("ogs_gather");
 }
 
@@ -452,9 +398,9 @@ extern "C" int libp_pcg_gather_pap_f64(int Nrows, const int* rowStarts, const in
                                        double* gv, const double* p, double* scalars, void* ws, void* stream) {
   if (Nrows <= 0) return LIBP_OK;
   if (!rowStarts || !colIds || !v || !gv || !p || !scalars || !ws) return arg_error("pcg_gather_pap", "null pointer");
-  const unsigned blocks = std::min<unsigned>(stream_blocks(Nrows, 1024), GATHER_PAP_MAXB);
-  hipLaunchKernelGGL(ogs_gather_pap_kernel, dim3(blocks), dim3(256), 0, as_stream(stream), Nrows, rowStarts, colIds,
-                     v, gv, p, scalars, (ReduceWS*)ws);
+  const unsigned blocks = (unsigned)std::min<long>(gather_row_blocks(Nrows), GATHER_PAP_MAXB);
+  hipLaunchKernelGGL(ogs_gather_pap_kernel, dim3(blocks), dim3(GB_THREADS), 0, as_stream(stream), Nrows, rowStarts,
+                     colIds, v, gv, p, scalars, (ReduceWS*)ws);
   hipLaunchKernelGGL(gather_pap_finish_kernel, dim3(1), dim3(256), 0, as_stream(stream), (int)blocks, scalars,
                      (const ReduceWS*)ws);
   return launch_status("pcg_gather_pap");
@@ -483,7 +429,7 @@ static int gather_cheb(int Nrows, const int* rs, const int* ci, const T* v, cons
   if (Nrows <= 0) return LIBP_OK;
   if (!rs || !ci || !v || !invDiag || !RES || !d || !X || (START && !r))
     return arg_error("ogs_gather_cheb", "null pointer");
-  hipLaunchKernelGGL((ogs_gather_cheb_kernel<T, START>), dim3(stream_blocks(Nrows, 1024)), dim3(256), 0,
+  hipLaunchKernelGGL((ogs_gather_cheb_kernel<T, START>), dim3(row_block_grid(Nrows)), dim3(GB_THREADS), 0,
                      as_stream(stream), Nrows, rs, ci, v, invDiag, r, c1, c2, RES, d, X);
   return launch_status("ogs_gather_cheb");
 }
@@ -545,8 +491,8 @@ extern "C" int libp_pcg_gather_jacobi_update_f64(int Nrows, const int* rowStarts
   if (!rowStarts || !colIds || !AqL || !invDiag || !p || !x || !r || !z || !scalars || !partials)
     return arg_error("pcg_gather_jacobi_update", "null pointer");
   const unsigned nb = gather_jacobi_blocks(Nrows);
-  hipLaunchKernelGGL(pcg_gather_jacobi_kernel, dim3(nb), dim3(256), 0, as_stream(stream), Nrows, rowStarts, colIds,
-                     AqL, invDiag, p, x, r, z, (const double*)scalars, partials);
+  hipLaunchKernelGGL(pcg_gather_jacobi_kernel, dim3(nb), dim3(GB_THREADS), 0, as_stream(stream), Nrows, rowStarts,
+                     colIds, AqL, invDiag, p, x, r, z, (const double*)scalars, partials);
   hipLaunchKernelGGL(pcg_gather_jacobi_finish_kernel, dim3(1), dim3(1024), 0, as_stream(stream), (int)nb, scalars,
                      (const double*)partials);
   return launch_status("pcg_gather_jacobi_update");

@@ -11,8 +11,14 @@
 // thread-per-row gather, so results are bit-identical to it. A thread per
 // row instead issues one load slot per copy position of the longest row of
 // its wavefront; on the 48^3, N=7 box this staging took the FP64 gather from
-// 340 to ~200 us. A block whose rows hold more than GB_CAP copies (possible
-// only on meshes with long rows) sums straight from global memory instead.
+// 340 to ~200 us. A block with more than GB_CAP copies (the N=1 level of a
+// hex mesh averages close to eight copies per row, ~4000 per block) is staged
+// in chunks of GB_CAP copies, in ascending order; each row adds the copies of
+// each chunk it overlaps in turn, so its sum keeps the same order. GB_CAP
+// sets the LDS per workgroup: 2048 copies (16 KB in FP64) keeps eight
+// workgroups per CU resident, where a 4096 cap took the 48^3, N=7 FP64
+// gather from 196 to 301 us. The chunks took the N=1, 96^3 FP64 gather from
+// 77 us (direct loads past the cap) to 28 us (26 us with a 4096 cap).
 #pragma once
 
 #include "kcommon.hpp"
@@ -21,7 +27,7 @@ namespace libp_amd {
 
 constexpr int GB_THREADS = 256;
 constexpr int GB_ROWS = 2 * GB_THREADS;  // two rows per thread
-constexpr int GB_CAP = 2048;             // staged copies per block (8 per thread)
+constexpr int GB_CAP = 2048;             // copies staged per chunk (8 per thread)
 
 __host__ __device__ inline long gather_row_blocks(long Nrows) { return (Nrows + GB_ROWS - 1) / GB_ROWS; }
 // grid of a gather that gives each workgroup one row block (a workgroup loops
@@ -63,35 +69,32 @@ __device__ inline void gather_block_sums_by(const long rb, const int Nrows, cons
     g.len[u] = b[u] - a[u];
   }
   const int n = __builtin_amdgcn_readfirstlane(j1 - j0);
-  if (n <= GB_CAP) {
-    // loads per thread, a scalar: the branches around the loads are uniform, so no
-    // load waits for another
-    const int nl = (n + GB_THREADS - 1) / GB_THREADS;
+  g.val[0] = g.val[1] = T(0);
+  // chunks of GB_CAP copies; n is a scalar, so the loop and the branches around the
+  // loads are uniform and no load waits for another
+  for (int c0 = 0; c0 < n; c0 += GB_CAP) {
+    const int lo = j0 + c0;
+    const int m = min(n - c0, GB_CAP);
+    const int nl = (m + GB_THREADS - 1) / GB_THREADS;
     int c[GB_CAP / GB_THREADS];
     T x[GB_CAP / GB_THREADS];
 #pragma unroll
     for (int u = 0; u < GB_CAP / GB_THREADS; ++u) {
-      const int j = min(j0 + t + u * GB_THREADS, j1 - 1);
+      const int j = min(lo + t + u * GB_THREADS, j1 - 1);
       c[u] = (u < nl) ? (colIds ? colIds[j] : j) : 0;
     }
 #pragma unroll
     for (int u = 0; u < GB_CAP / GB_THREADS; ++u) x[u] = (u < nl) ? load(c[u]) : T(0);
-    __syncthreads();  // the previous row block's sums have read s_v
+    __syncthreads();  // the previous chunk's or row block's sums have read s_v
 #pragma unroll
     for (int u = 0; u < GB_CAP / GB_THREADS; ++u)
-      if (u < nl && t + u * GB_THREADS < n) s_v[t + u * GB_THREADS] = x[u];
+      if (u < nl && t + u * GB_THREADS < m) s_v[t + u * GB_THREADS] = x[u];
     __syncthreads();
+    const int hi = lo + m;
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
-      T s = T(0);
-      for (int k = a[u]; k < b[u]; ++k) s += s_v[k - j0];
-      g.val[u] = s;
-    }
-  } else {
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      T s = T(0);
-      for (int k = a[u]; k < b[u]; ++k) s += load(colIds ? colIds[k] : k);
+      T s = g.val[u];
+      for (int k = max(a[u], lo); k < min(b[u], hi); ++k) s += s_v[k - lo];
       g.val[u] = s;
     }
   }

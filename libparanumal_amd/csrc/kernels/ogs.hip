@@ -59,7 +59,7 @@ __global__ void __launch_bounds__(GB_THREADS) ogs_gather_kernel(const int Nrows,
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
         const long row = min(rb * GB_ROWS + threadIdx.x + u * GB_THREADS, (long)Nrows - 1);
-        rr[u] = (OP == 1) ? r_vec[row] : gv[row];
+        rr[u] = __builtin_nontemporal_load((OP == 1 ? r_vec : gv) + row);
       }
     }
     GatherRows<T> g;
@@ -68,9 +68,10 @@ __global__ void __launch_bounds__(GB_THREADS) ogs_gather_kernel(const int Nrows,
     for (int u = 0; u < 2; ++u) {
       if (!g.in[u]) continue;
       const T val = g.val[u];
-      if (OP == 0) gv[g.row[u]] = val;
-      if (OP == 1) gv[g.row[u]] = T(1) * rr[u] + T(-1) * val;
-      if (OP == 2) gv[g.row[u]] = T(1) * val + T(1) * rr[u];
+      T o = val;
+      if (OP == 1) o = T(1) * rr[u] + T(-1) * val;
+      if (OP == 2) o = T(1) * val + T(1) * rr[u];
+      __builtin_nontemporal_store(o, gv + g.row[u]);
     }
   }
 }
@@ -83,25 +84,39 @@ __global__ void __launch_bounds__(GB_THREADS) ogs_gather_kernel(const int Nrows,
 //   step:  res = RES - invD a; d = c1 res + c2 d; RES = res; X += d
 // The expressions are those of the separate kernels, so results are
 // bit-identical.
+// The row's operands, loaded before the gather's loads complete (START: invD, r,
+// X; step: invD, RES, d, X), with non-temporal accesses as the rows are streamed.
 template <typename T, bool START>
-__device__ inline void cheb_row_update(const long row, const T val, const T* __restrict__ invDiag,
-                                       const T* __restrict__ rvec, const T c1, const T c2, T* __restrict__ RES,
-                                       T* __restrict__ d, T* __restrict__ X) {
+struct ChebRow {
+  T dv, a, b, x;  // START: a = r; step: a = RES, b = d
+};
+template <typename T, bool START>
+__device__ inline ChebRow<T, START> cheb_row_load(const long row, const T* __restrict__ invDiag,
+                                                   const T* __restrict__ rvec, const T* __restrict__ RES,
+                                                   const T* __restrict__ d, const T* __restrict__ X) {
+  ChebRow<T, START> o;
+  o.dv = __builtin_nontemporal_load(invDiag + row);
+  o.a = __builtin_nontemporal_load((START ? rvec : RES) + row);
+  o.b = START ? T(0) : __builtin_nontemporal_load(d + row);
+  o.x = __builtin_nontemporal_load(X + row);
+  return o;
+}
+template <typename T, bool START>
+__device__ inline void cheb_row_update(const long row, const T val, const ChebRow<T, START>& q, const T c1,
+                                       const T c2, T* __restrict__ RES, T* __restrict__ d, T* __restrict__ X) {
   // no a*b+c contraction, as in cheb_start_kernel / cheb_step_kernel
 #pragma clang fp contract(off)
+  T res, dn;
   if (START) {
-    const T res = T(1) * invDiag[row] * (T(1) * rvec[row] - T(1) * val);
-    const T dn = c1 * res;
-    RES[row] = res;
-    d[row] = dn;
-    X[row] = X[row] + dn;
+    res = T(1) * q.dv * (T(1) * q.a - T(1) * val);
+    dn = c1 * res;
   } else {
-    const T res = T(-1) * invDiag[row] * val + RES[row];
-    const T dn = c1 * res + c2 * d[row];
-    RES[row] = res;
-    d[row] = dn;
-    X[row] = X[row] + dn;
+    res = T(-1) * q.dv * val + q.a;
+    dn = c1 * res + c2 * q.b;
   }
+  __builtin_nontemporal_store(res, RES + row);
+  __builtin_nontemporal_store(dn, d + row);
+  __builtin_nontemporal_store(q.x + dn, X + row);
 }
 
 template <typename T, bool START>
@@ -115,11 +130,16 @@ __global__ void __launch_bounds__(GB_THREADS) ogs_gather_cheb_kernel(const int N
   __shared__ T s_v[GB_CAP];
   const long nrb = gather_row_blocks(Nrows);
   for (long rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+    ChebRow<T, START> q[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+      q[u] = cheb_row_load<T, START>(min(rb * GB_ROWS + threadIdx.x + u * GB_THREADS, (long)Nrows - 1), invDiag,
+                                     rvec, RES, d, X);
     GatherRows<T> g;
     gather_block_sums(rb, Nrows, rowStarts, colIds, v, s_v, g);
 #pragma unroll
     for (int u = 0; u < 2; ++u)
-      if (g.in[u]) cheb_row_update<T, START>(g.row[u], g.val[u], invDiag, rvec, c1, c2, RES, d, X);
+      if (g.in[u]) cheb_row_update<T, START>(g.row[u], g.val[u], q[u], c1, c2, RES, d, X);
   }
 }
 
@@ -148,13 +168,14 @@ __global__ void __launch_bounds__(GB_THREADS) ogs_gather_pap_kernel(const int Nr
   for (long rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
     double yv[2];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) yv[u] = y[min(rb * GB_ROWS + threadIdx.x + u * GB_THREADS, (long)Nrows - 1)];
+    for (int u = 0; u < 2; ++u)
+      yv[u] = __builtin_nontemporal_load(y + min(rb * GB_ROWS + threadIdx.x + u * GB_THREADS, (long)Nrows - 1));
     GatherRows<double> g;
     gather_block_sums(rb, Nrows, rowStarts, colIds, v, s_v, g);
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
       if (!g.in[u]) continue;
-      gv[g.row[u]] = g.val[u];
+      __builtin_nontemporal_store(g.val[u], gv + g.row[u]);
       acc += yv[u] * g.val[u];
     }
   }
@@ -259,10 +280,15 @@ inline unsigned face_gather_blocks(int Nrows) { return stream_blocks(Nrows, 1024
 // ellipticPreconJacobi.cpp:42-51). Ap is never stored. Each block stores its
 // r.r and r.z partials at part[b] and part[nb + b] (nb = grid size);
 // pcg_gather_jacobi_finish_kernel sums them in block order. Both return at
-// once while s[8] (STOP) is set. The rows run in row blocks (gather_block_sums);
-// a workgroup loops over row blocks, so the grid, and the partials, stay at
-// most GATHER_JACOBI_MAXB.
-constexpr long GATHER_JACOBI_MAXB = 8192;
+// once while s[8] (STOP) is set. The rows run in row blocks (gather_block_sums),
+// one per workgroup (a workgroup loops over row blocks only past
+// GATHER_JACOBI_MAXB of them), and the rows' own operands are read and written
+// with non-temporal accesses: on the 48^3, N=7 solve (73,429 row blocks) the
+// kernel took 501 us against 579 us with 8192 looping workgroups and cached
+// accesses (529 us with the non-temporal accesses alone). Non-temporal loads of
+// the gathered E-vector were slower (605 us): neighbouring row blocks share its
+// cache lines.
+constexpr long GATHER_JACOBI_MAXB = 1L << 20;
 inline unsigned gather_jacobi_blocks(int Nrows) {
   return (unsigned)std::max<long>(1, std::min<long>(gather_row_blocks(Nrows), GATHER_JACOBI_MAXB));
 }
@@ -290,10 +316,10 @@ __global__ void __launch_bounds__(GB_THREADS) pcg_gather_jacobi_kernel(const int
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
       const long rc = min(rb * GB_ROWS + threadIdx.x + u * GB_THREADS, (long)Nrows - 1);
-      pv[u] = p[rc];
-      xv[u] = x[rc];
-      rv[u] = r[rc];
-      dv[u] = invDiag[rc];
+      pv[u] = __builtin_nontemporal_load(p + rc);
+      xv[u] = __builtin_nontemporal_load(x + rc);
+      rv[u] = __builtin_nontemporal_load(r + rc);
+      dv[u] = __builtin_nontemporal_load(invDiag + rc);
     }
     GatherRows<double> g;
     gather_block_sums(rb, Nrows, rowStarts, colIds, v, s_v, g);
@@ -306,9 +332,9 @@ __global__ void __launch_bounds__(GB_THREADS) pcg_gather_jacobi_kernel(const int
       xn += alpha * pv[u];
       rn -= alpha * Ap;
       const double zn = 1.0 * dv[u] * rn;
-      x[row] = xn;
-      r[row] = rn;
-      z[row] = zn;
+      __builtin_nontemporal_store(xn, x + row);
+      __builtin_nontemporal_store(rn, r + row);
+      __builtin_nontemporal_store(zn, z + row);
       a0 += rn * rn;
       a1 += rn * zn;
     }

@@ -117,7 +117,19 @@ __global__ void __launch_bounds__(GTHREADS, 4) gemm_bf16_kernel(const GemmArgs g
   __shared__ __attribute__((aligned(16))) u16 sB[2][GBN * GLD];
   const int t = threadIdx.x, w = t >> 6, l = t & 63;
   const int wm = w >> 1, wn = w & 1;
-  const int m0 = blockIdx.y * GBM, n0 = blockIdx.x * GBN;
+  // Workgroups are dispatched round-robin over the eight XCDs, each with its
+  // own L2. When the M tiles come in multiples of eight, the tiles are renamed
+  // so that the gridDim.x N tiles of one M tile run on one XCD, one after
+  // another: the A rows (the large operand) then reach that L2 once instead of
+  // once per N tile. The tiles and their arithmetic are unchanged.
+  int mt = blockIdx.y, nt = blockIdx.x;
+  if ((gridDim.y & 7) == 0) {
+    const int b = blockIdx.x + gridDim.x * blockIdx.y;  // dispatch order within a K split
+    const int k = b >> 3;
+    nt = k % gridDim.x;
+    mt = (k / gridDim.x) * 8 + (b & 7);
+  }
+  const int m0 = mt * GBM, n0 = nt * GBN;
   const int kBeg = blockIdx.z * g.kChunk;
   const int kEnd = min(g.K, kBeg + g.kChunk);
   const bool vecA = (g.lda % 8) == 0 && ((uintptr_t)g.A & 15) == 0;

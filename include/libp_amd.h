@@ -248,7 +248,8 @@ int libp_pcg_jacobi_update_noalloc_f64(int N, const double* invDiag, const doubl
  * then scalars[PAP] = p.Ap summed from the local nodes (sum of p_L * AqL,
  * which equals p.Ap in exact arithmetic) and scalars[ALPHA] =
  * scalars[RDOTZ1] / p.Ap. papPartials holds libp_ax_hex_split_blocks(N,
- * Nelements) doubles. N in [1,14].
+ * Nelements) doubles (Nelements at N=15, where the FP64 MFMA kernel of
+ * libp_ax_hex_partial_mfma_n15 runs, one element per workgroup). N in [1,15].
  *
  * libp_pcg_gather_jacobi_update_f64: per gathered row, Ap = the sum of the
  * row's copies of AqL (the gather's arithmetic), then x += alpha p,

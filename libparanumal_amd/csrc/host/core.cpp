@@ -154,7 +154,7 @@ settings_t ellipticDefaultSettings() {
                "count are those of a test after every iteration");
   s.newSetting("PCG FUSED OPERATOR", "TRUE",
                "PCG + Jacobi in three passes per iteration: the p update, the Ax kernel with p.Ap, and the gather "
-               "fused with the x, r, z update (N <= 14, two-pass operator); FALSE runs the gather (with p.Ap) and "
+               "fused with the x, r, z update (N <= 15, two-pass operator); FALSE runs the gather (with p.Ap) and "
                "the update as separate passes",
                {"TRUE", "FALSE"});
   s.newSetting("OPERATOR SPLIT GATHER", "FALSE",

@@ -30,7 +30,7 @@ void elliptic_t::SetupPrecon() {
     precon.Setup<JacobiPrecon>(*this);
     // the fused Jacobi-PCG iteration's reduction partials, allocated here rather
     // than inside a timed solve
-    if (settings.compareSetting("PCG FUSED OPERATOR", "TRUE") && mesh.N >= 1 && mesh.N <= 14) {
+    if (settings.compareSetting("PCG FUSED OPERATOR", "TRUE") && mesh.N >= 1 && mesh.N <= 15) {
       o_axPartials = platform.malloc<double>(std::max(libp_ax_hex_split_blocks(mesh.N, mesh.NlocalGatherElements), 1));
       o_gatherPartials = platform.malloc<double>(std::max<long>(libp_pcg_gather_jacobi_partials(ogsMasked.NlocalT), 2));
     }
@@ -194,7 +194,7 @@ void elliptic_t::Operator(deviceMemory<dfloat>& o_q, deviceMemory<dfloat>& o_Aq)
 bool elliptic_t::OperatorJacobiPCG(deviceMemory<dfloat>& o_p, deviceMemory<dfloat>& o_x, deviceMemory<dfloat>& o_r,
                                    deviceMemory<dfloat>& o_z, const deviceMemory<dfloat>& o_invDiag,
                                    double* scalars) {
-  if (!settings.compareSetting("PCG FUSED OPERATOR", "TRUE") || mesh.N < 1 || mesh.N > 14 || SplitGather())
+  if (!settings.compareSetting("PCG FUSED OPERATOR", "TRUE") || mesh.N < 1 || mesh.N > 15 || SplitGather())
     return false;
   if (!o_gatherPartials.isInitialized()) {
     o_axPartials = platform.malloc<double>(std::max(libp_ax_hex_split_blocks(mesh.N, mesh.NlocalGatherElements), 1));

@@ -90,7 +90,7 @@ class elliptic_t : public operator_t {
   // the operator's gather fused with the PCG p.Ap (libp_pcg_gather_pap_f64)
   bool OperatorPAP(deviceMemory<dfloat>& o_q, deviceMemory<dfloat>& o_Aq, double* scalars, void* ws) override;
   // the Jacobi-PCG iteration with p.Ap in the Ax and the gather fused into the update
-  // ([PCG FUSED OPERATOR] TRUE, N <= 14, two-pass operator)
+  // ([PCG FUSED OPERATOR] TRUE, N <= 15, two-pass operator; N = 15 on the MFMA kernel)
   bool OperatorJacobiPCG(deviceMemory<dfloat>& o_p, deviceMemory<dfloat>& o_x, deviceMemory<dfloat>& o_r,
                          deviceMemory<dfloat>& o_z, const deviceMemory<dfloat>& o_invDiag, double* scalars) override;
   deviceMemory<double> o_axPartials, o_gatherPartials;

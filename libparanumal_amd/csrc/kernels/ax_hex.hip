@@ -391,6 +391,8 @@ extern "C" int libp_ax_hex_split_blocks(int N, int Nelements) {
     LIBP_AX_CASE(13)
     LIBP_AX_CASE(14)
 #undef LIBP_AX_CASE
+    case 15:  // the MFMA kernel of the PCG form: one partial per element
+      return Nelements;
     default:
       return -1;
   }
@@ -429,6 +431,13 @@ extern "C" int libp_ax_hex_split_f64(int N, int Nelements, const int* elementLis
   }
 }
 
+namespace libp_amd {
+// ax_hex_mfma.hip: the N=15 MFMA kernel in its PCG form (p.Ap partials, STOP)
+int launch_ax_mfma_n15_pap(int Nelements, const int* elementList, const int* G2L, const double* wJ,
+                           const double* ggeo, const double* D, double lambda, const double* p, double* AqL,
+                           double* part, const double* scal, hipStream_t st);
+}  // namespace libp_amd
+
 extern "C" int libp_pcg_ax_hex_pap_f64(int N, int Nelements, const int* elementList, const int* GlobalToLocal,
                                        const double* wJ, const double* ggeo, const double* D, double lambda,
                                        const double* p, double* AqL, double* papPartials, double* scalars,
@@ -456,7 +465,16 @@ extern "C" int libp_pcg_ax_hex_pap_f64(int N, int Nelements, const int* elementL
     LIBP_AX_CASE(13)
     LIBP_AX_CASE(14)
 #undef LIBP_AX_CASE
+    case 15: {  // the FP64 MFMA kernel, one element (and one partial) per workgroup
+      if (Nelements <= 0) return LIBP_OK;
+      const int rc = launch_ax_mfma_n15_pap(Nelements, elementList, GlobalToLocal, wJ, ggeo, D, lambda, p, AqL,
+                                            papPartials, scalars, st);
+      if (rc != LIBP_OK) return rc;
+      hipLaunchKernelGGL(ax_pap_finish_kernel, dim3(1), dim3(1024), 0, st, Nelements, scalars,
+                         (const double*)papPartials);
+      return launch_status("pcg_ax_hex_pap");
+    }
     default:
-      return arg_error("pcg_ax_hex_pap", "polynomial degree must be in [1,14]");
+      return arg_error("pcg_ax_hex_pap", "polynomial degree must be in [1,15]");
   }
 }

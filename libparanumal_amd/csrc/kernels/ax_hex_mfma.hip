@@ -21,7 +21,15 @@
 // LDS: q / partial Aq (32 KiB), qt / Gqt (32 KiB), per-wave transpose tile.
 // Reference math: okl/ellipticAxHex3D.okl:156-295 (same operator; the
 // summation order differs, results agree to rounding).
-#include "kcommon.hpp"
+//
+// PAP (the PCG form): the workgroup also stores sum over its element's nodes
+// of q_L * (A_L q)_L at pap[blockIdx.x], its share of p.Ap = p^T Q^T A_L Q p
+// for q = p, and returns at once while scal[8] (STOP) is set. The sum is
+// formed in phase K from values in registers: with A_L q = P + D_t^T Gqt (P
+// the r, s and lambda parts, D_t the derivative along k),
+// q . A_L q = q . P + (D_t q) . Gqt = sum of q P + qt Gqt over the nodes, so
+// no q has to outlive phase K. The E-vector is the same, bit for bit.
+#include "reduce.hpp"
 
 namespace libp_amd {
 
@@ -31,14 +39,21 @@ __device__ inline d4 mfma16(double a, double b, d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+template <bool PAP>
 __global__ void __launch_bounds__(256, 1) ax_hex_mfma_n15_kernel(const int* __restrict__ elementList,
                                                                  const int* __restrict__ G2L,
                                                                  const double* __restrict__ wJ,
                                                                  const double* __restrict__ ggeo,
                                                                  const double* __restrict__ D, const double lambda,
                                                                  const double* __restrict__ q,
-                                                                 double* __restrict__ Aq) {
+                                                                 double* __restrict__ Aq,
+                                                                 double* __restrict__ pap = nullptr,
+                                                                 const double* __restrict__ scal = nullptr) {
+  // a*b + c contracts only within one source expression, so both
+  // instantiations round the E-vector alike
+#pragma clang fp contract(on)
   constexpr int Nq = 16, NqNq = 256, Np = 4096;
+  if (PAP && scal[8] != 0.0) return;  // a PCG step after convergence (STOP)
   __shared__ double s_q[Np];    // q, then r+s+lambda partial of Aq
   __shared__ double s_t[Np];    // qt, then Gqt
   __shared__ double s_x[4][NqNq];  // per-wave transpose tile
@@ -101,6 +116,7 @@ __global__ void __launch_bounds__(256, 1) ax_hex_mfma_n15_kernel(const int* __re
   __syncthreads();
 
   // phase K: k = 4w..4w+3
+  double papAcc = 0;  // PAP: sum of q P + qt Gqt over this thread's nodes
 #pragma unroll 1
   for (int kk = 0; kk < 4; ++kk) {
     const int k = 4 * w + kk;
@@ -120,7 +136,7 @@ __global__ void __launch_bounds__(256, 1) ax_hex_mfma_n15_kernel(const int* __re
       // qs = D Q : a = D[li][4c+lg], b = q(i=li, j=4c+lg, k)
       qs = mfma16(dA[c], s_q[k * NqNq + (4 * c + lg) * Nq + li], qs);
     }
-    double Gqr[4], Gqs[4], lam[4];
+    double Gqr[4], Gqs[4], lam[4], qk[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int n = k * NqNq + (4 * r + lg) * Nq + li;
@@ -128,9 +144,14 @@ __global__ void __launch_bounds__(256, 1) ax_hex_mfma_n15_kernel(const int* __re
       const double qn = s_q[n];
       Gqr[r] = G[0][r] * qr[r] + G[1][r] * qs[r] + G[2][r] * qt;
       Gqs[r] = G[1][r] * qr[r] + G[3][r] * qs[r] + G[4][r] * qt;
-      s_t[n] = G[2][r] * qr[r] + G[4][r] * qs[r] + G[5][r] * qt;  // Gqt, in place
+      const double Gqt = G[2][r] * qr[r] + G[4][r] * qs[r] + G[5][r] * qt;
+      s_t[n] = Gqt;  // in place
       lam[r] = W[r] * lambda * qn;
       s_x[w][(4 * r + lg) * Nq + li] = Gqr[r];
+      if (PAP) {
+        qk[r] = qn;
+        papAcc += qt * Gqt;
+      }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -144,7 +165,11 @@ __global__ void __launch_bounds__(256, 1) ax_hex_mfma_n15_kernel(const int* __re
       acc = mfma16(s_x[w][li * Nq + 4 * c + lg], dT[c], acc);
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) s_q[k * NqNq + (4 * r + lg) * Nq + li] = acc[r] + lam[r];
+    for (int r = 0; r < 4; ++r) {
+      const double P = acc[r] + lam[r];
+      s_q[k * NqNq + (4 * r + lg) * Nq + li] = P;
+      if (PAP) papAcc += qk[r] * P;
+    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -164,6 +189,20 @@ __global__ void __launch_bounds__(256, 1) ax_hex_mfma_n15_kernel(const int* __re
       __builtin_nontemporal_store(acc[r] + s_q[n], Aq + eb + n);
     }
   }
+  if constexpr (PAP) {
+    __shared__ double s_w[16];
+    const double b = block_sum(papAcc, s_w);
+    if (t == 0) pap[blockIdx.x] = b;
+  }
+}
+
+// the PAP form, one partial per element; ax_hex.hip launches the finisher
+int launch_ax_mfma_n15_pap(int Nelements, const int* elementList, const int* G2L, const double* wJ,
+                           const double* ggeo, const double* D, double lambda, const double* p, double* AqL,
+                           double* part, const double* scal, hipStream_t st) {
+  hipLaunchKernelGGL(ax_hex_mfma_n15_kernel<true>, dim3(Nelements), dim3(256), 0, st, elementList, G2L, wJ, ggeo, D,
+                     lambda, p, AqL, part, scal);
+  return launch_status("pcg_ax_hex_pap (N=15)");
 }
 
 }  // namespace libp_amd
@@ -175,7 +214,7 @@ extern "C" int libp_ax_hex_partial_mfma_n15(int Nelements, const int* elementLis
                                             const double* q, double* AqL, void* stream) {
   if (Nelements <= 0) return LIBP_OK;
   if (!GlobalToLocal || !wJ || !ggeo || !D || !q || !AqL) return arg_error("ax_hex_mfma_n15", "null pointer");
-  hipLaunchKernelGGL(ax_hex_mfma_n15_kernel, dim3(Nelements), dim3(256), 0, as_stream(stream), elementList,
-                     GlobalToLocal, wJ, ggeo, D, lambda, q, AqL);
+  hipLaunchKernelGGL(ax_hex_mfma_n15_kernel<false>, dim3(Nelements), dim3(256), 0, as_stream(stream), elementList,
+                     GlobalToLocal, wJ, ggeo, D, lambda, q, AqL, nullptr, nullptr);
   return launch_status("ax_hex_mfma_n15");
 }

@@ -6,9 +6,9 @@ A p row by row and applies the update of linearSolverPCG.cpp:142-160 with the
 Jacobi preconditioner (ellipticPreconJacobi.cpp:42-51) without storing Ap.
 Checked through the C ABI against the oracle's partial Ax and operator
 (solvers/elliptic/okl/ellipticAxHex3D.okl:156-295, ellipticOperator.cpp:29-107)
-and, bit for bit, against the separate kernels (libp_ax_hex_partial_f64,
-libp_ogs_gather_f64 + libp_pcg_jacobi_update_f64); then whole solves with and
-without the fused form."""
+and, bit for bit, against the separate kernels (libp_ax_hex_partial_f64, or the
+FP64 MFMA kernel libp_ax_hex_partial_mfma_n15 at N=15, libp_ogs_gather_f64 +
+libp_pcg_jacobi_update_f64); then whole solves with and without the fused form."""
 import numpy as np
 import pytest
 
@@ -33,9 +33,9 @@ def box(nx, N, curve=0.05, lam=0.5, **kw):
     return s
 
 
-@pytest.mark.parametrize("N", [1, 2, 3, 7])
+@pytest.mark.parametrize("N", [1, 2, 3, 7, 15])
 def test_fused_iteration_through_c_abi(N):
-    o = OracleElliptic(box(3, N))
+    o = OracleElliptic(box(3 if N < 15 else 2, N))
     L = lp().lib()
     E, Np, Ng = o.geti("Nelements"), o.geti("Np"), o.geti("Ngather")
     NL = E * Np
@@ -58,8 +58,12 @@ def test_fused_iteration_through_c_abi(N):
     lp().check(L.libp_pcg_ax_hex_pap_f64(N, E, None, g2l.data_ptr(), wJ.data_ptr(), ggeo.data_ptr(), D.data_ptr(),
                                          0.5, pd.data_ptr(), AqL.data_ptr(), part.data_ptr(), s.data_ptr(), st))
     AqL2 = torch.full((NL,), float("nan"), dtype=torch.float64, device="cuda")
-    lp().check(L.libp_ax_hex_partial_f64(N, E, None, g2l.data_ptr(), wJ.data_ptr(), ggeo.data_ptr(), D.data_ptr(),
-                                         0.5, pd.data_ptr(), AqL2.data_ptr(), st))
+    if N == 15:  # the FP64 MFMA kernel, as the operator runs it at N=15
+        lp().check(L.libp_ax_hex_partial_mfma_n15(E, None, g2l.data_ptr(), wJ.data_ptr(), ggeo.data_ptr(),
+                                                  D.data_ptr(), 0.5, pd.data_ptr(), AqL2.data_ptr(), st))
+    else:
+        lp().check(L.libp_ax_hex_partial_f64(N, E, None, g2l.data_ptr(), wJ.data_ptr(), ggeo.data_ptr(),
+                                             D.data_ptr(), 0.5, pd.data_ptr(), AqL2.data_ptr(), st))
     torch.cuda.synchronize()
     # the same E-vector as the plain partial Ax, bit for bit, and the oracle's to 1e-12
     assert torch.equal(AqL, AqL2)
@@ -101,11 +105,11 @@ def test_fused_iteration_through_c_abi(N):
     assert abs(sah[3] - sah[0] / rdotz) <= 1e-15 * abs(sah[3])
 
 
-def test_stop_flag_skips_both_kernels():
+@pytest.mark.parametrize("N", [3, 15])
+def test_stop_flag_skips_both_kernels(N):
     """With scalars[STOP] set, neither kernel writes anything (chunked solves queue
     iterations past convergence)."""
-    N = 3
-    o = OracleElliptic(box(3, N))
+    o = OracleElliptic(box(3 if N < 15 else 2, N))
     L = lp().lib()
     E, Np, Ng = o.geti("Nelements"), o.geti("Np"), o.geti("Ngather")
     rs, ci = o.array("rowStartsT"), o.array("colIdsT")
@@ -135,14 +139,16 @@ def test_fused_kernels_reject_bad_arguments():
     L = lp().lib()
     v = torch.zeros(64, dtype=torch.float64, device="cuda")
     ptr = v.data_ptr()
-    assert L.libp_pcg_ax_hex_pap_f64(15, 1, None, ptr, ptr, ptr, ptr, 0.0, ptr, ptr, ptr, ptr, None) == 1
+    assert L.libp_pcg_ax_hex_pap_f64(16, 1, None, ptr, ptr, ptr, ptr, 0.0, ptr, ptr, ptr, ptr, None) == 1
     assert b"degree" in L.libp_last_error()
+    assert L.libp_ax_hex_split_blocks(15, 7) == 7 and L.libp_ax_hex_split_blocks(16, 7) == -1
     assert L.libp_pcg_gather_jacobi_update_f64(-1, ptr, ptr, ptr, ptr, ptr, ptr, ptr, ptr, ptr, ptr, None) == 1
     assert L.libp_pcg_gather_jacobi_update_f64(8, ptr, ptr, ptr, ptr, ptr, ptr, ptr, ptr, ptr, None, None) == 1
     assert L.libp_pcg_gather_jacobi_partials(-1) == -1
 
 
-@pytest.mark.parametrize("N,nx,curve,lam", [(4, 5, 0.0, 0.0), (7, 4, 0.05, 1.0), (3, 6, 0.05, 0.0)])
+@pytest.mark.parametrize("N,nx,curve,lam", [(4, 5, 0.0, 0.0), (7, 4, 0.05, 1.0), (3, 6, 0.05, 0.0),
+                                            (15, 2, 0.05, 1.0)])
 @pytest.mark.parametrize("stop", ["ABS/REL-INITRESID", "RELATIVE"])
 def test_fused_solve_matches_four_launch_form_and_oracle(N, nx, curve, lam, stop):
     """Whole PCG+Jacobi solves (elliptic_t::Run): the fused iteration against the

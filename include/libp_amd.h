@@ -160,6 +160,12 @@ int libp_amxpy_f64(int N, double alpha, const double* a, const double* x, double
 int libp_amxpy_f32(int N, float alpha, const float* a, const float* x, float beta, float* y, void* stream);
 /* x = alpha — linAlgSet.okl; x += alpha — linAlgAdd.okl; x *= alpha — linAlgScale.okl */
 int libp_set_f64(int N, double alpha, double* x, void* stream);
+/* hostDst[0, n) = src[0, n), n <= 64, written by a kernel into pinned host
+ * memory that is mapped for the device (hipHostMalloc with
+ * hipHostMallocMapped; hostDst is its device pointer). Replaces the host read
+ * of the solver's reduction results (rdotr.copyFrom(o_rdotr) at
+ * linearSolverPCG.cpp:163): the caller waits on an event recorded after it. */
+int libp_scalars_to_host_f64(int n, const double* src, double* hostDst, void* stream);
 int libp_add_f64(int N, double alpha, double* x, void* stream);
 int libp_scale_f64(int N, double alpha, double* x, void* stream);
 /* Precision conversion (no reference counterpart). */

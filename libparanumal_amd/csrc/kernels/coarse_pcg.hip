@@ -93,10 +93,37 @@ inline int coarse_row_blocks(const int Nrows) { return std::min((Nrows + RBS - 1
 
 // sum over the nb partials of one slot, in a fixed order, by the first wavefront
 // (valid in lane 0 of wave 0)
+// Partials of slots s0.. s0+NS-1 summed by one wavefront (threadIdx.x < 64):
+// lane l adds entries l, l+64, l+128, ... in that order, then a wave sum. The
+// loads of four steps of every slot are issued together (clamped indices;
+// entries past nb add nothing) instead of one dependent load per step.
+template <int NS>
+__device__ inline void coarse_sum_slots(const ReduceWS* __restrict__ ws, const int s0, const int nb,
+                                        double (&out)[NS]) {
+  const int l = threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < NS; ++k) out[k] = 0;
+  for (int b0 = l; b0 < nb; b0 += 4 * 64) {
+    double v[NS][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int b = min(b0 + 64 * u, nb - 1);
+#pragma unroll
+      for (int k = 0; k < NS; ++k) v[k][u] = ws->part[s0 + k][b];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (b0 + 64 * u < nb)
+#pragma unroll
+        for (int k = 0; k < NS; ++k) out[k] += v[k][u];
+  }
+#pragma unroll
+  for (int k = 0; k < NS; ++k) out[k] = wave_sum(out[k]);
+}
 __device__ inline double coarse_sum_slot(const ReduceWS* __restrict__ ws, const int slot, const int nb) {
-  double a = 0;
-  for (int b = threadIdx.x; b < nb; b += 64) a += ws->part[slot][b];
-  return wave_sum(a);
+  double a[1];
+  coarse_sum_slots<1>(ws, slot, nb, a);
+  return a[0];
 }
 
 __global__ void __launch_bounds__(256) coarse_ax_hex1_kernel(const int E, const int* __restrict__ G2L,
@@ -135,8 +162,10 @@ __global__ void __launch_bounds__(256) coarse_ax_hex1_kernel(const int E, const 
   if (threadIdx.x < 64) {
     double rr = 0, rz = 0;
     if (!first) {
-      rr = coarse_sum_slot(ws, 1, nb);
-      rz = coarse_sum_slot(ws, 2, nb);
+      double g2[2];
+      coarse_sum_slots<2>(ws, 1, nb, g2);
+      rr = g2[0];
+      rz = g2[1];
     }
     if (threadIdx.x == 0) {
       int stop = 0;
@@ -313,9 +342,11 @@ __device__ inline void cg1_scalars(const int mode, const int P, const double tol
   if (threadIdx.x < 64) {
     double ga = 0, de = 0, rr = 0;
     if (mode > 0) {
-      ga = coarse_sum_slot(ws, 0, nb);
-      de = coarse_sum_slot(ws, 1, nb);
-      rr = coarse_sum_slot(ws, 2, nb);
+      double g3[3];
+      coarse_sum_slots<3>(ws, 0, nb, g3);
+      ga = g3[0];
+      de = g3[1];
+      rr = g3[2];
     }
     if (threadIdx.x == 0) {
       int stop = 0;

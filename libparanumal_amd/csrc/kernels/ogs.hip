@@ -187,10 +187,7 @@ __global__ void __launch_bounds__(256) gather_pap_finish_kernel(const int nb, do
                                                                 const ReduceWS* __restrict__ ws) {
   __shared__ double s_w[16];
   if (s[8] != 0.0) return;
-  const double* flat = &ws->part[0][0];
-  double t = 0;
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) t += flat[i];
-  const double pap = block_sum(t, s_w);
+  const double pap = block_sum(strided_sum(&ws->part[0][0], nb), s_w);
   if (threadIdx.x == 0) {
     s[4] = pap;
     s[2] = s[0] / pap;
@@ -261,9 +258,7 @@ __global__ void __launch_bounds__(1024) split_pap_finish_kernel(const int nb, do
                                                                 const double* __restrict__ part) {
   __shared__ double s_w[16];
   if (s[8] != 0.0) return;
-  double t = 0;
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) t += part[i];
-  const double pap = block_sum(t, s_w);
+  const double pap = block_sum(strided_sum(part, nb), s_w);
   if (threadIdx.x == 0) {
     s[4] = pap;
     s[2] = s[0] / pap;
@@ -353,13 +348,8 @@ __global__ void __launch_bounds__(1024) pcg_gather_jacobi_finish_kernel(const in
                                                                        const double* __restrict__ part) {
   __shared__ double s_w[16];
   if (s[8] != 0.0) return;
-  double t0 = 0, t1 = 0;
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-    t0 += part[i];
-    t1 += part[nb + i];
-  }
-  const double rr = block_sum(t0, s_w);
-  const double rz = block_sum(t1, s_w);
+  const double rr = block_sum(strided_sum(part, nb), s_w);
+  const double rz = block_sum(strided_sum(part + nb, nb), s_w);
   if (threadIdx.x == 0) {
     s[5] = rr;
     s[9] += 1.0;

@@ -47,6 +47,26 @@ __device__ inline double block_sum(double v, double* s_w) {
   return r;
 }
 
+// Sum of x[i] for i = threadIdx.x, threadIdx.x + blockDim.x, ... below n,
+// added from zero in that order: the order of the plain strided loop, so the
+// result is the same bit for bit, but with eight loads in flight (at clamped
+// indices; entries past n add nothing) instead of a chain of dependent loads.
+// A one-block finisher over 73,429 partials (48^3 PCG gather-update) took
+// 34 us with the plain loop.
+__device__ inline double strided_sum(const double* __restrict__ x, const int n) {
+  double t = 0;
+  const int B = blockDim.x;
+  for (int i0 = threadIdx.x; i0 < n; i0 += 8 * B) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = x[min(i0 + u * B, n - 1)];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (i0 + u * B < n) t += v[u];
+  }
+  return t;
+}
+
 // One partial per block for each of NV sums: ws->part[k][blockIdx.x].
 template <int NV>
 __device__ inline void store_block_partials(ReduceWS* ws, const double (&v)[NV], double* s_w) {
@@ -63,9 +83,7 @@ template <int NV>
 __device__ inline void sum_block_partials(const ReduceWS* ws, const int nb, double (&out)[NV], double* s_w) {
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
-    double acc = 0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += ws->part[k][b];
-    out[k] = block_sum(acc, s_w);
+    out[k] = block_sum(strided_sum(ws->part[k], nb), s_w);
   }
 }
 

@@ -386,6 +386,8 @@ ellipticRunResult_t elliptic_t::Run(int verboseOverride) {
   settings.getSetting("LINEAR SOLVER TOLERANCE", tol);
   const int verbose = (verboseOverride >= 0) ? verboseOverride : (settings.compareSetting("VERBOSE", "TRUE") ? 1 : 0);
 
+  // x is the gather of a zero E-vector: the solver's initial residual is the rhs
+  linearSolver.get()->zeroInitialGuess = true;
   platform.finish();
   const auto t0 = std::chrono::steady_clock::now();
   const int iter = Solve(linearSolver, o_x, o_r, tol, maxIter, verbose);

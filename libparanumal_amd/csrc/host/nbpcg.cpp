@@ -34,8 +34,7 @@ int nbpcg::Solve(operator_t& A, operator_t& M, deviceMemory<dfloat>& o_x, device
   double* sc = o_sc.ptr();
   stats = solveStats_t();
   LIBP_CALL(libp_set_f64(NSC, 0.0, sc, st));
-  A.Operator(o_x, o_Ax);
-  LIBP_CALL(libp_axpy_f64(N, -1.0, o_Ax.ptr(), 1.0, o_r.ptr(), st));
+  initialResidual(A, o_x, o_Ax, o_r);
   M.solveStarting();
   M.Operator(o_r, o_z);
   LIBP_CALL(libp_nbpcg_update2_f64(N, 1, o_s.ptr(), o_S.ptr(), o_r.ptr(), o_z.ptr(), sc, ws, st));
@@ -82,8 +81,7 @@ int nbfpcg::Solve(operator_t& A, operator_t& M, deviceMemory<dfloat>& o_x, devic
   const size_t bytes = (size_t)N * sizeof(double);
   stats = solveStats_t();
   LIBP_CALL(libp_set_f64(NSC, 0.0, sc, st));
-  A.Operator(o_x, o_Ax);
-  LIBP_CALL(libp_axpy_f64(N, -1.0, o_Ax.ptr(), 1.0, o_r.ptr(), st));
+  initialResidual(A, o_x, o_Ax, o_r);
   M.solveStarting();
   M.Operator(o_r, o_u);
   LIBP_HIP(hipMemcpyAsync(o_p.ptr(), o_u.ptr(), bytes, hipMemcpyDeviceToDevice, platform.stream));

@@ -93,8 +93,7 @@ int pcg::Solve(operator_t& linearOperator, operator_t& precon, deviceMemory<dflo
     TOL = std::max(tol * tol * nb2, tol * tol);
   }
 
-  linearOperator.Operator(o_x, o_Ax);
-  LIBP_CALL(libp_axpy_f64(N, -1.0, o_Ax.ptr(), 1.0, o_r.ptr(), st));
+  initialResidual(linearOperator, o_x, o_Ax, o_r);
   LIBP_CALL(libp_norm2sq_f64(N, o_r.ptr(), tmp, ws, st));
   double rdotr0 = platform.readScalars(tmp, 1)[0];
   if (!rhsNorm) TOL = pureRelative ? tol * tol * rdotr0 : std::max(tol * tol * rdotr0, tol * tol);

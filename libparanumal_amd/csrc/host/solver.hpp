@@ -118,6 +118,21 @@ class linearSolverBase_t {
   bool pureRelative = false;  // stop at |r| <= tol |r0| (no absolute floor)
   int checkEvery = 1;         // host convergence test every k iterations (the device stops itself)
   bool recordHistory = false;
+  // Set by a caller whose initial guess is exactly zero (elliptic_t::Run gathers
+  // a zero E-vector into x): the next Solve then takes r = rhs without forming
+  // A x, which is zero, so r - A x is rhs bit for bit (up to the sign of zero
+  // entries). The Solve clears it.
+  bool zeroInitialGuess = false;
+  // r -= A x, the initial residual of linearSolverPCG.cpp:88-91, skipped for a
+  // zero initial guess
+  void initialResidual(operator_t& A, deviceMemory<dfloat>& o_x, deviceMemory<dfloat>& o_Ax,
+                       deviceMemory<dfloat>& o_r) {
+    const bool zero = zeroInitialGuess;
+    zeroInitialGuess = false;
+    if (zero) return;
+    A.Operator(o_x, o_Ax);
+    LIBP_CALL(libp_axpy_f64(N, -1.0, o_Ax.ptr(), 1.0, o_r.ptr(), platform.s()));
+  }
 
   linearSolverBase_t(dlong _N, dlong _Nhalo, platform_t& _platform, settings_t& _settings, comm_t _comm)
       : platform(_platform), settings(_settings), comm(_comm), N(_N), Nhalo(_Nhalo) {}

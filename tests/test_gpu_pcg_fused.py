@@ -168,3 +168,34 @@ def test_fused_solve_matches_four_launch_form_and_oracle(N, nx, curve, lam, stop
     assert abs(ra["iterations"] - rb["iterations"]) <= 1 and abs(ra["iterations"] - oit) <= 1
     assert abs(ra["norm"] - rb["norm"]) <= 1e-10 * abs(rb["norm"])
     assert abs(ra["norm"] - onorm) <= 1e-10 * abs(onorm)
+
+
+def test_scalars_to_host_through_mapped_memory():
+    """libp_scalars_to_host_f64 writes device scalars into pinned host memory mapped
+    for the device (the host read of the solver scalars, linearSolverPCG.cpp:163):
+    after an event, the host sees the values bit for bit; n outside [0, 64] is
+    rejected."""
+    import ctypes
+    L = lp().lib()
+    hip = ctypes.CDLL("libamdhip64.so")
+    host, dptr = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipHostMalloc(ctypes.byref(host), ctypes.c_size_t(64 * 8), ctypes.c_uint(0x2 | 0x40000000)) == 0
+    try:
+        assert hip.hipHostGetDevicePointer(ctypes.byref(dptr), host, ctypes.c_uint(0)) == 0
+        buf = (ctypes.c_double * 64).from_address(host.value)
+        for i in range(64):
+            buf[i] = -1.0
+        src = torch.from_numpy(np.random.default_rng(3).standard_normal(64)).cuda()
+        st = torch.cuda.current_stream()
+        for n in (1, 16, 64):
+            lp().check(L.libp_scalars_to_host_f64(n, src.data_ptr(), dptr.value, st.cuda_stream))
+            ev = torch.cuda.Event()
+            ev.record(st)
+            ev.synchronize()
+            got = np.array(buf[:n])
+            assert np.array_equal(got, src[:n].cpu().numpy())
+        assert L.libp_scalars_to_host_f64(65, src.data_ptr(), dptr.value, st.cuda_stream) == 1
+        assert L.libp_scalars_to_host_f64(-1, src.data_ptr(), dptr.value, st.cuda_stream) == 1
+        assert L.libp_scalars_to_host_f64(0, None, None, st.cuda_stream) == 0
+    finally:
+        hip.hipHostFree(host)

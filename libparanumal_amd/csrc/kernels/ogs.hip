@@ -342,8 +342,32 @@ __global__ void __launch_bounds__(GB_THREADS) pcg_gather_jacobi_kernel(const int
   }
 }
 
-// r.r and r.z from the partials in block order, then ITERS, STOP and the
-// r.z / beta bookkeeping of pcg_jacobi_update_finish_kernel.
+// First stage of the partial sums when there are many: block b sums entries
+// [1024 b, 1024 b + 1024) of both arrays (one load per thread, all in flight)
+// into out[b] and out[nb1 + b]. One block summing 73,429 pairs of partials
+// that other XCDs wrote took 29 us (about nine rounds of dependent far loads);
+// two stages take one round each.
+constexpr int GJ_STAGE1 = 1024;
+inline int gather_jacobi_stage1_blocks(long nb) { return nb > 2 * GJ_STAGE1 ? (int)((nb + GJ_STAGE1 - 1) / GJ_STAGE1) : 0; }
+__global__ void __launch_bounds__(GJ_STAGE1) pcg_gather_jacobi_stage1_kernel(const int nb, const double* __restrict__ s,
+                                                                            const double* __restrict__ part,
+                                                                            double* __restrict__ out) {
+  __shared__ double s_w[16];
+  if (s[8] != 0.0) return;
+  const int i = blockIdx.x * GJ_STAGE1 + threadIdx.x;
+  const int ic = min(i, nb - 1);
+  const double a = part[ic], b = part[nb + ic];
+  const double b0 = block_sum(i < nb ? a : 0.0, s_w);
+  const double b1 = block_sum(i < nb ? b : 0.0, s_w);
+  if (threadIdx.x == 0) {
+    out[blockIdx.x] = b0;
+    out[gridDim.x + blockIdx.x] = b1;
+  }
+}
+
+// r.r and r.z from the partials in block order (or from the first stage's
+// sums), then ITERS, STOP and the r.z / beta bookkeeping of
+// pcg_jacobi_update_finish_kernel.
 __global__ void __launch_bounds__(1024) pcg_gather_jacobi_finish_kernel(const int nb, double* __restrict__ s,
                                                                        const double* __restrict__ part) {
   __shared__ double s_w[16];
@@ -497,7 +521,11 @@ extern "C" int libp_pcg_gather_face_pap_f64(int NfaceRows, const int* faceRows, 
   return launch_status("pcg_gather_face_pap");
 }
 
-extern "C" long libp_pcg_gather_jacobi_partials(int Nrows) { return Nrows >= 0 ? 2L * gather_jacobi_blocks(Nrows) : -1; }
+extern "C" long libp_pcg_gather_jacobi_partials(int Nrows) {
+  if (Nrows < 0) return -1;
+  const long nb = gather_jacobi_blocks(Nrows);
+  return 2L * nb + 2L * gather_jacobi_stage1_blocks(nb);  // the blocks' partials, then the first stage's sums
+}
 
 extern "C" int libp_pcg_gather_jacobi_update_f64(int Nrows, const int* rowStarts, const int* colIds,
                                                  const double* AqL, const double* invDiag, const double* p, double* x,
@@ -509,7 +537,16 @@ extern "C" int libp_pcg_gather_jacobi_update_f64(int Nrows, const int* rowStarts
   const unsigned nb = gather_jacobi_blocks(Nrows);
   hipLaunchKernelGGL(pcg_gather_jacobi_kernel, dim3(nb), dim3(GB_THREADS), 0, as_stream(stream), Nrows, rowStarts,
                      colIds, AqL, invDiag, p, x, r, z, (const double*)scalars, partials);
-  hipLaunchKernelGGL(pcg_gather_jacobi_finish_kernel, dim3(1), dim3(1024), 0, as_stream(stream), (int)nb, scalars,
-                     (const double*)partials);
+  const int nb1 = gather_jacobi_stage1_blocks(nb);
+  if (nb1 > 0) {
+    double* sums = partials + 2L * nb;
+    hipLaunchKernelGGL(pcg_gather_jacobi_stage1_kernel, dim3(nb1), dim3(GJ_STAGE1), 0, as_stream(stream), (int)nb,
+                       (const double*)scalars, (const double*)partials, sums);
+    hipLaunchKernelGGL(pcg_gather_jacobi_finish_kernel, dim3(1), dim3(1024), 0, as_stream(stream), nb1, scalars,
+                       (const double*)sums);
+  } else {
+    hipLaunchKernelGGL(pcg_gather_jacobi_finish_kernel, dim3(1), dim3(1024), 0, as_stream(stream), (int)nb, scalars,
+                       (const double*)partials);
+  }
   return launch_status("pcg_gather_jacobi_update");
 }

@@ -31,9 +31,9 @@ def csr(Nrows, maxlen, rng, long_blocks=False):
     # random subset of 64 offsets
     base = rng.integers(0, NL - 64, size=Nrows)
     perm = np.argsort(rng.random((Nrows, 64)), axis=1)
-    ci = np.empty(rs[-1], np.int32)
-    for r in range(Nrows):
-        ci[rs[r]:rs[r + 1]] = np.sort(perm[r, :lens[r]]) + base[r]
+    take = np.arange(64)[None, :] < lens[:, None]
+    chosen = np.sort(np.where(take, perm, 64), axis=1)  # each row's first lens offsets, ascending
+    ci = (chosen + base[:, None])[np.arange(64)[None, :] < lens[:, None]].astype(np.int32)
     return rs, ci, NL, lens
 
 
@@ -87,7 +87,9 @@ def test_plain_residual_add_gathers(Nrows, maxlen, long_blocks):
         assert np.array_equal(add.cpu().numpy(), (ref + g0).astype(dt)), (Nrows, sfx)
 
 
-@pytest.mark.parametrize("Nrows,maxlen,long_blocks", [(513, 8, False), (40000, 8, False), (3001, 20, True)])
+# 1,200,000 rows: 2,344 row blocks, so the Jacobi update's partials are summed in two stages
+@pytest.mark.parametrize("Nrows,maxlen,long_blocks", [(513, 8, False), (40000, 8, False), (3001, 20, True),
+                                                      (1200000, 3, False)])
 def test_fused_pcg_and_chebyshev_gathers(Nrows, maxlen, long_blocks):
     rng = np.random.default_rng(7 * Nrows)
     rs, ci, NL, lens = csr(Nrows, maxlen, rng, long_blocks)

@@ -461,6 +461,13 @@ int libp_addbc_hex_f64(int NL, int dataKind, const double* coeffs, const double*
  * relu = 1: Y is BF16 = bf16(max(., 0)); relu = 0: Y is FP32 (no activation). */
 int libp_mlp_linear_fwd(int M, int K, int Nout, const unsigned short* X, long ldx, const unsigned short* Wt,
                         long ldw, const float* b, int relu, void* Y, long ldy, void* stream);
+/* The FP32 layer of libp_mlp_linear_fwd (relu = 0) with each output row
+ * multiplied by rowScale[m] in the epilogue: Y[m][n] = rowScale[m] *
+ * fl((X Wt^T + b)[m][n]), the same product that the assembly forms from an
+ * unscaled Y, so libp_learned_assemble_f32 with scale NULL on this Y gives the
+ * same result bit for bit with one load per copy instead of two. */
+int libp_mlp_linear_fwd_scaled(int M, int K, int Nout, const unsigned short* X, long ldx, const unsigned short* Wt,
+                               long ldw, const float* b, const float* rowScale, float* Y, long ldy, void* stream);
 /* Backward through the weights: dX[M x K] = bf16((dY Wt) .* (H > 0)), dY BF16
  * [M][lddy], H BF16 [M][ldh] (the layer input after its ReLU) or NULL for no mask. */
 int libp_mlp_linear_bwd_data(int M, int K, int Nout, const unsigned short* dY, long lddy, const unsigned short* Wt,
@@ -485,7 +492,8 @@ int libp_learned_features_f32(int Nelements, int Np, int Ntr, const int* GlobalT
                               const float* r, const float* invDiag, unsigned short* X, long ldx, float* scale,
                               void* stream);
 /* c[g] = (1/|row g|) sum_{j in row g} scale[j/Np] Y[j/Np][j%Np] + omega invDiag[g] r[g],
- * rows of the masked gather (rowStarts, colIds); omega is a device pointer (or NULL = 0). */
+ * rows of the masked gather (rowStarts, colIds); omega is a device pointer (or NULL = 0).
+ * scale NULL: Y holds the scaled values already (libp_mlp_linear_fwd_scaled). */
 int libp_learned_assemble_f32(int Nrows, const int* rowStarts, const int* colIds, int Np, const float* Y, long ldy,
                               const float* scale, const float* omega, const float* r, const float* invDiag, float* c,
                               void* stream);

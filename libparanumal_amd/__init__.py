@@ -127,6 +127,8 @@ SIGNATURES = {
     "libp_nbfpcg_update2_f64": (c_int, [c_int] + [c_void_p] * 10),
     "libp_mlp_linear_fwd": (c_int, [c_int, c_int, c_int, c_void_p, c_long, c_void_p, c_long, c_void_p, c_int,
                                     c_void_p, c_long, c_void_p]),
+    "libp_mlp_linear_fwd_scaled": (c_int, [c_int, c_int, c_int, c_void_p, c_long, c_void_p, c_long, c_void_p,
+                                           c_void_p, c_void_p, c_long, c_void_p]),
     "libp_mlp_linear_bwd_data": (c_int, [c_int, c_int, c_int, c_void_p, c_long, c_void_p, c_long, c_void_p, c_long,
                                          c_void_p, c_long, c_void_p]),
     "libp_mlp_linear_bwd_weight": (c_int, [c_int, c_int, c_int, c_void_p, c_long, c_void_p, c_long, c_void_p,

@@ -257,21 +257,25 @@ void learnedSmoother_t::correction(const float* r, const float* invDiag, float* 
   const float* th = o_theta.ptr();
   LIBP_CALL(libp_learned_features_f32(E, Np, 6 * Nfp, elliptic.o_GlobalToLocal.ptr(), o_trace.ptr(), r, invDiag,
                                       o_X.ptr(), nIn, o_scale.ptr(), s));
+  // the output layer multiplies each element's row by its input scale s_e in its
+  // epilogue, so the assembly below reads one value per copy (scale NULL)
   if (arch == ARCH_LINEAR) {
-    LIBP_CALL(libp_mlp_linear_fwd(E, nIn, nOut, o_X.ptr(), nIn, Wb + oW3, nIn, nullptr, 0, o_Y.ptr(), nOut, s));
+    LIBP_CALL(libp_mlp_linear_fwd_scaled(E, nIn, nOut, o_X.ptr(), nIn, Wb + oW3, nIn, nullptr, o_scale.ptr(),
+                                         o_Y.ptr(), nOut, s));
   } else {
     LIBP_CALL(libp_mlp_linear_fwd(E, nIn, H, o_X.ptr(), nIn, Wb + oW1, nIn, th + ob1, 1, o_H1.ptr(), H, s));
     LIBP_CALL(libp_mlp_linear_fwd(E, H, H, o_H1.ptr(), H, Wb + oW2, H, th + ob2, 1, o_H2.ptr(), H, s));
-    LIBP_CALL(libp_mlp_linear_fwd(E, H, nOut, o_H2.ptr(), H, Wb + oW3, H, th + ob3, 0, o_Y.ptr(), nOut, s));
+    LIBP_CALL(libp_mlp_linear_fwd_scaled(E, H, nOut, o_H2.ptr(), H, Wb + oW3, H, th + ob3, o_scale.ptr(), o_Y.ptr(),
+                                         nOut, s));
   }
   // the linear smoother's omega only changes through setParameters: skip the term when it is zero
   const float* om = (arch == ARCH_LINEAR && omegaHost == 0.f) ? nullptr : th + oOm;
   const int* rs = elliptic.ogsMasked.o_rowStartsT.ptr();
   const int* ci = elliptic.ogsMasked.o_colIdsT.ptr();
   if (accumulate)
-    LIBP_CALL(libp_learned_assemble_add_f32(Ndofs, rs, ci, Np, o_Y.ptr(), nOut, o_scale.ptr(), om, r, invDiag, out, s));
+    LIBP_CALL(libp_learned_assemble_add_f32(Ndofs, rs, ci, Np, o_Y.ptr(), nOut, nullptr, om, r, invDiag, out, s));
   else
-    LIBP_CALL(libp_learned_assemble_f32(Ndofs, rs, ci, Np, o_Y.ptr(), nOut, o_scale.ptr(), om, r, invDiag,
+    LIBP_CALL(libp_learned_assemble_f32(Ndofs, rs, ci, Np, o_Y.ptr(), nOut, nullptr, om, r, invDiag,
                                         out ? out : o_c.ptr(), s));
 }
 

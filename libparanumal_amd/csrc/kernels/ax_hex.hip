@@ -290,7 +290,8 @@ __global__ void __launch_bounds__(1024) ax_pap_finish_kernel(const int nb, doubl
                                                              const double* __restrict__ part) {
   __shared__ double s_w[16];
   if (s[8] != 0.0) return;
-  const double pap = block_sum(strided_sum(part, nb), s_w);
+  // 27,648 partials at 48^3, N=7: one round of 32 loads per thread (7.3 us with 8)
+  const double pap = block_sum(strided_sum<32>(part, nb), s_w);
   if (threadIdx.x == 0) {
     s[4] = pap;
     s[2] = s[0] / pap;

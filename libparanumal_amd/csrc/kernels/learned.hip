@@ -78,8 +78,10 @@ __global__ void __launch_bounds__(256) learned_features_kernel(const int E, cons
 // memory, the term skipped when omega is null. SHIFT = log2(Np) when Np is a
 // power of two (the element of a local id is then a shift), else -1.
 // ACC: c += the correction (the smoother's x += c in the same pass; the sum
-// rounds once, as the separate axpy it replaces does).
-template <int SHIFT, bool ACC = false>
+// rounds once, as the separate axpy it replaces does). SCALED = false: Y
+// already holds s_e y (libp_mlp_linear_fwd_scaled forms the same products in
+// its epilogue), so each copy is one load.
+template <int SHIFT, bool ACC = false, bool SCALED = true>
 __global__ void __launch_bounds__(GB_THREADS) learned_assemble_kernel(const int Nrows, const int* __restrict__ rowStarts,
                                                                       const int* __restrict__ colIds, const int Np,
                                                                       const float* __restrict__ Y, const long ldy,
@@ -95,7 +97,7 @@ __global__ void __launch_bounds__(GB_THREADS) learned_assemble_kernel(const int 
   auto copy_value = [=](int id) {
     const int e = SHIFT >= 0 ? (id >> SHIFT) : id / Np;
     const int n = id - e * Np;
-    return scale[e] * Y[(long)e * ldy + n];
+    return SCALED ? scale[e] * Y[(long)e * ldy + n] : Y[(long)e * ldy + n];
   };
   const long nrb = gather_row_blocks(Nrows);
   for (long rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
@@ -298,24 +300,35 @@ extern "C" int libp_learned_features_f32(int Nelements, int Np, int Ntr, const i
   return launch_status("learned_features");
 }
 
+template <bool ACC, bool SCALED>
+static void assemble_launch(dim3 grid, dim3 block, hipStream_t st, int Nrows, const int* rowStarts,
+                            const int* colIds, int Np, const float* Y, long ldy, const float* scale,
+                            const float* omega, const float* r, const float* invDiag, float* c) {
+  if (Np == 512)
+    hipLaunchKernelGGL((learned_assemble_kernel<9, ACC, SCALED>), grid, block, 0, st, Nrows, rowStarts, colIds, Np,
+                       Y, ldy, scale, omega, r, invDiag, c);
+  else if (Np == 64)
+    hipLaunchKernelGGL((learned_assemble_kernel<6, ACC, SCALED>), grid, block, 0, st, Nrows, rowStarts, colIds, Np,
+                       Y, ldy, scale, omega, r, invDiag, c);
+  else
+    hipLaunchKernelGGL((learned_assemble_kernel<-1, ACC, SCALED>), grid, block, 0, st, Nrows, rowStarts, colIds, Np,
+                       Y, ldy, scale, omega, r, invDiag, c);
+}
+
+// scale null: Y holds the scaled outputs already
 template <bool ACC>
 static int assemble(int Nrows, const int* rowStarts, const int* colIds, int Np, const float* Y, long ldy,
                     const float* scale, const float* omega, const float* r, const float* invDiag, float* c,
                     void* stream) {
   if (Nrows <= 0) return LIBP_OK;
-  if (!rowStarts || !colIds || !Y || !scale || !r || !invDiag || !c) return arg_error("learned_assemble", "null pointer");
+  if (!rowStarts || !colIds || !Y || !r || !invDiag || !c) return arg_error("learned_assemble", "null pointer");
   if (Np <= 0 || ldy < Np) return arg_error("learned_assemble", "bad size");
   const dim3 grid(row_block_grid(Nrows)), block(GB_THREADS);
   hipStream_t st = as_stream(stream);
-  if (Np == 512)
-    hipLaunchKernelGGL((learned_assemble_kernel<9, ACC>), grid, block, 0, st, Nrows, rowStarts, colIds, Np, Y, ldy,
-                       scale, omega, r, invDiag, c);
-  else if (Np == 64)
-    hipLaunchKernelGGL((learned_assemble_kernel<6, ACC>), grid, block, 0, st, Nrows, rowStarts, colIds, Np, Y, ldy,
-                       scale, omega, r, invDiag, c);
+  if (scale)
+    assemble_launch<ACC, true>(grid, block, st, Nrows, rowStarts, colIds, Np, Y, ldy, scale, omega, r, invDiag, c);
   else
-    hipLaunchKernelGGL((learned_assemble_kernel<-1, ACC>), grid, block, 0, st, Nrows, rowStarts, colIds, Np, Y, ldy,
-                       scale, omega, r, invDiag, c);
+    assemble_launch<ACC, false>(grid, block, st, Nrows, rowStarts, colIds, Np, Y, ldy, scale, omega, r, invDiag, c);
   return launch_status("learned_assemble");
 }
 

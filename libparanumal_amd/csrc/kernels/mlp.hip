@@ -52,6 +52,7 @@ struct GemmArgs {
   const u16* mask;      // [M][ldm] or null: output zeroed where mask <= 0
   long ldm;
   int kChunk;           // K range of one split (multiple of GBK)
+  const float* rowScale = nullptr;  // EPI_F32_BIAS: C[m][n] = rowScale[m] * (A B + bias)[m][n] when set
 };
 
 // One operand tile (rows x GBK) in registers: two 16-byte chunks per thread.
@@ -197,7 +198,8 @@ __global__ void __launch_bounds__(GTHREADS, 4) gemm_bf16_kernel(const GemmArgs g
         if (row >= g.M) continue;
         const float v = acc[i][j][r];
         if (EPI == EPI_F32_BIAS) {
-          reinterpret_cast<float*>(g.C)[(long)row * g.ldc + col] = v + bcol;
+          const float o = v + bcol;
+          reinterpret_cast<float*>(g.C)[(long)row * g.ldc + col] = g.rowScale ? g.rowScale[row] * o : o;
         } else if (EPI == EPI_BF16_BIAS_RELU) {
           reinterpret_cast<u16*>(g.C)[(long)row * g.ldc + col] = f2bf(fmaxf(v + bcol, 0.f));
         } else if (EPI == EPI_BF16_MASK) {
@@ -287,6 +289,16 @@ extern "C" int libp_mlp_linear_fwd(int M, int K, int Nout, const unsigned short*
   GemmArgs g{M, Nout, K, X, ldx, Wt, ldw, Y, ldy, b, nullptr, 0, ((K + GBK - 1) / GBK) * GBK};
   if (relu) return gemm<false, false, EPI_BF16_BIAS_RELU>(g, 1, stream, "mlp_linear_fwd");
   return gemm<false, false, EPI_F32_BIAS>(g, 1, stream, "mlp_linear_fwd");
+}
+
+extern "C" int libp_mlp_linear_fwd_scaled(int M, int K, int Nout, const unsigned short* X, long ldx,
+                                          const unsigned short* Wt, long ldw, const float* b, const float* rowScale,
+                                          float* Y, long ldy, void* stream) {
+  if (M < 0 || K <= 0 || Nout <= 0) return arg_error("mlp_linear_fwd_scaled", "bad size");
+  if (!X || !Wt || !Y || !rowScale) return arg_error("mlp_linear_fwd_scaled", "null pointer");
+  if (ldx < K || ldw < K || ldy < Nout) return arg_error("mlp_linear_fwd_scaled", "leading dimension too small");
+  GemmArgs g{M, Nout, K, X, ldx, Wt, ldw, Y, ldy, b, nullptr, 0, ((K + GBK - 1) / GBK) * GBK, rowScale};
+  return gemm<false, false, EPI_F32_BIAS>(g, 1, stream, "mlp_linear_fwd_scaled");
 }
 
 extern "C" int libp_mlp_linear_bwd_data(int M, int K, int Nout, const unsigned short* dY, long lddy,

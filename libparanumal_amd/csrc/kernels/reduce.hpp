@@ -52,16 +52,19 @@ __device__ inline double block_sum(double v, double* s_w) {
 // result is the same bit for bit, but with eight loads in flight (at clamped
 // indices; entries past n add nothing) instead of a chain of dependent loads.
 // A one-block finisher over 73,429 partials (48^3 PCG gather-update) took
-// 34 us with the plain loop.
+// 34 us with the plain loop. U loads per thread per round (more for the
+// finishers whose partials come from other XCDs' blocks: each round is a far
+// load's latency).
+template <int U = 8>
 __device__ inline double strided_sum(const double* __restrict__ x, const int n) {
   double t = 0;
   const int B = blockDim.x;
-  for (int i0 = threadIdx.x; i0 < n; i0 += 8 * B) {
-    double v[8];
+  for (int i0 = threadIdx.x; i0 < n; i0 += U * B) {
+    double v[U];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = x[min(i0 + u * B, n - 1)];
+    for (int u = 0; u < U; ++u) v[u] = x[min(i0 + u * B, n - 1)];
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
+    for (int u = 0; u < U; ++u)
       if (i0 + u * B < n) t += v[u];
   }
   return t;

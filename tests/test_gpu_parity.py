@@ -304,8 +304,14 @@ def test_fused_gather_coarsen_assemble_match_two_calls(prec):
         assert lib.libp_axpy_f32(n, 1.0, cc.data_ptr(), 1.0, a.data_ptr(), st) == 0
         assert lib.libp_learned_assemble_add_f32(n, rs.data_ptr(), ci.data_ptr(), Np, Y.data_ptr(), Np, sc.data_ptr(),
                                                  None, r.data_ptr(), invD.data_ptr(), b.data_ptr(), st) == 0
+        # Y already scaled per element (what libp_mlp_linear_fwd_scaled writes), scale NULL: the same sums
+        Ys = (Y.view(E, Np) * sc.view(E, 1)).contiguous()
+        cs = torch.empty(n, dtype=dt, device="cuda")
+        assert lib.libp_learned_assemble_f32(n, rs.data_ptr(), ci.data_ptr(), Np, Ys.data_ptr(), Np, None,
+                                             None, r.data_ptr(), invD.data_ptr(), cs.data_ptr(), st) == 0
         torch.cuda.synchronize()
         assert torch.equal(a, b)
+        assert torch.equal(cs, cc)
 
 def test_rdotz_from_f32_matches_convert_and_rdotz():
     """The FP32 preconditioner output widened in the pass that forms r.z and

@@ -335,6 +335,12 @@ int libp_pcg_jacobi_csr_cg1_f64(int Nrows, const int* rowStarts, const int* colI
                                 const double* invDiag, double tol, int mode, int parity, double* x, double* const* w2,
                                 double* const* p2, double* const* s2, double* const* r2, double* const* u2,
                                 double* scalars, void* ws, void* stream);
+/* The stopping test of mode 3 of libp_pcg_jacobi_csr_cg1_f64 (parity of the
+ * last queued iteration), which then copies scalars[0, LIBP_PCG_NSCALARS) into
+ * hostMirror, pinned host memory mapped for the device (its device pointer),
+ * so that a host check after a chunk of iterations needs no further launch. */
+int libp_pcg_jacobi_csr_cg1_check_f64(int Nrows, int parity, double* scalars, void* ws, double* hostMirror,
+                                      void* stream);
 
 /* Non-blocking solvers (libs/linearSolver/linearSolverNBPCG.cpp,
  * linearSolverNBFPCG.cpp): fused update + multi-dot kernels replacing

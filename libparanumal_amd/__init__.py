@@ -98,6 +98,7 @@ SIGNATURES = {
                                     + [c_void_p] * 10),
     "libp_pcg_jacobi_csr_cg1_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int, c_int]
                                    + [c_void_p] * 9),
+    "libp_pcg_jacobi_csr_cg1_check_f64": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "libp_pcg_gather_pap_f64": (c_int, [c_int] + [c_void_p] * 8),
     "libp_pcg_jacobi_update_f64": (c_int, [c_int] + [c_void_p] * 9),
     "libp_pcg_jacobi_update_noalloc_f64": (c_int, [c_int] + [c_void_p] * 8),

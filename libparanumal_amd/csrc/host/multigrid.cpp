@@ -540,7 +540,8 @@ class coarseJacobiPCG_t {
     double* x;
     int first, n;
     hipGraphExec_t exec;
-    int kind = 0, parity = 0;  // kind 1: single-reduction chunk starting at this parity
+    int kind = 0, parity = 0;   // kind 1: single-reduction chunk starting at this parity
+    double* mirror = nullptr;  // host mirror its last check writes (assembled form)
   };
   std::vector<chunkGraph> graphs;
   // pinned, device-mapped copies of the scalars after each of two chunks in
@@ -552,7 +553,8 @@ class coarseJacobiPCG_t {
   int runChunks(int n0, Enqueue enqueue);
   void enqueueChunk(deviceMemory<double>& o_X, int firstIt, int n);
   int solveFusedHex1(deviceMemory<double>& o_X);
-  void enqueueCG1Chunk(deviceMemory<double>& o_X, int itStart, int n);
+  // returns true when the chunk's last check wrote the scalars to mirror itself
+  bool enqueueCG1Chunk(deviceMemory<double>& o_X, int itStart, int n, double* mirror);
   int solveCG1Hex1(deviceMemory<double>& o_X);
   void allocFused();
   // iterations of the last coarse solve
@@ -597,9 +599,9 @@ int coarseJacobiPCG_t::runChunks(int n0, Enqueue enqueue) {
   const int step = std::min(std::max(1, solver->checkEvery), 4);
   int queued = 0, slot = 0, it = 0;
   auto launch = [&](int n, int sl) {
-    enqueue(queued, n);
+    const bool mirrored = enqueue(queued, n, d_pipe[sl]);
     queued += n;
-    LIBP_CALL(libp_scalars_to_host_f64(LIBP_PCG_NSCALARS, o_s.ptr(), d_pipe[sl], p.s()));
+    if (!mirrored) LIBP_CALL(libp_scalars_to_host_f64(LIBP_PCG_NSCALARS, o_s.ptr(), d_pipe[sl], p.s()));
     LIBP_HIP(hipEventRecord(ev_pipe[sl], p.stream));
   };
   launch(std::max(1, std::min(n0, maxit)), 0);
@@ -633,7 +635,10 @@ int coarseJacobiPCG_t::solveFusedHex1(deviceMemory<double>& o_X) {
   // 4. Iterations past convergence return at once (device STOP flag).
   const int chunk = std::max(1, solver->checkEvery);
   const int n0 = (lastIterations >= 8) ? (lastIterations / 4) * 4 : chunk;
-  return runChunks(n0, [&](int itStart, int n) { enqueueChunk(o_X, itStart == 0, n); });
+  return runChunks(n0, [&](int itStart, int n, double*) {
+    enqueueChunk(o_X, itStart == 0, n);
+    return false;
+  });
 }
 
 /* n fused iterations (the first with first = 1 when firstIt). With
@@ -709,10 +714,10 @@ int coarseJacobiPCG_t::solveCG1Hex1(deviceMemory<double>& o_X) {
   // first chunk sized from the previous solve, as in solveFusedHex1
   const int chunk = std::max(1, solver->checkEvery);
   const int n0 = (lastIterations >= 8) ? (lastIterations / 4) * 4 : chunk;
-  return runChunks(n0, [&](int itStart, int n) { enqueueCG1Chunk(o_X, itStart, n); });
+  return runChunks(n0, [&](int itStart, int n, double* mirror) { return enqueueCG1Chunk(o_X, itStart, n, mirror); });
 }
 
-void coarseJacobiPCG_t::enqueueCG1Chunk(deviceMemory<double>& o_X, int itStart, int n) {
+bool coarseJacobiPCG_t::enqueueCG1Chunk(deviceMemory<double>& o_X, int itStart, int n, double* mirror) {
   platform_t& p = elliptic->platform;
   void* st = p.s();
   void* ws = p.o_reduceWS.ptr();
@@ -736,33 +741,41 @@ void coarseJacobiPCG_t::enqueueCG1Chunk(deviceMemory<double>& o_X, int itStart, 
         jacobi->o_invDiagA.ptr(), tol, mode, parity, o_X.ptr(), o_w.ptr(), P2, S2, R2, U2, o_AqL.ptr(), o_s.ptr(), ws,
         st));
   };
+  // the assembled form's last check also writes the scalars to the host mirror
+  double* const hostMirror = cg1csr ? mirror : nullptr;
   auto launch = [&]() {
     if (first) call(0, 0);
     for (int c = 0; c < n; ++c) {
       const int itc = itStart + c;
       call(itc == 0 ? 1 : 2, itc & 1);
     }
-    call(3, (itStart + n - 1) & 1);  // the stopping test of the last iteration (of this parity)
+    // the stopping test of the last iteration (of this parity)
+    if (hostMirror)
+      LIBP_CALL(libp_pcg_jacobi_csr_cg1_check_f64(N, (itStart + n - 1) & 1, o_s.ptr(), o_ws2.ptr(), hostMirror, st));
+    else
+      call(3, (itStart + n - 1) & 1);
   };
   if (!useGraph) {
     launch();
-    return;
+    return hostMirror != nullptr;
   }
   const int parity = itStart & 1;
   for (auto& g : graphs)
-    if (g.kind == 1 && g.x == o_X.ptr() && g.first == first && g.n == n && g.parity == parity) {
+    if (g.kind == 1 && g.x == o_X.ptr() && g.first == first && g.n == n && g.parity == parity &&
+        g.mirror == hostMirror) {
       LIBP_HIP(hipGraphLaunch(g.exec, p.stream));
-      return;
+      return hostMirror != nullptr;
     }
   hipGraph_t graph;
   LIBP_HIP(hipStreamBeginCapture(p.stream, hipStreamCaptureModeThreadLocal));
   launch();
   LIBP_HIP(hipStreamEndCapture(p.stream, &graph));
-  chunkGraph g{o_X.ptr(), first, n, nullptr, 1, parity};
+  chunkGraph g{o_X.ptr(), first, n, nullptr, 1, parity, hostMirror};
   LIBP_HIP(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
   LIBP_HIP(hipGraphDestroy(graph));
   graphs.push_back(g);
   LIBP_HIP(hipGraphLaunch(g.exec, p.stream));
+  return hostMirror != nullptr;
 }
 
 template <>

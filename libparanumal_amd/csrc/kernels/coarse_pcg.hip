@@ -499,14 +499,21 @@ __global__ void __launch_bounds__(RBS) coarse_cg1_gather_kernel(const int Nrows,
   }
 }
 
-// stopping test on the r.r partials of the last cg1_gather (one wavefront)
-__global__ void __launch_bounds__(64) coarse_cg1_check_kernel(const int nb, double* s, const ReduceWS* __restrict__ ws) {
-  if (s[8] != 0.0) return;
-  const double rr = coarse_sum_slot(ws, 2, nb);
-  if (threadIdx.x == 0) {
-    s[5] = rr;
-    if (s[7] > 0.0 && rr <= s[7]) s[8] = 1.0;
+// stopping test on the r.r partials of the last cg1_gather (one wavefront);
+// with mirror set, the scalars are then copied into pinned host memory mapped
+// for the device, whether or not the solve had already stopped, so the host
+// needs no separate copy after the chunk
+__global__ void __launch_bounds__(64) coarse_cg1_check_kernel(const int nb, double* s, const ReduceWS* __restrict__ ws,
+                                                              double* __restrict__ mirror) {
+  if (s[8] == 0.0) {
+    const double rr = coarse_sum_slot(ws, 2, nb);
+    if (threadIdx.x == 0) {
+      s[5] = rr;
+      if (s[7] > 0.0 && rr <= s[7]) s[8] = 1.0;
+    }
   }
+  if (mirror && threadIdx.x == 0)
+    for (int i = 0; i < LIBP_PCG_NSCALARS; ++i) mirror[i] = s[i];
 }
 
 // The single-reduction iteration on the assembled operator (CSR rows of the
@@ -695,7 +702,7 @@ extern "C" int libp_pcg_jacobi_hex1_cg1_f64(int Nelements, const int* GlobalToLo
   hipStream_t st = as_stream(stream);
   const int nb = coarse_row_blocks(Nrows);
   if (mode == 3) {
-    hipLaunchKernelGGL(coarse_cg1_check_kernel, dim3(1), dim3(64), 0, st, nb, scalars, (const ReduceWS*)ws);
+    hipLaunchKernelGGL(coarse_cg1_check_kernel, dim3(1), dim3(64), 0, st, nb, scalars, (const ReduceWS*)ws, nullptr);
     return launch_status("pcg_jacobi_hex1_cg1 (check)");
   }
   const int a = parity, b = parity ^ 1;
@@ -725,7 +732,8 @@ extern "C" int libp_pcg_jacobi_csr_cg1_f64(int Nrows, const int* rowStarts, cons
   const int a = parity, b = parity ^ 1;
   if (mode == 3) {
     // iteration (mode 1 or 2) of parity a wrote its partials to w[b]
-    hipLaunchKernelGGL(coarse_cg1_check_kernel, dim3(1), dim3(64), 0, st, nb, scalars, (const ReduceWS*)(w + b));
+    hipLaunchKernelGGL(coarse_cg1_check_kernel, dim3(1), dim3(64), 0, st, nb, scalars, (const ReduceWS*)(w + b),
+                       nullptr);
     return launch_status("pcg_jacobi_csr_cg1 (check)");
   }
   const CG1Vec cur{p2[a], s2[a], r2[a], u2[a]}, nxt{p2[b], s2[b], r2[b], u2[b]};
@@ -734,4 +742,16 @@ extern "C" int libp_pcg_jacobi_csr_cg1_f64(int Nrows, const int* rowStarts, cons
                      parity, tol * tol, nb, invDiag, x, cur, nxt, w2[a], w2[b], scalars, (const ReduceWS*)(w + a),
                      w + (mode == 0 ? a : b));
   return launch_status("pcg_jacobi_csr_cg1");
+}
+
+extern "C" int libp_pcg_jacobi_csr_cg1_check_f64(int Nrows, int parity, double* scalars, void* ws, double* hostMirror,
+                                                 void* stream) {
+  if (Nrows <= 0) return LIBP_OK;
+  if (!scalars || !ws) return arg_error("pcg_jacobi_csr_cg1_check", "null pointer");
+  if (parity & ~1) return arg_error("pcg_jacobi_csr_cg1_check", "parity 0 or 1");
+  const int nb = (Nrows + CSR_ROWS - 1) / CSR_ROWS < RMAXB ? (Nrows + CSR_ROWS - 1) / CSR_ROWS : RMAXB;
+  // iteration (mode 1 or 2) of parity p wrote its partials to workspace p ^ 1
+  hipLaunchKernelGGL(coarse_cg1_check_kernel, dim3(1), dim3(64), 0, as_stream(stream), nb, scalars,
+                     (const ReduceWS*)ws + (parity ^ 1), hostMirror);
+  return launch_status("pcg_jacobi_csr_cg1_check");
 }

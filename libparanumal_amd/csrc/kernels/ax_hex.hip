@@ -40,13 +40,14 @@ struct AxShape {
   static constexpr int Np = Nq * Nq * Nq;
   // elements per workgroup: fill a wavefront for small N. At N=7 the FP64
   // kernel measured fastest with four one-wavefront elements per workgroup,
-  // the FP32 kernel with one element and a higher occupancy target (32^3 box:
-  // 114 us against 128 us with the FP64 shape).
-  static constexpr int NE = (Nq == 8) ? (f32 ? 1 : 4) : ((NqNq >= 64) ? 1 : (64 / NqNq));
+  // the FP32 kernel with two (32^3 box: 101.1 us, against 105.0 us with one
+  // element and eight waves per SIMD, 101.4 us with four, 211 us with two and
+  // eight waves).
+  static constexpr int NE = (Nq == 8) ? (f32 ? 2 : 4) : ((NqNq >= 64) ? 1 : (64 / NqNq));
   static constexpr int threads = NE * NqNq;
   // occupancy target: bounds the register budget so that the compiler
   // re-reads loop-invariant LDS operands instead of pinning them in VGPRs
-  static constexpr int waves = (Nq == 8 && f32) ? 8 : ((Nq <= 8) ? 4 : 2);
+  static constexpr int waves = (Nq <= 8) ? 4 : 2;
 };
 
 // Number of element-boundary nodes of an Nq^3 element, and the position of

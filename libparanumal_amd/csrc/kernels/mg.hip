@@ -5,13 +5,16 @@
 // okl/ellipticPreconCoarsenHex3D.okl:209-294 and
 // okl/ellipticPreconProlongateHex3D.okl:216-301. The reference uses NqF^3
 // threads, which exceeds the 1024-lane workgroup limit for N >= 10; the 2-D
-// tile looping over the third index covers every degree up to 15.
+// tile looping over the third index covers every degree up to 15. NQC > 0
+// fixes the coarse size at compile time (the 7->3 and 3->1 steps of the
+// default schedule), so the sweeps unroll; NQC = 0 takes it at run time. The
+// arithmetic and its order are the same either way.
 #include "kcommon.hpp"
 
 namespace libp_amd {
 
-template <int NqF, typename T>
-__global__ void __launch_bounds__(NqF * NqF) mg_coarsen_kernel(const int NqC, const int* __restrict__ elementList,
+template <int NqF, int NQC, typename T>
+__global__ void __launch_bounds__(NqF * NqF) mg_coarsen_kernel(const int NqC_, const int* __restrict__ elementList,
                                                                const int* __restrict__ G2L, const T* __restrict__ P,
                                                                const T* __restrict__ qf, const T* __restrict__ w,
                                                                T* __restrict__ qc) {
@@ -22,6 +25,7 @@ __global__ void __launch_bounds__(NqF * NqF) mg_coarsen_kernel(const int NqC, co
   __shared__ T s_cff[NqF - 1][NqF][NqF];
   __shared__ T s_P[NqF][NqF - 1];
   T(*s_ccf)[NqF][NqF] = s_f;
+  const int NqC = NQC > 0 ? NQC : NqC_;
   const int e = blockIdx.x;
   const int element = elementList ? elementList[e] : e;
   const int t = threadIdx.x, i = t % NqF, j = t / NqF;
@@ -64,8 +68,8 @@ __global__ void __launch_bounds__(NqF * NqF) mg_coarsen_kernel(const int NqC, co
   }
 }
 
-template <int NqF, typename T>
-__global__ void __launch_bounds__(NqF * NqF) mg_prolong_kernel(const int NqC, const int* __restrict__ elementList,
+template <int NqF, int NQC, typename T>
+__global__ void __launch_bounds__(NqF * NqF) mg_prolong_kernel(const int NqC_, const int* __restrict__ elementList,
                                                                const int* __restrict__ G2LC, const T* __restrict__ P,
                                                                const T* __restrict__ qcIn, T* __restrict__ qN) {
   constexpr int NpF = NqF * NqF * NqF;
@@ -75,6 +79,7 @@ __global__ void __launch_bounds__(NqF * NqF) mg_prolong_kernel(const int NqC, co
   __shared__ T s_P[NqF][NqF - 1];
   T(*s_ccc)[NqF][NqF] = s_A;
   T(*s_cff)[NqF][NqF] = s_A;
+  const int NqC = NQC > 0 ? NQC : NqC_;
   const int e = blockIdx.x;
   const int element = elementList ? elementList[e] : e;
   const int t = threadIdx.x, i = t % NqF, j = t / NqF;
@@ -123,15 +128,32 @@ static int transfer(int Nf, int Nc, int Nelements, const int* elist, const int* 
   if (!g2l || !P || !in || !out) return arg_error("mg_transfer", "null pointer");
   hipStream_t st = as_stream(stream);
   const int NqC = Nc + 1;
+  // the default schedule's steps with the coarse size fixed at compile time
+  if (Nf == 7 && Nc == 3) {
+    if (COARSEN)
+      hipLaunchKernelGGL((mg_coarsen_kernel<8, 4, T>), dim3(Nelements), dim3(64), 0, st, NqC, elist, g2l, P, in, w,
+                         out);
+    else
+      hipLaunchKernelGGL((mg_prolong_kernel<8, 4, T>), dim3(Nelements), dim3(64), 0, st, NqC, elist, g2l, P, in, out);
+    return launch_status(COARSEN ? "mg_coarsen" : "mg_prolong");
+  }
+  if (Nf == 3 && Nc == 1) {
+    if (COARSEN)
+      hipLaunchKernelGGL((mg_coarsen_kernel<4, 2, T>), dim3(Nelements), dim3(16), 0, st, NqC, elist, g2l, P, in, w,
+                         out);
+    else
+      hipLaunchKernelGGL((mg_prolong_kernel<4, 2, T>), dim3(Nelements), dim3(16), 0, st, NqC, elist, g2l, P, in, out);
+    return launch_status(COARSEN ? "mg_coarsen" : "mg_prolong");
+  }
   switch (Nf + 1) {
-#define LIBP_MG_CASE(NQ)                                                                                         \
-  case NQ:                                                                                                       \
-    if (COARSEN)                                                                                                 \
-      hipLaunchKernelGGL((mg_coarsen_kernel<NQ, T>), dim3(Nelements), dim3(NQ * NQ), 0, st, NqC, elist, g2l, P, \
-                         in, w, out);                                                                            \
-    else                                                                                                         \
-      hipLaunchKernelGGL((mg_prolong_kernel<NQ, T>), dim3(Nelements), dim3(NQ * NQ), 0, st, NqC, elist, g2l, P,  \
-                         in, out);                                                                               \
+#define LIBP_MG_CASE(NQ)                                                                                            \
+  case NQ:                                                                                                          \
+    if (COARSEN)                                                                                                    \
+      hipLaunchKernelGGL((mg_coarsen_kernel<NQ, 0, T>), dim3(Nelements), dim3(NQ * NQ), 0, st, NqC, elist, g2l, P, \
+                         in, w, out);                                                                               \
+    else                                                                                                            \
+      hipLaunchKernelGGL((mg_prolong_kernel<NQ, 0, T>), dim3(Nelements), dim3(NQ * NQ), 0, st, NqC, elist, g2l, P,  \
+                         in, out);                                                                                  \
     break;
     LIBP_MG_CASE(3)
     LIBP_MG_CASE(4)

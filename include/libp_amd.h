@@ -497,6 +497,19 @@ int libp_f32_to_bf16(long n, const float* x, unsigned short* y, void* stream);
 int libp_learned_features_f32(int Nelements, int Np, int Ntr, const int* GlobalToLocal, const int* traceIds,
                               const float* r, const float* invDiag, unsigned short* X, long ldx, float* scale,
                               void* stream);
+/* Weights W[Nout][ldw] (BF16, K inputs) in the order the fused layer reads them:
+ * 16-byte chunk (cb*K/16 + t)*64 + lane = W[32 cb + lane%32][16 t + 8 (lane/32) .. +7].
+ * Nout a multiple of 32, K of 16; Wpacked holds Nout*K values, 16-byte aligned. */
+int libp_learned_pack_weights_bf16(int Nout, int K, const unsigned short* W, long ldw, unsigned short* Wpacked,
+                                   void* stream);
+/* The linear smoother's layer fused with its input: Y[e][n] = scale[e] * sum_k X[e][k] W[n][k]
+ * with X and scale as libp_learned_features_f32 forms them (scale may be NULL), W packed
+ * by libp_learned_pack_weights_bf16. The same Y, to the bit, as libp_learned_features_f32
+ * followed by libp_mlp_linear_fwd_scaled (bias NULL); X is never stored. N=7 shape only:
+ * Np = 512, Ntr = 384, Nout = 512 (other shapes return an error). */
+int libp_learned_linear_layer_f32(int Nelements, int Np, int Ntr, int Nout, const int* GlobalToLocal,
+                                  const int* traceIds, const float* r, const float* invDiag,
+                                  const unsigned short* Wpacked, float* Y, long ldy, float* scale, void* stream);
 /* c[g] = (1/|row g|) sum_{j in row g} scale[j/Np] Y[j/Np][j%Np] + omega invDiag[g] r[g],
  * rows of the masked gather (rowStarts, colIds); omega is a device pointer (or NULL = 0).
  * scale NULL: Y holds the scaled values already (libp_mlp_linear_fwd_scaled). */

@@ -140,6 +140,9 @@ SIGNATURES = {
     "libp_f32_to_bf16": (c_int, [c_long, c_void_p, c_void_p, c_void_p]),
     "libp_learned_features_f32": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                           c_long, c_void_p, c_void_p]),
+    "libp_learned_pack_weights_bf16": (c_int, [c_int, c_int, c_void_p, c_long, c_void_p, c_void_p]),
+    "libp_learned_linear_layer_f32": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                              c_void_p, c_void_p, c_long, c_void_p, c_void_p]),
     "libp_learned_assemble_f32": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_long, c_void_p, c_void_p,
                                           c_void_p, c_void_p, c_void_p, c_void_p]),
     "libp_learned_alpha_f32": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

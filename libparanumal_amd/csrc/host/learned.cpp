@@ -71,6 +71,8 @@ learnedSmoother_t::learnedSmoother_t(elliptic_t& el, int _arch) : elliptic(el), 
   o_m = platform.malloc<float>(nParams);
   o_v = platform.malloc<float>(nParams);
   o_thetaBf = platform.malloc<unsigned short>(nParams);
+  fusedLayer = arch == ARCH_LINEAR && Np == 512 && 6 * Nfp == 384;
+  if (fusedLayer) o_Wpacked = platform.malloc<unsigned short>((size_t)nOut * nIn);
   o_X = platform.malloc<unsigned short>((size_t)E * nIn);
   if (H > 0) {
     o_H1 = platform.malloc<unsigned short>((size_t)E * H);
@@ -174,6 +176,8 @@ void learnedSmoother_t::setParameters(const std::vector<float>& th) {
   o_theta.copyFrom(th.data(), nParams, 0, platform.stream);
   omegaHost = th[oOm];
   LIBP_CALL(libp_f32_to_bf16(nParams, o_theta.ptr(), o_thetaBf.ptr(), platform.s()));
+  if (fusedLayer)  // the only writer of the linear weights (Adam trains the MLP only)
+    LIBP_CALL(libp_learned_pack_weights_bf16(nOut, nIn, o_thetaBf.ptr() + oW3, nIn, o_Wpacked.ptr(), platform.s()));
   LIBP_HIP(hipMemsetAsync(o_m.ptr(), 0, nParams * sizeof(float), platform.stream));
   LIBP_HIP(hipMemsetAsync(o_v.ptr(), 0, nParams * sizeof(float), platform.stream));
   adamStep = 0;
@@ -255,14 +259,19 @@ void learnedSmoother_t::correction(const float* r, const float* invDiag, float* 
   void* s = platform.s();
   const unsigned short* Wb = o_thetaBf.ptr();
   const float* th = o_theta.ptr();
-  LIBP_CALL(libp_learned_features_f32(E, Np, 6 * Nfp, elliptic.o_GlobalToLocal.ptr(), o_trace.ptr(), r, invDiag,
-                                      o_X.ptr(), nIn, o_scale.ptr(), s));
   // the output layer multiplies each element's row by its input scale s_e in its
   // epilogue, so the assembly below reads one value per copy (scale NULL)
-  if (arch == ARCH_LINEAR) {
+  if (fusedLayer) {  // same Y to the bit as features + GEMM, without the input round trip
+    LIBP_CALL(libp_learned_linear_layer_f32(E, Np, 6 * Nfp, nOut, elliptic.o_GlobalToLocal.ptr(), o_trace.ptr(), r,
+                                            invDiag, o_Wpacked.ptr(), o_Y.ptr(), nOut, o_scale.ptr(), s));
+  } else if (arch == ARCH_LINEAR) {
+    LIBP_CALL(libp_learned_features_f32(E, Np, 6 * Nfp, elliptic.o_GlobalToLocal.ptr(), o_trace.ptr(), r, invDiag,
+                                        o_X.ptr(), nIn, o_scale.ptr(), s));
     LIBP_CALL(libp_mlp_linear_fwd_scaled(E, nIn, nOut, o_X.ptr(), nIn, Wb + oW3, nIn, nullptr, o_scale.ptr(),
                                          o_Y.ptr(), nOut, s));
   } else {
+    LIBP_CALL(libp_learned_features_f32(E, Np, 6 * Nfp, elliptic.o_GlobalToLocal.ptr(), o_trace.ptr(), r, invDiag,
+                                        o_X.ptr(), nIn, o_scale.ptr(), s));
     LIBP_CALL(libp_mlp_linear_fwd(E, nIn, H, o_X.ptr(), nIn, Wb + oW1, nIn, th + ob1, 1, o_H1.ptr(), H, s));
     LIBP_CALL(libp_mlp_linear_fwd(E, H, H, o_H1.ptr(), H, Wb + oW2, H, th + ob2, 1, o_H2.ptr(), H, s));
     LIBP_CALL(libp_mlp_linear_fwd_scaled(E, H, nOut, o_H2.ptr(), H, Wb + oW3, H, th + ob3, o_scale.ptr(), o_Y.ptr(),

@@ -62,6 +62,10 @@ class learnedSmoother_t {
   deviceMemory<float> o_invLen;  // [Ndofs] 1 / multiplicity
   deviceMemory<float> o_theta, o_grad, o_m, o_v;
   deviceMemory<unsigned short> o_thetaBf;
+  // LINEAR at N=7: the layer fused with its input (libp_learned_linear_layer_f32) reads
+  // the weights in the MFMA-packed order, repacked whenever the BF16 weights change
+  bool fusedLayer = false;
+  deviceMemory<unsigned short> o_Wpacked;
   deviceMemory<unsigned short> o_X, o_H1, o_H2, o_dY, o_dH1, o_dH2, o_T;
   deviceMemory<float> o_scale, o_Y, o_c, o_Ac, o_res, o_gc, o_d, o_Ad;
   deviceMemory<double> o_sc;  // [0..3] step statistics, [4] last loss, [5] running loss sum

@@ -26,11 +26,58 @@ __device__ inline u16 f2bf_l(float f) {
   return (u16)(u >> 16);
 }
 
-// One wavefront per element, four per block. NPER = ceil(nIn/64) values per
-// lane. NP, NTR > 0 fix Np and Ntr at compile time (the N=7 instance, 512 +
-// 384): which map a lane's feature comes from is then known statically, and
-// every index load of the lane is issued before the value loads that depend
-// on them. NP = 0 takes the sizes at run time.
+// Network input of NE elements, one wavefront: lane l holds the features
+// f = l + 64 i (i < NPER) of each element as BF16 and returns the elements'
+// RMS scales. e[n] < 0 stands for no element (zero input, scale 0). NP, NTR > 0
+// fix Np and Ntr at compile time (the N=7 instance, 512 + 384): which map a
+// lane's feature comes from is then known statically, and every index load of
+// the lane is issued before the value loads that depend on them. NP = 0 takes
+// the sizes at run time. The arithmetic of one element does not depend on NE.
+template <int NPER, int NP, int NTR, int NE>
+__device__ inline void element_inputs(const int (&e)[NE], const int l, const int Np_, const int Ntr_,
+                                      const int* __restrict__ G2L, const int* __restrict__ traceIds,
+                                      const float* __restrict__ r, const float* __restrict__ invD, u16 (&xb)[NE][NPER],
+                                      float (&scale)[NE]) {
+  const int Np = NP > 0 ? NP : Np_, Ntr = NP > 0 ? NTR : Ntr_;
+  const int nIn = Np + Ntr;
+  int gid[NE][NPER];
+#pragma unroll
+  for (int n = 0; n < NE; ++n)
+#pragma unroll
+    for (int i = 0; i < NPER; ++i) {
+      const int f = l + 64 * i;
+      int g = -1;
+      if (e[n] >= 0) {
+        if (f < Np) g = G2L[(long)e[n] * Np + f];
+        else if (f < nIn) g = traceIds[(long)e[n] * Ntr + (f - Np)];
+      }
+      gid[n][i] = g;
+    }
+  float z[NE][NPER];
+#pragma unroll
+  for (int n = 0; n < NE; ++n)
+#pragma unroll
+    for (int i = 0; i < NPER; ++i) {
+      const int g = gid[n][i];
+      const int gc = max(g, 0);  // unconditional loads: a missing node reads entry 0 and takes 0
+      const float dv = invD[gc], rv = r[gc];
+      z[n][i] = (g >= 0) ? dv * rv : 0.f;
+    }
+#pragma unroll
+  for (int n = 0; n < NE; ++n) {
+    double ss = 0;
+#pragma unroll
+    for (int i = 0; i < NPER; ++i) ss += (double)z[n][i] * z[n][i];
+    ss = wave_sum(ss);
+    const float s = (float)sqrt(ss / nIn);
+    const float inv = (s > 0.f) ? 1.f / s : 0.f;
+#pragma unroll
+    for (int i = 0; i < NPER; ++i) xb[n][i] = f2bf_l(z[n][i] * inv);
+    scale[n] = s;
+  }
+}
+
+// One wavefront per element, four per block.
 template <int NPER, int NP, int NTR>
 __global__ void __launch_bounds__(256) learned_features_kernel(const int E, const int Np_, const int Ntr_,
                                                                const int* __restrict__ G2L,
@@ -41,37 +88,144 @@ __global__ void __launch_bounds__(256) learned_features_kernel(const int E, cons
   const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int l = threadIdx.x & 63;
   if (e >= E) return;
-  const int Np = NP > 0 ? NP : Np_, Ntr = NP > 0 ? NTR : Ntr_;
-  const int nIn = Np + Ntr;
-  int gid[NPER];
+  const int nIn = NP > 0 ? NP + NTR : Np_ + Ntr_;
+  const int ee[1] = {e};
+  u16 xb[1][NPER];
+  float s[1];
+  element_inputs<NPER, NP, NTR, 1>(ee, l, Np_, Ntr_, G2L, traceIds, r, invD, xb, s);
 #pragma unroll
   for (int i = 0; i < NPER; ++i) {
     const int f = l + 64 * i;
-    int g = -1;
-    if (f < Np) g = G2L[(long)e * Np + f];
-    else if (f < nIn) g = traceIds[(long)e * Ntr + (f - Np)];
-    gid[i] = g;
+    if (f < nIn) X[(long)e * ldx + f] = xb[0][i];
   }
-  float z[NPER];
-  double ss = 0;
+  if (l == 0) scale[e] = s[0];
+}
+
+// Linear layer of the N=7 instance fused with its input: Y[e][n] =
+// s_e * sum_k X[e][k] W[n][k] for 896 inputs and 512 outputs, where X is the
+// BF16 input of element_inputs and s_e its scale. The products, their grouping
+// into v_mfma_f32_32x32x16_bf16 steps over k = 16t..16t+15 in ascending t, the
+// operand lanes and the epilogue are those of gemm_bf16_kernel with
+// libp_mlp_linear_fwd_scaled, so Y is the same to the bit; X never goes to
+// memory.
+//
+// One workgroup of eight wavefronts per 64 elements. Phase 1: each wavefront
+// builds the inputs of eight elements, two at a time, into LDS rows of 904
+// BF16 (the 8-element pad puts consecutive rows four banks apart, so the
+// MFMA operand reads are conflict-free). Phase 2: wavefront w forms columns
+// 64w..64w+63 of all 64 rows (2x2 MFMA blocks of 32x32). Its weights come from
+// the packed copy (learned_pack_weights_kernel: one 1 KiB contiguous block
+// per 32 columns and 16 inputs, in the MFMA's lane order) through a ring of
+// LL_PF steps of register prefetch; the 917 KB of packed weights stay in L2.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int LL_ROWS = 64, LL_NP = 512, LL_NTR = 384, LL_K = LL_NP + LL_NTR, LL_N = 512;
+constexpr int LL_LDA = LL_K + 8, LL_STEPS = LL_K / 16, LL_PF = 8;
+static_assert(LL_STEPS % LL_PF == 0, "prefetch ring must divide the steps");
+
+__global__ void __launch_bounds__(512, 1) learned_linear_layer_kernel(const int E, const int* __restrict__ G2L,
+                                                                       const int* __restrict__ traceIds,
+                                                                       const float* __restrict__ r,
+                                                                       const float* __restrict__ invD,
+                                                                       const u32x4* __restrict__ Wp,
+                                                                       float* __restrict__ Y, const long ldy,
+                                                                       float* __restrict__ scale) {
+  __shared__ __attribute__((aligned(16))) u16 sA[LL_ROWS * LL_LDA];
+  __shared__ float sS[LL_ROWS];
+  const int t = threadIdx.x, w = t >> 6, l = t & 63;
+  const long e0 = (long)blockIdx.x * LL_ROWS;
+
+  // phase 1: rows 8w..8w+7
+#pragma unroll 1
+  for (int q = 0; q < 8; q += 2) {
+    const int row0 = 8 * w + q;
+    const int ee[2] = {e0 + row0 < E ? (int)(e0 + row0) : -1, e0 + row0 + 1 < E ? (int)(e0 + row0 + 1) : -1};
+    u16 xb[2][14];
+    float sc[2];
+    element_inputs<14, LL_NP, LL_NTR, 2>(ee, l, LL_NP, LL_NTR, G2L, traceIds, r, invD, xb, sc);
 #pragma unroll
-  for (int i = 0; i < NPER; ++i) {
-    const int g = gid[i];
-    const int gc = max(g, 0);  // unconditional loads: a missing node reads entry 0 and takes 0
-    const float dv = invD[gc], rv = r[gc];
-    z[i] = (g >= 0) ? dv * rv : 0.f;
+    for (int n = 0; n < 2; ++n) {
+#pragma unroll
+      for (int i = 0; i < 14; ++i) sA[(row0 + n) * LL_LDA + l + 64 * i] = xb[n][i];
+      if (l == 0) {
+        sS[row0 + n] = sc[n];
+        if (scale && ee[n] >= 0) scale[ee[n]] = sc[n];
+      }
+    }
   }
+  __syncthreads();
+
+  // phase 2: columns 64w..64w+63
+  f32x16 acc[2][2];
 #pragma unroll
-  for (int i = 0; i < NPER; ++i) ss += (double)z[i] * z[i];
-  ss = wave_sum(ss);
-  const float s = (float)sqrt(ss / nIn);
-  const float inv = (s > 0.f) ? 1.f / s : 0.f;
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
-  for (int i = 0; i < NPER; ++i) {
-    const int f = l + 64 * i;
-    if (f < nIn) X[(long)e * ldx + f] = f2bf_l(z[i] * inv);
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int k = 0; k < 16; ++k) acc[i][j][k] = 0.f;
+  const int lr = l & 31, lk = 8 * (l >> 5);
+  const u32x4* wp0 = Wp + (long)(2 * w) * LL_STEPS * 64 + l;
+  const u32x4* wp1 = wp0 + LL_STEPS * 64;
+  const u16* a0p = sA + lr * LL_LDA + lk;
+  const u16* a1p = a0p + 32 * LL_LDA;
+  u32x4 bq[LL_PF][2];
+#pragma unroll
+  for (int p = 0; p < LL_PF; ++p) {
+    bq[p][0] = wp0[p * 64];
+    bq[p][1] = wp1[p * 64];
   }
-  if (l == 0) scale[e] = s;
+#pragma unroll 1
+  for (int s0 = 0; s0 < LL_STEPS; s0 += LL_PF) {
+#pragma unroll
+    for (int p = 0; p < LL_PF; ++p) {
+      const int st = s0 + p;
+      const bf16x8 b0 = __builtin_bit_cast(bf16x8, bq[p][0]);
+      const bf16x8 b1 = __builtin_bit_cast(bf16x8, bq[p][1]);
+      if (st + LL_PF < LL_STEPS) {
+        bq[p][0] = wp0[(st + LL_PF) * 64];
+        bq[p][1] = wp1[(st + LL_PF) * 64];
+      }
+      const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(a0p + st * 16);
+      const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(a1p + st * 16);
+      acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, acc[1][1], 0, 0, 0);
+    }
+  }
+
+  // epilogue: s_e * (sum + 0), the bias-free form of the GEMM's scaled epilogue
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = 64 * w + 32 * j + lr;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const int row = 32 * i + 8 * (k >> 2) + 4 * (l >> 5) + (k & 3);
+        if (e0 + row >= E) continue;
+        const float o = acc[i][j][k] + 0.f;
+        Y[(e0 + row) * ldy + col] = sS[row] * o;
+      }
+    }
+}
+
+// Packed weights of the fused layer: 16-byte chunk (cb*K/16 + t)*64 + lane
+// holds W[32 cb + lane%32][16 t + 8 (lane/32) .. +7], the B operand that lane
+// supplies to step t of the MFMA for the 32 columns of block cb.
+__global__ void __launch_bounds__(256) learned_pack_weights_kernel(const int nChunks, const int ksteps,
+                                                                   const u16* __restrict__ W, const long ldw,
+                                                                   u32x4* __restrict__ Wp) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= nChunks) return;
+  const int lane = c & 63, st = (c >> 6) % ksteps, cb = (c >> 6) / ksteps;
+  const u16* src = W + (long)(32 * cb + (lane & 31)) * ldw + 16 * st + 8 * (lane >> 5);
+  u32x4 v;
+#pragma unroll
+  for (int h = 0; h < 4; ++h) v[h] = (unsigned)src[2 * h] | ((unsigned)src[2 * h + 1] << 16);
+  Wp[c] = v;
 }
 
 // c[g] = invLen * sum_j s_e Y[j] + omega * invD[g] r[g]; omega read from device
@@ -298,6 +452,35 @@ extern "C" int libp_learned_features_f32(int Nelements, int Np, int Ntr, const i
   }
 #undef LF
   return launch_status("learned_features");
+}
+
+extern "C" int libp_learned_pack_weights_bf16(int Nout, int K, const unsigned short* W, long ldw,
+                                              unsigned short* Wpacked, void* stream) {
+  if (Nout <= 0 || K <= 0) return LIBP_OK;
+  if (!W || !Wpacked) return arg_error("learned_pack_weights", "null pointer");
+  if (Nout % 32 || K % 16 || ldw < K) return arg_error("learned_pack_weights", "Nout must be a multiple of 32, K of 16");
+  if (((uintptr_t)Wpacked & 15) != 0) return arg_error("learned_pack_weights", "packed weights must be 16-byte aligned");
+  const int nChunks = (Nout / 32) * (K / 16) * 64;
+  hipLaunchKernelGGL(learned_pack_weights_kernel, dim3((nChunks + 255) / 256), dim3(256), 0, as_stream(stream),
+                     nChunks, K / 16, W, ldw, reinterpret_cast<u32x4*>(Wpacked));
+  return launch_status("learned_pack_weights");
+}
+
+extern "C" int libp_learned_linear_layer_f32(int Nelements, int Np, int Ntr, int Nout, const int* GlobalToLocal,
+                                             const int* traceIds, const float* r, const float* invDiag,
+                                             const unsigned short* Wpacked, float* Y, long ldy, float* scale,
+                                             void* stream) {
+  if (Nelements <= 0) return LIBP_OK;
+  if (Np != LL_NP || Ntr != LL_NTR || Nout != LL_N)
+    return arg_error("learned_linear_layer", "only the N=7 shape (512 + 384 inputs, 512 outputs)");
+  if (!GlobalToLocal || !traceIds || !r || !invDiag || !Wpacked || !Y)
+    return arg_error("learned_linear_layer", "null pointer");
+  if (ldy < Nout) return arg_error("learned_linear_layer", "bad size");
+  if (((uintptr_t)Wpacked & 15) != 0) return arg_error("learned_linear_layer", "packed weights must be 16-byte aligned");
+  hipLaunchKernelGGL(learned_linear_layer_kernel, dim3((Nelements + LL_ROWS - 1) / LL_ROWS), dim3(512), 0,
+                     as_stream(stream), Nelements, GlobalToLocal, traceIds, r, invDiag,
+                     reinterpret_cast<const u32x4*>(Wpacked), Y, ldy, scale);
+  return launch_status("learned_linear_layer");
 }
 
 template <bool ACC, bool SCALED>

@@ -155,6 +155,63 @@ def test_features_n7_instance_against_numpy():
     assert diff.max() <= 1 and (diff > 0).mean() < 1e-3
 
 
+@pytest.mark.parametrize("n", [5, 6])
+def test_fused_linear_layer_matches_features_and_gemm(n):
+    """libp_learned_linear_layer_f32 (the N=7 layer fused with its input, weights in
+    the packed MFMA order) gives the same Y and scales, to the bit, as
+    libp_learned_features_f32 followed by libp_mlp_linear_fwd_scaled. E = 120 and 210
+    elements leave the last 64-element workgroup partly empty; an all-zero residual
+    gives scale 0 and Y = 0. The packed weights are checked against numpy."""
+    import ctypes
+    lib = lp().lib()
+    p = lp().Elliptic(learned_settings(n, 7, "7,3,1", **LIN))
+    E, Np, Ng = p.geti("Nelements"), p.geti("Np"), p.geti("Ngather")
+    assert Np == 512 and E % 64 != 0
+    g2l = dev(p.host_array("GlobalToLocal"))
+    tr = dev(p.host_array("learnedTrace"))
+    rng = np.random.default_rng(100 + n)
+    W = bf16_bits(rng.uniform(-1, 1, (512, 896)).astype(np.float32) / 30)
+    Wd = dev(W)
+    Wp = torch.zeros(512 * 896, dtype=torch.int16, device="cuda")
+    s0 = ctypes.c_void_p(0)
+    assert lib.libp_learned_pack_weights_bf16(512, 896, Wd.data_ptr(), 896, Wp.data_ptr(), s0) == 0
+    # chunk (cb*56 + t)*64 + lane = W[32 cb + lane%32][16 t + 8 (lane/32) .. +7]
+    ref = W.reshape(16, 32, 56, 2, 8).transpose(0, 2, 3, 1, 4).reshape(-1)
+    torch.cuda.synchronize()
+    assert np.array_equal(Wp.cpu().numpy().view(np.uint16), ref)
+    invD = dev(rng.uniform(0.5, 2.0, Ng).astype(np.float32))
+    for r_host in (rng.standard_normal(Ng).astype(np.float32), np.zeros(Ng, dtype=np.float32)):
+        r = dev(r_host)
+        X = torch.zeros(E * 896, dtype=torch.int16, device="cuda")
+        sc1 = torch.full((E,), -1.0, device="cuda")
+        sc2 = torch.full((E,), -2.0, device="cuda")
+        Y1 = torch.full((E * 512,), 7.0, device="cuda")
+        Y2 = torch.full((E * 512,), 9.0, device="cuda")
+        assert lib.libp_learned_features_f32(E, 512, 384, g2l.data_ptr(), tr.data_ptr(), r.data_ptr(),
+                                             invD.data_ptr(), X.data_ptr(), 896, sc1.data_ptr(), s0) == 0
+        assert lib.libp_mlp_linear_fwd_scaled(E, 896, 512, X.data_ptr(), 896, Wd.data_ptr(), 896, None,
+                                              sc1.data_ptr(), Y1.data_ptr(), 512, s0) == 0
+        assert lib.libp_learned_linear_layer_f32(E, 512, 384, 512, g2l.data_ptr(), tr.data_ptr(), r.data_ptr(),
+                                                 invD.data_ptr(), Wp.data_ptr(), Y2.data_ptr(), 512,
+                                                 sc2.data_ptr(), s0) == 0
+        torch.cuda.synchronize()
+        y1, y2 = Y1.cpu().numpy(), Y2.cpu().numpy()
+        print(f"E={E}: max |Y| {np.abs(y1).max():.3e}, differing bits {int((y1.view(np.int32) != y2.view(np.int32)).sum())}",
+              flush=True)
+        assert np.array_equal(y1.view(np.int32), y2.view(np.int32))
+        assert np.array_equal(sc1.cpu().numpy().view(np.int32), sc2.cpu().numpy().view(np.int32))
+        if not r_host.any():
+            assert not y2.any() and not sc2.cpu().numpy().any()
+        else:
+            assert np.abs(y1).max() > 0
+    # other shapes and a misaligned packed buffer are refused
+    assert lib.libp_learned_linear_layer_f32(E, 64, 96, 64, g2l.data_ptr(), tr.data_ptr(), r.data_ptr(),
+                                             invD.data_ptr(), Wp.data_ptr(), Y2.data_ptr(), 512, None, s0) != 0
+    assert lib.libp_learned_linear_layer_f32(E, 512, 384, 512, g2l.data_ptr(), tr.data_ptr(), r.data_ptr(),
+                                             invD.data_ptr(), Wp.data_ptr() + 2, Y2.data_ptr(), 512, None, s0) != 0
+    assert lib.libp_learned_pack_weights_bf16(500, 896, Wd.data_ptr(), 896, Wp.data_ptr(), s0) != 0
+
+
 def test_committed_linear_weights_beat_chebyshev_on_configuration_5():
     """Configuration 5 itself: curved 32^3 box, N=7, lambda=0, Fourier forcing of seed 1
     (no fit used it), FPCG + p-MG 7,3,1 in FP32 to 1e-8 relative. The committed linear

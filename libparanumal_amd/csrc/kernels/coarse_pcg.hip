@@ -137,7 +137,6 @@ __global__ void __launch_bounds__(256) coarse_ax_hex1_kernel(const int E, const 
                                                              double* __restrict__ AqL) {
   __shared__ double s_beta;
   __shared__ int s_stop;
-  if (s[8] != 0.0) return;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int e = t >> 3, n = t & 7;
   const bool live = e < E;  // every lane takes part in the shuffles
@@ -168,9 +167,12 @@ __global__ void __launch_bounds__(256) coarse_ax_hex1_kernel(const int E, const 
       rz = g2[1];
     }
     if (threadIdx.x == 0) {
-      int stop = 0;
+      // STOP raised by an earlier launch: return without writing anything (one
+      // thread reads the flag for the block, as in cg1_scalars)
+      const bool done = s[8] != 0.0;
+      int stop = done ? 1 : 0;
       double beta = 0.0;
-      if (!first) {
+      if (!first && !done) {
         if (s[7] > 0.0 && rr <= s[7])
           stop = 1;
         else
@@ -178,7 +180,7 @@ __global__ void __launch_bounds__(256) coarse_ax_hex1_kernel(const int E, const 
       }
       s_stop = stop;
       s_beta = beta;
-      if (blockIdx.x == 0) {
+      if (blockIdx.x == 0 && !done) {
         if (first) {
           s[3] = 0.0;
           s[1] = s[0];
@@ -349,9 +351,15 @@ __device__ inline void cg1_scalars(const int mode, const int P, const double tol
       rr = g3[2];
     }
     if (threadIdx.x == 0) {
-      int stop = 0;
+      // STOP raised by an earlier launch: return without writing anything. One
+      // thread reads the flag for the whole block (block 0 of the launch that
+      // stops raises it while other blocks may still be starting), and the
+      // decision reaches every wave through LDS after the barrier below.
+      const bool done = s[8] != 0.0;
+      int stop = done ? 1 : 0;
       double alpha = 0.0, beta = 0.0;
-      if (mode == 1) {
+      if (done) {
+      } else if (mode == 1) {
         // r0.r0 gives the relative threshold; a zero residual stops at once
         const double T = tol2 * rr;
         if (rr == 0.0) stop = 1;
@@ -369,7 +377,7 @@ __device__ inline void cg1_scalars(const int mode, const int P, const double tol
       s_stop = stop;
       s_ab[0] = alpha;
       s_ab[1] = beta;
-      if (blockIdx.x == 0 && mode > 0) {
+      if (blockIdx.x == 0 && mode > 0 && !done) {
         s[5] = rr;
         if (stop) {
           s[8] = 1.0;
@@ -394,7 +402,6 @@ __global__ void __launch_bounds__(256) coarse_cg1_ax_kernel(
   // mode 0: start (u = invD r, x = 0, no update); 1: first update (beta = 0); 2: later updates
   __shared__ double s_ab[2];
   __shared__ int s_stop;
-  if (s[8] != 0.0) return;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int e = t >> 3, n = t & 7;
   const bool live = e < E;
@@ -540,7 +547,6 @@ __global__ void __launch_bounds__(CSR_THREADS) coarse_cg1_csr_kernel(
   __shared__ double s_ab[2];
   __shared__ int s_stop;
   __shared__ double s_w[16];
-  if (s[8] != 0.0) return;
   // u at gathered dof j after this iteration's update, from its loaded
   // r, w, s and invD
   auto unew = [&](double ro, double wo, double so, double dg, double alpha, double beta) -> double {

@@ -457,13 +457,15 @@ void learnedSmoother_t::monitorStart() {
   monitorHist.clear();
   if (!active) return;
   fallback = false;  // every solve starts with the network
-  monitorRejected = stats()[3];
+  // only the MLP's energy-optimal step can be rejected: the linear smoother's
+  // fixed step needs no device read (and no stream synchronisation) here
+  monitorRejected = (arch == ARCH_MLP) ? stats()[3] : 0.0;
 }
 
 void learnedSmoother_t::monitorIteration(int it, double rdotr) {
   (void)it;
   if (!active || fallback) return;
-  const double rej = stats()[3];
+  const double rej = (arch == ARCH_MLP) ? stats()[3] : 0.0;
   std::string why;
   if (!std::isfinite(rdotr)) why = "non-finite residual";
   if (rej > monitorRejected) why = "rejected step";

@@ -100,7 +100,7 @@ __device__ inline void block_partial(const double acc, double* __restrict__ part
 // whole blocks first and the remainder with NE=1 (element offset e0). A
 // clamped/predicated tail inside this kernel defeats the compiler's uniform
 // address analysis and roughly quadruples register use.
-template <int Nq, int NE, typename T, int MODE, int OUT = 0, bool PAP = false>
+template <int Nq, int NE, typename T, int MODE, int OUT = 0, bool PAP = false, bool LAM = true>
 __global__ void __launch_bounds__(NE * Nq * Nq) __attribute__((amdgpu_waves_per_eu(AxShape<Nq, T>::waves, 8)))
     ax_hex_kernel(const int e0, const int* __restrict__ elementList, const int* __restrict__ G2L,
                   const T* __restrict__ wJ, const T* __restrict__ ggeo, const T* __restrict__ D, const T lambda,
@@ -165,7 +165,8 @@ __global__ void __launch_bounds__(NE * Nq * Nq) __attribute__((amdgpu_waves_per_
     const T G11 = __builtin_nontemporal_load(gb + 3 * Np + k * NqNq);
     const T G12 = __builtin_nontemporal_load(gb + 4 * Np + k * NqNq);
     const T G22 = __builtin_nontemporal_load(gb + 5 * Np + k * NqNq);
-    const T GwJ = __builtin_nontemporal_load(wb + k * NqNq);
+    // LAM = false (lambda = 0): wJ is not read; lambda * GwJ * q is +-0 with either GwJ
+    const T GwJ = LAM ? __builtin_nontemporal_load(wb + k * NqNq) : T(0);
 
     s_q[es][j][i] = r_q[k];
     T qt = 0;
@@ -235,19 +236,30 @@ __global__ void __launch_bounds__(NE * Nq * Nq) __attribute__((amdgpu_waves_per_
   }
 }
 
+// lambda = 0 (the Poisson operator) takes the instances that do not read wJ:
+// the mass term lambda * wJ * q is +-0 in both, so the results are the same to
+// the bit, and the kernel moves 8 (FP64) or 4 (FP32) bytes less per local node.
+template <int Nq, typename T, int MODE, bool LAM>
+static void launch_ax_lam(int Nelements, const int* elist, const int* g2l, const T* wJ, const T* ggeo, const T* D,
+                          T lambda, const T* q, T* Aq, hipStream_t st) {
+  using S = AxShape<Nq, T>;
+  const int full = Nelements / S::NE;
+  if (full > 0)
+    hipLaunchKernelGGL((ax_hex_kernel<Nq, S::NE, T, MODE, 0, false, LAM>), dim3(full), dim3(S::threads), 0, st, 0,
+                       elist, g2l, wJ, ggeo, D, lambda, q, Aq);
+  const int rest = Nelements - full * S::NE;
+  if (S::NE > 1 && rest > 0)
+    hipLaunchKernelGGL((ax_hex_kernel<Nq, 1, T, MODE, 0, false, LAM>), dim3(rest), dim3(S::NqNq), 0, st,
+                       full * S::NE, elist, g2l, wJ, ggeo, D, lambda, q, Aq);
+}
 template <int Nq, typename T, int MODE>
 static int launch_ax(int Nelements, const int* elist, const int* g2l, const T* wJ, const T* ggeo, const T* D,
                      T lambda, const T* q, T* Aq, hipStream_t st) {
-  using S = AxShape<Nq, T>;
   if (Nelements <= 0) return LIBP_OK;
-  const int full = Nelements / S::NE;
-  if (full > 0)
-    hipLaunchKernelGGL((ax_hex_kernel<Nq, S::NE, T, MODE>), dim3(full), dim3(S::threads), 0, st, 0, elist, g2l,
-                       wJ, ggeo, D, lambda, q, Aq);
-  const int rest = Nelements - full * S::NE;
-  if (S::NE > 1 && rest > 0)
-    hipLaunchKernelGGL((ax_hex_kernel<Nq, 1, T, MODE>), dim3(rest), dim3(S::NqNq), 0, st, full * S::NE, elist,
-                       g2l, wJ, ggeo, D, lambda, q, Aq);
+  if (lambda == T(0))
+    launch_ax_lam<Nq, T, MODE, false>(Nelements, elist, g2l, wJ, ggeo, D, lambda, q, Aq, st);
+  else
+    launch_ax_lam<Nq, T, MODE, true>(Nelements, elist, g2l, wJ, ggeo, D, lambda, q, Aq, st);
   return launch_status("ax_hex");
 }
 
@@ -259,8 +271,8 @@ static int split_blocks(int Nelements) {
   return Nelements / S::NE + ((S::NE > 1) ? Nelements % S::NE : 0);
 }
 
-template <int Nq>
-static int launch_ax_split(int Nelements, const int* elist, const int* g2l, const double* wJ, const double* ggeo,
+template <int Nq, bool LAM>
+static int launch_ax_split_lam(int Nelements, const int* elist, const int* g2l, const double* wJ, const double* ggeo,
                            const double* D, double lambda, const double* q, double* AqF, double* Ag, double* pap,
                            const int* fslot, hipStream_t st) {
   using S = AxShape<Nq, double>;
@@ -269,20 +281,28 @@ static int launch_ax_split(int Nelements, const int* elist, const int* g2l, cons
   const int rest = Nelements - full * S::NE;
   if (pap) {
     if (full > 0)
-      hipLaunchKernelGGL((ax_hex_kernel<Nq, S::NE, double, 0, 1, true>), dim3(full), dim3(S::threads), 0, st, 0,
+      hipLaunchKernelGGL((ax_hex_kernel<Nq, S::NE, double, 0, 1, true, LAM>), dim3(full), dim3(S::threads), 0, st, 0,
                          elist, g2l, wJ, ggeo, D, lambda, q, AqF, Ag, pap, 0, fslot);
     if (S::NE > 1 && rest > 0)
-      hipLaunchKernelGGL((ax_hex_kernel<Nq, 1, double, 0, 1, true>), dim3(rest), dim3(S::NqNq), 0, st,
+      hipLaunchKernelGGL((ax_hex_kernel<Nq, 1, double, 0, 1, true, LAM>), dim3(rest), dim3(S::NqNq), 0, st,
                          full * S::NE, elist, g2l, wJ, ggeo, D, lambda, q, AqF, Ag, pap, full, fslot);
   } else {
     if (full > 0)
-      hipLaunchKernelGGL((ax_hex_kernel<Nq, S::NE, double, 0, 1, false>), dim3(full), dim3(S::threads), 0, st, 0,
+      hipLaunchKernelGGL((ax_hex_kernel<Nq, S::NE, double, 0, 1, false, LAM>), dim3(full), dim3(S::threads), 0, st, 0,
                          elist, g2l, wJ, ggeo, D, lambda, q, AqF, Ag, nullptr, 0, fslot);
     if (S::NE > 1 && rest > 0)
-      hipLaunchKernelGGL((ax_hex_kernel<Nq, 1, double, 0, 1, false>), dim3(rest), dim3(S::NqNq), 0, st,
+      hipLaunchKernelGGL((ax_hex_kernel<Nq, 1, double, 0, 1, false, LAM>), dim3(rest), dim3(S::NqNq), 0, st,
                          full * S::NE, elist, g2l, wJ, ggeo, D, lambda, q, AqF, Ag, nullptr, 0, fslot);
   }
   return launch_status("ax_hex_split");
+}
+template <int Nq>
+static int launch_ax_split(int Nelements, const int* elist, const int* g2l, const double* wJ, const double* ggeo,
+                           const double* D, double lambda, const double* q, double* AqF, double* Ag, double* pap,
+                           const int* fslot, hipStream_t st) {
+  if (lambda == 0.0)
+    return launch_ax_split_lam<Nq, false>(Nelements, elist, g2l, wJ, ggeo, D, lambda, q, AqF, Ag, pap, fslot, st);
+  return launch_ax_split_lam<Nq, true>(Nelements, elist, g2l, wJ, ggeo, D, lambda, q, AqF, Ag, pap, fslot, st);
 }
 
 // p.Ap from the partials of the PCG-form Ax kernel, summed in block order,
@@ -301,8 +321,8 @@ __global__ void __launch_bounds__(1024) ax_pap_finish_kernel(const int nb, doubl
 
 // Partial Ax of p (gathered, through GlobalToLocal) into the E-vector with
 // p.Ap from the local nodes and alpha (MODE 0, OUT 0, PAP).
-template <int Nq>
-static int launch_ax_pap(int Nelements, const int* elist, const int* g2l, const double* wJ, const double* ggeo,
+template <int Nq, bool LAM>
+static int launch_ax_pap_lam(int Nelements, const int* elist, const int* g2l, const double* wJ, const double* ggeo,
                          const double* D, double lambda, const double* p, double* AqL, double* part, double* s,
                          hipStream_t st) {
   using S = AxShape<Nq, double>;
@@ -310,14 +330,21 @@ static int launch_ax_pap(int Nelements, const int* elist, const int* g2l, const 
   const int full = Nelements / S::NE;
   const int rest = Nelements - full * S::NE;
   if (full > 0)
-    hipLaunchKernelGGL((ax_hex_kernel<Nq, S::NE, double, 0, 0, true>), dim3(full), dim3(S::threads), 0, st, 0, elist,
+    hipLaunchKernelGGL((ax_hex_kernel<Nq, S::NE, double, 0, 0, true, LAM>), dim3(full), dim3(S::threads), 0, st, 0, elist,
                        g2l, wJ, ggeo, D, lambda, p, AqL, nullptr, part, 0, nullptr, s);
   if (S::NE > 1 && rest > 0)
-    hipLaunchKernelGGL((ax_hex_kernel<Nq, 1, double, 0, 0, true>), dim3(rest), dim3(S::NqNq), 0, st, full * S::NE,
+    hipLaunchKernelGGL((ax_hex_kernel<Nq, 1, double, 0, 0, true, LAM>), dim3(rest), dim3(S::NqNq), 0, st, full * S::NE,
                        elist, g2l, wJ, ggeo, D, lambda, p, AqL, nullptr, part, full, nullptr, s);
   hipLaunchKernelGGL(ax_pap_finish_kernel, dim3(1), dim3(1024), 0, st, split_blocks<Nq>(Nelements), s,
                      (const double*)part);
   return launch_status("pcg_ax_hex_pap");
+}
+template <int Nq>
+static int launch_ax_pap(int Nelements, const int* elist, const int* g2l, const double* wJ, const double* ggeo,
+                         const double* D, double lambda, const double* p, double* AqL, double* part, double* s,
+                         hipStream_t st) {
+  if (lambda == 0.0) return launch_ax_pap_lam<Nq, false>(Nelements, elist, g2l, wJ, ggeo, D, lambda, p, AqL, part, s, st);
+  return launch_ax_pap_lam<Nq, true>(Nelements, elist, g2l, wJ, ggeo, D, lambda, p, AqL, part, s, st);
 }
 
 template <typename T, int MODE>

@@ -56,6 +56,39 @@ def test_ax_kernel_f64(N):
     assert relerr(out.cpu().numpy(), ref) < 1e-12
 
 
+@pytest.mark.parametrize("N", [1, 3, 7, 11])
+def test_ax_kernel_lambda_zero(N):
+    """lambda = 0 (the Poisson operator) takes the kernel instance that does not read
+    wJ: it matches the oracle to 1e-12, and a wJ full of NaN changes nothing (any read
+    of it would turn the result into NaN). FP32 agrees with FP64 to FP32 rounding."""
+    n = 3 if N <= 8 else 2
+    o = OracleElliptic(box(n, N, curve=0.05, lam=0.0))
+    rng = np.random.default_rng(100 + N)
+    q = rng.uniform(-1, 1, o.geti("Ngather"))
+    ref = o.partial_ax(q)
+    E = o.geti("Nelements")
+    g2l, ggeo, D = dev(o.array("GlobalToLocal")), dev(o.array("ggeo")), dev(o.array("D"))
+    wJnan = torch.full((o.array("wJ").size,), np.nan, dtype=torch.float64, device="cuda")
+    qd = dev(q)
+    out = torch.full((ref.size,), 7.0, dtype=torch.float64, device="cuda")
+    st = L().libp_ax_hex_partial_f64(N, E, None, g2l.data_ptr(), wJnan.data_ptr(), ggeo.data_ptr(), D.data_ptr(),
+                                     0.0, qd.data_ptr(), out.data_ptr(), stream())
+    assert st == 0
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    assert np.isfinite(got).all()
+    assert relerr(got, ref) < 1e-12
+    out32 = torch.full((ref.size,), 7.0, dtype=torch.float32, device="cuda")
+    wJ32, ggeo32, D32, q32 = wJnan.float(), ggeo.float(), D.float(), qd.float()  # held until the kernel has run
+    st = L().libp_ax_hex_partial_f32(N, E, None, g2l.data_ptr(), wJ32.data_ptr(), ggeo32.data_ptr(), D32.data_ptr(),
+                                     0.0, q32.data_ptr(), out32.data_ptr(), stream())
+    assert st == 0
+    torch.cuda.synchronize()
+    got32 = out32.cpu().numpy().astype(np.float64)
+    assert np.isfinite(got32).all()
+    assert relerr(got32, ref) < 1e-5
+
+
 @pytest.mark.parametrize("use_list", [False, True])
 def test_ax_mfma_n15(use_list):
     o = OracleElliptic(box(2, 15, curve=0.05, lam=0.9))

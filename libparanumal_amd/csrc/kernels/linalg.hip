@@ -152,7 +152,27 @@ __global__ void __launch_bounds__(RBS) pcg_rdotz_f32in_kernel(const int N, const
   __shared__ double s_w[16];
   if (s[8] != 0.0) return;
   double a0 = 0, a1 = 0;
-  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
+  // four grid-stride steps per round with their loads issued together, in the
+  // order of a plain grid-stride loop (the partial sums are unchanged)
+  const int stride = gridDim.x * blockDim.x;
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  for (; (long)n + 3L * stride < N; n += 4 * stride) {
+    double rv[4], zv[4], av[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int m = n + u * stride;
+      rv[u] = __builtin_nontemporal_load(r + m);
+      zv[u] = (double)__builtin_nontemporal_load(z32 + m);
+      av[u] = flexible ? __builtin_nontemporal_load(Ap + m) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      z[n + u * stride] = zv[u];
+      a0 += rv[u] * zv[u];
+      if (flexible) a1 += zv[u] * av[u];
+    }
+  }
+  for (; n < N; n += stride) {
     const double rn = r[n];
     const double zn = (double)z32[n];
     z[n] = zn;
@@ -204,7 +224,34 @@ __global__ void __launch_bounds__(RBS) pcg_update_kernel(const int N, const doub
   if (s[8] != 0.0) return;  // converged in an earlier iteration of this chunk
   const double alpha = s[2];
   double a0 = 0;
-  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
+  // four grid-stride steps per round, their loads issued together (streamed
+  // once: non-temporal); each thread visits its rows in the same order as a
+  // plain grid-stride loop, so the partial sums are unchanged
+  const int stride = gridDim.x * blockDim.x;
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  for (; (long)n + 3L * stride < N; n += 4 * stride) {
+    double rv[4], xv[4], pv[4], av[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int m = n + u * stride;
+      rv[u] = __builtin_nontemporal_load(r + m);
+      xv[u] = __builtin_nontemporal_load(x + m);
+      pv[u] = __builtin_nontemporal_load(p + m);
+      av[u] = __builtin_nontemporal_load(Ap + m);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int m = n + u * stride;
+      double xn = xv[u];
+      double rn = rv[u];
+      xn += alpha * pv[u];
+      rn -= alpha * av[u];
+      a0 += rn * rn;
+      __builtin_nontemporal_store(xn, x + m);
+      __builtin_nontemporal_store(rn, r + m);
+    }
+  }
+  for (; n < N; n += stride) {
     double rn = r[n];
     x[n] += alpha * p[n];
     rn -= alpha * Ap[n];

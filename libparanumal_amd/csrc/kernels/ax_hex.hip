@@ -18,6 +18,12 @@
 //    the dominant arrays);
 //  * the E-vector result is written with non-temporal stores (it is consumed
 //    by the gather kernel, not re-read by this one).
+//  * lambda = 0 (the Poisson operator) runs an instance that does not read wJ
+//    (the reference reads it whatever lambda is): the mass term is +-0 in both,
+//    so the result is the same to the bit, and the FP64 kernel streams 60
+//    instead of 68 bytes per local node (48^3, N=7: 649 -> 604 us). Testing
+//    lambda inside the k-loop instead gave the same bits at 2.4x the time: the
+//    branches split the loop into blocks the scheduler no longer overlapped.
 //
 // The split output form (OUT = 1) serves the whole C0 operator A = Q^T A_L Q
 // (ellipticOperator.cpp:29-107) without writing the full E-vector: a node

@@ -135,6 +135,59 @@ def test_stop_flag_skips_both_kernels(N):
     assert torch.equal(s, s0)
 
 
+def test_stop_flag_coarse_iterations_write_nothing():
+    """Coarse Jacobi-PCG launches queued after STOP (the chunks run ahead of the host's
+    check) write nothing: the single-reduction iteration on a 70,000-row CSR matrix
+    (547 blocks) in every mode and parity, and the three-kernel N=1 iteration. Each
+    block reads the flag once and all its waves return together."""
+    import ctypes
+    L = lp().lib()
+    st = torch.cuda.current_stream().cuda_stream
+    f64 = dict(dtype=torch.float64, device="cuda")
+    n = 70000
+    rows = np.arange(n)
+    starts = np.concatenate([[0], np.cumsum(np.where((rows == 0) | (rows == n - 1), 2, 3))]).astype(np.int32)
+    cols = np.concatenate([[j for j in (i - 1, i, i + 1) if 0 <= j < n] for i in range(n)]).astype(np.int32)
+    vals = np.concatenate([[-1.0 if j != i else 2.0 for j in (i - 1, i, i + 1) if 0 <= j < n] for i in range(n)])
+    rs, ci, va = (torch.from_numpy(a).cuda() for a in (starts, cols, vals))
+    invD = torch.full((n,), 0.5, **f64)
+    vec = lambda: torch.full((n,), 7.0, **f64)
+    x = vec()
+    pairs = {k: [vec(), vec()] for k in "wpsru"}
+    arr = {k: (ctypes.c_void_p * 2)(*(t.data_ptr() for t in v)) for k, v in pairs.items()}
+    wsb = L.libp_reduce_ws_bytes()
+    ws = torch.full((2 * wsb // 8 + 1,), 3.0, **f64)
+    s = torch.zeros(16, **f64)
+    s[7], s[8] = 1e-3, 1.0
+    s0, ws0 = s.clone(), ws.clone()
+    for mode in (0, 1, 2):
+        for parity in (0, 1):
+            lp().check(L.libp_pcg_jacobi_csr_cg1_f64(n, rs.data_ptr(), ci.data_ptr(), va.data_ptr(), invD.data_ptr(),
+                                                     1e-2, mode, parity, x.data_ptr(), arr["w"], arr["p"], arr["s"],
+                                                     arr["r"], arr["u"], s.data_ptr(), ws.data_ptr(), st))
+    torch.cuda.synchronize()
+    assert bool((x == 7.0).all())
+    assert all(bool((t == 7.0).all()) for v in pairs.values() for t in v)
+    assert torch.equal(s, s0) and torch.equal(ws, ws0)
+
+    o = OracleElliptic(box(5, 1))
+    E, Ng = o.geti("Nelements"), o.geti("Ngather")
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    g2l, wJ, ggeo, D = dev(o.array("GlobalToLocal")), dev(o.array("wJ")), dev(o.array("ggeo")), dev(o.array("D"))
+    grs, gci = dev(o.array("rowStartsT")), dev(o.array("colIdsT"))
+    v1 = [torch.full((Ng,), 7.0, **f64) for _ in range(6)]  # invD, x, r, z, p, Ap
+    AqL = torch.full((E * 8,), 7.0, **f64)
+    for first in (1, 0):
+        lp().check(L.libp_pcg_jacobi_hex1_iteration_f64(E, g2l.data_ptr(), wJ.data_ptr(), ggeo.data_ptr(),
+                                                        D.data_ptr(), 1.0, Ng, grs.data_ptr(), gci.data_ptr(),
+                                                        *[t.data_ptr() for t in v1[:1]], first,
+                                                        *[t.data_ptr() for t in v1[1:]], AqL.data_ptr(),
+                                                        s.data_ptr(), ws.data_ptr(), st))
+    torch.cuda.synchronize()
+    assert all(bool((t == 7.0).all()) for t in v1) and bool((AqL == 7.0).all())
+    assert torch.equal(s, s0) and torch.equal(ws, ws0)
+
+
 def test_fused_kernels_reject_bad_arguments():
     L = lp().lib()
     v = torch.zeros(64, dtype=torch.float64, device="cuda")

@@ -370,6 +370,14 @@ def main():
         stream.synchronize()
     settle_s = time.time() - t_settle
     ms_ax = timed(ax, args.steps, args.warmup)
+    # the same Ax through the instance that reads all seven geometric factors per
+    # node (wJ included), as the reference kernel does whatever lambda is: the
+    # operator with lambda = 1 on the same mesh and arrays
+    g2l_p, wJ_p, gg_p, D_p = (el.array_info(nm)[0] for nm in ("GlobalToLocal", "wJ", "ggeo", "D"))
+    L_ax = lp.lib()
+    ax_wj = lambda: lp.check(L_ax.libp_ax_hex_partial_f64(7, E, None, g2l_p, wJ_p, gg_p, D_p, 1.0, q.data_ptr(),
+                                                          AqL.data_ptr(), el.stream))
+    ms_ax_wj = timed(ax_wj, args.steps, args.warmup)
     ms_op = timed(lambda: el.operator(q, Aq), max(10, args.steps // 2), 2)
     L = lp.lib()
     rs, _, _, _ = el.array_info("rowStartsT")
@@ -418,6 +426,11 @@ def main():
             "ms_per_step_of_launches_after_warmup": sum(cold[args.warmup:]) / max(len(cold) - args.warmup, 1),
             "gdofs": Ng / (sum(cold[args.warmup:]) / max(len(cold) - args.warmup, 1) * 1e-3) / 1e9,
             "launch_us": [round(1e3 * c, 1) for c in cold]},
+        "ax_reading_wJ": {"what": "the same Ax launched with lambda = 1: the kernel instance that reads all seven "
+                                  "geometric factors per node, as the reference kernel does at any lambda",
+                          "ms": ms_ax_wj, "gdofs": Ng / (ms_ax_wj * 1e-3) / 1e9,
+                          "GBs": (68 * NL + 8 * Ng) / (ms_ax_wj * 1e-3) / 1e9,
+                          "frac": (68 * NL + 8 * Ng) / (ms_ax_wj * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "operator": {"ms": ms_op, "gdofs": Ng / (ms_op * 1e-3) / 1e9,
                      "form": "split" if el.geti("splitOperator") == 1 else "two-pass",
                      "frac_of_two_pass_bytes": bytes_op / (ms_op * 1e-3) / 1e9 / HBM_PEAK_GBS},
